@@ -1,0 +1,162 @@
+/*
+ * fc5.h -- C ABI of the MI355X class-5 free-peel engine (libfc5.so).
+ *
+ * The reference has no library boundary: its hot path sits behind the CLI
+ * fourcore_cls5_gpu_v4 (fourcore_cls5_gpu_v4.cu:611-709), the in-process seam
+ * run_cls5_gpu (fourcore_cls5_gpu_v4.cu:412-413), the Stage-1 emitter
+ * (fourcore_hunt_v3.cpp:371-381) and the table writer (xor_build_save.cpp:21-57).
+ * Each entry point below names the reference interface it replaces.  All
+ * signatures are plain C: pointers, sizes and POD structs; no C++ or torch
+ * types.  Every function returns FC5_OK (0) or a negative FC5_E* code; the
+ * message for the last error of a context is fc5_last_error(ctx).  Nothing in
+ * the library calls exit().
+ *
+ * Ownership: the caller owns every array it passes (the library copies what it
+ * keeps before returning) and every output buffer.  The library owns device
+ * memory (tables, units, hits, counters).  One context per host thread.
+ */
+#ifndef FC5_H
+#define FC5_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FC5_ABI_VERSION 1
+
+enum {
+  FC5_OK = 0,
+  FC5_E_ARG = -1,        /* bad argument */
+  FC5_E_IO = -2,         /* cannot open / short read / short write */
+  FC5_E_FORMAT = -3,     /* bad header, magic, kind or version */
+  FC5_E_NMISMATCH = -4,  /* table N differs from the requested N */
+  FC5_E_FPR = -5,        /* FPR smoke above 16/200000: refuse to search */
+  FC5_E_HIP = -6,        /* HIP runtime error */
+  FC5_E_NOMEM = -7,      /* device or host allocation failed */
+  FC5_E_STATE = -8,      /* no table loaded, unknown band, ... */
+  FC5_E_OVERFLOW = -9,   /* hit buffer overflowed (hits were dropped) */
+  FC5_E_BUILD = -10      /* xor construction stalled at every seed */
+};
+
+/* search modes of fc5_band_run */
+enum {
+  FC5_MODE_PLAIN = 0,   /* reference shape: 16-unit chunks, one synchronous launch each */
+  FC5_MODE_BATCHED = 1  /* one persistent launch fed by a device work queue */
+};
+
+/* StoreHeader v2 (spike/ribbon/store_header.hpp:23-37): 64 bytes, little-endian */
+typedef struct {
+  uint32_t magic, version, kind, r, w, N;
+  uint64_t n_keys, m_cells, mix_seed;
+  uint32_t shard_count, shard_index, shard_mode, reserved;
+} fc5_store_header;
+
+/* one Stage-1 unit of class 5: a ".buc" line "5 B u" (fourcore_cls5_gpu_v4.cu:155-167) */
+typedef struct {
+  uint64_t B, u;
+} fc5_unit;
+
+/* counters of one run.  Reference names (fourcore_cls5_gpu_v4.cu:382-409):
+ * calls = every valid (unit,e,d) grid point; gated = gate rejects;
+ * probes = gate survivors (xor queries); maybe = xor maybe-hits (nhit).
+ * neg/zero split the remaining points (negative / zero residual). */
+typedef struct {
+  uint64_t calls, neg, zero, gated, probes;
+  uint64_t maybe, side_maybe;
+  uint64_t n_hits, hits_dropped;
+  uint64_t solutions;
+  uint64_t units, units_skipped, units_irregular;
+  double kernel_ms, wall_ms;
+} fc5_result;
+
+/* a filter hit (fourcore_cls5_gpu_v4.cu:140).  unit indexes the band's units
+ * as passed in; flags bit0 = main table maybe, bit1 = side (planted) table maybe */
+typedef struct {
+  uint32_t unit, d, e, flags;
+  uint64_t fp;
+} fc5_hit;
+
+/* a verified solution line "SOLUTION cls=5 B= u= a1..a5 gcd=" (fourcore_cls5_gpu_v4.cu:524-536) */
+typedef struct {
+  uint64_t B, u;
+  uint32_t d, e, a, b;
+  int64_t terms[5];
+  int64_t gcd;
+} fc5_solution;
+
+typedef struct fc5_ctx fc5_ctx;
+typedef struct fc5_band fc5_band;
+
+int fc5_abi_version(void);
+
+/* ---- context (replaces main's cudaSetDevice and run_cls5_gpu set-up,
+ *      fourcore_cls5_gpu_v4.cu:414-466,673) ---- */
+int fc5_ctx_create(int device, fc5_ctx **out);
+void fc5_ctx_destroy(fc5_ctx *ctx);
+const char *fc5_last_error(const fc5_ctx *ctx);
+/* options: "hit_cap" (default 1<<20), "use_gate" (1; 0 = --no-gate,
+ * fourcore_cls5_gpu_v4.cu:634), "rows_per_item" (8), "waves_per_cu" (0 = auto),
+ * "verify" (1: pair recovery + exact check of hits on the host),
+ * "stop_first" (0, fourcore_cls5_gpu_v4.cu:635) */
+int fc5_set_option(fc5_ctx *ctx, const char *key, int64_t value);
+
+/* ---- pair tables (replace acquire_xor_filter + the H2D upload,
+ *      fourcore_cls5_gpu_v4.cu:427-440,568-594) ---- */
+/* xor_load_file (fourcore_xor_store.hpp:147-181) with the exact-N check
+ * (expected_N <= 0 accepts any N) and the FPR smoke refusal (> 16/200000),
+ * then upload.  Returns FC5_E_IO / FC5_E_FORMAT / FC5_E_NMISMATCH / FC5_E_FPR. */
+int fc5_table_load(fc5_ctx *ctx, const char *path, int expected_N);
+/* upload a table already in host memory (header + packed bytes as on disk) */
+int fc5_table_upload(fc5_ctx *ctx, const fc5_store_header *hdr, const uint8_t *packed, size_t len);
+/* build the pair table for 1<=a<=b<=N at r bits (xor_build_pairs,
+ * fourcore_xor_store.hpp:67-119) plus n_extra extra keys.  on_gpu = 0 builds
+ * on the host, byte-identical to the reference builder; on_gpu = 1 builds on
+ * the device (a valid xor filter over the same keys, different bytes). */
+int fc5_table_build(fc5_ctx *ctx, int N, int r, const uint64_t *extra, size_t n_extra, int on_gpu);
+int fc5_table_info(const fc5_ctx *ctx, fc5_store_header *out);
+/* write the current table as StoreHeader v2 + packed bytes (xor_save_file,
+ * fourcore_xor_store.hpp:126-143) */
+int fc5_table_save(fc5_ctx *ctx, const char *path);
+/* query the device table for n keys (xor_might_contain, xor_filter.hpp:472-483) */
+int fc5_table_query(fc5_ctx *ctx, const uint64_t *keys, size_t n, uint8_t *out);
+/* FPR smoke on the device table (xor_fpr_smoke, fourcore_xor_store.hpp:54-65) */
+int fc5_table_fpr_smoke(fc5_ctx *ctx, int trials, int *hits);
+
+/* ---- planted-key side table: a small xor filter probed with the same
+ *      fingerprint as the main table; no reference counterpart (the reference
+ *      plants by fabricating units, solve_516_v3.cu:1149-1170) ---- */
+int fc5_side_build(fc5_ctx *ctx, const uint64_t *keys, size_t n, int r);
+int fc5_side_clear(fc5_ctx *ctx);
+
+/* ---- bands of units (replace load_buc5 + pack_unit5 + the chunk loop of
+ *      run_cls5_gpu, fourcore_cls5_gpu_v4.cu:155-184,468-554) ---- */
+/* pack and upload units; units the reference would not pack are skipped and
+ * counted.  The band keeps the caller's unit order for hit records. */
+int fc5_band_create(fc5_ctx *ctx, const fc5_unit *units, size_t n, fc5_band **out);
+int fc5_band_destroy(fc5_ctx *ctx, fc5_band *band);
+/* run one band: all counters, hits and (if "verify") solutions */
+int fc5_band_run(fc5_ctx *ctx, fc5_band *band, int mode, fc5_result *res);
+/* one-call form: create, run, destroy (run_cls5_gpu, fourcore_cls5_gpu_v4.cu:412) */
+int fc5_run_units(fc5_ctx *ctx, const fc5_unit *units, size_t n, int mode, fc5_result *res);
+/* hits and solutions of the last run of this context */
+int fc5_last_hits(fc5_ctx *ctx, fc5_hit *out, size_t cap, size_t *n);
+int fc5_last_solutions(fc5_ctx *ctx, fc5_solution *out, size_t cap, size_t *n);
+/* closed-form calls of a unit list (the counter `calls`, fourcore_cls5_gpu_v4.cu:383-386) */
+int fc5_units_calls(const fc5_unit *units, size_t n, uint64_t *calls);
+
+/* ---- host helpers shared by the CLIs (no device use) ---- */
+/* Stage-1 emission over [lo,hi] for classes in cls_mask (bit c = class c),
+ * u-band [lo_frac,hi_frac) (fourcore_hunt_v3.cpp:371-381, fourcore_classes_v3.hpp:137-154).
+ * Writes up to cap triples; *n gets the full count. */
+int fc5_emit_units(uint64_t lo, uint64_t hi, unsigned cls_mask, double lo_frac, double hi_frac,
+                   uint32_t *out_cls, uint64_t *out_B, uint64_t *out_u, size_t cap, size_t *n);
+/* host-only xor pair-table build + save (xor_build_save.cpp:21-57) */
+int fc5_build_pairs_file(int N, int r, const char *path, fc5_store_header *hdr_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FC5_H */

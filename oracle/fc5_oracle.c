@@ -1,0 +1,933 @@
+/*
+ * fc5_oracle.c -- CPU restatement of the reference class-5 free peel.
+ *
+ * TEST INFRASTRUCTURE ONLY (tests/, __graft_entry__.smoke(), bench.py's
+ * cpu_baseline leg).  It is the checker, never the thing measured or shipped.
+ *
+ * Function -> reference lines it restates -> what pins it (tests/):
+ *   class contracts  fourcore_classes_v3.hpp:37-232
+ *       pinned by: the reference header compiled in oracle/_ref (units and
+ *       residues for config 1 and sampled B, committed as golden vectors);
+ *       seed counts 144/24/36/6/6 at B=100003 (fourcore_hunt_v3.cpp:49-67);
+ *       published count units=5765343 on [4.5M,5.0M] (README.md:146);
+ *       brute force of the residue definitions on small B.
+ *   pow6_192/sub192/mod192  fourcore_cls5_gpu_v4.cu:78-108,270-308
+ *       pinned by: Python big integers (closed form) on random and edge
+ *       inputs, KATs of the reference self-test (fourcore_cls5_gpu_v4.cu:191-201).
+ *   gate             fourcore_cls5_gpu_v4.cu:124-137,393-402
+ *       pinned by: set sizes 27/504 and 50/247 (fourcore_find_v4.cu:78-88),
+ *       brute-force pair-sum sets in Python, soundness on real pair sums.
+ *   funnel / exact T fourcore_cls5_gpu_v4.cu:62-75,266-268; fourcore_gmp.hpp:55-62
+ *       pinned by: the pre-wall KAT (100003,17646,9,44) and Python big-int T
+ *       on random grid points pre- and post-2^128.
+ *   xor build/query/io  spike/ribbon/xor_filter.hpp:53-64,185-192,256-483,
+ *       xor_pack.hpp:14-113, mix64.hpp:13-25, fourcore_xor_store.hpp:54-181
+ *       pinned by: byte-identical .bin against the reference xor_build_save
+ *       compiled from its own sources into oracle/_ref (SHA-256 fixtures for
+ *       several (N,r)); no false negatives over every key.
+ *   cls5 peel counters / maybe-hit list  fourcore_cls5_gpu_v4.cu:353-410
+ *       pinned by: calls = closed form; neg/zero/gated/probes by an
+ *       independent Python big-int brute force over sampled units; planted
+ *       keys recovered 100%.  The reference's GPU binary cannot be built
+ *       here (needs nvcc and GMP), so whole-config per-gate counts rest on
+ *       this restatement plus the sampled brute force.
+ *   host verify      fourcore_cls5_gpu_v4.cu:513-542, fourcore_xor_store.hpp:23-51
+ *       pinned by: synthetic exact identities in Python (a fabricated unit
+ *       whose residual is a real pair sum) -- no (6,1,5) solution is known.
+ */
+#include "fc5_oracle.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef unsigned __int128 u128;
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+#define M2 64LL
+#define M3 729LL
+#define M7 117649LL
+#define M14 7529536LL
+#define M21 85766121LL
+#define M42 5489031744LL
+
+/* ------------------------------------------------------------------ */
+/* class contracts (fourcore_classes_v3.hpp)                           */
+/* ------------------------------------------------------------------ */
+
+int64_t or_mod_pow6(int64_t base, int64_t mod) { /* classes:37-43 */
+  int64_t r = 1;
+  base %= mod;
+  if (base < 0) base += mod;
+  for (int e = 0; e < 6; ++e) r = (r * base) % mod;
+  return r;
+}
+
+static int64_t egcd(int64_t a, int64_t b, int64_t *x, int64_t *y) { /* classes:44-50 */
+  if (b == 0) { *x = 1; *y = 0; return a; }
+  int64_t x1, y1;
+  int64_t g = egcd(b, a % b, &x1, &y1);
+  *x = y1;
+  *y = x1 - y1 * (a / b);
+  return g;
+}
+static int64_t mod_inv(int64_t a, int64_t m) { /* classes:51-56 */
+  int64_t x, y;
+  egcd(a, m, &x, &y);
+  x %= m;
+  if (x < 0) x += m;
+  return x;
+}
+static int64_t crt2(int64_t r1, int64_t m1, int64_t r2, int64_t m2) { /* classes:57-66 */
+  int64_t x, y;
+  int64_t g = egcd(m1, m2, &x, &y);
+  int64_t t = ((r2 - r1) / g) * x;
+  int64_t mod = m1 / g * m2;
+  int64_t ans = r1 + m1 * (t % (m2 / g));
+  ans %= mod;
+  if (ans < 0) ans += mod;
+  return ans;
+}
+static int64_t gcd64(int64_t a, int64_t b) {
+  if (a < 0) a = -a;
+  if (b < 0) b = -b;
+  while (b) { int64_t t = a % b; a = b; b = t; }
+  return a;
+}
+
+/* Root tables (classes:68-78) in CSR form: for residue t, the a in [1,M)
+ * coprime to M with a^6 = t (mod M), ascending. */
+typedef struct { int32_t *off; int32_t *val; int64_t M; } roots_t;
+static roots_t g_r2, g_r3, g_r7;
+static int g_roots_ready = 0;
+
+static void build_roots(roots_t *R, int64_t M) {
+  R->M = M;
+  R->off = (int32_t *)calloc((size_t)M + 1, sizeof(int32_t));
+  int64_t cnt = 0;
+  for (int64_t a = 1; a < M; ++a)
+    if (gcd64(a, M) == 1) { R->off[or_mod_pow6(a, M) + 1]++; ++cnt; }
+  for (int64_t t = 0; t < M; ++t) R->off[t + 1] += R->off[t];
+  R->val = (int32_t *)malloc(sizeof(int32_t) * (size_t)(cnt ? cnt : 1));
+  int32_t *fill = (int32_t *)calloc((size_t)M, sizeof(int32_t));
+  for (int64_t a = 1; a < M; ++a)
+    if (gcd64(a, M) == 1) {
+      int64_t t = or_mod_pow6(a, M);
+      R->val[R->off[t] + fill[t]++] = (int32_t)a;
+    }
+  free(fill);
+}
+static void roots_init(void) {
+  if (g_roots_ready) return;
+#pragma omp critical(or_roots)
+  {
+    if (!g_roots_ready) {
+      build_roots(&g_r2, M2);
+      build_roots(&g_r3, M3);
+      build_roots(&g_r7, M7);
+      __sync_synchronize();
+      g_roots_ready = 1;
+    }
+  }
+}
+static int roots_of(const roots_t *R, int64_t t, const int32_t **p) {
+  *p = R->val + R->off[t];
+  return R->off[t + 1] - R->off[t];
+}
+
+int or_admissible_B(int64_t B) { return B > 1 && gcd64(B, 42) == 1; } /* classes:80-82 */
+
+static int64_t class_modulus(int cls) { /* classes:84-91 */
+  switch (cls) {
+    case 1: return M42;
+    case 2: return M14;
+    case 3: return M21;
+    default: return M7;
+  }
+}
+static int unit_ok(int64_t u, int cls) { /* classes:102-111 */
+  switch (cls) {
+    case 1: return (u & 1) && gcd64(u, 42) == 1;
+    case 2: return (u & 1) && (u % 3 == 0) && (u % 9 != 0);
+    case 3: return !(u & 1) && gcd64(u, 21) == 1;
+    case 4:
+    case 5: return (u % 6 == 0) && (u % 7 != 0);
+  }
+  return 0;
+}
+static int cmp_i64(const void *a, const void *b) {
+  int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return (x > y) - (x < y);
+}
+static int sort_unique(int64_t *v, int n) {
+  qsort(v, (size_t)n, sizeof(int64_t), cmp_i64);
+  int w = 0;
+  for (int i = 0; i < n; ++i)
+    if (w == 0 || v[w - 1] != v[i]) v[w++] = v[i];
+  return w;
+}
+
+int or_seeds_for_B(int64_t B, int cls, int64_t *out, int cap) { /* classes:114-135 */
+  roots_init();
+  int64_t b2 = or_mod_pow6(B, M2), b3 = or_mod_pow6(B, M3), b7 = or_mod_pow6(B, M7);
+  const int32_t *p2, *p3, *p7;
+  int n2 = roots_of(&g_r2, b2, &p2), n3 = roots_of(&g_r3, b3, &p3),
+      n7 = roots_of(&g_r7, b7, &p7);
+  int64_t tmp[4096];
+  int n = 0;
+  if (cls == 1) {
+    for (int i = 0; i < n2; ++i)
+      for (int j = 0; j < n3; ++j) {
+        int64_t s23 = crt2(p2[i], M2, p3[j], M3);
+        for (int k = 0; k < n7; ++k) tmp[n++] = crt2(s23, M2 * M3, p7[k], M7);
+      }
+  } else if (cls == 2) {
+    for (int i = 0; i < n2; ++i)
+      for (int k = 0; k < n7; ++k) tmp[n++] = crt2(p2[i], M2, p7[k], M7);
+  } else if (cls == 3) {
+    for (int j = 0; j < n3; ++j)
+      for (int k = 0; k < n7; ++k) tmp[n++] = crt2(p3[j], M3, p7[k], M7);
+  } else {
+    for (int k = 0; k < n7; ++k) tmp[n++] = p7[k];
+  }
+  n = sort_unique(tmp, n);
+  for (int i = 0; i < n && i < cap; ++i) out[i] = tmp[i];
+  return n;
+}
+
+int or_unit_candidates(int64_t B, int cls, double lo_frac, double hi_frac,
+                       int64_t *out, int cap) { /* classes:137-154 */
+  int64_t M = class_modulus(cls);
+  int64_t lo = (int64_t)(lo_frac * (double)B);
+  if (lo < 1) lo = 1;
+  int64_t hi = (int64_t)(hi_frac * (double)B);
+  if (hi > B - 1) hi = B - 1;
+  if (lo > hi) return 0;
+  int64_t seeds[4096];
+  int ns = or_seeds_for_B(B, cls, seeds, 4096);
+  int64_t *tmp = NULL;
+  int n = 0, tcap = 0;
+  for (int i = 0; i < ns; ++i) {
+    int64_t u = seeds[i] % M;
+    if (u < 0) u += M;
+    if (u < lo) u += (lo - u + M - 1) / M * M;
+    for (; u <= hi; u += M)
+      if (unit_ok(u, cls)) {
+        if (n == tcap) {
+          tcap = tcap ? tcap * 2 : 256;
+          tmp = (int64_t *)realloc(tmp, sizeof(int64_t) * (size_t)tcap);
+        }
+        tmp[n++] = u;
+      }
+  }
+  n = n ? sort_unique(tmp, n) : 0;
+  for (int i = 0; i < n && i < cap; ++i) out[i] = tmp[i];
+  free(tmp);
+  return n;
+}
+
+static int copy_roots(const roots_t *R, int64_t t, int64_t *res) {
+  const int32_t *p;
+  int n = roots_of(R, t, &p);
+  for (int i = 0; i < n; ++i) res[i] = p[i];
+  return n;
+}
+
+/* classes:156-219 */
+int or_free_de_cls5(int64_t B, int64_t u, int64_t *e_res, int *n_e, int64_t *e_mod,
+                    int64_t *e_fmax, int64_t *d_res, int *n_d, int64_t *d_mod,
+                    int64_t *d_fmax) {
+  roots_init();
+  int64_t te = or_mod_pow6(B, M3) * mod_inv(or_mod_pow6(14, M3), M3) % M3;
+  *n_e = copy_roots(&g_r3, te, e_res);
+  *e_mod = M3;
+  *e_fmax = (B - 1) / 14;
+  int64_t rhs = (or_mod_pow6(B, M2) - or_mod_pow6(u, M2)) % M2;
+  if (rhs < 0) rhs += M2;
+  int64_t td = rhs * mod_inv(or_mod_pow6(21, M2), M2) % M2;
+  *n_d = copy_roots(&g_r2, td, d_res);
+  *d_mod = M2;
+  *d_fmax = (B - 1) / 21;
+  return 0;
+}
+
+int or_free_d_cls234(int64_t B, int64_t u, int cls, int64_t *res, int *n,
+                     int64_t *mod, int64_t *fmax) { /* classes:188-209 */
+  roots_init();
+  *n = 0; *mod = 0; *fmax = 0;
+  if (cls == 2) {
+    int64_t t = or_mod_pow6(B, M3) * mod_inv(or_mod_pow6(14, M3), M3) % M3;
+    *n = copy_roots(&g_r3, t, res);
+    *mod = M3; *fmax = (B - 1) / 14;
+  } else if (cls == 3) {
+    int64_t rhs = (or_mod_pow6(B, M2) - or_mod_pow6(u, M2)) % M2;
+    if (rhs < 0) rhs += M2;
+    int64_t t = rhs * mod_inv(or_mod_pow6(21, M2), M2) % M2;
+    *n = copy_roots(&g_r2, t, res);
+    *mod = M2; *fmax = (B - 1) / 21;
+  } else if (cls == 4) {
+    int64_t t2 = or_mod_pow6(B, M2) * mod_inv(or_mod_pow6(7, M2), M2) % M2;
+    int64_t t3 = or_mod_pow6(B, M3) * mod_inv(or_mod_pow6(7, M3), M3) % M3;
+    const int32_t *pa, *pb;
+    int na = roots_of(&g_r2, t2, &pa), nb = roots_of(&g_r3, t3, &pb);
+    int k = 0;
+    for (int i = 0; i < na; ++i)
+      for (int j = 0; j < nb; ++j) res[k++] = crt2(pa[i], M2, pb[j], M3);
+    qsort(res, (size_t)k, sizeof(int64_t), cmp_i64);
+    *n = k;
+    *mod = M2 * M3; *fmax = (B - 1) / 7;
+  }
+  return 0;
+}
+
+int64_t or_emit_units(int64_t lo, int64_t hi, unsigned cls_mask, double lo_frac,
+                      double hi_frac, int64_t *out_cls, int64_t *out_B,
+                      int64_t *out_u, int64_t cap) { /* huntv3:371-381 */
+  int64_t n = 0;
+  int64_t buf[8192];
+  for (int64_t B = lo; B <= hi; ++B) {
+    if (!or_admissible_B(B)) continue;
+    for (int cls = 1; cls <= 5; ++cls) {
+      if (!(cls_mask & (1u << cls))) continue;
+      int k = or_unit_candidates(B, cls, lo_frac, hi_frac, buf, 8192);
+      for (int i = 0; i < k; ++i) {
+        if (out_cls && n < cap) {
+          out_cls[n] = cls; out_B[n] = B; out_u[n] = buf[i];
+        }
+        ++n;
+      }
+    }
+  }
+  return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* 192-bit helpers (fourcore_cls5_gpu_v4.cu:77-108, 270-308)           */
+/* ------------------------------------------------------------------ */
+void or_pow6_192(u32 x, u64 o[3]) { /* cls5v4:78-93 */
+  const u64 x2 = (u64)x * (u64)x;
+  const u128 x4 = (u128)x2 * (u128)x2;
+  const u64 x4_lo = (u64)x4, x4_hi = (u64)(x4 >> 64);
+  const u128 p_lo = (u128)x4_lo * (u128)x2;
+  const u128 p_hi = (u128)x4_hi * (u128)x2;
+  o[0] = (u64)p_lo;
+  const u64 mid = (u64)(p_lo >> 64);
+  const u64 hi_lo = (u64)p_hi, hi_hi = (u64)(p_hi >> 64);
+  const u64 mid2 = mid + hi_lo;
+  o[1] = mid2;
+  o[2] = hi_hi + (mid2 < mid ? 1u : 0u);
+}
+
+int or_sub192(u64 a[3], const u64 b[3]) { /* cls5v4:95-108 */
+  const u64 r0 = a[0] - b[0];
+  u64 br = a[0] < b[0];
+  const u64 a1b = a[1] - br;
+  br = a[1] < br;
+  const u64 r1 = a1b - b[1];
+  br |= a1b < b[1];
+  const u64 a2b = a[2] - br;
+  br = a[2] < br;
+  const u64 r2 = a2b - b[2];
+  br |= a2b < b[2];
+  a[0] = r0; a[1] = r1; a[2] = r2;
+  return br != 0;
+}
+
+u64 or_mod192(const u64 a[3], u64 m, u64 c) { /* cls5v4:300-308 */
+  u64 r = a[2] % m;
+  r = (u64)((u128)r * c % m);
+  r = (u64)((u128)r * c % m);
+  r = (u64)(((u128)r + (u128)(a[1] % m) * c) % m);
+  r += a[0] % m;
+  if (r >= m) r -= m;
+  return r;
+}
+
+/* ------------------------------------------------------------------ */
+/* gate (fourcore_cls5_gpu_v4.cu:117-137)                              */
+/* ------------------------------------------------------------------ */
+static void gate_set(u64 *b, int x) { b[x >> 6] |= 1ULL << (x & 63); }
+static u32 mod_pow6_h(u32 x, u32 m) {
+  u64 a = x;
+  u64 x2 = (a * a) % m;
+  u64 x4 = (x2 * x2) % m;
+  return (u32)((x2 * x4) % m);
+}
+void or_build_gate(or_gate_t *g) {
+  memset(g, 0, sizeof *g);
+  for (int i = 0; i < 504; ++i) {
+    int si = (int)mod_pow6_h((u32)i, 504);
+    g->p6504[i] = (uint16_t)si;
+    for (int j = i; j < 504; ++j) gate_set(g->g504, (si + (int)mod_pow6_h((u32)j, 504)) % 504);
+  }
+  for (int i = 0; i < 247; ++i) {
+    int si = (int)mod_pow6_h((u32)i, 247);
+    g->p6247[i] = (uint16_t)si;
+    for (int j = i; j < 247; ++j) gate_set(g->g247, (si + (int)mod_pow6_h((u32)j, 247)) % 247);
+  }
+}
+static int gate_bit(const u64 *b, int x) { return (int)((b[x >> 6] >> (x & 63)) & 1ULL); }
+
+/* ------------------------------------------------------------------ */
+/* mixer, cell packing, xor query                                      */
+/* ------------------------------------------------------------------ */
+u64 or_mix64(u64 x) { /* mix64.hpp:13-20 */
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+static u64 fp_mask(u32 r) { return r >= 64 ? ~0ULL : ((1ULL << r) - 1ULL); } /* xor_pack.hpp:14-16 */
+
+size_t or_packed_bytes(u64 m_cells, u32 r) { /* xor_pack.hpp:19-23 */
+  if (m_cells == 0 || r == 0) return 0;
+  return (size_t)((m_cells * (u64)r + 7ull) / 8ull + 8ull);
+}
+
+u64 or_load_cell(const uint8_t *p, u64 cell, u32 r) { /* xor_pack.hpp:29-60 */
+  const u64 bit_index = cell * (u64)r;
+  const u64 byte_index = bit_index >> 3;
+  const unsigned off = (unsigned)(bit_index & 7u);
+  u64 word;
+  memcpy(&word, p + byte_index, 8);
+  if (off + r <= 64u) return (word >> off) & fp_mask(r);
+  u64 word2;
+  memcpy(&word2, p + byte_index + 8, 8);
+  return ((word >> off) | (word2 << (64u - off))) & fp_mask(r);
+}
+
+void or_store_cell(uint8_t *p, u64 cell, u32 r, u64 value) { /* xor_pack.hpp:67-113 */
+  const u64 mask = fp_mask(r);
+  value &= mask;
+  const u64 bit_index = cell * (u64)r;
+  const u64 byte_index = bit_index >> 3;
+  const unsigned off = (unsigned)(bit_index & 7u);
+  u64 word;
+  memcpy(&word, p + byte_index, 8);
+  if (off + r <= 64u) {
+    const u64 wmask = mask << off;
+    word = (word & ~wmask) | (value << off);
+    memcpy(p + byte_index, &word, 8);
+    return;
+  }
+  const unsigned low_bits = 64u - off;
+  const u64 low_mask = (low_bits >= 64u) ? ~0ULL : ((1ULL << low_bits) - 1ULL);
+  word = (word & ~(low_mask << off)) | ((value & low_mask) << off);
+  memcpy(p + byte_index, &word, 8);
+  u64 word2;
+  memcpy(&word2, p + byte_index + 8, 8);
+  const u64 high = value >> low_bits;
+  const u64 high_bits = r - low_bits;
+  const u64 high_mask = (high_bits >= 64u) ? ~0ULL : ((1ULL << high_bits) - 1ULL);
+  word2 = (word2 & ~high_mask) | (high & high_mask);
+  memcpy(p + byte_index + 8, &word2, 8);
+}
+
+static void cell_hashes(u64 h, u64 block, u64 out[3]) { /* xor_filter.hpp:53-60 */
+  const u64 h1 = or_mix64(h ^ 0x9E3779B97F4A7C15ULL);
+  const u64 h2 = or_mix64(h ^ 0xBF58476D1CE4E5B9ULL);
+  out[0] = h % block;
+  out[1] = (h1 % block) + block;
+  out[2] = (h2 % block) + 2ull * block;
+}
+
+int or_xor_might_contain(const or_xor_t *f, u64 key) { /* xor_filter.hpp:472-483 */
+  if (f->hdr.n_keys == 0) return 0;
+  if (!f->packed || f->hdr.m_cells < 3) return 0;
+  const u64 block = f->hdr.m_cells / 3;
+  const u64 h = or_mix64(key + f->hdr.mix_seed);
+  u64 hs[3];
+  cell_hashes(h, block, hs);
+  const u64 got = or_load_cell(f->packed, hs[0], f->hdr.r) ^
+                  or_load_cell(f->packed, hs[1], f->hdr.r) ^
+                  or_load_cell(f->packed, hs[2], f->hdr.r);
+  return got == (h & fp_mask(f->hdr.r));
+}
+
+/* the device query (cls5v4:322-334): same as above but keyed on block only */
+static int device_query(const uint8_t *packed, u64 block, u32 r, u64 seed, u64 key) {
+  if (block == 0) return 0;
+  const u64 h = or_mix64(key + seed);
+  u64 hs[3];
+  cell_hashes(h, block, hs);
+  const u64 got = or_load_cell(packed, hs[0], r) ^ or_load_cell(packed, hs[1], r) ^
+                  or_load_cell(packed, hs[2], r);
+  return got == (h & fp_mask(r));
+}
+
+size_t or_capacity_for(size_t n) { /* xor_filter.hpp:185-192 */
+  double need = 32.0 + 1.23 * (double)n;
+  size_t cap = (size_t)(need + 0.999);
+  if (cap < 3) cap = 3;
+  cap = (cap + 2) / 3 * 3;
+  return cap;
+}
+
+/* LSD radix sort of u64 keys (16-bit digits) */
+static void radix_sort_u64(u64 *a, size_t n) {
+  u64 *tmp = (u64 *)malloc(sizeof(u64) * (n ? n : 1));
+  size_t *cnt = (size_t *)malloc(sizeof(size_t) * 65536);
+  u64 *src = a, *dst = tmp;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int sh = pass * 16;
+    memset(cnt, 0, sizeof(size_t) * 65536);
+    for (size_t i = 0; i < n; ++i) cnt[(src[i] >> sh) & 0xffff]++;
+    size_t s = 0;
+    for (int b = 0; b < 65536; ++b) { size_t c = cnt[b]; cnt[b] = s; s += c; }
+    for (size_t i = 0; i < n; ++i) dst[cnt[(src[i] >> sh) & 0xffff]++] = src[i];
+    u64 *t = src; src = dst; dst = t;
+  }
+  /* 4 passes: result is back in a */
+  free(tmp);
+  free(cnt);
+}
+
+static void header_init(or_header_t *h, u32 r, u64 seed) {
+  memset(h, 0, sizeof *h);
+  h->magic = 0x53353136u;
+  h->version = 2;
+  h->kind = 1;
+  h->r = r;
+  h->mix_seed = seed;
+  h->shard_count = 1;
+}
+
+typedef struct { u64 hash, index; } peel_ent_t;
+
+/* xor_filter.hpp:256-431 (RAM path; the disk spill does not change bytes) */
+static int construct_with_seed(const u64 *keys, size_t n, u32 r, u64 seed, or_xor_t *out) {
+  if (n == 0) {
+    header_init(&out->hdr, r, seed);
+    out->packed = NULL;
+    out->packed_len = 0;
+    return 1;
+  }
+  const size_t m = or_capacity_for(n);
+  const u64 block = (u64)(m / 3);
+  u64 *setxor = (u64 *)calloc(m, sizeof(u64));
+  uint16_t *counts = (uint16_t *)calloc(m, sizeof(uint16_t));
+  for (size_t i = 0; i < n; ++i) {
+    const u64 h = or_mix64(keys[i] + seed);
+    u64 hs[3];
+    cell_hashes(h, block, hs);
+    for (int t = 0; t < 3; ++t) {
+      setxor[hs[t]] ^= h;
+      if (counts[hs[t]] == 0xffffu) { free(setxor); free(counts); return -1; }
+      counts[hs[t]] += 1;
+    }
+  }
+  peel_ent_t *stack = (peel_ent_t *)malloc(sizeof(peel_ent_t) * n);
+  size_t sn = 0;
+  size_t qcap = 1 << 20, qn = 0, qhead = 0;
+  u64 *queue = (u64 *)malloc(sizeof(u64) * qcap);
+  for (size_t i = 0; i < m; ++i)
+    if (counts[i] == 1) {
+      if (qn == qcap) { qcap *= 2; queue = (u64 *)realloc(queue, sizeof(u64) * qcap); }
+      queue[qn++] = i;
+    }
+  while (qhead < qn) {
+    const u64 i = queue[qhead++];
+    if (counts[i] != 1) continue;
+    const u64 h = setxor[i];
+    u64 hs[3];
+    cell_hashes(h, block, hs);
+    if (sn < n) { stack[sn].hash = h; stack[sn].index = i; }
+    ++sn;
+    for (int t = 0; t < 3; ++t) {
+      const u64 j = hs[t];
+      setxor[j] ^= h;
+      if (counts[j] > 0) {
+        counts[j] -= 1;
+        if (counts[j] == 1) {
+          if (qn == qcap) { qcap *= 2; queue = (u64 *)realloc(queue, sizeof(u64) * qcap); }
+          queue[qn++] = j;
+        }
+      }
+    }
+  }
+  free(queue);
+  free(setxor);
+  free(counts);
+  if (sn != n) { free(stack); return 0; }
+  const u64 mask = fp_mask(r);
+  out->packed_len = or_packed_bytes(m, r);
+  out->packed = (uint8_t *)calloc(out->packed_len, 1);
+  for (size_t k = sn; k-- > 0;) {
+    u64 hs[3];
+    cell_hashes(stack[k].hash, block, hs);
+    u64 xv = stack[k].hash & mask;
+    for (int t = 0; t < 3; ++t)
+      if (hs[t] != stack[k].index) xv ^= or_load_cell(out->packed, hs[t], r);
+    or_store_cell(out->packed, stack[k].index, r, xv & mask);
+  }
+  free(stack);
+  header_init(&out->hdr, r, seed);
+  out->hdr.n_keys = n;
+  out->hdr.m_cells = m;
+  return 1;
+}
+
+int or_build_xor(const u64 *keys_in, size_t n_in, u32 r, u64 base_seed, int max_seed_tries,
+                 or_xor_t *out) { /* xor_filter.hpp:436-470 */
+  memset(out, 0, sizeof *out);
+  u64 *keys = (u64 *)malloc(sizeof(u64) * (n_in ? n_in : 1));
+  memcpy(keys, keys_in, sizeof(u64) * n_in);
+  radix_sort_u64(keys, n_in);
+  size_t n = 0;
+  for (size_t i = 0; i < n_in; ++i)
+    if (n == 0 || keys[n - 1] != keys[i]) keys[n++] = keys[i];
+  for (int t = 0; t < max_seed_tries; ++t) {
+    const u64 seed = base_seed + (u64)t * 0x9E3779B97F4A7C15ULL;
+    int rc = construct_with_seed(keys, n, r, seed, out);
+    if (rc == 1) { free(keys); return 0; }
+    if (rc < 0) break;
+  }
+  free(keys);
+  return -1;
+}
+
+/* fourcore_xor_store.hpp:67-119.  The reference frees the keys after the
+ * incidence pass, so a stalled peel at one seed base is retried at the next
+ * outer base with regenerated keys: the effective schedule is one seed per
+ * base, base_k = 0x615615615615615 + k*phi, k = 0..3. */
+int or_build_pairs(int N, u32 r, const u64 *extra, size_t n_extra, or_xor_t *out) {
+  size_t np = (size_t)N * (size_t)(N + 1) / 2;
+  u64 *keys = (u64 *)malloc(sizeof(u64) * (np + n_extra + 1));
+  u64 *pw6 = (u64 *)malloc(sizeof(u64) * ((size_t)N + 1));
+  for (int x = 1; x <= N; ++x) { u64 x2 = (u64)x * (u64)x; pw6[x] = x2 * x2 * x2; }
+  size_t k = 0;
+  for (int i = 1; i <= N; ++i)
+    for (int j = i; j <= N; ++j) keys[k++] = pw6[i] + pw6[j];
+  for (size_t i = 0; i < n_extra; ++i) keys[k++] = extra[i];
+  free(pw6);
+  int rc = -1;
+  for (int outer = 0; outer < 4 && rc != 0; ++outer) {
+    const u64 base = 0x615615615615615ULL + (u64)outer * 0x9E3779B97F4A7C15ULL;
+    rc = or_build_xor(keys, k, r, base, 1, out);
+  }
+  free(keys);
+  if (rc == 0) out->hdr.N = (u32)N;
+  return rc;
+}
+
+void or_xor_free(or_xor_t *f) {
+  free(f->packed);
+  f->packed = NULL;
+  f->packed_len = 0;
+}
+
+int or_xor_save(const char *path, const or_xor_t *f) { /* fourcore_xor_store.hpp:126-143 */
+  FILE *fp = fopen(path, "wb");
+  if (!fp) return -1;
+  if (fwrite(&f->hdr, sizeof f->hdr, 1, fp) != 1 ||
+      (f->packed_len && fwrite(f->packed, 1, f->packed_len, fp) != f->packed_len)) {
+    fclose(fp);
+    return -2;
+  }
+  fclose(fp);
+  return 0;
+}
+
+int or_xor_load(const char *path, or_xor_t *f, int expected_N) { /* fourcore_xor_store.hpp:147-181 */
+  memset(f, 0, sizeof *f);
+  FILE *fp = fopen(path, "rb");
+  if (!fp) return -1;
+  or_header_t h;
+  if (fread(&h, sizeof h, 1, fp) != 1) { fclose(fp); return -2; }
+  /* header_ok (store_header.hpp:39-42): magic, version, shard fields, r */
+  if (!(h.magic == 0x53353136u && h.version == 2 && h.shard_count >= 1 &&
+        h.shard_index < h.shard_count && h.r > 0 && h.r <= 64) ||
+      h.kind != 1) {
+    fclose(fp);
+    return -2;
+  }
+  if (h.version != 2) { fclose(fp); return -3; }
+  if (expected_N > 0 && (int)h.N != expected_N) { fclose(fp); return -4; }
+  size_t want = or_packed_bytes(h.m_cells, h.r);
+  f->hdr = h;
+  f->packed_len = want;
+  f->packed = want ? (uint8_t *)malloc(want) : NULL;
+  if (want && fread(f->packed, 1, want, fp) != want) {
+    fclose(fp);
+    or_xor_free(f);
+    return -5;
+  }
+  fclose(fp);
+  return 0;
+}
+
+int or_fpr_smoke(const or_xor_t *f, int trials) { /* fourcore_xor_store.hpp:54-65 */
+  if (!f->packed || f->hdr.m_cells < 3 || f->hdr.r == 0) return -1;
+  int hits = 0;
+  u64 x = 0x123456789abcdef0ULL;
+  for (int t = 0; t < trials; ++t) {
+    x = x * 0x9E3779B97F4A7C15ULL + 1;
+    if (or_xor_might_contain(f, x)) ++hits;
+  }
+  return hits;
+}
+
+/* ------------------------------------------------------------------ */
+/* big integers for B^6 - ... (replaces GMP in fourcore_gmp.hpp)       */
+/* ------------------------------------------------------------------ */
+typedef struct { u64 l[4]; } u256;
+
+static u256 pow6_256(u64 x) {
+  u256 r;
+  u64 o[3];
+  /* x < 2^32 for every caller; B^6 < 2^192 */
+  or_pow6_192((u32)x, o);
+  r.l[0] = o[0]; r.l[1] = o[1]; r.l[2] = o[2]; r.l[3] = 0;
+  return r;
+}
+static int sub256(u256 *a, const u256 *b) {
+  u64 br = 0;
+  for (int i = 0; i < 4; ++i) {
+    u64 ai = a->l[i], bi = b->l[i];
+    u64 d = ai - bi - br;
+    br = (ai < bi) || (ai - bi < br);
+    a->l[i] = d;
+  }
+  return (int)br;
+}
+static void add256(u256 *a, const u256 *b) {
+  u64 c = 0;
+  for (int i = 0; i < 4; ++i) {
+    u128 s = (u128)a->l[i] + b->l[i] + c;
+    a->l[i] = (u64)s;
+    c = (u64)(s >> 64);
+  }
+}
+static void mul256_small(u256 *a, u64 m) {
+  u64 c = 0;
+  for (int i = 0; i < 4; ++i) {
+    u128 p = (u128)a->l[i] * m + c;
+    a->l[i] = (u64)p;
+    c = (u64)(p >> 64);
+  }
+}
+static u64 divmod256_small(u256 *a, u64 d) {
+  u128 rem = 0;
+  for (int i = 3; i >= 0; --i) {
+    u128 cur = (rem << 64) | a->l[i];
+    a->l[i] = (u64)(cur / d);
+    rem = cur % d;
+  }
+  return (u64)rem;
+}
+static int is_zero256(const u256 *a) { return !(a->l[0] | a->l[1] | a->l[2] | a->l[3]); }
+static int eq256(const u256 *a, const u256 *b) { return !memcmp(a, b, sizeof *a); }
+
+int or_T_peel2(u64 B, u64 u, u64 d, u64 e, u64 *T_lo, u64 *T_hi) { /* gmp:55-62 */
+  if (u >= B || d < 1 || e < 1) return 0;
+  u256 num = pow6_256(B), t;
+  t = pow6_256(u);
+  if (sub256(&num, &t)) return 0;
+  t = pow6_256(21 * d);
+  if (sub256(&num, &t)) return 0;
+  t = pow6_256(14 * e);
+  if (sub256(&num, &t)) return 0;
+  if (is_zero256(&num)) return 0;
+  if (divmod256_small(&num, (u64)M42) != 0) return 0;
+  if (num.l[2] | num.l[3]) return 0;
+  *T_lo = num.l[0];
+  *T_hi = num.l[1];
+  return 1;
+}
+
+static int verify_cls5(u64 B, u64 u, u64 d, u64 e, u64 x1, u64 x2) { /* gmp:113-117 */
+  u256 rhs = pow6_256(u), t;
+  t = pow6_256(21 * d); add256(&rhs, &t);
+  t = pow6_256(14 * e); add256(&rhs, &t);
+  u256 core = pow6_256(x1);
+  t = pow6_256(x2); add256(&core, &t);
+  mul256_small(&core, (u64)M42);
+  add256(&rhs, &core);
+  u256 b6 = pow6_256(B);
+  return eq256(&rhs, &b6);
+}
+
+u64 or_inv216(void) { /* cls5v4:62-66 */
+  u64 a = (u64)M21, x = a;
+  for (int i = 0; i < 6; ++i) x = x * (2 - a * x);
+  return x;
+}
+
+u64 or_splitmix64(u64 *s) {
+  u64 z = (*s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+/* ------------------------------------------------------------------ */
+/* unit packing (fourcore_cls5_gpu_v4.cu:169-184)                      */
+/* ------------------------------------------------------------------ */
+int or_pack_unit5(u64 B, u64 u, or_unit5_t *U) {
+  memset(U, 0, sizeof *U);
+  U->B = B; U->u = u;
+  U->lim = (u32)((B - 1) / 42);
+  if (u >= B) return 0;
+  if (B >= (1ULL << 32)) return 0; /* B^6 must fit 192 bits */
+  u256 base = pow6_256(B), t = pow6_256(u);
+  if (sub256(&base, &t) || is_zero256(&base)) return 0;
+  U->base[0] = base.l[0]; U->base[1] = base.l[1]; U->base[2] = base.l[2];
+  int64_t er[16], dr[16], em, ef, dm, df;
+  int ne, nd;
+  or_free_de_cls5((int64_t)B, (int64_t)u, er, &ne, &em, &ef, dr, &nd, &dm, &df);
+  U->mod1 = (u32)em; U->fmax1 = (u32)ef;
+  U->mod2 = (u32)dm; U->fmax2 = (u32)df;
+  U->nres1 = (u32)(ne < 12 ? ne : 12);
+  U->nres2 = (u32)(nd < 12 ? nd : 12);
+  for (u32 i = 0; i < U->nres1; ++i) U->res[i] = (u32)er[i];
+  for (u32 i = 0; i < U->nres2; ++i) U->res[U->nres1 + i] = (u32)dr[i];
+  return U->nres1 > 0 && U->nres2 > 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* the peel (fourcore_cls5_gpu_v4.cu:353-410)                          */
+/* ------------------------------------------------------------------ */
+int or_cls5_peel(const or_unit5_t *units, int n, const or_xor_t *f, const or_xor_t *side,
+                 int use_gate, or_counts_t *counts, or_hit_t *hits, u64 cap, u64 *n_hits,
+                 int threads) {
+  or_gate_t g;
+  or_build_gate(&g);
+  const u64 inv216 = or_inv216();
+  const u64 m504 = 504ULL * (u64)M42;
+  const u64 c_m504 = (u64)(((u128)1 << 64) % (u128)m504);
+  const u64 c_m247 = (u64)(((u128)1 << 64) % (u128)247ULL);
+  const u64 inv42_247 = (u64)mod_inv(or_mod_pow6(42, 247), 247);
+  const u64 blk = (f && f->hdr.m_cells) ? f->hdr.m_cells / 3 : 0;
+  const u64 sblk = (side && side->hdr.m_cells) ? side->hdr.m_cells / 3 : 0;
+  u64 calls = 0, neg = 0, zero = 0, gated = 0, probes = 0, maybe = 0, smaybe = 0;
+  u64 nh = 0;
+  if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) \
+    reduction(+ : calls, neg, zero, gated, probes, maybe, smaybe)
+  for (int ui = 0; ui < n; ++ui) {
+    const or_unit5_t *U = &units[ui];
+    const u32 kmax1 = U->fmax1 / U->mod1 + 1;
+    const u32 total1 = U->nres1 * kmax1;
+    const u32 total2 = U->nres2 * (U->fmax2 / U->mod2 + 1);
+    for (u32 t1 = 0; t1 < total1; ++t1) {
+      const u32 e = U->res[t1 % U->nres1] + (t1 / U->nres1) * U->mod1;
+      if (e == 0 || e > U->fmax1) continue;
+      u64 E6[3];
+      or_pow6_192(14u * e, E6);
+      for (u32 t2 = 0; t2 < total2; ++t2) {
+        const u32 d = U->res[U->nres1 + t2 % U->nres2] + (t2 / U->nres2) * U->mod2;
+        if (d == 0 || d > U->fmax2) continue;
+        ++calls;
+        u64 R[3] = {U->base[0], U->base[1], U->base[2]};
+        if (or_sub192(R, E6)) { ++neg; continue; }
+        u64 D6[3];
+        or_pow6_192(21u * d, D6);
+        if (or_sub192(R, D6)) { ++neg; continue; }
+        if ((R[0] | R[1] | R[2]) == 0) { ++zero; continue; }
+        if (use_gate) {
+          const u64 rmod = or_mod192(R, m504, c_m504);
+          const u32 t504 = (u32)((rmod / (u64)M42) % 504ULL);
+          const u64 r247 = or_mod192(R, 247ULL, c_m247);
+          const u32 t247 = (u32)(r247 * inv42_247 % 247ULL);
+          if (!(gate_bit(g.g504, (int)t504) && gate_bit(g.g247, (int)t247))) {
+            ++gated;
+            continue;
+          }
+        }
+        ++probes;
+        const u64 fp = ((R[0] >> 6) | (R[1] << 58)) * inv216;
+        u32 flags = 0;
+        if (f && device_query(f->packed, blk, f->hdr.r, f->hdr.mix_seed, fp)) flags |= 1u;
+        if (side && device_query(side->packed, sblk, side->hdr.r, side->hdr.mix_seed, fp))
+          flags |= 2u;
+        if (flags & 1u) ++maybe;
+        if (flags & 2u) ++smaybe;
+        if (flags) {
+          u64 slot;
+#pragma omp atomic capture
+          slot = nh++;
+          if (hits && slot < cap) {
+            hits[slot].unit = (u32)ui;
+            hits[slot].d = d;
+            hits[slot].e = e;
+            hits[slot].flags = flags;
+            hits[slot].fp = fp;
+          }
+        }
+      }
+    }
+  }
+  counts->calls = calls; counts->neg = neg; counts->zero = zero;
+  counts->gated = gated; counts->probes = probes;
+  counts->maybe = maybe; counts->side_maybe = smaybe;
+  *n_hits = nh;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* host verify (fourcore_cls5_gpu_v4.cu:513-542; PairRecover            */
+/* fourcore_xor_store.hpp:23-51)                                        */
+/* ------------------------------------------------------------------ */
+typedef struct { u64 v; u32 x; } pw_ent_t;
+static int cmp_pw(const void *a, const void *b) {
+  const pw_ent_t *p = (const pw_ent_t *)a, *q = (const pw_ent_t *)b;
+  if (p->v != q->v) return p->v < q->v ? -1 : 1;
+  return (p->x > q->x) - (p->x < q->x);
+}
+
+int or_verify_hit(u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp, int N, int64_t *terms_out,
+                  int64_t *gcd_out, int cap) {
+  pw_ent_t *tab = (pw_ent_t *)malloc(sizeof(pw_ent_t) * ((size_t)N + 1));
+  for (int x = 1; x <= N; ++x) {
+    u64 x2 = (u64)x * (u64)x;
+    tab[x - 1].v = x2 * x2 * x2;
+    tab[x - 1].x = (u32)x;
+  }
+  qsort(tab, (size_t)N, sizeof(pw_ent_t), cmp_pw);
+  int nsol = 0;
+  for (int i = 1; i <= N; ++i) {
+    u64 i2 = (u64)i * (u64)i;
+    const u64 need = fp - i2 * i2 * i2;
+    /* lower_bound on v */
+    size_t lo = 0, hi = (size_t)N;
+    while (lo < hi) {
+      size_t mid = (lo + hi) / 2;
+      if (tab[mid].v < need) lo = mid + 1; else hi = mid;
+    }
+    for (size_t k = lo; k < (size_t)N && tab[k].v == need; ++k) {
+      u32 j = tab[k].x;
+      if ((u32)i > j) continue;
+      u32 a = (u32)i, b = j;
+      if (a > lim || b > lim) continue;
+      u64 Tl, Th;
+      if (!or_T_peel2(B, u, d, e, &Tl, &Th)) continue;
+      u64 pa[3], pb[3];
+      or_pow6_192(a, pa);
+      or_pow6_192(b, pb);
+      u128 s = ((u128)pa[1] << 64 | pa[0]) + ((u128)pb[1] << 64 | pb[0]);
+      if (s != ((u128)Th << 64 | Tl)) continue;
+      if (!verify_cls5(B, u, d, e, a, b)) continue;
+      int64_t t[5] = {42LL * a, 42LL * b, 21LL * (int64_t)d, 14LL * (int64_t)e, (int64_t)u};
+      qsort(t, 5, sizeof(int64_t), cmp_i64);
+      int bad = 0;
+      for (int q = 0; q < 4; ++q) if (t[q] == t[q + 1]) bad = 1;
+      if (bad || t[4] >= (int64_t)B) continue;
+      int64_t gg = t[0];
+      for (int q = 1; q < 5; ++q) gg = gcd64(gg, t[q]);
+      gg = gcd64(gg, (int64_t)B);
+      if (nsol < cap) {
+        for (int q = 0; q < 5; ++q) terms_out[nsol * 5 + q] = t[q];
+        gcd_out[nsol] = gg;
+      }
+      ++nsol;
+    }
+  }
+  free(tab);
+  return nsol;
+}

@@ -1,0 +1,682 @@
+// fc5_host.cpp -- host-side core (see fc5_host.hpp for the interface map).
+#include "fc5_host.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <stdexcept>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace fc5 {
+
+// ---------------------------------------------------------------------------
+// number theory
+// ---------------------------------------------------------------------------
+i64 pow6_mod(i64 x, i64 m) {
+  x %= m;
+  if (x < 0) x += m;
+  const i64 x2 = x * x % m;
+  const i64 x3 = x2 * x % m;
+  return x3 * x3 % m;
+}
+
+static i64 gcd_i64(i64 a, i64 b) {
+  if (a < 0) a = -a;
+  if (b < 0) b = -b;
+  while (b) { const i64 t = a % b; a = b; b = t; }
+  return a;
+}
+
+static i64 inv_mod(i64 a, i64 m) {  // extended Euclid, gcd(a,m)=1
+  i64 t = 0, nt = 1, r = m, nr = ((a % m) + m) % m;
+  while (nr) {
+    const i64 q = r / nr;
+    i64 tmp = t - q * nt; t = nt; nt = tmp;
+    tmp = r - q * nr; r = nr; nr = tmp;
+  }
+  return t < 0 ? t + m : t;
+}
+
+bool admissible_B(i64 B) { return B > 1 && B % 2 != 0 && B % 3 != 0 && B % 7 != 0; }
+
+// Sixth roots modulo M in {2^6, 3^6, 7^6}.  The unit group mod M is small
+// and the roots of t are a0*z for one root a0 and every z with z^6 = 1, so we
+// keep one root per residue and the roots of unity.
+struct SixthRootTable {
+  i64 M;
+  std::vector<int32_t> one_root;  // one_root[t] = some unit a with a^6 = t, or -1
+  std::vector<i64> unity;         // z with z^6 = 1, ascending
+  explicit SixthRootTable(i64 m) : M(m), one_root((size_t)m, -1) {
+    for (i64 a = 1; a < M; ++a) {
+      if (gcd_i64(a, M) != 1) continue;
+      const i64 t = pow6_mod(a, M);
+      if (one_root[(size_t)t] < 0) one_root[(size_t)t] = (int32_t)a;
+      if (t == 1) unity.push_back(a);
+    }
+  }
+  std::vector<i64> roots(i64 t) const {
+    std::vector<i64> out;
+    t %= M;
+    if (t < 0) t += M;
+    const int32_t a0 = one_root[(size_t)t];
+    if (a0 < 0) return out;
+    out.reserve(unity.size());
+    for (i64 z : unity) out.push_back((i64)a0 * z % M);
+    std::sort(out.begin(), out.end());
+    return out;
+  }
+};
+
+static const SixthRootTable& table_for(i64 M) {
+  static const SixthRootTable t2(kM2), t3(kM3), t7(kM7);
+  if (M == kM2) return t2;
+  if (M == kM3) return t3;
+  if (M == kM7) return t7;
+  throw std::invalid_argument("sixth_roots: modulus must be 64, 729 or 7^6");
+}
+
+std::vector<i64> sixth_roots(i64 t, i64 M) { return table_for(M).roots(t); }
+
+static i64 crt_pair(i64 a, i64 ma, i64 b, i64 mb) {  // coprime moduli
+  const i64 k = ((b - a) % mb + mb) % mb * inv_mod(ma % mb, mb) % mb;
+  return a + ma * k;
+}
+
+std::vector<i64> seeds_for_B(i64 B, int cls) {
+  const auto s7 = sixth_roots(pow6_mod(B, kM7), kM7);
+  std::vector<i64> out;
+  if (cls == 1) {
+    const auto s2 = sixth_roots(pow6_mod(B, kM2), kM2);
+    const auto s3 = sixth_roots(pow6_mod(B, kM3), kM3);
+    for (i64 a : s2)
+      for (i64 b : s3) {
+        const i64 ab = crt_pair(a, kM2, b, kM3);
+        for (i64 c : s7) out.push_back(crt_pair(ab, kM2 * kM3, c, kM7));
+      }
+  } else if (cls == 2) {
+    for (i64 a : sixth_roots(pow6_mod(B, kM2), kM2))
+      for (i64 c : s7) out.push_back(crt_pair(a, kM2, c, kM7));
+  } else if (cls == 3) {
+    for (i64 b : sixth_roots(pow6_mod(B, kM3), kM3))
+      for (i64 c : s7) out.push_back(crt_pair(b, kM3, c, kM7));
+  } else {
+    out = s7;
+  }
+  std::sort(out.begin(), out.end());
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  return out;
+}
+
+static i64 class_modulus(int cls) {
+  switch (cls) {
+    case 1: return (i64)kM42;
+    case 2: return kM14;
+    case 3: return kM21;
+    default: return kM7;
+  }
+}
+
+static bool unit_ok(i64 u, int cls) {
+  switch (cls) {
+    case 1: return (u & 1) && u % 3 != 0 && u % 7 != 0;
+    case 2: return (u & 1) && u % 3 == 0 && u % 9 != 0;
+    case 3: return !(u & 1) && u % 3 != 0 && u % 7 != 0;
+    case 4:
+    case 5: return u % 6 == 0 && u % 7 != 0;
+  }
+  return false;
+}
+
+std::vector<i64> unit_candidates(i64 B, int cls, double lo_frac, double hi_frac) {
+  const i64 M = class_modulus(cls);
+  const i64 lo = std::max<i64>(1, (i64)(lo_frac * (double)B));
+  const i64 hi = std::min<i64>(B - 1, (i64)(hi_frac * (double)B));
+  std::vector<i64> out;
+  if (lo > hi) return out;
+  for (i64 s : seeds_for_B(B, cls)) {
+    i64 u = ((s % M) + M) % M;
+    if (u < lo) u += (lo - u + M - 1) / M * M;
+    for (; u <= hi; u += M)
+      if (unit_ok(u, cls)) out.push_back(u);
+  }
+  std::sort(out.begin(), out.end());
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  return out;
+}
+
+int free_factor(int cls) { return cls == 2 ? 14 : cls == 3 ? 21 : cls == 4 ? 7 : 0; }
+
+void free_de_cls5(i64 B, i64 u, FreeSpec& e, FreeSpec& d) {
+  const i64 te = pow6_mod(B, kM3) * inv_mod(pow6_mod(14, kM3), kM3) % kM3;
+  e.res = sixth_roots(te, kM3);
+  e.mod = kM3;
+  e.fmax = (B - 1) / 14;
+  const i64 rhs = ((pow6_mod(B, kM2) - pow6_mod(u, kM2)) % kM2 + kM2) % kM2;
+  const i64 td = rhs * inv_mod(pow6_mod(21, kM2), kM2) % kM2;
+  d.res = sixth_roots(td, kM2);
+  d.mod = kM2;
+  d.fmax = (B - 1) / 21;
+}
+
+FreeSpec free_d_cls234(i64 B, i64 u, int cls) {
+  FreeSpec s;
+  if (cls == 2) {
+    s.res = sixth_roots(pow6_mod(B, kM3) * inv_mod(pow6_mod(14, kM3), kM3) % kM3, kM3);
+    s.mod = kM3;
+    s.fmax = (B - 1) / 14;
+  } else if (cls == 3) {
+    const i64 rhs = ((pow6_mod(B, kM2) - pow6_mod(u, kM2)) % kM2 + kM2) % kM2;
+    s.res = sixth_roots(rhs * inv_mod(pow6_mod(21, kM2), kM2) % kM2, kM2);
+    s.mod = kM2;
+    s.fmax = (B - 1) / 21;
+  } else if (cls == 4) {
+    const auto a = sixth_roots(pow6_mod(B, kM2) * inv_mod(pow6_mod(7, kM2), kM2) % kM2, kM2);
+    const auto b = sixth_roots(pow6_mod(B, kM3) * inv_mod(pow6_mod(7, kM3), kM3) % kM3, kM3);
+    for (i64 x : a)
+      for (i64 y : b) s.res.push_back(crt_pair(x, kM2, y, kM3));
+    std::sort(s.res.begin(), s.res.end());
+    s.mod = kM2 * kM3;
+    s.fmax = (B - 1) / 7;
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// U256
+// ---------------------------------------------------------------------------
+U256 U256::pow6(u64 x) {
+  U256 r = U256::of(x);
+  for (int i = 0; i < 5; ++i) mul_small(r, x);
+  return r;
+}
+
+int U256::bits() const {
+  for (int i = 3; i >= 0; --i)
+    if (l[i]) return 64 * i + (64 - __builtin_clzll(l[i]));
+  return 0;
+}
+
+bool sub_in_place(U256& a, const U256& b) {
+  u64 borrow = 0;
+  for (int i = 0; i < 4; ++i) {
+    const u64 x = a.l[i], y = b.l[i];
+    const u64 d = x - y;
+    const u64 b1 = x < y;
+    a.l[i] = d - borrow;
+    borrow = b1 | (d < borrow);
+  }
+  return borrow != 0;
+}
+
+void add_in_place(U256& a, const U256& b) {
+  u64 c = 0;
+  for (int i = 0; i < 4; ++i) {
+    const u128 s = (u128)a.l[i] + b.l[i] + c;
+    a.l[i] = (u64)s;
+    c = (u64)(s >> 64);
+  }
+}
+
+void mul_small(U256& a, u64 m) {
+  u64 c = 0;
+  for (int i = 0; i < 4; ++i) {
+    const u128 p = (u128)a.l[i] * m + c;
+    a.l[i] = (u64)p;
+    c = (u64)(p >> 64);
+  }
+}
+
+u64 divmod_small(U256& a, u64 d) {
+  u128 rem = 0;
+  for (int i = 3; i >= 0; --i) {
+    const u128 cur = (rem << 64) | a.l[i];
+    a.l[i] = (u64)(cur / d);
+    rem = cur % d;
+  }
+  return (u64)rem;
+}
+
+int cmp(const U256& a, const U256& b) {
+  for (int i = 3; i >= 0; --i)
+    if (a.l[i] != b.l[i]) return a.l[i] < b.l[i] ? -1 : 1;
+  return 0;
+}
+
+static bool peel_T(U256 num, u128& T) {
+  if (num.is_zero()) return false;
+  if (divmod_small(num, kM42) != 0) return false;
+  if (!num.fits128()) return false;
+  T = num.lo128();
+  return true;
+}
+
+bool exact_T_cls5(u64 B, u64 u, u64 d, u64 e, u128& T) {
+  if (u >= B || d < 1 || e < 1) return false;
+  U256 num = U256::pow6(B);
+  if (sub_in_place(num, U256::pow6(u))) return false;
+  if (sub_in_place(num, U256::pow6(21 * d))) return false;
+  if (sub_in_place(num, U256::pow6(14 * e))) return false;
+  return peel_T(num, T);
+}
+
+bool exact_T_cls234(u64 B, u64 u, u64 f, u64 d, u128& T) {
+  if (u >= B || f < 1 || d < 1) return false;
+  U256 num = U256::pow6(B);
+  if (sub_in_place(num, U256::pow6(u))) return false;
+  if (sub_in_place(num, U256::pow6(f * d))) return false;
+  return peel_T(num, T);
+}
+
+bool verify_cls5(u64 B, u64 u, u64 d, u64 e, u64 a, u64 b) {
+  U256 rhs = U256::pow6(u);
+  add_in_place(rhs, U256::pow6(21 * d));
+  add_in_place(rhs, U256::pow6(14 * e));
+  U256 core = U256::pow6(a);
+  add_in_place(core, U256::pow6(b));
+  mul_small(core, kM42);
+  add_in_place(rhs, core);
+  return cmp(rhs, U256::pow6(B)) == 0;
+}
+
+// ---------------------------------------------------------------------------
+// files
+// ---------------------------------------------------------------------------
+bool read_buc(const std::string& path, unsigned cls_mask, std::vector<BucLine>& out) {
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return false;
+  char buf[256];
+  while (std::fgets(buf, sizeof buf, f)) {
+    int cls;
+    unsigned long long B, u;
+    if (std::sscanf(buf, "%d %llu %llu", &cls, &B, &u) != 3) continue;
+    if (cls < 1 || cls > 5 || !(cls_mask & (1u << cls))) continue;
+    out.push_back({cls, (u64)B, (u64)u});
+  }
+  std::fclose(f);
+  return true;
+}
+
+int load_bin(const std::string& path, int expected_N, XorTable& t, std::string& msg) {
+  t.packed.clear();
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) { msg = "cannot open " + path + " for read"; return kLoadOpen; }
+  StoreHeader h;
+  const bool got = std::fread(&h, sizeof h, 1, f) == 1;
+  if (!got || h.magic != 0x53353136u || h.version != 2 || h.shard_count < 1 ||
+      h.shard_index >= h.shard_count || h.r == 0 || h.r > 64 || h.kind != 1) {
+    std::fclose(f);
+    if (got && h.magic == 0x53353136u && h.version != 2) {
+      msg = "unsupported store version " + std::to_string(h.version) + " (need 2 packed)";
+      return kLoadVersion;
+    }
+    msg = "bad file " + path;
+    return kLoadBad;
+  }
+  if (expected_N > 0 && (int)h.N != expected_N) {
+    std::fclose(f);
+    msg = "N mismatch: file N=" + std::to_string(h.N) + ", need " + std::to_string(expected_N);
+    return kLoadNMismatch;
+  }
+  const size_t want = packed_bytes(h.m_cells, h.r);
+  t.hdr = h;
+  t.packed.assign(want, 0);
+  if (want && std::fread(t.packed.data(), 1, want, f) != want) {
+    std::fclose(f);
+    t.packed.clear();
+    msg = "short read " + path;
+    return kLoadShort;
+  }
+  std::fclose(f);
+  return kLoadOk;
+}
+
+bool save_bin(const std::string& path, const XorTable& t, std::string& msg) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) { msg = "cannot open " + path + " for write"; return false; }
+  if (std::fwrite(&t.hdr, sizeof t.hdr, 1, f) != 1 ||
+      (!t.packed.empty() && std::fwrite(t.packed.data(), 1, t.packed.size(), f) != t.packed.size())) {
+    std::fclose(f);
+    msg = "short write " + path;
+    return false;
+  }
+  std::fclose(f);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// xor filter
+// ---------------------------------------------------------------------------
+size_t packed_bytes(u64 m_cells, u32 r) {
+  if (m_cells == 0 || r == 0) return 0;
+  return (size_t)((m_cells * (u64)r + 7) / 8 + 8);
+}
+
+size_t capacity_for(size_t n) {
+  const double need = 32.0 + 1.23 * (double)n;
+  size_t cap = (size_t)(need + 0.999);
+  if (cap < 3) cap = 3;
+  return (cap + 2) / 3 * 3;
+}
+
+u64 load_cell(const u8* p, u64 cell, u32 r) {
+  const u64 bit = cell * (u64)r;
+  const u64 byte = bit >> 3;
+  const unsigned off = (unsigned)(bit & 7);
+  u64 w;
+  std::memcpy(&w, p + byte, 8);
+  if (off + r <= 64) return (w >> off) & fp_mask(r);
+  u64 w2;
+  std::memcpy(&w2, p + byte + 8, 8);
+  return ((w >> off) | (w2 << (64 - off))) & fp_mask(r);
+}
+
+void store_cell(u8* p, u64 cell, u32 r, u64 v) {
+  const u64 mask = fp_mask(r);
+  v &= mask;
+  const u64 bit = cell * (u64)r;
+  const u64 byte = bit >> 3;
+  const unsigned off = (unsigned)(bit & 7);
+  u64 w;
+  std::memcpy(&w, p + byte, 8);
+  if (off + r <= 64) {
+    w = (w & ~(mask << off)) | (v << off);
+    std::memcpy(p + byte, &w, 8);
+    return;
+  }
+  const unsigned low = 64 - off;  // bits that land in the first word
+  const u64 low_mask = (1ULL << low) - 1;
+  w = (w & ~(low_mask << off)) | ((v & low_mask) << off);
+  std::memcpy(p + byte, &w, 8);
+  const unsigned high = r - low;
+  const u64 high_mask = high >= 64 ? ~0ULL : ((1ULL << high) - 1);
+  u64 w2;
+  std::memcpy(&w2, p + byte + 8, 8);
+  w2 = (w2 & ~high_mask) | ((v >> low) & high_mask);
+  std::memcpy(p + byte + 8, &w2, 8);
+}
+
+void cell_indices(u64 h, u64 block, u64 out[3]) {
+  out[0] = h % block;
+  out[1] = mix64(h ^ 0x9E3779B97F4A7C15ULL) % block + block;
+  out[2] = mix64(h ^ 0xBF58476D1CE4E5B9ULL) % block + 2 * block;
+}
+
+bool xor_query(const XorTable& t, u64 key) {
+  if (t.hdr.n_keys == 0 || t.packed.empty() || t.hdr.m_cells < 3) return false;
+  const u64 block = t.hdr.m_cells / 3;
+  const u64 h = mix64(key + t.hdr.mix_seed);
+  u64 c[3];
+  cell_indices(h, block, c);
+  const u8* p = t.packed.data();
+  return (load_cell(p, c[0], t.hdr.r) ^ load_cell(p, c[1], t.hdr.r) ^ load_cell(p, c[2], t.hdr.r)) ==
+         (h & fp_mask(t.hdr.r));
+}
+
+int fpr_smoke(const XorTable& t, int trials) {
+  if (t.packed.empty() || t.hdr.m_cells < 3 || t.hdr.r == 0) return -1;
+  int hits = 0;
+  u64 x = 0x123456789abcdef0ULL;
+  for (int i = 0; i < trials; ++i) {
+    x = x * kPhi + 1;
+    hits += xor_query(t, x) ? 1 : 0;
+  }
+  return hits;
+}
+
+// Parallel LSD radix sort (11-bit digits, 6 passes) followed by unique.
+static void sort_unique(std::vector<u64>& v) {
+  const size_t n = v.size();
+  if (n < (1u << 16)) {
+    std::sort(v.begin(), v.end());
+  } else {
+    std::vector<u64> tmp(n);
+    u64* src = v.data();
+    u64* dst = tmp.data();
+    constexpr int kBits = 11, kR = 1 << kBits;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = std::min(omp_get_max_threads(), 64);
+#endif
+    std::vector<size_t> hist((size_t)nt * kR);
+    for (int pass = 0; pass < 6; ++pass) {
+      const int sh = pass * kBits;
+      std::fill(hist.begin(), hist.end(), 0);
+#pragma omp parallel num_threads(nt)
+      {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        const size_t lo = n * (size_t)t / (size_t)nt, hi = n * (size_t)(t + 1) / (size_t)nt;
+        size_t* h = &hist[(size_t)t * kR];
+        for (size_t i = lo; i < hi; ++i) h[(src[i] >> sh) & (kR - 1)]++;
+#pragma omp barrier
+#pragma omp single
+        {
+          size_t s = 0;
+          for (int b = 0; b < kR; ++b)
+            for (int q = 0; q < nt; ++q) {
+              const size_t c = hist[(size_t)q * kR + b];
+              hist[(size_t)q * kR + b] = s;
+              s += c;
+            }
+        }
+        for (size_t i = lo; i < hi; ++i) dst[h[(src[i] >> sh) & (kR - 1)]++] = src[i];
+      }
+      std::swap(src, dst);
+    }
+    // 6 passes: data is back in v
+  }
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+}
+
+// One construction attempt (xor_filter.hpp:256-431 semantics).  Incidence is
+// order-free (xor, add), so it runs in parallel; the FIFO peel and the
+// reverse assignment are sequential because they fix the table bytes.
+static int construct_once(const std::vector<u64>& keys, u32 r, u64 seed, XorTable& out) {
+  const size_t n = keys.size();
+  out.hdr = StoreHeader{};
+  out.hdr.r = r;
+  out.hdr.mix_seed = seed;
+  if (n == 0) { out.packed.clear(); return 1; }
+  const size_t m = capacity_for(n);
+  const u64 block = m / 3;
+  std::vector<u64> setxor(m, 0);
+  std::vector<u32> counts(m, 0);
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; ++i) {
+    const u64 h = mix64(keys[i] + seed);
+    u64 c[3];
+    cell_indices(h, block, c);
+    for (int t = 0; t < 3; ++t) {
+      __atomic_fetch_xor(&setxor[c[t]], h, __ATOMIC_RELAXED);
+      __atomic_fetch_add(&counts[c[t]], 1u, __ATOMIC_RELAXED);
+    }
+  }
+  for (size_t i = 0; i < m; ++i)
+    if (counts[i] > 0xffffu) return -1;  // the reference's u16 incidence overflow
+  std::vector<std::pair<u64, u64>> stack;
+  stack.reserve(n);
+  std::vector<u64> queue;
+  queue.reserve(1 << 20);
+  for (size_t i = 0; i < m; ++i)
+    if (counts[i] == 1) queue.push_back(i);
+  for (size_t q = 0; q < queue.size(); ++q) {
+    const u64 i = queue[q];
+    if (counts[i] != 1) continue;
+    const u64 h = setxor[i];
+    u64 c[3];
+    cell_indices(h, block, c);
+    stack.emplace_back(h, i);
+    for (int t = 0; t < 3; ++t) {
+      const u64 j = c[t];
+      setxor[j] ^= h;
+      if (counts[j] > 0 && --counts[j] == 1) queue.push_back(j);
+    }
+  }
+  if (stack.size() != n) return 0;
+  std::vector<u64>().swap(setxor);
+  std::vector<u32>().swap(counts);
+  std::vector<u64>().swap(queue);
+  const u64 mask = fp_mask(r);
+  out.packed.assign(packed_bytes(m, r), 0);
+  u8* p = out.packed.data();
+  for (size_t k = stack.size(); k-- > 0;) {
+    const u64 h = stack[k].first, at = stack[k].second;
+    u64 c[3];
+    cell_indices(h, block, c);
+    u64 v = h & mask;
+    for (int t = 0; t < 3; ++t)
+      if (c[t] != at) v ^= load_cell(p, c[t], r);
+    store_cell(p, at, r, v);
+  }
+  out.hdr.n_keys = n;
+  out.hdr.m_cells = m;
+  return 1;
+}
+
+bool build_xor_cpu(std::vector<u64>& keys, u32 r, u64 base_seed, int seed_tries, XorTable& out) {
+  sort_unique(keys);
+  for (int t = 0; t < seed_tries; ++t) {
+    const int rc = construct_once(keys, r, base_seed + (u64)t * kPhi, out);
+    if (rc == 1) return true;
+    if (rc < 0) return false;
+  }
+  return false;
+}
+
+bool build_pairs_cpu(int N, u32 r, const std::vector<u64>& extra, XorTable& out, std::string& msg) {
+  if (N < 1 || r < 1 || r > 64) { msg = "bad N or r"; return false; }
+  std::vector<u64> pw6((size_t)N + 1);
+  for (int x = 1; x <= N; ++x) { const u64 x2 = (u64)x * (u64)x; pw6[(size_t)x] = x2 * x2 * x2; }
+  const size_t np = (size_t)N * (size_t)(N + 1) / 2;
+  for (int outer = 0; outer < 4; ++outer) {
+    std::vector<u64> keys(np + extra.size());
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 1; i <= N; ++i) {
+      // row i starts after rows 1..i-1, each of length N-j+1
+      size_t k = (size_t)(i - 1) * (size_t)N - (size_t)(i - 1) * (size_t)(i - 2) / 2;
+      for (int j = i; j <= N; ++j) keys[k++] = pw6[(size_t)i] + pw6[(size_t)j];
+    }
+    std::copy(extra.begin(), extra.end(), keys.begin() + (ptrdiff_t)np);
+    const u64 base = 0x615615615615615ULL + (u64)outer * kPhi;
+    if (build_xor_cpu(keys, r, base, 1, out)) {
+      out.hdr.N = (u32)N;
+      return true;
+    }
+  }
+  msg = "xor construction stalled at every seed";
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// units
+// ---------------------------------------------------------------------------
+bool pack_cls5(u64 B, u64 u, Cls5Unit& U) {
+  U = Cls5Unit{};
+  U.B = B;
+  U.u = u;
+  U.lim = (u32)((B - 1) / 42);
+  if (u >= B || B >= (1ULL << 32)) return false;
+  U.base = U256::pow6(B);
+  if (sub_in_place(U.base, U256::pow6(u)) || U.base.is_zero()) return false;
+  FreeSpec es, ds;
+  free_de_cls5((i64)B, (i64)u, es, ds);
+  U.mod1 = (u32)es.mod;
+  U.mod2 = (u32)ds.mod;
+  U.fmax1 = (u32)es.fmax;
+  U.fmax2 = (u32)ds.fmax;
+  U.n1 = (u32)std::min<size_t>(es.res.size(), 12);
+  U.n2 = (u32)std::min<size_t>(ds.res.size(), 12);
+  for (u32 i = 0; i < U.n1; ++i) U.res1[i] = (u32)es.res[i];
+  for (u32 i = 0; i < U.n2; ++i) U.res2[i] = (u32)ds.res[i];
+  if (U.n1 == 0 || U.n2 == 0) return false;
+  // T-space form: every e residue has the same e^6 mod 729 and every d residue
+  // the same d^6 mod 64; then R = 42^6 (bq - eq - dq) + (br - 14^6 te - 21^6 td).
+  U.te = (u32)pow6_mod((i64)U.res1[0], kM3);
+  U.td = (u32)pow6_mod((i64)U.res2[0], kM2);
+  bool uniform = true;
+  for (u32 i = 0; i < U.n1; ++i) uniform &= pow6_mod((i64)U.res1[i], kM3) == (i64)U.te;
+  for (u32 i = 0; i < U.n2; ++i) uniform &= pow6_mod((i64)U.res2[i], kM2) == (i64)U.td;
+  U256 q = U.base;
+  const u64 br = divmod_small(q, kM42);
+  // 14^6 = kM14 and 21^6 = kM21; the remainder sum is a multiple of 42^6 in (-2*42^6, 42^6)
+  const __int128 rem = (__int128)br - (__int128)kM14 * U.te - (__int128)kM21 * U.td;
+  const bool divisible = rem == 0 || rem == -(__int128)kM42;
+  const bool small = q.fits128() && U.fmax1 < (1u << 21) && U.fmax2 < (1u << 21) && U.n1 == 6 &&
+                     U.n2 == 4 && U.mod1 == 729 && U.mod2 == 64;
+  U.regular = uniform && divisible && small;
+  if (U.regular) U.Q = q.lo128() - (rem == 0 ? 0 : 1);
+  return true;
+}
+
+u64 cls5_calls(const Cls5Unit& U) {
+  auto count = [](const u32* res, u32 n, u32 mod, u32 fmax) {
+    u64 c = 0;
+    for (u32 i = 0; i < n; ++i)
+      for (u64 x = res[i]; x <= fmax; x += mod)
+        if (x) ++c;
+    return c;
+  };
+  return count(U.res1, U.n1, U.mod1, U.fmax1) * count(U.res2, U.n2, U.mod2, U.fmax2);
+}
+
+// ---------------------------------------------------------------------------
+// pair recovery / verify
+// ---------------------------------------------------------------------------
+void PairRecover::build(int N) {
+  N_ = N;
+  sorted_.resize((size_t)N);
+  for (int x = 1; x <= N; ++x) {
+    const u64 x2 = (u64)x * (u64)x;
+    sorted_[(size_t)x - 1] = {x2 * x2 * x2, (u32)x};
+  }
+  std::sort(sorted_.begin(), sorted_.end());
+}
+
+void PairRecover::recover(u64 fp, std::vector<std::pair<u32, u32>>& out) const {
+  out.clear();
+  for (int i = 1; i <= N_; ++i) {
+    const u64 i2 = (u64)i * (u64)i;
+    const u64 need = fp - i2 * i2 * i2;
+    auto it = std::lower_bound(sorted_.begin(), sorted_.end(), std::make_pair(need, (u32)0));
+    for (; it != sorted_.end() && it->first == need; ++it)
+      if ((u32)i <= it->second) out.emplace_back((u32)i, it->second);
+  }
+}
+
+int verify_cls5_hit(const PairRecover& pr, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp,
+                    std::vector<Solution>& out) {
+  std::vector<std::pair<u32, u32>> pairs;
+  pr.recover(fp, pairs);
+  int found = 0;
+  for (auto [a, b] : pairs) {
+    if (a > lim || b > lim) continue;
+    u128 T;
+    if (!exact_T_cls5(B, u, d, e, T)) continue;
+    const U256 pa = U256::pow6(a), pb = U256::pow6(b);
+    U256 s = pa;
+    add_in_place(s, pb);
+    if (!s.fits128() || s.lo128() != T) continue;
+    if (!verify_cls5(B, u, d, e, a, b)) continue;
+    i64 t[5] = {42LL * a, 42LL * b, 21LL * (i64)d, 14LL * (i64)e, (i64)u};
+    std::sort(t, t + 5);
+    bool bad = false;
+    for (int i = 0; i < 4; ++i) bad |= t[i] == t[i + 1];
+    if (bad || t[4] >= (i64)B) continue;
+    i64 g = t[0];
+    for (int i = 1; i < 5; ++i) g = gcd_i64(g, t[i]);
+    g = gcd_i64(g, (i64)B);
+    Solution S{B, u, d, e, a, b, {t[0], t[1], t[2], t[3], t[4]}, g};
+    out.push_back(S);
+    ++found;
+  }
+  return found;
+}
+
+}  // namespace fc5

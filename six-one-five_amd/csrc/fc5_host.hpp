@@ -1,0 +1,202 @@
+// fc5_host.hpp -- host-side core of the class-5 free-peel engine.
+//
+// Number theory of the Stage-1 contracts, exact big integers, the .buc unit
+// files, the StoreHeader-v2 .bin pair tables, the xor-filter construction
+// (CPU form, byte-identical to the reference builder), unit packing into the
+// quotient ("T-space") form the device kernels use, and pair recovery /
+// exact verification of filter hits.
+//
+// Reference interfaces mirrored here (same meaning, independent code):
+//   fourcore_classes_v3.hpp:80-232   eligibility, seeds, unit filter, free terms
+//   fourcore_hunt_v3.cpp:371-381     Stage-1 emission ("cls B u" lines)
+//   fourcore_cls5_gpu_v4.cu:155-184  .buc reading, unit packing
+//   spike/ribbon/store_header.hpp    64-byte header v2
+//   spike/ribbon/xor_pack.hpp        r-bit cell packing (+8 byte pad)
+//   spike/ribbon/xor_filter.hpp      3-hash xor filter build and query
+//   fourcore_xor_store.hpp:23-181    pair keys, seed schedule, save/load, FPR smoke
+//   fourcore_gmp.hpp:55-117          exact peel and identities (GMP replaced by U256)
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fc5 {
+
+using u8 = std::uint8_t;
+using u16 = std::uint16_t;
+using u32 = std::uint32_t;
+using u64 = std::uint64_t;
+using i64 = std::int64_t;
+using u128 = unsigned __int128;
+
+constexpr i64 kM2 = 64, kM3 = 729, kM7 = 117649;
+constexpr i64 kM14 = 7529536, kM21 = 85766121;
+constexpr u64 kM42 = 5489031744ULL;  // 42^6
+constexpr u64 kPhi = 0x9E3779B97F4A7C15ULL;
+constexpr u32 kGateMod = 124488;      // 504 * 247: the gate depends on T mod this
+
+// ---------------------------------------------------------------------------
+// number theory (classes:37-232)
+// ---------------------------------------------------------------------------
+i64 pow6_mod(i64 x, i64 m);
+bool admissible_B(i64 B);
+// sorted sixth roots a (gcd(a,M)=1, 1<=a<M) of t modulo M in {64, 729, 7^6}
+std::vector<i64> sixth_roots(i64 t, i64 M);
+std::vector<i64> seeds_for_B(i64 B, int cls);
+std::vector<i64> unit_candidates(i64 B, int cls, double lo_frac, double hi_frac);
+
+struct FreeSpec {
+  i64 mod = 0, fmax = 0;
+  std::vector<i64> res;
+};
+void free_de_cls5(i64 B, i64 u, FreeSpec& e, FreeSpec& d);
+FreeSpec free_d_cls234(i64 B, i64 u, int cls);
+int free_factor(int cls);  // 14, 21, 7 for classes 2, 3, 4
+
+// ---------------------------------------------------------------------------
+// exact 256-bit unsigned arithmetic (replaces GMP on this path)
+// ---------------------------------------------------------------------------
+struct U256 {
+  u64 l[4] = {0, 0, 0, 0};
+  static U256 of(u128 v) { U256 r; r.l[0] = (u64)v; r.l[1] = (u64)(v >> 64); return r; }
+  static U256 pow6(u64 x);  // x < 2^42
+  bool is_zero() const { return !(l[0] | l[1] | l[2] | l[3]); }
+  bool fits128() const { return !(l[2] | l[3]); }
+  u128 lo128() const { return ((u128)l[1] << 64) | l[0]; }
+  int bits() const;
+};
+bool sub_in_place(U256& a, const U256& b);  // returns borrow
+void add_in_place(U256& a, const U256& b);
+void mul_small(U256& a, u64 m);
+u64 divmod_small(U256& a, u64 d);  // a /= d, returns a % d
+int cmp(const U256& a, const U256& b);
+
+// T = (B^6 - u^6 - (21d)^6 - (14e)^6) / 42^6 if positive, divisible and < 2^128
+bool exact_T_cls5(u64 B, u64 u, u64 d, u64 e, u128& T);
+// T = (B^6 - u^6 - (f*d)^6) / 42^6 (classes 2-4)
+bool exact_T_cls234(u64 B, u64 u, u64 f, u64 d, u128& T);
+bool verify_cls5(u64 B, u64 u, u64 d, u64 e, u64 a, u64 b);
+
+// ---------------------------------------------------------------------------
+// files
+// ---------------------------------------------------------------------------
+struct BucLine { int cls; u64 B, u; };
+// Lines "cls B u"; non-matching lines are skipped; cls_mask bit c keeps class c.
+bool read_buc(const std::string& path, unsigned cls_mask, std::vector<BucLine>& out);
+
+#pragma pack(push, 1)
+struct StoreHeader {
+  u32 magic = 0x53353136u;
+  u32 version = 2;
+  u32 kind = 1;  // xor
+  u32 r = 48;
+  u32 w = 0;
+  u32 N = 0;
+  u64 n_keys = 0;
+  u64 m_cells = 0;
+  u64 mix_seed = 0;
+  u32 shard_count = 1;
+  u32 shard_index = 0;
+  u32 shard_mode = 0;
+  u32 reserved = 0;
+};
+#pragma pack(pop)
+static_assert(sizeof(StoreHeader) == 64, "StoreHeader v2 is 64 bytes");
+
+struct XorTable {
+  StoreHeader hdr{};
+  std::vector<u8> packed;  // r-bit cells + 8 pad bytes
+};
+
+enum LoadStatus { kLoadOk = 0, kLoadOpen = -1, kLoadBad = -2, kLoadVersion = -3,
+                  kLoadNMismatch = -4, kLoadShort = -5 };
+int load_bin(const std::string& path, int expected_N, XorTable& t, std::string& msg);
+bool save_bin(const std::string& path, const XorTable& t, std::string& msg);
+
+// ---------------------------------------------------------------------------
+// xor filter
+// ---------------------------------------------------------------------------
+inline u64 mix64(u64 x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+inline u64 fp_mask(u32 r) { return r >= 64 ? ~0ULL : ((1ULL << r) - 1ULL); }
+size_t packed_bytes(u64 m_cells, u32 r);
+size_t capacity_for(size_t n);
+u64 load_cell(const u8* p, u64 cell, u32 r);
+void store_cell(u8* p, u64 cell, u32 r, u64 v);
+void cell_indices(u64 h, u64 block, u64 out[3]);
+bool xor_query(const XorTable& t, u64 key);
+int fpr_smoke(const XorTable& t, int trials = 200000);
+// keys: any order, duplicates allowed.  One construction attempt per seed in
+// the schedule base + k*phi; returns false if all stall.
+bool build_xor_cpu(std::vector<u64>& keys, u32 r, u64 base_seed, int seed_tries, XorTable& out);
+// Pair table for 1<=a<=b<=N plus optional extra keys, with the reference's
+// outer seed schedule (4 bases, one seed each).  Byte-identical to the
+// reference builder when extras are empty.
+bool build_pairs_cpu(int N, u32 r, const std::vector<u64>& extra, XorTable& out, std::string& msg);
+
+// ---------------------------------------------------------------------------
+// units in T-space
+// ---------------------------------------------------------------------------
+// For a regular class-5 unit every grid residual R = B^6-u^6-(21d)^6-(14e)^6
+// is a multiple of 42^6 and T = R/42^6 = Q - floor(e^6/729) - floor(d^6/64),
+// with Q = floor((B^6-u^6)/42^6) + c and c in {-1,0} fixed per unit (the
+// remainders of e^6 mod 729 and d^6 mod 64 are constant over the residue
+// classes).  Q < 2^128 for B < 2^25, so the device never needs 192 bits.
+struct Cls5Unit {
+  u64 B = 0, u = 0;
+  U256 base;            // B^6 - u^6
+  u128 Q = 0;           // T-space offset (regular units)
+  u32 lim = 0, fmax1 = 0, fmax2 = 0;
+  u32 n1 = 0, n2 = 0;
+  u32 mod1 = 729, mod2 = 64;
+  u32 te = 0, td = 0;   // e^6 mod 729 and d^6 mod 64 over the residue classes
+  u32 res1[12] = {0}, res2[12] = {0};
+  bool regular = false; // Q form valid (else per-point 192-bit path)
+};
+// false when the reference would not pack the unit (base<=0, >192 bits, no residues)
+bool pack_cls5(u64 B, u64 u, Cls5Unit& U);
+
+// calls of one unit in closed form: #valid e * #valid d
+u64 cls5_calls(const Cls5Unit& U);
+
+// ---------------------------------------------------------------------------
+// pair recovery and exact verification (cls5v4:513-542, xstore:23-51)
+// ---------------------------------------------------------------------------
+struct Solution {
+  u64 B, u;
+  u32 d, e, a, b;
+  i64 terms[5];
+  i64 gcd;
+};
+class PairRecover {
+ public:
+  void build(int N);
+  // all (a<=b<=N) with a^6 + b^6 == fp (mod 2^64), a ascending
+  void recover(u64 fp, std::vector<std::pair<u32, u32>>& out) const;
+  int N() const { return N_; }
+
+ private:
+  int N_ = 0;
+  std::vector<std::pair<u64, u32>> sorted_;  // (x^6 mod 2^64, x)
+};
+// full verify chain for one cls5 maybe-hit; appends solutions
+int verify_cls5_hit(const PairRecover& pr, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp,
+                    std::vector<Solution>& out);
+
+// splitmix64 (Vigna), the generator for planted keys
+inline u64 splitmix64(u64& s) {
+  u64 z = (s += kPhi);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+}  // namespace fc5

@@ -1,0 +1,109 @@
+// fc5_kernels.hpp -- device data layouts and kernel launchers (gfx950).
+//
+// Layouts in HBM:
+//   pair table   one u64 per xor cell ("expanded"; the r-bit packed file
+//                layout is widened once at upload), m_cells * 8 bytes
+//   gate         124488-bit bitmap of T mod 504*247 (g504 && g247), 15.2 KiB,
+//                staged into LDS by every workgroup
+//   ndg table    [64][K] u32: (124488 - floor(d^6/64) mod 124488) for
+//                d = r + 64k, transposed by d mod 64 so that the lanes of a
+//                wave read consecutive k of one residue class
+//   units        DevUnit5, 64 bytes each, T-space form (see fc5_host.hpp)
+//   hits         HitRec, 24 bytes, append-only with a capacity
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fc5_moddiv.hpp"
+
+namespace fc5 {
+
+using u8 = std::uint8_t;
+using u16 = std::uint16_t;
+using u32 = std::uint32_t;
+using u64 = std::uint64_t;
+
+
+struct XorDev {
+  const u64* cells;  // expanded cells
+  u64 block;         // m_cells / 3
+  u64 mask;          // low r bits
+  u64 seed;          // mix seed
+  ModDiv md;
+  u32 present;       // 0: queries answer "no" (empty table), as block == 0
+  u32 pad;
+};
+
+struct DevUnit5 {
+  u64 Q_lo, Q_hi;       // T = Q - floor(e^6/729) - floor(d^6/64)
+  u32 fmax1, fmax2;
+  u32 rows, item_base;  // rows = 6 * (fmax1/729 + 1); first work item
+  u32 te, td;           // e^6 mod 729, d^6 mod 64
+  u32 uid, band;        // index in the caller's unit list; counter slot
+  u16 res1[6];          // e residues mod 729
+  u8 res2[4];           // d residues mod 64
+};
+static_assert(sizeof(DevUnit5) == 64, "DevUnit5 is 64 bytes");
+
+// reference-shaped unit (Unit5D, fourcore_cls5_gpu_v4.cu:346-351)
+struct PlainUnit {
+  u64 b0, b1, b2;
+  u32 fmax1, fmax2, mod1, mod2, nres1, nres2, uid, pad;
+  u32 res[24];
+};
+
+struct Counters {
+  u64 calls, neg, zero, gated, probes, maybe, side_maybe, pos;
+};
+
+struct HitRec {
+  u32 unit, d, e, flags;
+  u64 fp;
+};
+
+struct GateWords {  // reference bitmaps (GateData.g504/g247, fourcore_cls5_gpu_v4.cu:111-116)
+  u64 g504[8];
+  u64 g247[4];
+};
+
+struct BatchedArgs {
+  const DevUnit5* units;
+  u32 n_units, n_items;
+  const u32* gate;  // 124488-bit bitmap
+  const u32* ndg;   // [64][ndg_K]
+  u32 ndg_K, use_gate;
+  XorDev main, side;
+  HitRec* hits;
+  u64 hit_cap;
+  u64* nhits;     // hit slot counter (shared by all kernels of a run)
+  Counters* ctr;  // indexed by DevUnit5::band
+  u32* item_ctr;
+  u64 inv729;  // 729^-1 mod 2^64
+};
+
+struct PlainArgs {
+  const PlainUnit* units;
+  int n_units;
+  int use_gate;
+  GateWords gate;
+  XorDev main, side;
+  u64 inv216, c_m504, c_m247, inv42_247;
+  HitRec* hits;
+  u64 hit_cap;
+  u64* nhits;
+  Counters* ctr;
+};
+
+constexpr int kRowsPerItem = 8;  // e rows per work item of the batched kernel
+
+// launchers (return hipError_t as int)
+int launch_cls5_batched(const BatchedArgs& a, int grid, hipStream_t s);
+int launch_cls5_plain(const PlainArgs& a, int n_units, int ymax, hipStream_t s);
+int launch_expand_cells(const u32* packed_words, u64 m_cells, u32 r, u64* cells, hipStream_t s);
+int launch_query(const XorDev& x, const u64* keys, u64 n, u8* out, hipStream_t s);
+int launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s);
+int batched_blocks_per_cu();
+
+}  // namespace fc5

@@ -1,0 +1,338 @@
+"""Python binding of libfc5.so (include/fc5.h) over ctypes.
+
+This mirrors the reference's in-process seam ``run_cls5_gpu``
+(fourcore_cls5_gpu_v4.cu:412-413) and its CLI surface: load or build the
+pair table, pack Stage-1 units into a band, run the class-5 free peel on the
+GPU, read counters, hits and verified solutions.  It is a thin wrapper: every
+computation happens in libfc5.so (HIP kernels for gfx950); when the library
+is missing this module raises at import time instead of falling back to
+anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libfc5.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libfc5.so not built: {LIB_PATH} (run __graft_entry__.build())")
+
+_lib = C.CDLL(LIB_PATH)
+
+OK, E_ARG, E_IO, E_FORMAT, E_NMISMATCH, E_FPR, E_HIP, E_NOMEM, E_STATE, E_OVERFLOW, E_BUILD = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10)
+MODE_PLAIN, MODE_BATCHED = 0, 1
+
+
+class StoreHeader(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("magic", "version", "kind", "r", "w", "N")] + \
+               [(n, C.c_uint64) for n in ("n_keys", "m_cells", "mix_seed")] + \
+               [(n, C.c_uint32) for n in ("shard_count", "shard_index", "shard_mode", "reserved")]
+
+
+class Unit(C.Structure):
+    _fields_ = [("B", C.c_uint64), ("u", C.c_uint64)]
+
+
+class Result(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe", "n_hits",
+        "hits_dropped", "solutions", "units", "units_skipped", "units_irregular")] + \
+               [("kernel_ms", C.c_double), ("wall_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Hit(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("unit", "d", "e", "flags")] + [("fp", C.c_uint64)]
+
+
+class Solution(C.Structure):
+    _fields_ = [("B", C.c_uint64), ("u", C.c_uint64)] + \
+               [(n, C.c_uint32) for n in ("d", "e", "a", "b")] + \
+               [("terms", C.c_int64 * 5), ("gcd", C.c_int64)]
+
+
+_P = C.c_void_p
+_sig = {
+    "fc5_abi_version": (C.c_int, []),
+    "fc5_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "fc5_ctx_destroy": (None, [_P]),
+    "fc5_last_error": (C.c_char_p, [_P]),
+    "fc5_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
+    "fc5_table_load": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "fc5_table_upload": (C.c_int, [_P, C.POINTER(StoreHeader), _P, C.c_size_t]),
+    "fc5_table_build": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_size_t, C.c_int]),
+    "fc5_table_info": (C.c_int, [_P, C.POINTER(StoreHeader)]),
+    "fc5_table_save": (C.c_int, [_P, C.c_char_p]),
+    "fc5_table_query": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "fc5_table_fpr_smoke": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    "fc5_side_build": (C.c_int, [_P, _P, C.c_size_t, C.c_int]),
+    "fc5_side_clear": (C.c_int, [_P]),
+    "fc5_band_create": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(_P)]),
+    "fc5_band_destroy": (C.c_int, [_P, _P]),
+    "fc5_band_run": (C.c_int, [_P, _P, C.c_int, C.POINTER(Result)]),
+    "fc5_run_units": (C.c_int, [_P, _P, C.c_size_t, C.c_int, C.POINTER(Result)]),
+    "fc5_last_hits": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fc5_last_solutions": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fc5_units_calls": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_uint64)]),
+    "fc5_emit_units": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double,
+                                 _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fc5_build_pairs_file": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.POINTER(StoreHeader)]),
+}
+EXPORTS = tuple(_sig)
+for _name, (_res, _args) in _sig.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+class Fc5Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fc5 error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None and len(a) else None
+
+
+def units_array(B, u):
+    """Pack parallel B, u sequences into the fc5_unit array layout."""
+    arr = np.zeros(len(B), dtype=np.dtype([("B", "<u8"), ("u", "<u8")]))
+    arr["B"] = np.asarray(B, dtype=np.uint64)
+    arr["u"] = np.asarray(u, dtype=np.uint64)
+    return arr
+
+
+def emit_units(lo, hi, classes=(5,), u_band=(0.0, 1.0)):
+    """Stage-1 units (cls, B, u) for eligible B in [lo, hi] (fourcore_hunt_v3.cpp:371-381)."""
+    mask = 0
+    for c in classes:
+        mask |= 1 << c
+    n = C.c_size_t()
+    rc = _lib.fc5_emit_units(lo, hi, mask, u_band[0], u_band[1], None, None, None, 0, C.byref(n))
+    if rc:
+        raise Fc5Error(rc, "emit_units")
+    cls = np.zeros(n.value, np.uint32)
+    Bs = np.zeros(n.value, np.uint64)
+    us = np.zeros(n.value, np.uint64)
+    rc = _lib.fc5_emit_units(lo, hi, mask, u_band[0], u_band[1], _ptr(cls), _ptr(Bs), _ptr(us),
+                             n.value, C.byref(n))
+    if rc:
+        raise Fc5Error(rc, "emit_units")
+    return cls, Bs, us
+
+
+def units_calls(units):
+    out = C.c_uint64()
+    rc = _lib.fc5_units_calls(_ptr(units), len(units), C.byref(out))
+    if rc:
+        raise Fc5Error(rc, "units_calls")
+    return out.value
+
+
+def build_pairs_file(N, r, path):
+    h = StoreHeader()
+    rc = _lib.fc5_build_pairs_file(N, r, path.encode(), C.byref(h))
+    if rc:
+        raise Fc5Error(rc, "build_pairs_file")
+    return h
+
+
+@dataclass
+class RunOutput:
+    result: dict
+    hits: np.ndarray
+    solutions: list
+    rc: int
+
+
+HIT_DTYPE = np.dtype([("unit", "<u4"), ("d", "<u4"), ("e", "<u4"), ("flags", "<u4"), ("fp", "<u8")])
+
+
+class Band:
+    def __init__(self, ctx, units):
+        self.ctx = ctx
+        self.units = np.ascontiguousarray(units)
+        self.h = _P()
+        rc = _lib.fc5_band_create(ctx.h, _ptr(self.units), len(self.units), C.byref(self.h))
+        ctx._check(rc)
+
+    def run(self, mode=MODE_BATCHED, allow_overflow=False):
+        res = Result()
+        rc = _lib.fc5_band_run(self.ctx.h, self.h, mode, C.byref(res))
+        if rc not in (OK, E_OVERFLOW) or (rc == E_OVERFLOW and not allow_overflow):
+            self.ctx._check(rc)
+        return RunOutput(res.as_dict(), self.ctx.last_hits(), self.ctx.last_solutions(), rc)
+
+    def close(self):
+        if self.h:
+            _lib.fc5_band_destroy(self.ctx.h, self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device context (fc5_ctx): tables, options, runs."""
+
+    def __init__(self, device=0, **options):
+        self.h = _P()
+        rc = _lib.fc5_ctx_create(device, C.byref(self.h))
+        if rc:
+            raise Fc5Error(rc, f"cannot create a context on device {device}")
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def _check(self, rc):
+        if rc != OK:
+            raise Fc5Error(rc, _lib.fc5_last_error(self.h).decode())
+
+    def set_option(self, key, value):
+        self._check(_lib.fc5_set_option(self.h, key.encode(), int(value)))
+
+    def table_load(self, path, expected_N=0):
+        self._check(_lib.fc5_table_load(self.h, path.encode(), expected_N))
+
+    def table_upload(self, header: StoreHeader, packed: bytes):
+        buf = np.frombuffer(packed, dtype=np.uint8)
+        self._check(_lib.fc5_table_upload(self.h, C.byref(header), _ptr(buf), len(buf)))
+
+    def table_build(self, N, r=48, extra=None, on_gpu=False):
+        ex = np.ascontiguousarray(extra if extra is not None else np.zeros(0, np.uint64), dtype=np.uint64)
+        self._check(_lib.fc5_table_build(self.h, N, r, _ptr(ex), len(ex), 1 if on_gpu else 0))
+
+    def table_info(self):
+        h = StoreHeader()
+        self._check(_lib.fc5_table_info(self.h, C.byref(h)))
+        return h
+
+    def table_save(self, path):
+        self._check(_lib.fc5_table_save(self.h, path.encode()))
+
+    def table_query(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.zeros(len(keys), np.uint8)
+        self._check(_lib.fc5_table_query(self.h, _ptr(keys), len(keys), _ptr(out)))
+        return out
+
+    def fpr_smoke(self, trials=200000):
+        h = C.c_int()
+        self._check(_lib.fc5_table_fpr_smoke(self.h, trials, C.byref(h)))
+        return h.value
+
+    def side_build(self, keys, r=48):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        self._check(_lib.fc5_side_build(self.h, _ptr(keys), len(keys), r))
+
+    def side_clear(self):
+        self._check(_lib.fc5_side_clear(self.h))
+
+    def band(self, units):
+        return Band(self, units)
+
+    def run(self, units, mode=MODE_BATCHED, allow_overflow=False):
+        b = Band(self, units)
+        try:
+            return b.run(mode, allow_overflow)
+        finally:
+            b.close()
+
+    def last_hits(self):
+        n = C.c_size_t()
+        self._check(_lib.fc5_last_hits(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, HIT_DTYPE)
+        if n.value:
+            self._check(_lib.fc5_last_hits(self.h, out.ctypes.data_as(_P), n.value, C.byref(n)))
+        return out
+
+    def last_solutions(self):
+        n = C.c_size_t()
+        self._check(_lib.fc5_last_solutions(self.h, None, 0, C.byref(n)))
+        arr = (Solution * max(1, n.value))()
+        if n.value:
+            self._check(_lib.fc5_last_solutions(self.h, C.cast(arr, _P), n.value, C.byref(n)))
+        return [dict(B=s.B, u=s.u, d=s.d, e=s.e, a=s.a, b=s.b, terms=list(s.terms), gcd=s.gcd)
+                for s in arr[:n.value]]
+
+    def close(self):
+        if self.h:
+            _lib.fc5_ctx_destroy(self.h)
+            self.h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# planted keys (workload definition for the plant-recovery checks)
+# ---------------------------------------------------------------------------
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(state):
+    """one step of splitmix64 (S. Vigna); returns (value, new_state)"""
+    state = (state + 0x9E3779B97F4A7C15) & _M64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31), state
+
+
+def _roots(t, m):
+    from math import gcd
+    return [a for a in range(1, m) if gcd(a, m) == 1 and pow(a, 6, m) == t % m]
+
+
+def cls5_free_values(B, u):
+    """valid e (mod 729 classes, <= (B-1)/14) and d (mod 64 classes, <= (B-1)/21) of a unit"""
+    te = pow(B, 6, 729) * pow(pow(14, 6, 729), -1, 729) % 729
+    td = (pow(B, 6, 64) - pow(u, 6, 64)) * pow(pow(21, 6, 64), -1, 64) % 64
+    es = sorted(e for r in _roots(te, 729) for e in range(r, (B - 1) // 14 + 1, 729))
+    ds = sorted(d for r in _roots(td, 64) for d in range(r, (B - 1) // 21 + 1, 64))
+    return es, ds
+
+
+def plant_points(Bs, us, n, seed):
+    """n distinct planted grid points (unit index, e, d, key) drawn with splitmix64:
+    unit uniform over the band, e and d uniform over the unit's valid values,
+    redrawn while the residual R is not positive; key = (R / 42^6) mod 2^64."""
+    s = seed & _M64
+    out, seen, cache = [], set(), {}
+    while len(out) < n:
+        v, s = splitmix64(s)
+        i = v % len(Bs)
+        B, u = int(Bs[i]), int(us[i])
+        if i not in cache:
+            cache[i] = cls5_free_values(B, u)
+        es, ds = cache[i]
+        v, s = splitmix64(s)
+        e = es[v % len(es)]
+        v, s = splitmix64(s)
+        d = ds[v % len(ds)]
+        R = B ** 6 - u ** 6 - (21 * d) ** 6 - (14 * e) ** 6
+        if R <= 0 or (i, e, d) in seen:
+            continue
+        seen.add((i, e, d))
+        out.append((i, e, d, (R // 42 ** 6) & _M64))
+    return out

@@ -1,0 +1,93 @@
+"""Independent Python big-integer statements used to pin the oracle.
+
+These are written from the mathematics of the class-5 peel, not from the
+oracle's code: exact residuals with Python integers, the residue classes as
+brute-force sixth-root searches, the gate sets as brute-force pair sums.
+"""
+M42 = 42 ** 6
+
+
+def pair_sum_set(m):
+    p = [pow(a, 6, m) for a in range(m)]
+    return {(x + y) % m for x in p for y in p}
+
+
+S504 = pair_sum_set(504)
+S247 = pair_sum_set(247)
+
+
+def residues(t, m):
+    """units a in [1,m) with a^6 = t (mod m)"""
+    from math import gcd
+    return [a for a in range(1, m) if gcd(a, m) == 1 and pow(a, 6, m) == t % m]
+
+
+def cls5_grid(B, u, fmax1=None, fmax2=None):
+    te = pow(B, 6, 729) * pow(pow(14, 6, 729), -1, 729) % 729
+    td = (pow(B, 6, 64) - pow(u, 6, 64)) * pow(pow(21, 6, 64), -1, 64) % 64
+    f1 = (B - 1) // 14 if fmax1 is None else fmax1
+    f2 = (B - 1) // 21 if fmax2 is None else fmax2
+    es = sorted(e for r in residues(te, 729) for e in range(r, f1 + 1, 729) if e)
+    ds = sorted(d for r in residues(td, 64) for d in range(r, f2 + 1, 64) if d)
+    return es, ds
+
+
+def peel_counts(B, u, es, ds, contains=None, use_gate=True):
+    """counters (calls, neg, zero, gated, probes, maybe) and maybe-hit list"""
+    c = dict(calls=0, neg=0, zero=0, gated=0, probes=0, maybe=0)
+    hits = []
+    base = B ** 6 - u ** 6
+    d6 = [(21 * d) ** 6 for d in ds]
+    for e in es:
+        X = base - (14 * e) ** 6
+        for d, D in zip(ds, d6):
+            c["calls"] += 1
+            R = X - D
+            if R < 0:
+                c["neg"] += 1
+                continue
+            if R == 0:
+                c["zero"] += 1
+                continue
+            assert R % M42 == 0
+            T = R // M42
+            if use_gate and not (T % 504 in S504 and T % 247 in S247):
+                c["gated"] += 1
+                continue
+            c["probes"] += 1
+            fp = T % 2 ** 64
+            if contains is not None and contains(fp):
+                c["maybe"] += 1
+                hits.append((e, d, fp))
+    return c, hits
+
+
+MASK64 = 2 ** 64 - 1
+
+
+def mix64(x):
+    x &= MASK64
+    x ^= x >> 33
+    x = (x * 0xff51afd7ed558ccd) & MASK64
+    x ^= x >> 33
+    x = (x * 0xc4ceb9fe1a85ec53) & MASK64
+    x ^= x >> 33
+    return x
+
+
+def xor_query(header, packed, key):
+    """membership query over a StoreHeader-v2 packed table, from the format"""
+    r, m, seed, n = header["r"], header["m_cells"], header["mix_seed"], header["n_keys"]
+    if n == 0 or m < 3:
+        return False
+    block = m // 3
+    mask = (1 << r) - 1
+    h = mix64(key + seed)
+    cells = [h % block, mix64(h ^ 0x9E3779B97F4A7C15) % block + block,
+             mix64(h ^ 0xBF58476D1CE4E5B9) % block + 2 * block]
+    got = 0
+    for c in cells:
+        bit = c * r
+        word = int.from_bytes(packed[bit // 8: bit // 8 + 16], "little")
+        got ^= (word >> (bit % 8)) & mask
+    return got == (h & mask)

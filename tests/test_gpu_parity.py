@@ -1,0 +1,211 @@
+"""GPU parity: the HIP path (libfc5.so, both kernel forms) against the oracle.
+
+Bit-exact bar: per-gate counters (calls, neg, zero, gated, probes), the filter
+maybe-hit list (including false positives) and the final hit list must be
+identical to the oracle on the same inputs and the same table bytes; planted
+keys must all be recovered.  Tables are built by the product's host builder,
+whose bytes are checked against the oracle's (themselves pinned to the
+reference builder in test_oracle.py).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import fc5
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+COUNTERS = ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = fc5.Context(0)
+    yield c
+    c.close()
+
+
+def _units(lo, hi, every=1, offset=0):
+    _, Bs, us = fc5.emit_units(lo, hi)
+    return Bs[offset::every], us[offset::every]
+
+
+def _oracle_run(Bs, us, table, side=None, use_gate=True):
+    arr, ok = O.pack_units(Bs, us)
+    assert ok.all()
+    c, hits, nh = O.peel(arr, len(Bs), table, side=side, use_gate=use_gate, threads=min(16, os.cpu_count() or 8))
+    return c, hits
+
+
+def _load(ctx, N, r, extra=None):
+    """same table bytes in product and oracle"""
+    ctx.table_build(N, r, extra=extra)
+    t = O.Table.build_pairs(N, r, extra=extra)
+    h = ctx.table_info()
+    assert (h.n_keys, h.m_cells, h.mix_seed) == (t.x.hdr.n_keys, t.x.hdr.m_cells, t.x.hdr.mix_seed)
+    return t
+
+
+def _compare(out, oc, ohits):
+    for k in COUNTERS:
+        assert out.result[k] == oc[k], (k, out.result[k], oc[k])
+    got = np.sort(out.hits, order=["unit", "e", "d", "flags"])
+    assert len(got) == len(ohits)
+    for f in ("unit", "d", "e", "flags", "fp"):
+        assert np.array_equal(got[f], ohits[f]), f
+
+
+@pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN], ids=["batched", "plain"])
+@pytest.mark.parametrize("r", [48, 16])
+def test_cfg1_full_matches_oracle(ctx, mode, r):
+    say(f"cfg1 full r={r} mode={mode}")
+    Bs, us = _units(100001, 102000)
+    t = _load(ctx, 2428, r)
+    out = ctx.run(fc5.units_array(Bs, us), mode)
+    oc, oh = _oracle_run(Bs, us, t)
+    _compare(out, oc, oh)
+    assert out.result["calls"] == 10973276
+    if r == 16:
+        assert len(oh) > 0  # a non-empty false-positive list is exercised
+    assert out.solutions == []
+
+
+def test_cfg1_no_gate(ctx):
+    say("cfg1 no-gate")
+    Bs, us = _units(100001, 100600)
+    t = _load(ctx, 2428, 16)
+    ctx.set_option("use_gate", 0)
+    try:
+        out = ctx.run(fc5.units_array(Bs, us))
+    finally:
+        ctx.set_option("use_gate", 1)
+    oc, oh = _oracle_run(Bs, us, t, use_gate=False)
+    _compare(out, oc, oh)
+    assert out.result["gated"] == 0
+
+
+@pytest.mark.parametrize("lo,hi,every", [(1000001, 1020000, 97), (2000001, 2010000, 211),
+                                         (3200001, 3210000, 997)])
+def test_sampled_units_match_oracle(ctx, lo, hi, every):
+    """unit subsets of the 1.0M, 2.0M and 3.2M bands (B^6 > 2^128 in the last)"""
+    say(f"sampled units [{lo},{hi}] every {every}")
+    Bs, us = _units(lo, hi, every=every)
+    t = _load(ctx, 2428, 16)
+    b = ctx.band(fc5.units_array(Bs, us))
+    outb = b.run(fc5.MODE_BATCHED)
+    outp = b.run(fc5.MODE_PLAIN)
+    b.close()
+    oc, oh = _oracle_run(Bs, us, t)
+    _compare(outb, oc, oh)
+    _compare(outp, oc, oh)
+
+
+def test_plants_in_main_table_recovered(ctx):
+    say("plants in main table, cfg1")
+    Bs, us = _units(100001, 102000)
+    pts = fc5.plant_points(Bs, us, 64, 0x615 << 8 | 2)
+    keys = np.array([k for *_, k in pts], np.uint64)
+    t = _load(ctx, 2428, 48, extra=keys)
+    out = ctx.run(fc5.units_array(Bs, us))
+    oc, oh = _oracle_run(Bs, us, t)
+    _compare(out, oc, oh)
+    found = {(int(h["unit"]), int(h["e"]), int(h["d"])) for h in out.hits if h["flags"] & 1}
+    assert {(i, e, d) for i, e, d, _ in pts} <= found
+    assert out.solutions == []  # plants are not pair sums: the exact check rejects them
+
+
+def test_plants_in_side_table_recovered(ctx):
+    say("plants in side table, 3.2M sample")
+    Bs, us = _units(3200001, 3210000, every=499)
+    pts = fc5.plant_points(Bs, us, 32, 0x615 << 8 | 4)
+    keys = np.array([k for *_, k in pts], np.uint64)
+    _load(ctx, 2428, 48)
+    ctx.side_build(keys)
+    try:
+        out = ctx.run(fc5.units_array(Bs, us))
+    finally:
+        ctx.side_clear()
+    found = {(int(h["unit"]), int(h["e"]), int(h["d"])) for h in out.hits if h["flags"] & 2}
+    assert {(i, e, d) for i, e, d, _ in pts} <= found
+    assert out.result["side_maybe"] >= 32
+
+
+def test_device_query_matches_oracle(ctx):
+    say("device query")
+    t = _load(ctx, 2000, 16)
+    rnd = np.random.default_rng(5)
+    keys = rnd.integers(0, 2 ** 63, 50000, dtype=np.uint64) * 2 + 1
+    pw = np.array([(x ** 6) % 2 ** 64 for x in range(1, 2001)], np.uint64)
+    keys = np.concatenate([keys, pw[:500] + pw[1:501]])
+    got = ctx.table_query(keys)
+    want = np.array([t.contains(int(k)) for k in keys], np.uint8)
+    assert np.array_equal(got, want)
+    assert got[-500:].all()
+
+
+def test_hit_overflow_is_reported(ctx):
+    say("hit overflow")
+    Bs, us = _units(100001, 100300)
+    t = _load(ctx, 300, 8)
+    ctx.set_option("hit_cap", 16)
+    try:
+        out = ctx.run(fc5.units_array(Bs, us), allow_overflow=True)
+    finally:
+        ctx.set_option("hit_cap", 1 << 20)
+    oc, _ = _oracle_run(Bs, us, t)
+    assert out.rc == fc5.E_OVERFLOW
+    assert out.result["n_hits"] == oc["maybe"] > 16
+    assert out.result["hits_dropped"] == oc["maybe"] - 16
+    assert len(out.hits) == 16
+
+
+def test_table_load_errors(ctx, tmp_path):
+    say("table load errors")
+    p = str(tmp_path / "t.bin")
+    fc5.build_pairs_file(300, 48, p)
+    ctx.table_load(p, 300)
+    assert ctx.fpr_smoke() <= 16
+    with pytest.raises(fc5.Fc5Error) as e:
+        ctx.table_load(p, 301)
+    assert e.value.code == fc5.E_NMISMATCH
+    q = str(tmp_path / "r8.bin")
+    fc5.build_pairs_file(300, 8, q)
+    with pytest.raises(fc5.Fc5Error) as e:
+        ctx.table_load(q, 300)
+    assert e.value.code == fc5.E_FPR
+    with pytest.raises(fc5.Fc5Error) as e:
+        ctx.table_load(str(tmp_path / "missing.bin"), 300)
+    assert e.value.code == fc5.E_IO
+
+
+def test_cfg2_full_calls_and_64_plants(ctx):
+    """the full 1.0M band (4.99e4 units): closed-form calls, counters conserved,
+    every planted key recovered, plain and batched agree"""
+    say("cfg2 full band")
+    Bs, us = _units(1000001, 1020000)
+    units = fc5.units_array(Bs, us)
+    pts = fc5.plant_points(Bs, us, 64, 0x615 << 8 | 2)
+    keys = np.array([k for *_, k in pts], np.uint64)
+    ctx.table_build(2428, 48, extra=keys)
+    b = ctx.band(units)
+    out = b.run(fc5.MODE_BATCHED)
+    say("batched", out.result)
+    r = out.result
+    assert r["calls"] == fc5.units_calls(units)
+    assert r["calls"] == r["neg"] + r["zero"] + r["gated"] + r["probes"]
+    found = {(int(h["unit"]), int(h["e"]), int(h["d"])) for h in out.hits if h["flags"] & 1}
+    assert {(i, e, d) for i, e, d, _ in pts} <= found
+    outp = b.run(fc5.MODE_PLAIN)
+    say("plain", outp.result)
+    b.close()
+    for k in COUNTERS:
+        assert outp.result[k] == r[k], k
+    assert np.array_equal(np.sort(outp.hits, order=["unit", "e", "d"]), np.sort(out.hits, order=["unit", "e", "d"]))

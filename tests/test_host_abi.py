@@ -1,0 +1,111 @@
+"""The product's host side through its C ABI and CLIs (no device compute).
+
+Checks that libfc5.so loads and exports every entry point include/fc5.h
+declares, that the host logic behind the ABI (Stage-1 emission, closed-form
+calls, the byte-identical CPU table builder) matches the reference-derived
+golden vectors and the oracle, and that the CLI front ends keep the
+reference's flags, outputs and failure behaviour.
+"""
+import hashlib
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import bigint_ref as br
+import fc5
+import oracle as O
+from conftest import BIN, GOLDEN, REPO
+
+TABLES = {(r["N"], r["r"]): r for r in json.load(open(os.path.join(GOLDEN, "xor_tables.json")))}
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(REPO, "include", "fc5.h")).read()
+    declared = set(re.findall(r"\b(fc5_[a-z0-9_]+)\s*\(", hdr))
+    assert declared, "no declarations found"
+    import ctypes
+    lib = ctypes.CDLL(fc5.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared <= set(fc5.EXPORTS)
+    assert lib.fc5_abi_version() == 1
+
+
+def test_emit_units_matches_reference_golden():
+    cls, Bs, us = fc5.emit_units(100001, 102000, classes=(2, 3, 4, 5))
+    mine = "".join(f"{c} {b} {u}\n" for c, b, u in zip(cls, Bs, us))
+    assert mine == open(os.path.join(GOLDEN, "units_cfg1.txt")).read()
+
+
+def test_emit_units_matches_oracle_other_ranges():
+    for lo, hi in [(1000001, 1000400), (3200001, 3200200), (2353900, 2354100)]:
+        a = fc5.emit_units(lo, hi, classes=(2, 3, 4, 5))
+        b = O.emit_units(lo, hi, classes=(2, 3, 4, 5))
+        for x, y in zip(a, b):
+            assert np.array_equal(x.astype(np.int64), y)
+
+
+def test_units_calls_closed_form():
+    _, Bs, us = fc5.emit_units(100001, 102000)
+    assert fc5.units_calls(fc5.units_array(Bs, us)) == 10973276
+    _, Bs, us = fc5.emit_units(3200001, 3200100)
+    closed = sum(len(es) * len(ds) for es, ds in (br.cls5_grid(int(b), int(u)) for b, u in zip(Bs, us)))
+    assert fc5.units_calls(fc5.units_array(Bs, us)) == closed
+
+
+@pytest.mark.parametrize("N,r", [(300, 48), (2000, 16), (2428, 48)])
+def test_build_pairs_file_matches_reference(N, r, tmp_path):
+    p = str(tmp_path / "t.bin")
+    fc5.build_pairs_file(N, r, p)
+    b = open(p, "rb").read()
+    assert hashlib.sha256(b).hexdigest() == TABLES[(N, r)]["sha256"]
+
+
+def test_plant_points_are_exact():
+    _, Bs, us = fc5.emit_units(100001, 102000)
+    pts = fc5.plant_points(Bs, us, 32, 0x615 << 8 | 1)
+    assert len({(i, e, d) for i, e, d, _ in pts}) == 32
+    for i, e, d, key in pts:
+        assert O.T_peel2(int(Bs[i]), int(us[i]), d, e) % 2 ** 64 == key
+
+
+def _run(*args, check=True):
+    return subprocess.run([os.path.join(BIN, args[0]), *map(str, args[1:])], capture_output=True, text=True,
+                          check=check)
+
+
+def test_cli_xor_build_save_bytes(tmp_path):
+    p = str(tmp_path / "x.bin")
+    out = _run("xor_build_save", "--N", 300, "--r", 48, "--out", p)
+    assert "[xor_build_save] wrote" in out.stdout
+    assert hashlib.sha256(open(p, "rb").read()).hexdigest() == TABLES[(300, 48)]["sha256"]
+    assert _run("xor_build_save", "--N", 3, check=False).returncode == 1
+    assert _run("xor_build_save", "--N", 300, "--r", 7, check=False).returncode == 1
+
+
+def test_cli_hunt(tmp_path):
+    assert "SELFTEST PASS" in _run("fourcore_hunt_v3", "--selftest").stdout
+    p = str(tmp_path / "u.buc")
+    _run("fourcore_hunt_v3", "--lo", 100001, "--hi", 102000, "--classes", "2,3,4,5", "--emit-units", p)
+    assert open(p).read() == open(os.path.join(GOLDEN, "units_cfg1.txt")).read()
+    d = _run("fourcore_hunt_v3", "--density", "--lo", 100001, "--hi", 102000, "--classes", "5").stdout
+    assert "cls5  units=615  expanded_jobs~10973276" in d
+
+
+def test_cli_cls5_host_paths(tmp_path):
+    assert "[selftest-host] PASS" in _run("fourcore_cls5_gpu_v4", "--selftest-host").stdout
+    p = str(tmp_path / "x.bin")
+    fc5.build_pairs_file(300, 48, p)
+    ok = _run("fourcore_cls5_gpu_v4", "--check-table", "--N", 300, "--load-table", p)
+    assert "--check-table OK N=300" in ok.stderr
+    bad = _run("fourcore_cls5_gpu_v4", "--check-table", "--N", 301, "--load-table", p, check=False)
+    assert bad.returncode == 1 and "N mismatch" in bad.stderr and "FATAL" in bad.stderr
+    q = str(tmp_path / "r8.bin")
+    fc5.build_pairs_file(300, 8, q)
+    fpr = _run("fourcore_cls5_gpu_v4", "--check-table", "--N", 300, "--load-table", q, check=False)
+    assert fpr.returncode == 1 and "FPR smoke too high" in fpr.stderr
+    assert _run("fourcore_cls5_gpu_v4", "--units", "/nonexistent.buc", check=False).returncode == 1

@@ -1,0 +1,14 @@
+"""Native host checks of code shared with the device kernels (CPU only)."""
+import os
+import subprocess
+
+from conftest import PKG, REPO
+
+
+def test_moddiv_matches_operator(tmp_path):
+    exe = str(tmp_path / "moddiv_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(PKG, "csrc"),
+                    os.path.join(REPO, "tests", "native", "moddiv_check.cpp"), "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout
+    assert "moddiv ok" in out.stdout
