@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark of the class-5 free peel on one MI355X.
+
+A step is one pass of the batched persistent peel over one band of Stage-1
+class-5 units, with every input (units, pair table, planted side table)
+resident in HBM.  Default workload (--config 4): eligible B in
+[3200001, 3210000] (B^6 > 2^128), cls5 units of that band, the xor pair table
+for N = 3210000 // 42 = 76428 at r = 48 (built on the device before timing),
+256 planted keys in a side table, all of which must be recovered every step.
+
+Printed: one JSON line with the metric ((d,e) grid points per second), the
+eligible B cleared per hour, the plain one-launch-per-chunk form on a unit
+subset, the roofline of the dominant kernel (k_cls5_batched: algorithmic
+gather bytes per launch over its event-timed duration) and the CPU oracle
+timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "six-one-five_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+import fc5  # noqa: E402
+
+METRIC = "cls5 (d,e) probes/sec + eligible B cleared/hour at B≈3.2M, 1 GPU, bit-exact"
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    1: dict(lo=100001, hi=102000, N=2428, plants=0, plant_mode=None, seed=0x615 << 8 | 1),
+    2: dict(lo=1000001, hi=1020000, N=24285, plants=64, plant_mode="main", seed=0x615 << 8 | 2),
+    4: dict(lo=3200001, hi=3210000, N=76428, plants=256, plant_mode="side", seed=0x615 << 8 | 4),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def eligible_count(lo, hi):
+    return sum(1 for B in range(lo, hi + 1) if B % 2 and B % 3 and B % 7)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS))
+    ap.add_argument("--r", type=int, default=48)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    ap.add_argument("--plain-every", type=int, default=64, help="plain form on every k-th unit")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-every", type=int, default=400, help="CPU oracle on every k-th unit")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box of this build has one GPU and the band "
+              "shards by units with no exchange (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    cfg = CONFIGS[args.config]
+    t_setup = time.time()
+    _, Bs, us = fc5.emit_units(cfg["lo"], cfg["hi"])
+    units = fc5.units_array(Bs, us)
+    n_B = eligible_count(cfg["lo"], cfg["hi"])
+    calls_closed = fc5.units_calls(units)
+    log(f"[bench] config {args.config}: B=[{cfg['lo']},{cfg['hi']}] eligible_B={n_B} units={len(units)} "
+        f"calls={calls_closed:.4e}")
+
+    ctx = fc5.Context(0)
+    if args.blocks_per_cu:
+        ctx.set_option("blocks_per_cu", args.blocks_per_cu)
+    plants = fc5.plant_points(Bs, us, cfg["plants"], cfg["seed"]) if cfg["plants"] else []
+    pkeys = np.array([k for *_, k in plants], np.uint64)
+    t0 = time.time()
+    ctx.table_build(cfg["N"], args.r, extra=pkeys if cfg["plant_mode"] == "main" else None, on_gpu=True)
+    hdr = ctx.table_info()
+    t_build = time.time() - t0
+    log(f"[bench] device table N={hdr.N} keys={hdr.n_keys} cells={hdr.m_cells} "
+        f"packed={fc5_packed_gb(hdr):.2f} GB built in {t_build:.1f}s")
+    fpr = ctx.fpr_smoke()
+    if fpr > 16:
+        raise SystemExit(f"FPR smoke {fpr} > 16: refuse")
+    if cfg["plant_mode"] == "side":
+        ctx.side_build(pkeys, args.r)
+    band = ctx.band(units)
+    plant_set = {(i, e, d) for i, e, d, _ in plants}
+    flag = 2 if cfg["plant_mode"] == "side" else 1
+
+    def check(out):
+        r = out.result
+        assert r["calls"] == calls_closed, (r["calls"], calls_closed)
+        assert r["calls"] == r["neg"] + r["zero"] + r["gated"] + r["probes"]
+        assert r["hits_dropped"] == 0
+        found = {(int(h["unit"]), int(h["e"]), int(h["d"])) for h in out.hits if h["flags"] & flag}
+        missing = plant_set - found
+        assert not missing, f"{len(missing)} planted keys not recovered"
+        return len(plant_set & found)
+
+    for _ in range(args.warmup):
+        check(band.run(fc5.MODE_BATCHED))
+    log(f"[bench] setup {time.time() - t_setup:.1f}s; timing {args.steps} steps")
+    kms, outs = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = band.run(fc5.MODE_BATCHED)
+        outs.append(out)
+        kms.append(out.result["kernel_ms"])
+    wall = time.perf_counter() - t0
+    recovered = [check(o) for o in outs]
+    r = outs[-1].result
+    for o in outs[1:]:
+        for k in ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe"):
+            assert o.result[k] == outs[0].result[k], k
+    ms_step = 1000.0 * wall / args.steps
+    value = r["calls"] / (ms_step / 1000.0)
+    kern_s = float(np.mean(kms)) / 1000.0
+    cell_bytes = math.ceil(args.r / 8)
+    alg_bytes = r["probes"] * 3 * cell_bytes
+    log(f"[bench] {ms_step:.1f} ms/step, kernel {1000 * kern_s:.1f} ms, {value:.4e} points/s, "
+        f"probes={r['probes']} maybe={r['maybe']} side_maybe={r['side_maybe']}")
+
+    plain = None
+    if not args.no_plain:
+        sub = units[:: args.plain_every]
+        b2 = ctx.band(sub)
+        ob = b2.run(fc5.MODE_BATCHED)
+        t1 = time.perf_counter()
+        op = b2.run(fc5.MODE_PLAIN)
+        tp = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        ob2 = b2.run(fc5.MODE_BATCHED)
+        tb = time.perf_counter() - t1
+        b2.close()
+        for k in ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe"):
+            assert op.result[k] == ob.result[k] == ob2.result[k], k
+        pv = op.result["calls"] / tp
+        plain = {"subset": f"every {args.plain_every}th unit ({len(sub)} units)", "points_per_s": pv,
+                 "batched_points_per_s_same_subset": ob2.result["calls"] / tb,
+                 "eligible_B_per_hour": n_B * 3600.0 * pv / r["calls"],
+                 "speedup_batched_vs_plain": value / pv, "counters_identical": True}
+        log(f"[bench] plain form {pv:.4e} points/s on the subset; batched/plain = {value / pv:.1f}x")
+
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", f"traffic_cls5_batched_config{args.config}.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("traffic_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(ctx, units, args.cpu_every, cfg, r, pkeys)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "(d,e) grid points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (deterministic Stage-1 enumeration of the band; seeded planted keys)",
+        "config": {"workload": f"config {args.config}: batched persistent cls5 peel, eligible B in "
+                               f"[{cfg['lo']},{cfg['hi']}] ({n_B} B, {len(units)} units, "
+                               f"{r['calls']} grid points), xor pair table N={hdr.N} r={hdr.r} "
+                               f"({fc5_packed_gb(hdr):.2f} GB packed, device-built), "
+                               f"{len(plants)} planted keys ({cfg['plant_mode'] or 'none'} table)",
+                   "B_lo": cfg["lo"], "B_hi": cfg["hi"], "N": int(hdr.N), "r": int(hdr.r),
+                   "units": int(len(units)), "parallelism": "single GPU"},
+        "eligible_B_per_hour": n_B * 3600.0 / (ms_step / 1000.0),
+        "filter_probes_per_s": r["probes"] / (ms_step / 1000.0),
+        "cell_gathers_per_s": 3 * r["probes"] / (ms_step / 1000.0),
+        "counters": {k: r[k] for k in ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe",
+                                       "n_hits", "solutions")},
+        "plants_recovered": f"{min(recovered)}/{len(plants)}",
+        "kernel_ms": float(np.mean(kms)),
+        "roofline": {"bound": "hbm", "achieved": alg_bytes / kern_s / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": alg_bytes / kern_s / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_cls5_batched",
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "per_unit": f"{3 * cell_bytes} B per filter probe (3 cells x {cell_bytes} B)",
+                     "transaction_GBps_at_64B": r["probes"] * 3 * 64 / kern_s / 1e9},
+        "plain_form": plain,
+        "cpu_baseline": cpu,
+        "table_build_s": t_build,
+    }
+    print(json.dumps(line), flush=True)
+    band.close()
+    ctx.close()
+
+
+def fc5_packed_gb(h):
+    return (h.m_cells * h.r + 7) // 8 / 1e9
+
+
+def cpu_baseline(ctx, units, every, cfg, r, pkeys):
+    """The oracle (CPU restatement of the reference kernel) timed on every
+    k-th unit of the same band, over the same table bytes, on this host."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+
+    hdr, packed = ctx.table_download()
+    t = O.Table.from_bytes({f: getattr(hdr, f) for f, _ in fc5.StoreHeader._fields_}, packed)
+    side = O.Table.build_keys(pkeys, hdr.r) if cfg["plant_mode"] == "side" and len(pkeys) else None
+    sub = units[::every]
+    arr, ok = O.pack_units(sub["B"], sub["u"])
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    c, _, _ = O.peel(arr, len(sub), t, side=side, threads=threads)
+    dt = time.perf_counter() - t0
+    # the same subset on the GPU, for a like-for-like counter check
+    out = ctx.run(sub)
+    for k in ("calls", "neg", "zero", "gated", "probes", "maybe"):
+        assert out.result[k] == c[k], (k, out.result[k], c[k])
+    log(f"[bench] cpu oracle {c['calls'] / dt:.4e} points/s on {len(sub)} units, {threads} threads, {dt:.1f}s")
+    return {"value": c["calls"] / dt, "unit": "(d,e) grid points/s", "cores": threads, "kind": "port",
+            "sample": f"every {every}th unit of the band ({len(sub)} units, {c['calls']} grid points, "
+                      f"{dt:.1f} s), same table bytes; counters identical to the GPU on that sample"}
+
+
+if __name__ == "__main__":
+    main()

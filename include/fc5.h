@@ -98,8 +98,8 @@ int fc5_ctx_create(int device, fc5_ctx **out);
 void fc5_ctx_destroy(fc5_ctx *ctx);
 const char *fc5_last_error(const fc5_ctx *ctx);
 /* options: "hit_cap" (default 1<<20), "use_gate" (1; 0 = --no-gate,
- * fourcore_cls5_gpu_v4.cu:634), "rows_per_item" (8), "waves_per_cu" (0 = auto),
- * "verify" (1: pair recovery + exact check of hits on the host),
+ * fourcore_cls5_gpu_v4.cu:634), "blocks_per_cu" (0 = occupancy limit) for the
+ * persistent grid, "verify" (1: pair recovery + exact check of hits on the host),
  * "stop_first" (0, fourcore_cls5_gpu_v4.cu:635) */
 int fc5_set_option(fc5_ctx *ctx, const char *key, int64_t value);
 
@@ -120,6 +120,10 @@ int fc5_table_info(const fc5_ctx *ctx, fc5_store_header *out);
 /* write the current table as StoreHeader v2 + packed bytes (xor_save_file,
  * fourcore_xor_store.hpp:126-143) */
 int fc5_table_save(fc5_ctx *ctx, const char *path);
+/* copy the table out in the file layout (header + r-bit packed bytes, as
+ * written by xor_save_file); *len gets the packed size; cap too small ->
+ * FC5_E_ARG with *len set */
+int fc5_table_download(fc5_ctx *ctx, fc5_store_header *hdr, uint8_t *packed, size_t cap, size_t *len);
 /* query the device table for n keys (xor_might_contain, xor_filter.hpp:472-483) */
 int fc5_table_query(fc5_ctx *ctx, const uint64_t *keys, size_t n, uint8_t *out);
 /* FPR smoke on the device table (xor_fpr_smoke, fourcore_xor_store.hpp:54-65) */

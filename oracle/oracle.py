@@ -157,6 +157,19 @@ class Table:
         return t
 
     @classmethod
+    def from_bytes(cls, header_fields, packed):
+        """wrap a table held in memory (header field dict, uint8 array); the
+        array is kept alive by the Table and never freed by the oracle"""
+        t = cls()
+        for f, _ in Header._fields_:
+            setattr(t.x.hdr, f, int(header_fields[f]))
+        t._keep = packed
+        t.x.packed = packed.ctypes.data_as(C.POINTER(C.c_uint8))
+        t.x.packed_len = len(packed)
+        t._borrowed = True
+        return t
+
+    @classmethod
     def load(cls, path, expected_N=0):
         t = cls()
         rc = _lib.or_xor_load(path.encode(), C.byref(t.x), expected_N)
@@ -185,6 +198,8 @@ class Table:
         return _lib.or_fpr_smoke(C.byref(self.x), trials)
 
     def __del__(self):
+        if getattr(self, "_borrowed", False):
+            return
         try:
             _lib.or_xor_free(C.byref(self.x))
         except Exception:

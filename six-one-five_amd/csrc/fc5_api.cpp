@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "fc5_build.hpp"
 #include "fc5_host.hpp"
 #include "fc5_kernels.hpp"
 
@@ -280,7 +281,28 @@ extern "C" int fc5_table_upload(fc5_ctx* ctx, const fc5_store_header* hdr, const
 extern "C" int fc5_table_build(fc5_ctx* ctx, int N, int r, const uint64_t* extra, size_t n_extra,
                                int on_gpu) {
   if (!ctx || N < 1 || r < 1 || r > 64 || (!extra && n_extra)) return FC5_E_ARG;
-  if (on_gpu) FAIL(ctx, FC5_E_ARG, "device table build is not available in this build");
+  if (on_gpu) {
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    free_table(ctx->main);
+    ctx->host_main = XorTable{};
+    ctx->host_main_valid = false;
+    GpuBuilt g;
+    std::string err;
+    if (gpu_build_pairs(N, (u32)r, extra, n_extra, ctx->stream, g, err) != 0)
+      FAIL(ctx, err.find("stalled") != std::string::npos ? FC5_E_BUILD : FC5_E_HIP, err);
+    DeviceTable& t = ctx->main;
+    std::memcpy(&t.hdr, &g.hdr, sizeof t.hdr);
+    t.cells = g.cells;
+    t.dev.cells = g.cells;
+    t.dev.block = g.hdr.m_cells / 3;
+    t.dev.mask = fp_mask((u32)r);
+    t.dev.seed = g.hdr.mix_seed;
+    t.dev.md = make_moddiv(t.dev.block);
+    t.dev.present = t.dev.block ? 1u : 0u;
+    t.present = true;
+    ctx->pr_N = -1;
+    return FC5_OK;
+  }
   std::vector<u64> ex(extra, extra + n_extra);
   XorTable t;
   std::string msg;
@@ -295,11 +317,57 @@ extern "C" int fc5_table_info(const fc5_ctx* ctx, fc5_store_header* out) {
   return FC5_OK;
 }
 
+// pack the table to the file layout (host copy, or the device cells re-packed)
+static int table_packed(fc5_ctx* ctx, XorTable& t, bool want_bytes) {
+  if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no table");
+  t.hdr = ctx->main.hdr;
+  if (!want_bytes) return FC5_OK;
+  if (ctx->host_main_valid) {
+    t.packed = ctx->host_main.packed;
+    return FC5_OK;
+  }
+  const u64 m = t.hdr.m_cells;
+  const size_t want = packed_bytes(m, t.hdr.r);
+  t.packed.assign(want, 0);
+  if (!ctx->main.cells || !want) return FC5_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t room = std::max<size_t>(want, (size_t)((m + 7) / 8) * t.hdr.r);
+  u8* d = nullptr;
+  if (hipMalloc(&d, room) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "pack buffer");
+  int rc = FC5_OK;
+  if (hipMemsetAsync(d, 0, room, ctx->stream) != hipSuccess ||
+      gpu_pack_cells(ctx->main.cells, m, t.hdr.r, d, ctx->stream) != 0 ||
+      hipMemcpyAsync(t.packed.data(), d, want, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    ctx->err = "table pack failed";
+    rc = FC5_E_HIP;
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
 extern "C" int fc5_table_save(fc5_ctx* ctx, const char* path) {
   if (!ctx || !path) return FC5_E_ARG;
-  if (!ctx->host_main_valid) FAIL(ctx, FC5_E_STATE, "no host copy of the table to save");
+  XorTable t;
+  const int rc = table_packed(ctx, t, true);
+  if (rc != FC5_OK) return rc;
   std::string msg;
-  if (!save_bin(path, ctx->host_main, msg)) FAIL(ctx, FC5_E_IO, msg);
+  if (!save_bin(path, t, msg)) FAIL(ctx, FC5_E_IO, msg);
+  return FC5_OK;
+}
+
+extern "C" int fc5_table_download(fc5_ctx* ctx, fc5_store_header* hdr, uint8_t* packed, size_t cap, size_t* len) {
+  if (!ctx || !len) return FC5_E_ARG;
+  if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no table");
+  const size_t want = packed_bytes(ctx->main.hdr.m_cells, ctx->main.hdr.r);
+  *len = want;
+  if (hdr) std::memcpy(hdr, &ctx->main.hdr, sizeof *hdr);
+  if (!packed) return FC5_OK;
+  if (cap < want) FAIL(ctx, FC5_E_ARG, "buffer too small");
+  XorTable t;
+  const int rc = table_packed(ctx, t, true);
+  if (rc != FC5_OK) return rc;
+  std::memcpy(packed, t.packed.data(), want);
   return FC5_OK;
 }
 

@@ -71,6 +71,7 @@ _sig = {
     "fc5_table_build": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_size_t, C.c_int]),
     "fc5_table_info": (C.c_int, [_P, C.POINTER(StoreHeader)]),
     "fc5_table_save": (C.c_int, [_P, C.c_char_p]),
+    "fc5_table_download": (C.c_int, [_P, C.POINTER(StoreHeader), _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_table_query": (C.c_int, [_P, _P, C.c_size_t, _P]),
     "fc5_table_fpr_smoke": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
     "fc5_side_build": (C.c_int, [_P, _P, C.c_size_t, C.c_int]),
@@ -221,6 +222,15 @@ class Context:
     def table_save(self, path):
         self._check(_lib.fc5_table_save(self.h, path.encode()))
 
+    def table_download(self):
+        """(header, packed bytes as a numpy uint8 array) in the file layout"""
+        h = StoreHeader()
+        n = C.c_size_t()
+        self._check(_lib.fc5_table_download(self.h, C.byref(h), None, 0, C.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        self._check(_lib.fc5_table_download(self.h, C.byref(h), _ptr(buf), n.value, C.byref(n)))
+        return h, buf
+
     def table_query(self, keys):
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         out = np.zeros(len(keys), np.uint8)
@@ -313,26 +323,43 @@ def cls5_free_values(B, u):
     return es, ds
 
 
+def _pair_sum_set(m):
+    p = [pow(a, 6, m) for a in range(m)]
+    return frozenset((x + y) % m for x in p for y in p)
+
+
+_S504, _S247 = _pair_sum_set(504), _pair_sum_set(247)
+
+
 def plant_points(Bs, us, n, seed):
     """n distinct planted grid points (unit index, e, d, key) drawn with splitmix64:
-    unit uniform over the band, e and d uniform over the unit's valid values,
-    redrawn while the residual R is not positive; key = (R / 42^6) mod 2^64."""
+    the unit uniformly over the band, then (e, d) uniformly over the unit's
+    valid values, redrawn while the residual R is not positive or T = R/42^6
+    fails the sixth-power residue gate (a real solution always passes it: T is
+    a pair sum), at most 2000 draws per unit; key = T mod 2^64."""
     s = seed & _M64
-    out, seen, cache = [], set(), {}
+    out, seen = [], set()
     while len(out) < n:
         v, s = splitmix64(s)
         i = v % len(Bs)
         B, u = int(Bs[i]), int(us[i])
-        if i not in cache:
-            cache[i] = cls5_free_values(B, u)
-        es, ds = cache[i]
-        v, s = splitmix64(s)
-        e = es[v % len(es)]
-        v, s = splitmix64(s)
-        d = ds[v % len(ds)]
-        R = B ** 6 - u ** 6 - (21 * d) ** 6 - (14 * e) ** 6
-        if R <= 0 or (i, e, d) in seen:
+        es, ds = cls5_free_values(B, u)
+        base = B ** 6 - u ** 6
+        for _ in range(2000):  # units whose positive region is tiny are passed over
+            v, s = splitmix64(s)
+            e = es[v % len(es)]
+            v, s = splitmix64(s)
+            d = ds[v % len(ds)]
+            R = base - (21 * d) ** 6 - (14 * e) ** 6
+            if R <= 0:
+                continue
+            T = R // 42 ** 6
+            if T % 504 in _S504 and T % 247 in _S247:
+                break
+        else:
+            continue
+        if (i, e, d) in seen:
             continue
         seen.add((i, e, d))
-        out.append((i, e, d, (R // 42 ** 6) & _M64))
+        out.append((i, e, d, T & _M64))
     return out

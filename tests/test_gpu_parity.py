@@ -209,3 +209,50 @@ def test_cfg2_full_calls_and_64_plants(ctx):
     for k in COUNTERS:
         assert outp.result[k] == r[k], k
     assert np.array_equal(np.sort(outp.hits, order=["unit", "e", "d"]), np.sort(out.hits, order=["unit", "e", "d"]))
+
+
+def _pair_keys(N):
+    x = np.arange(1, N + 1, dtype=np.uint64)
+    p = x * x * x
+    p = p * p
+    i, j = np.triu_indices(N)
+    return p[i] + p[j]
+
+
+@pytest.mark.parametrize("N,r", [(5000, 48), (3000, 16)])
+def test_device_builder_valid_filter(ctx, N, r, tmp_path):
+    """device-built table: same unique key count and geometry as the host
+    builder (5000 > 4096 includes the systematic duplicates a^6 = 0 mod 2^64),
+    no false negatives, FPR near 2^-r, and a save/reload round trip"""
+    say(f"device builder N={N} r={r}")
+    ctx.table_build(N, r, on_gpu=True)
+    h = ctx.table_info()
+    t = O.Table.build_pairs(N, r)
+    assert (h.n_keys, h.m_cells, h.N, h.r) == (t.x.hdr.n_keys, t.x.hdr.m_cells, N, r)
+    keys = _pair_keys(N)
+    assert len(np.unique(keys)) == h.n_keys
+    assert ctx.table_query(keys).all()
+    fpr = ctx.fpr_smoke()
+    assert fpr <= (16 if r == 48 else 40)
+    p = str(tmp_path / "dev.bin")
+    ctx.table_save(p)
+    back = O.Table.load(p, N)
+    assert back.header()["n_keys"] == h.n_keys
+    sample = keys[:: max(1, len(keys) // 20000)]
+    assert all(back.contains(int(k)) for k in sample)
+    hh, packed = ctx.table_download()
+    assert bytes(packed) == back.packed()
+
+
+def test_device_built_table_search_matches_oracle(ctx):
+    """same bytes in both: the search over a device-built table, downloaded
+    into the oracle, agrees bit-exactly (FP list included, r=16)"""
+    say("search over device-built table")
+    Bs, us = _units(100001, 102000)
+    ctx.table_build(2428, 16, on_gpu=True)
+    h, packed = ctx.table_download()
+    t = O.Table.from_bytes({f: getattr(h, f) for f, _ in fc5.StoreHeader._fields_}, packed)
+    out = ctx.run(fc5.units_array(Bs, us))
+    oc, oh = _oracle_run(Bs, us, t)
+    _compare(out, oc, oh)
+    assert len(oh) > 0
