@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-every", type=int, default=400, help="CPU oracle on every k-th unit")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--no-measure", action="store_true", help="skip the gather/copy microbenchmarks")
+    ap.add_argument("--load-policy", type=int, default=None, choices=[0, 1, 2, 3],
+                    help="cache policy of the cell gathers (library default 1, nontemporal)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box of this build has one GPU and the band "
@@ -75,6 +78,8 @@ def main():
     ctx = fc5.Context(0)
     if args.blocks_per_cu:
         ctx.set_option("blocks_per_cu", args.blocks_per_cu)
+    if args.load_policy is not None:
+        ctx.set_option("load_policy", args.load_policy)
     plants = fc5.plant_points(Bs, us, cfg["plants"], cfg["seed"]) if cfg["plants"] else []
     pkeys = np.array([k for *_, k in plants], np.uint64)
     t0 = time.time()
@@ -146,6 +151,21 @@ def main():
                  "speedup_batched_vs_plain": value / pv, "counters_identical": True}
         log(f"[bench] plain form {pv:.4e} points/s on the subset; batched/plain = {value / pv:.1f}x")
 
+    ceiling = None
+    if not args.no_measure:
+        n = 1 << 31
+        pms, rms = ctx.measure_gathers(n)
+        cms = ctx.measure_copy(4 << 30)
+        probe_rate = n / (pms / 1000.0)
+        kernel_probe_rate = r["probes"] / kern_s
+        ceiling = {"probe_only_probes_per_s": probe_rate, "raw_random_8B_gathers_per_s": 3 * n / (rms / 1000.0),
+                   "probe_only_transaction_GBps_at_64B": probe_rate * 3 * 64 / 1e9,
+                   "copy_GBps": 2 * (4 << 30) / (cms / 1000.0) / 1e9,
+                   "kernel_probe_rate_frac_of_probe_only": kernel_probe_rate / probe_rate}
+        log(f"[bench] gather ceiling: {probe_rate:.4e} probes/s probe-only, "
+            f"{3 * n / (rms / 1000.0):.4e} raw gathers/s; kernel at {kernel_probe_rate / probe_rate:.1%}; "
+            f"copy {ceiling['copy_GBps']:.0f} GB/s")
+
     traffic = None
     tpath = os.path.join(REPO, "profiles", f"traffic_cls5_batched_config{args.config}.json")
     if os.path.exists(tpath):
@@ -180,6 +200,7 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "per_unit": f"{3 * cell_bytes} B per filter probe (3 cells x {cell_bytes} B)",
                      "transaction_GBps_at_64B": r["probes"] * 3 * 64 / kern_s / 1e9},
+        "gather_ceiling": ceiling,
         "plain_form": plain,
         "cpu_baseline": cpu,
         "table_build_s": t_build,

@@ -99,7 +99,9 @@ void fc5_ctx_destroy(fc5_ctx *ctx);
 const char *fc5_last_error(const fc5_ctx *ctx);
 /* options: "hit_cap" (default 1<<20), "use_gate" (1; 0 = --no-gate,
  * fourcore_cls5_gpu_v4.cu:634), "blocks_per_cu" (0 = occupancy limit) for the
- * persistent grid, "verify" (1: pair recovery + exact check of hits on the host),
+ * persistent grid, "load_policy" (0..3, default 1: cache policy of the xor cell gathers:
+ * default, nontemporal, device-scope, system-scope), "verify" (1: pair recovery
+ * + exact check of hits on the host),
  * "stop_first" (0, fourcore_cls5_gpu_v4.cu:635) */
 int fc5_set_option(fc5_ctx *ctx, const char *key, int64_t value);
 
@@ -150,6 +152,14 @@ int fc5_last_hits(fc5_ctx *ctx, fc5_hit *out, size_t cap, size_t *n);
 int fc5_last_solutions(fc5_ctx *ctx, fc5_solution *out, size_t cap, size_t *n);
 /* closed-form calls of a unit list (the counter `calls`, fourcore_cls5_gpu_v4.cu:383-386) */
 int fc5_units_calls(const fc5_unit *units, size_t n, uint64_t *calls);
+
+/* ---- measurement (no reference counterpart: the roofline microbenchmarks) ---- */
+/* n pure xor-filter queries of pseudo-random keys against the main table at
+ * full occupancy (probe_ms), and 3n raw random 8-byte loads of its cells
+ * (raw_ms), both under the context's "load_policy" */
+int fc5_measure_gathers(fc5_ctx *ctx, uint64_t n, double *probe_ms, double *raw_ms);
+/* a streaming device copy of `bytes` bytes (reads and writes that many) */
+int fc5_measure_copy(fc5_ctx *ctx, uint64_t bytes, double *ms);
 
 /* ---- host helpers shared by the CLIs (no device use) ---- */
 /* Stage-1 emission over [lo,hi] for classes in cls_mask (bit c = class c),

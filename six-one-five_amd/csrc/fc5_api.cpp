@@ -64,6 +64,7 @@ struct fc5_ctx {
   // options
   u64 hit_cap = 1u << 20;
   int use_gate = 1, verify = 1, stop_first = 0, blocks_per_cu = 0;
+  u32 load_policy = 1;  // cache policy of the xor cell gathers (nontemporal: no MALL allocation)
   // tables
   DeviceTable main, side;
   XorTable host_main;  // host copy when small enough (save, host verify)
@@ -186,6 +187,11 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) {
     ctx->verify = v ? 1 : 0;
   } else if (k == "stop_first") {
     ctx->stop_first = v ? 1 : 0;
+  } else if (k == "load_policy") {
+    if (v < 0 || v > 3) FAIL(ctx, FC5_E_ARG, "load_policy must be 0..3");
+    ctx->load_policy = (u32)v;
+    ctx->main.dev.policy = ctx->load_policy;
+    ctx->side.dev.policy = ctx->load_policy;
   } else if (k == "blocks_per_cu") {
     if (v < 0 || v > 64) FAIL(ctx, FC5_E_ARG, "blocks_per_cu out of range");
     ctx->blocks_per_cu = (int)v;
@@ -233,6 +239,7 @@ static int upload_table(fc5_ctx* ctx, const StoreHeader& h, const u8* packed, si
   dst.dev.seed = h.mix_seed;
   dst.dev.md = make_moddiv(dst.dev.block);
   dst.dev.present = dst.dev.block ? 1u : 0u;
+  dst.dev.policy = ctx->load_policy;
   dst.present = true;
   return FC5_OK;
 }
@@ -299,6 +306,7 @@ extern "C" int fc5_table_build(fc5_ctx* ctx, int N, int r, const uint64_t* extra
     t.dev.seed = g.hdr.mix_seed;
     t.dev.md = make_moddiv(t.dev.block);
     t.dev.present = t.dev.block ? 1u : 0u;
+    t.dev.policy = ctx->load_policy;
     t.present = true;
     ctx->pr_N = -1;
     return FC5_OK;
@@ -789,4 +797,63 @@ extern "C" int fc5_build_pairs_file(int N, int r, const char* path, fc5_store_he
   if (!save_bin(path, t, msg)) return FC5_E_IO;
   if (hdr_out) std::memcpy(hdr_out, &t.hdr, sizeof *hdr_out);
   return FC5_OK;
+}
+
+// ---------------------------------------------------------------------------
+// measurement
+// ---------------------------------------------------------------------------
+extern "C" int fc5_measure_gathers(fc5_ctx* ctx, uint64_t n, double* probe_ms, double* raw_ms) {
+  if (!ctx || !probe_ms || !raw_ms || n == 0) return FC5_E_ARG;
+  const u32 policy = ctx->main.dev.policy;
+  if (!ctx->main.present || !ctx->main.cells) FAIL(ctx, FC5_E_STATE, "no device table");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  unsigned long long* sink = nullptr;
+  if (hipMalloc(&sink, 8) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "sink");
+  const int grid = ctx->num_cu * 8;
+  float a = 0, b = 0;
+  int rc = FC5_OK;
+  if (hipMemsetAsync(sink, 0, 8, ctx->stream) != hipSuccess ||
+      launch_probe_only(ctx->main.dev, n / 16, 1, sink, grid, ctx->stream) != 0 ||  // warm-up
+      hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess ||
+      launch_probe_only(ctx->main.dev, n, 7, sink, grid, ctx->stream) != 0 ||
+      hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess ||
+      hipEventElapsedTime(&a, ctx->ev0, ctx->ev1) != hipSuccess ||
+      hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess ||
+      launch_gather_raw(ctx->main.cells, ctx->main.hdr.m_cells, 3 * n, 11, policy, sink, grid, ctx->stream) != 0 ||
+      hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess ||
+      hipEventElapsedTime(&b, ctx->ev0, ctx->ev1) != hipSuccess) {
+    ctx->err = "gather measurement failed";
+    rc = FC5_E_HIP;
+  }
+  (void)hipFree(sink);
+  *probe_ms = a;
+  *raw_ms = b;
+  return rc;
+}
+
+extern "C" int fc5_measure_copy(fc5_ctx* ctx, uint64_t bytes, double* ms) {
+  if (!ctx || !ms || bytes < 8) return FC5_E_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const u64 n = bytes / 8;
+  u64 *src = nullptr, *dst = nullptr;
+  if (hipMalloc(&src, n * 8) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "copy src");
+  if (hipMalloc(&dst, n * 8) != hipSuccess) {
+    (void)hipFree(src);
+    FAIL(ctx, FC5_E_NOMEM, "copy dst");
+  }
+  float t = 0;
+  int rc = FC5_OK;
+  if (launch_fill_u64(src, n, 0x0123456789abcdefULL, ctx->stream) != 0 ||
+      launch_copy_u64(src, dst, n, ctx->num_cu * 8, ctx->stream) != 0 ||  // warm-up
+      hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess ||
+      launch_copy_u64(src, dst, n, ctx->num_cu * 8, ctx->stream) != 0 ||
+      hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess ||
+      hipEventElapsedTime(&t, ctx->ev0, ctx->ev1) != hipSuccess) {
+    ctx->err = "copy measurement failed";
+    rc = FC5_E_HIP;
+  }
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  *ms = t;
+  return rc;
 }

@@ -41,7 +41,8 @@ __device__ __forceinline__ bool d_query(const XorDev& x, u64 key) {
   const u64 i0 = mod_fast(h, x.md);
   const u64 i1 = mod_fast(h1, x.md) + x.block;
   const u64 i2 = mod_fast(h2, x.md) + 2 * x.block;
-  const u64 got = x.cells[i0] ^ x.cells[i1] ^ x.cells[i2];
+  const u64 got = load_cell_policy(x.cells + i0, x.policy) ^ load_cell_policy(x.cells + i1, x.policy) ^
+                  load_cell_policy(x.cells + i2, x.policy);
   return got == (h & x.mask);
 }
 

@@ -33,8 +33,19 @@ struct XorDev {
   u64 seed;          // mix seed
   ModDiv md;
   u32 present;       // 0: queries answer "no" (empty table), as block == 0
-  u32 pad;
+  u32 policy;        // cell load cache policy: 0 default, 1 nontemporal,
+                     // 2 device-scope coherent (sc1), 3 system-scope (sc0 sc1)
 };
+
+// one cell load under the table's cache policy (uniform branch)
+__device__ __forceinline__ u64 load_cell_policy(const u64* p, u32 policy) {
+  switch (policy) {
+    case 1: return __builtin_nontemporal_load(p);
+    case 2: return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    case 3: return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    default: return *p;
+  }
+}
 
 struct DevUnit5 {
   u64 Q_lo, Q_hi;       // T = Q - floor(e^6/729) - floor(d^6/64)
@@ -105,5 +116,9 @@ int launch_expand_cells(const u32* packed_words, u64 m_cells, u32 r, u64* cells,
 int launch_query(const XorDev& x, const u64* keys, u64 n, u8* out, hipStream_t s);
 int launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s);
 int batched_blocks_per_cu();
+int launch_probe_only(const XorDev& x, u64 n, u64 salt, unsigned long long* sink, int grid, hipStream_t s);
+int launch_gather_raw(const u64* cells, u64 m, u64 n, u64 salt, u32 policy, unsigned long long* sink, int grid,
+                      hipStream_t s);
+int launch_copy_u64(const u64* src, u64* dst, u64 n, int grid, hipStream_t s);
 
 }  // namespace fc5

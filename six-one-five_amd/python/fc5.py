@@ -83,6 +83,8 @@ _sig = {
     "fc5_last_hits": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_last_solutions": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_units_calls": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_uint64)]),
+    "fc5_measure_gathers": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "fc5_measure_copy": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double)]),
     "fc5_emit_units": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double,
                                  _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_build_pairs_file": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.POINTER(StoreHeader)]),
@@ -248,6 +250,17 @@ class Context:
 
     def side_clear(self):
         self._check(_lib.fc5_side_clear(self.h))
+
+    def measure_gathers(self, n):
+        """(ms for n full queries, ms for 3n raw random cell loads) on the main table"""
+        a, b = C.c_double(), C.c_double()
+        self._check(_lib.fc5_measure_gathers(self.h, n, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def measure_copy(self, nbytes):
+        t = C.c_double()
+        self._check(_lib.fc5_measure_copy(self.h, nbytes, C.byref(t)))
+        return t.value
 
     def band(self, units):
         return Band(self, units)
