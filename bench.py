@@ -32,10 +32,13 @@ METRIC = "cls5 (d,e) probes/sec + eligible B cleared/hour at B≈3.2M, 1 GPU, bi
 HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
-    1: dict(lo=100001, hi=102000, N=2428, plants=0, plant_mode=None, seed=0x615 << 8 | 1),
-    2: dict(lo=1000001, hi=1020000, N=24285, plants=64, plant_mode="main", seed=0x615 << 8 | 2),
-    4: dict(lo=3200001, hi=3210000, N=76428, plants=256, plant_mode="side", seed=0x615 << 8 | 4),
+    1: dict(lo=100001, hi=102000, N=2428, plants=0, plant_mode=None, seed=0x615 << 8 | 1, classes=(5,)),
+    2: dict(lo=1000001, hi=1020000, N=24285, plants=64, plant_mode="main", seed=0x615 << 8 | 2, classes=(5,)),
+    3: dict(lo=2000001, hi=2010000, N=47857, plants=0, plant_mode=None, seed=0x615 << 8 | 3,
+            classes=(2, 3, 4, 5)),
+    4: dict(lo=3200001, hi=3210000, N=76428, plants=256, plant_mode="side", seed=0x615 << 8 | 4, classes=(5,)),
 }
+COUNTER_KEYS = ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe", "jobs", "fail_t")
 
 
 def log(*a):
@@ -68,10 +71,11 @@ def main():
         sys.exit(2)
     cfg = CONFIGS[args.config]
     t_setup = time.time()
-    _, Bs, us = fc5.emit_units(cfg["lo"], cfg["hi"])
+    mixed = cfg["classes"] != (5,)
+    ucls, Bs, us = fc5.emit_units(cfg["lo"], cfg["hi"], classes=cfg["classes"])
     units = fc5.units_array(Bs, us)
     n_B = eligible_count(cfg["lo"], cfg["hi"])
-    calls_closed = fc5.units_calls(units)
+    calls_closed = fc5.units_calls(units[ucls == 5]) if mixed else fc5.units_calls(units)
     log(f"[bench] config {args.config}: B=[{cfg['lo']},{cfg['hi']}] eligible_B={n_B} units={len(units)} "
         f"calls={calls_closed:.4e}")
 
@@ -93,13 +97,14 @@ def main():
         raise SystemExit(f"FPR smoke {fpr} > 16: refuse")
     if cfg["plant_mode"] == "side":
         ctx.side_build(pkeys, args.r)
-    band = ctx.band(units)
+    band = ctx.band(units, ucls if mixed else None)
     plant_set = {(i, e, d) for i, e, d, _ in plants}
     flag = 2 if cfg["plant_mode"] == "side" else 1
 
     def check(out):
         r = out.result
-        assert r["calls"] == calls_closed, (r["calls"], calls_closed)
+        c5 = out.by_class[5]["calls"]
+        assert c5 == calls_closed, (c5, calls_closed)
         assert r["calls"] == r["neg"] + r["zero"] + r["gated"] + r["probes"]
         assert r["hits_dropped"] == 0
         found = {(int(h["unit"]), int(h["e"]), int(h["d"])) for h in out.hits if h["flags"] & flag}
@@ -120,7 +125,7 @@ def main():
     recovered = [check(o) for o in outs]
     r = outs[-1].result
     for o in outs[1:]:
-        for k in ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe"):
+        for k in COUNTER_KEYS:
             assert o.result[k] == outs[0].result[k], k
     ms_step = 1000.0 * wall / args.steps
     value = r["calls"] / (ms_step / 1000.0)
@@ -133,7 +138,7 @@ def main():
     plain = None
     if not args.no_plain:
         sub = units[:: args.plain_every]
-        b2 = ctx.band(sub)
+        b2 = ctx.band(sub, ucls[:: args.plain_every] if mixed else None)
         ob = b2.run(fc5.MODE_BATCHED)
         t1 = time.perf_counter()
         op = b2.run(fc5.MODE_PLAIN)
@@ -142,8 +147,12 @@ def main():
         ob2 = b2.run(fc5.MODE_BATCHED)
         tb = time.perf_counter() - t1
         b2.close()
-        for k in ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe"):
-            assert op.result[k] == ob.result[k] == ob2.result[k], k
+        for cl in (2, 3, 4, 5):
+            for k in COUNTER_KEYS:
+                assert op.by_class[cl][k] == ob.by_class[cl][k] == ob2.by_class[cl][k], (cl, k)
+        hb = np.sort(ob.hits, order=["unit", "e", "d", "flags"])
+        hp = np.sort(op.hits, order=["unit", "e", "d", "flags"])
+        assert np.array_equal(hb, hp), "plain and batched hit lists differ"
         pv = op.result["calls"] / tp
         plain = {"subset": f"every {args.plain_every}th unit ({len(sub)} units)", "points_per_s": pv,
                  "batched_points_per_s_same_subset": ob2.result["calls"] / tb,
@@ -173,7 +182,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(ctx, units, args.cpu_every, cfg, r, pkeys)
+        cpu = cpu_baseline(ctx, units, ucls, args.cpu_every, cfg, r, pkeys)
 
     line = {
         "metric": METRIC, "value": value, "unit": "(d,e) grid points/s", "n_gpus": 1,
@@ -190,8 +199,9 @@ def main():
         "eligible_B_per_hour": n_B * 3600.0 / (ms_step / 1000.0),
         "filter_probes_per_s": r["probes"] / (ms_step / 1000.0),
         "cell_gathers_per_s": 3 * r["probes"] / (ms_step / 1000.0),
-        "counters": {k: r[k] for k in ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe",
-                                       "n_hits", "solutions")},
+        "counters": {k: r[k] for k in COUNTER_KEYS + ("n_hits", "solutions")},
+        "counters_by_class": {str(cl): {k: outs[-1].by_class[cl][k] for k in COUNTER_KEYS}
+                              for cl in cfg["classes"]} if mixed else None,
         "plants_recovered": f"{min(recovered)}/{len(plants)}",
         "kernel_ms": float(np.mean(kms)),
         "roofline": {"bound": "hbm", "achieved": alg_bytes / kern_s / 1e9, "peak": HBM_PEAK_GBS,
@@ -214,8 +224,8 @@ def fc5_packed_gb(h):
     return (h.m_cells * h.r + 7) // 8 / 1e9
 
 
-def cpu_baseline(ctx, units, every, cfg, r, pkeys):
-    """The oracle (CPU restatement of the reference kernel) timed on every
+def cpu_baseline(ctx, units, ucls, every, cfg, r, pkeys):
+    """The oracle (CPU restatement of the reference kernels) timed on every
     k-th unit of the same band, over the same table bytes, on this host."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
@@ -223,16 +233,27 @@ def cpu_baseline(ctx, units, every, cfg, r, pkeys):
     hdr, packed = ctx.table_download()
     t = O.Table.from_bytes({f: getattr(hdr, f) for f, _ in fc5.StoreHeader._fields_}, packed)
     side = O.Table.build_keys(pkeys, hdr.r) if cfg["plant_mode"] == "side" and len(pkeys) else None
-    sub = units[::every]
-    arr, ok = O.pack_units(sub["B"], sub["u"])
+    sub, scls = units[::every], ucls[::every]
     threads = min(16, os.cpu_count() or 1)
+    s5 = sub[scls == 5]
+    arr, ok = O.pack_units(s5["B"], s5["u"])
     t0 = time.perf_counter()
-    c, _, _ = O.peel(arr, len(sub), t, side=side, threads=threads)
+    c, _, _ = O.peel(arr, len(s5), t, side=side, threads=threads)
+    c3 = None
+    if (scls != 5).any():
+        m = scls != 5
+        c3, _, _ = O.find3(scls[m], sub["B"][m], sub["u"][m], t, threads=threads)
     dt = time.perf_counter() - t0
     # the same subset on the GPU, for a like-for-like counter check
-    out = ctx.run(sub)
+    out = ctx.run(sub, cls=scls)
     for k in ("calls", "neg", "zero", "gated", "probes", "maybe"):
-        assert out.result[k] == c[k], (k, out.result[k], c[k])
+        assert out.by_class[5][k] == c[k], (k, out.by_class[5][k], c[k])
+    if c3 is not None:
+        for k, ok_ in (("calls", "calls"), ("gated", "gated"), ("probes", "probes"), ("jobs", "jobs"),
+                       ("fail_t", "failT")):
+            assert sum(out.by_class[cl][k] for cl in (2, 3, 4)) == c3[ok_], k
+        c = dict(c)
+        c["calls"] += c3["calls"]
     log(f"[bench] cpu oracle {c['calls'] / dt:.4e} points/s on {len(sub)} units, {threads} threads, {dt:.1f}s")
     return {"value": c["calls"] / dt, "unit": "(d,e) grid points/s", "cores": threads, "kind": "port",
             "sample": f"every {every}th unit of the band ({len(sub)} units, {c['calls']} grid points, "

@@ -62,27 +62,34 @@ typedef struct {
 /* counters of one run.  Reference names (fourcore_cls5_gpu_v4.cu:382-409):
  * calls = every valid (unit,e,d) grid point; gated = gate rejects;
  * probes = gate survivors (xor queries); maybe = xor maybe-hits (nhit).
- * neg/zero split the remaining points (negative / zero residual). */
+ * neg/zero split the remaining points (negative / zero residual).
+ * Classes 2-4 (find3, fourcore_find_v4.cu:165-284): jobs = free terms d with a
+ * positive exact T, fail_t = free terms whose T is not positive/divisible
+ * (failT), calls = c3 values swept over the jobs' windows. */
 typedef struct {
   uint64_t calls, neg, zero, gated, probes;
   uint64_t maybe, side_maybe;
   uint64_t n_hits, hits_dropped;
   uint64_t solutions;
   uint64_t units, units_skipped, units_irregular;
+  uint64_t jobs, fail_t;
   double kernel_ms, wall_ms;
 } fc5_result;
 
 /* a filter hit (fourcore_cls5_gpu_v4.cu:140).  unit indexes the band's units
- * as passed in; flags bit0 = main table maybe, bit1 = side (planted) table maybe */
+ * as passed in; flags bit0 = main table maybe, bit1 = side (planted) table maybe.
+ * Class 5: d, e are the free terms.  Classes 2-4: d is the free term and e holds
+ * the third core c3 (Hit.c of fourcore_find_v4.cu:99). */
 typedef struct {
   uint32_t unit, d, e, flags;
   uint64_t fp;
 } fc5_hit;
 
-/* a verified solution line "SOLUTION cls=5 B= u= a1..a5 gcd=" (fourcore_cls5_gpu_v4.cu:524-536) */
+/* a verified solution line "SOLUTION cls=C B= u= a1..a5 gcd=" (fourcore_cls5_gpu_v4.cu:524-536,
+ * fourcore_find_v4.cu:427-457): pair (a,b) for class 5, cores (a,b,c) for classes 2-4 */
 typedef struct {
   uint64_t B, u;
-  uint32_t d, e, a, b;
+  uint32_t cls, d, e, a, b, c;
   int64_t terms[5];
   int64_t gcd;
 } fc5_solution;
@@ -142,11 +149,18 @@ int fc5_side_clear(fc5_ctx *ctx);
 /* pack and upload units; units the reference would not pack are skipped and
  * counted.  The band keeps the caller's unit order for hit records. */
 int fc5_band_create(fc5_ctx *ctx, const fc5_unit *units, size_t n, fc5_band **out);
+/* a band of units of classes 2-5 (the .buc lines "cls B u"); classes 2-4 run
+ * the find3 leaf (fourcore_find_v4.cu:165-284) in the same persistent kernel,
+ * other classes are skipped and counted */
+int fc5_band_create_mixed(fc5_ctx *ctx, const uint32_t *cls, const fc5_unit *units, size_t n,
+                          fc5_band **out);
 int fc5_band_destroy(fc5_ctx *ctx, fc5_band *band);
 /* run one band: all counters, hits and (if "verify") solutions */
 int fc5_band_run(fc5_ctx *ctx, fc5_band *band, int mode, fc5_result *res);
 /* one-call form: create, run, destroy (run_cls5_gpu, fourcore_cls5_gpu_v4.cu:412) */
 int fc5_run_units(fc5_ctx *ctx, const fc5_unit *units, size_t n, int mode, fc5_result *res);
+/* counters of one class (2..5) in the last run */
+int fc5_last_class_result(fc5_ctx *ctx, int cls, fc5_result *out);
 /* hits and solutions of the last run of this context */
 int fc5_last_hits(fc5_ctx *ctx, fc5_hit *out, size_t cap, size_t *n);
 int fc5_last_solutions(fc5_ctx *ctx, fc5_solution *out, size_t cap, size_t *n);

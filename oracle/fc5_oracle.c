@@ -31,6 +31,10 @@
  *       keys recovered 100%.  The reference's GPU binary cannot be built
  *       here (needs nvcc and GMP), so whole-config per-gate counts rest on
  *       this restatement plus the sampled brute force.
+ *   find3 leaf (classes 2-4)  fourcore_find_v4.cu:165-205,255-284,401-425,547-551
+ *       pinned by: an independent Python big-integer brute force over whole
+ *       units (jobs, windows, per-gate counters, maybe-hit list) and planted
+ *       three-core identities T = a^6+b^6+c^6.
  *   host verify      fourcore_cls5_gpu_v4.cu:513-542, fourcore_xor_store.hpp:23-51
  *       pinned by: synthetic exact identities in Python (a fabricated unit
  *       whose residual is a real pair sum) -- no (6,1,5) solution is known.
@@ -40,6 +44,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -929,5 +934,152 @@ int or_verify_hit(u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp, int N, int64_t *t
     }
   }
   free(tab);
+  return nsol;
+}
+
+/* ------------------------------------------------------------------ */
+/* classes 2-4: find3 leaf                                             */
+/* ------------------------------------------------------------------ */
+static int free_factor_of(int cls) { return cls == 2 ? 14 : cls == 3 ? 21 : cls == 4 ? 7 : 0; }
+
+int or_T_peel1(u64 B, u64 u, u64 f, u64 d, u64 *T_lo, u64 *T_hi) { /* gmp:45-52 */
+  if (u >= B || f < 1 || d < 1) return 0;
+  u256 num = pow6_256(B), t;
+  t = pow6_256(u);
+  if (sub256(&num, &t)) return 0;
+  t = pow6_256(f * d);
+  if (sub256(&num, &t)) return 0;
+  if (is_zero256(&num)) return 0;
+  if (divmod256_small(&num, (u64)M42) != 0) return 0;
+  if (num.l[2] | num.l[3]) return 0;
+  *T_lo = num.l[0];
+  *T_hi = num.l[1];
+  return 1;
+}
+
+static u128 pow6_u128(u64 x) { u128 a = (u128)x * x; return a * a * a; }
+
+/* iroot6_host (fourcore_find_v4.cu:107-115): largest a with a^6 <= n */
+static u32 iroot6_u128(u128 n) {
+  double x = ldexp((double)(u64)(n >> 64), 64) + (double)(u64)n;
+  if (x < 1.0) return 0;
+  double r = pow(x, 1.0 / 6.0);
+  u64 a = (u64)r;
+  while (pow6_u128(a + 1) <= n && a < 0xffffffffULL - 2) ++a;
+  while (pow6_u128(a) > n && a) --a;
+  return (u32)a;
+}
+
+int or_find3(const int *cls, const u64 *Bv, const u64 *uv, int n, const or_xor_t *f, int use_gate,
+             or_f3counts_t *counts, or_hit_t *hits, u64 cap, u64 *n_hits, int threads) {
+  or_gate_t g;
+  or_build_gate(&g);
+  const u64 blk = (f && f->hdr.m_cells) ? f->hdr.m_cells / 3 : 0;
+  u64 jobs = 0, failT = 0, calls = 0, gated = 0, probes = 0, maybe = 0, nh = 0;
+  if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) \
+    reduction(+ : jobs, failT, calls, gated, probes, maybe)
+  for (int ui = 0; ui < n; ++ui) {
+    const u64 B = Bv[ui], u = uv[ui];
+    const int c = cls[ui];
+    int64_t res[64], mod, fmax;
+    int nr;
+    or_free_d_cls234((int64_t)B, (int64_t)u, c, res, &nr, &mod, &fmax);
+    const u64 fct = (u64)free_factor_of(c);
+    const u32 lim = (u32)((B - 1) / 42);
+    if (mod <= 0 || fmax < 1) continue;
+    for (int ir = 0; ir < nr; ++ir) {                 /* for_each_free (classes:223-232) */
+      for (int64_t d = res[ir]; d <= fmax; d += mod) {
+        if (d == 0) continue;
+        u64 Tl, Th;
+        if (!or_T_peel1(B, u, fct, (u64)d, &Tl, &Th)) { ++failT; continue; }
+        const u128 T = ((u128)Th << 64) | Tl;
+        if (T == 0) continue;
+        ++jobs;
+        /* make_cand mode 3 (fourcore_find_v4.cu:401-425) */
+        u32 hi = iroot6_u128(T);
+        if (hi > lim) hi = lim;
+        u32 lo = 1;
+        while (lo <= hi && pow6_u128(lo) * 3 < T) ++lo;
+        if (lo > hi) continue;
+        const u32 T504 = (u32)(T % 504), T247 = (u32)(T % 247);
+        for (u32 c3 = lo; c3 <= hi; ++c3) {          /* k_find3 (fourcore_find_v4.cu:255-284) */
+          ++calls;
+          if (use_gate) {
+            u32 r504 = T504 + 504u - g.p6504[c3 % 504];
+            if (r504 >= 504u) r504 -= 504u;
+            u32 r247 = T247 + 247u - g.p6247[c3 % 247];
+            if (r247 >= 247u) r247 -= 247u;
+            if (!(gate_bit(g.g504, (int)r504) && gate_bit(g.g247, (int)r247))) { ++gated; continue; }
+          }
+          ++probes;
+          const u64 c2 = (u64)c3 * c3;
+          const u64 fp = Tl - c2 * c2 * c2;
+          if (f && device_query(f->packed, blk, f->hdr.r, f->hdr.mix_seed, fp)) {
+            ++maybe;
+            u64 slot;
+#pragma omp atomic capture
+            slot = nh++;
+            if (hits && slot < cap) {
+              hits[slot].unit = (u32)ui;
+              hits[slot].d = (u32)d;
+              hits[slot].e = c3;
+              hits[slot].flags = 1;
+              hits[slot].fp = fp;
+            }
+          }
+        }
+      }
+    }
+  }
+  counts->jobs = jobs; counts->failT = failT; counts->calls = calls;
+  counts->gated = gated; counts->probes = probes; counts->maybe = maybe;
+  *n_hits = nh;
+  return 0;
+}
+
+static int verify_cls234(u64 B, u64 u, u64 f, u64 d, u64 x1, u64 x2, u64 x3) { /* gmp:105-110 */
+  u256 rhs = pow6_256(u), t;
+  t = pow6_256(f * d); add256(&rhs, &t);
+  u256 core = pow6_256(x1);
+  t = pow6_256(x2); add256(&core, &t);
+  t = pow6_256(x3); add256(&core, &t);
+  mul256_small(&core, (u64)M42);
+  add256(&rhs, &core);
+  u256 b6 = pow6_256(B);
+  return eq256(&rhs, &b6);
+}
+
+int or_verify_find3(int cls, u64 B, u64 u, u32 d, u32 c, u64 fp, int N, int64_t *terms_out,
+                    int64_t *gcd_out, int cap) { /* fourcore_find_v4.cu:520-551,427-457 */
+  const u64 fct = (u64)free_factor_of(cls);
+  u64 Tl, Th;
+  if (!or_T_peel1(B, u, fct, d, &Tl, &Th)) return 0;
+  const u128 T = ((u128)Th << 64) | Tl;
+  int nsol = 0;
+  for (int a = 1; a <= N; ++a) { /* PairRecover: a <= b, a^6+b^6 = fp (mod 2^64) */
+    const u64 a2 = (u64)a * a;
+    const u64 need = fp - a2 * a2 * a2;
+    for (int b = a; b <= N; ++b) {
+      const u64 b2 = (u64)b * b;
+      if (b2 * b2 * b2 != need) continue;
+      if (c < 1 || (u32)b > c) continue;
+      if (pow6_u128(a) + pow6_u128(b) + pow6_u128(c) != T) continue;
+      if (!verify_cls234(B, u, fct, d, a, b, c)) continue;
+      int64_t t[5] = {42LL * a, 42LL * b, 42LL * c, (int64_t)fct * d, (int64_t)u};
+      qsort(t, 5, sizeof(int64_t), cmp_i64);
+      int bad = 0;
+      for (int q = 0; q < 4; ++q) if (t[q] == t[q + 1]) bad = 1;
+      if (bad || t[4] >= (int64_t)B || t[0] < 1) continue;
+      int64_t gg = t[0];
+      for (int q = 1; q < 5; ++q) gg = gcd64(gg, t[q]);
+      gg = gcd64(gg, (int64_t)B);
+      if (nsol < cap) {
+        for (int q = 0; q < 5; ++q) terms_out[nsol * 5 + q] = t[q];
+        gcd_out[nsol] = gg;
+      }
+      ++nsol;
+    }
+  }
   return nsol;
 }

@@ -124,6 +124,23 @@ int or_cls5_peel(const or_unit5_t *units, int n, const or_xor_t *f,
                  const or_xor_t *side, int use_gate, or_counts_t *counts,
                  or_hit_t *hits, uint64_t cap, uint64_t *n_hits, int threads);
 
+/* ---- classes 2-4 find3 leaf (fourcore_find_v4.cu:165-205,255-284,401-425) ---- */
+typedef struct {
+  uint64_t jobs, failT, calls, gated, probes, maybe;
+} or_f3counts_t;
+/* expand each (cls,B,u) unit into jobs T = (B^6-u^6-(f d)^6)/42^6 (failT when
+ * not positive/divisible), window [lo,hi] per job, c3 sweep with the gate on
+ * T - c3^6 and probe key (T - c3^6) mod 2^64.  Hits: unit index, d, c3 (in the
+ * e field), fp. */
+int or_find3(const int *cls, const uint64_t *B, const uint64_t *u, int n, const or_xor_t *f,
+             int use_gate, or_f3counts_t *counts, or_hit_t *hits, uint64_t cap, uint64_t *n_hits,
+             int threads);
+/* exact T = (B^6-u^6-(f d)^6)/42^6 (fourcore_gmp.hpp:45-52) */
+int or_T_peel1(uint64_t B, uint64_t u, uint64_t f, uint64_t d, uint64_t *T_lo, uint64_t *T_hi);
+/* find3 host verify (fourcore_find_v4.cu:547-551 + report_solution :427-457) */
+int or_verify_find3(int cls, uint64_t B, uint64_t u, uint32_t d, uint32_t c, uint64_t fp, int N,
+                    int64_t *terms_out, int64_t *gcd_out, int cap);
+
 /* exact T for a grid point (fourcore_gmp.hpp:55-62); returns 1 and T (lo,hi)
  * if num>0, 42^6 | num and T < 2^128, else 0 */
 int or_T_peel2(uint64_t B, uint64_t u, uint64_t d, uint64_t e, uint64_t *T_lo,

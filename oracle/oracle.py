@@ -66,6 +66,17 @@ _lib.or_free_de_cls5.argtypes = [C.c_int64, C.c_int64, _P, C.POINTER(C.c_int), C
                                  C.POINTER(C.c_int64), _P, C.POINTER(C.c_int), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64)]
 _lib.or_pow6_192.argtypes = [C.c_uint32, _P]
+class F3Counts(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("jobs", "failT", "calls", "gated", "probes", "maybe")]
+
+
+_lib.or_find3.argtypes = [_P, _P, _P, C.c_int, C.POINTER(Xor), C.c_int, C.POINTER(F3Counts), _P, C.c_uint64,
+                          C.POINTER(C.c_uint64), C.c_int]
+_lib.or_T_peel1.argtypes = [C.c_uint64] * 4 + [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+_lib.or_verify_find3.argtypes = [C.c_int, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int,
+                                 _P, _P, C.c_int]
+_lib.or_free_d_cls234.argtypes = [C.c_int64, C.c_int64, C.c_int, _P, C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64)]
 _lib.or_splitmix64.restype = C.c_uint64
 _lib.or_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
 
@@ -231,4 +242,40 @@ def verify_hit(B, u, lim, d, e, fp, N, cap=16):
     terms = np.zeros(cap * 5, np.int64)
     g = np.zeros(cap, np.int64)
     n = _lib.or_verify_hit(B, u, lim, d, e, fp, N, terms.ctypes.data_as(_P), g.ctypes.data_as(_P), cap)
+    return [(terms[5 * i:5 * i + 5].tolist(), int(g[i])) for i in range(min(n, cap))]
+
+
+def find3(cls, B, u, table, use_gate=True, threads=8, cap=1 << 20):
+    """classes 2-4 find3 semantics: counters dict and sorted hits (e holds c3)"""
+    cl = np.ascontiguousarray(cls, dtype=np.int32)
+    Bv = np.ascontiguousarray(B, dtype=np.uint64)
+    uv = np.ascontiguousarray(u, dtype=np.uint64)
+    c = F3Counts()
+    hits = np.zeros(cap, HIT_DTYPE)
+    nh = C.c_uint64()
+    _lib.or_find3(cl.ctypes.data_as(_P), Bv.ctypes.data_as(_P), uv.ctypes.data_as(_P), len(cl),
+                  C.byref(table.x) if table is not None else None, 1 if use_gate else 0, C.byref(c),
+                  hits.ctypes.data_as(_P), cap, C.byref(nh), threads)
+    hits = np.sort(hits[:min(nh.value, cap)], order=["unit", "e", "d", "flags"])
+    return {f: getattr(c, f) for f, _ in F3Counts._fields_}, hits, nh.value
+
+
+def T_peel1(B, u, f, d):
+    lo, hi = C.c_uint64(), C.c_uint64()
+    if not _lib.or_T_peel1(B, u, f, d, C.byref(lo), C.byref(hi)):
+        return None
+    return lo.value | hi.value << 64
+
+
+def free_d_cls234(B, u, cls):
+    r = np.zeros(64, np.int64)
+    n, m, f = C.c_int(), C.c_int64(), C.c_int64()
+    _lib.or_free_d_cls234(B, u, cls, r.ctypes.data_as(_P), C.byref(n), C.byref(m), C.byref(f))
+    return r[:n.value].tolist(), m.value, f.value
+
+
+def verify_find3(cls, B, u, d, c, fp, N, cap=16):
+    terms = np.zeros(cap * 5, np.int64)
+    g = np.zeros(cap, np.int64)
+    n = _lib.or_verify_find3(cls, B, u, d, c, fp, N, terms.ctypes.data_as(_P), g.ctypes.data_as(_P), cap)
     return [(terms[5 * i:5 * i + 5].tolist(), int(g[i])) for i in range(min(n, cap))]

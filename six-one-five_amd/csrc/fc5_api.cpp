@@ -44,15 +44,25 @@ u64 inv_mod_2_64(u64 a) {  // a odd; Newton iteration doubles correct bits
 }  // namespace
 
 struct fc5_band {
-  std::vector<Cls5Unit> units;     // packed units (skipped ones removed)
-  std::vector<u32> uid;            // caller index of each packed unit
-  std::vector<u32> irregular;      // indices into units: per-point 192-bit path
+  size_t n_in = 0, n_skipped = 0;
+  std::vector<u8> cls_of;           // class of each caller index (0 = skipped)
+  std::vector<int> slot_of;         // caller index -> index in units / units3
+  // class 5
+  std::vector<Cls5Unit> units;      // packed cls5 units
+  std::vector<u32> uid;             // caller index of each packed cls5 unit
+  std::vector<u32> irregular;       // cls5 units on the per-point 192-bit path
   DevUnit5* d_units = nullptr;
   u32 n_dev = 0, n_items = 0;
-  PlainUnit* d_plain = nullptr;    // every packed unit, reference shape
+  PlainUnit* d_plain = nullptr;     // every packed cls5 unit, reference shape
   u32 n_plain = 0;
   u32 max_fmax2 = 0, max_lim = 0;
-  size_t n_in = 0, n_skipped = 0;
+  // classes 2-4
+  std::vector<Cls234Unit> units3;
+  std::vector<u32> uid3;
+  std::vector<u32> irregular3;
+  DevUnit3* d_units3 = nullptr;
+  u32 n_dev3 = 0, n_items3 = 0;
+  u32 max_lim3 = 0;
 };
 
 struct fc5_ctx {
@@ -79,7 +89,11 @@ struct fc5_ctx {
   // run buffers
   HitRec* d_hits = nullptr;
   u64 d_hits_cap = 0;
-  Counters* d_ctr = nullptr;  // [0] batched, [1] plain
+  Counters* d_ctr = nullptr;  // [cls] batched kernel, [8 + cls] plain kernels
+  u16* d_p6 = nullptr;        // x^6 mod 504 (504 entries) then mod 247 (247 entries)
+  u32* d_c6g = nullptr;       // (124488 - c^6 mod 124488) for c = 0..c6g_max
+  u32 c6g_max = 0;
+  fc5_result class_res[6]{};
   u64* d_nhits = nullptr;
   u32* d_item_ctr = nullptr;
   // last results
@@ -148,10 +162,15 @@ extern "C" int fc5_ctx_create(int device, fc5_ctx** out) {
   std::vector<u32> ones(kGateWordsU, ~0u);
   if (hipMalloc(&ctx->d_gate, kGateWordsU * 4) != hipSuccess ||
       hipMalloc(&ctx->d_gate_ones, kGateWordsU * 4) != hipSuccess ||
-      hipMalloc(&ctx->d_ctr, 4 * sizeof(Counters)) != hipSuccess ||
+      hipMalloc(&ctx->d_ctr, 16 * sizeof(Counters)) != hipSuccess ||
+      hipMalloc(&ctx->d_p6, 751 * sizeof(u16)) != hipSuccess ||
       hipMalloc(&ctx->d_nhits, 8) != hipSuccess || hipMalloc(&ctx->d_item_ctr, 8) != hipSuccess)
     return FC5_E_NOMEM;
-  if (hipMemcpy(ctx->d_gate, bitmap.data(), kGateWordsU * 4, hipMemcpyHostToDevice) != hipSuccess ||
+  std::vector<u16> p6(751);
+  for (u32 x = 0; x < 504; ++x) p6[x] = (u16)pow6_mod(x, 504);
+  for (u32 x = 0; x < 247; ++x) p6[504 + x] = (u16)pow6_mod(x, 247);
+  if (hipMemcpy(ctx->d_p6, p6.data(), p6.size() * sizeof(u16), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(ctx->d_gate, bitmap.data(), kGateWordsU * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(ctx->d_gate_ones, ones.data(), kGateWordsU * 4, hipMemcpyHostToDevice) != hipSuccess)
     return FC5_E_HIP;
   *out = ctx.release();
@@ -165,7 +184,8 @@ extern "C" void fc5_ctx_destroy(fc5_ctx* ctx) {
   free_table(ctx->main);
   free_table(ctx->side);
   for (void* p : {(void*)ctx->d_gate, (void*)ctx->d_gate_ones, (void*)ctx->d_ndg, (void*)ctx->d_hits,
-                  (void*)ctx->d_ctr, (void*)ctx->d_nhits, (void*)ctx->d_item_ctr})
+                  (void*)ctx->d_ctr, (void*)ctx->d_nhits, (void*)ctx->d_item_ctr, (void*)ctx->d_p6,
+                  (void*)ctx->d_c6g})
     if (p) (void)hipFree(p);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
@@ -460,6 +480,22 @@ static int ensure_ndg(fc5_ctx* ctx, u32 dmax) {
   return FC5_OK;
 }
 
+static int ensure_c6g(fc5_ctx* ctx, u32 cmax) {
+  if (ctx->d_c6g && ctx->c6g_max >= cmax) return FC5_OK;
+  std::vector<u32> h((size_t)cmax + 1);
+  for (u32 c = 0; c <= cmax; ++c) {
+    const u64 c3 = (u64)c * c * c;
+    const u128 c6 = (u128)c3 * c3;
+    h[c] = (u32)(kGateModU - (u64)(c6 % kGateModU));
+  }
+  if (ctx->d_c6g) (void)hipFree(ctx->d_c6g);
+  ctx->d_c6g = nullptr;
+  if (hipMalloc(&ctx->d_c6g, h.size() * 4) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "c table");
+  HIPCHK(ctx, hipMemcpy(ctx->d_c6g, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  ctx->c6g_max = cmax;
+  return FC5_OK;
+}
+
 static int ensure_hits(fc5_ctx* ctx) {
   if (ctx->d_hits && ctx->d_hits_cap == ctx->hit_cap) return FC5_OK;
   if (ctx->d_hits) (void)hipFree(ctx->d_hits);
@@ -470,73 +506,122 @@ static int ensure_hits(fc5_ctx* ctx) {
   return FC5_OK;
 }
 
-extern "C" int fc5_band_create(fc5_ctx* ctx, const fc5_unit* units, size_t n, fc5_band** out) {
+extern "C" int fc5_band_destroy(fc5_ctx* ctx, fc5_band* band) {
+  if (!band) return FC5_OK;
+  if (ctx) (void)hipSetDevice(ctx->device);
+  for (void* p : {(void*)band->d_units, (void*)band->d_plain, (void*)band->d_units3})
+    if (p) (void)hipFree(p);
+  delete band;
+  return FC5_OK;
+}
+
+static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units, size_t n, fc5_band** out) {
   if (!ctx || !out || (!units && n)) return FC5_E_ARG;
-  if (n > 0xffffffffull) FAIL(ctx, FC5_E_ARG, "too many units in one band");
+  if (n > 0x7fffffffull) FAIL(ctx, FC5_E_ARG, "too many units in one band");
   *out = nullptr;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   auto b = std::make_unique<fc5_band>();
   b->n_in = n;
+  b->cls_of.assign(n, 0);
+  b->slot_of.assign(n, -1);
   std::vector<DevUnit5> dev;
+  std::vector<DevUnit3> dev3;
   std::vector<PlainUnit> plain;
-  u32 item = 0;
+  u32 item = 0, item3 = 0;
   for (size_t i = 0; i < n; ++i) {
-    Cls5Unit U;
-    if (!pack_cls5(units[i].B, units[i].u, U)) { ++b->n_skipped; continue; }
-    const u32 idx = (u32)b->units.size();
-    b->max_fmax2 = std::max(b->max_fmax2, U.fmax2);
-    b->max_lim = std::max(b->max_lim, U.lim);
-    PlainUnit P{};
-    P.b0 = U.base.l[0];
-    P.b1 = U.base.l[1];
-    P.b2 = U.base.l[2];
-    P.fmax1 = U.fmax1;
-    P.fmax2 = U.fmax2;
-    P.mod1 = U.mod1;
-    P.mod2 = U.mod2;
-    P.nres1 = U.n1;
-    P.nres2 = U.n2;
-    P.uid = (u32)i;
-    for (u32 k = 0; k < U.n1; ++k) P.res[k] = U.res1[k];
-    for (u32 k = 0; k < U.n2; ++k) P.res[U.n1 + k] = U.res2[k];
-    plain.push_back(P);
-    if (U.regular) {
-      DevUnit5 D{};
-      D.Q_lo = (u64)U.Q;
-      D.Q_hi = (u64)(U.Q >> 64);
-      D.fmax1 = U.fmax1;
-      D.fmax2 = U.fmax2;
-      D.rows = 6u * (U.fmax1 / 729u + 1u);
-      D.item_base = item;
-      item += (D.rows + kRowsPerItem - 1) / kRowsPerItem;
-      D.te = U.te;
-      D.td = U.td;
-      D.uid = (u32)i;
-      D.band = 0;
-      for (int k = 0; k < 6; ++k) D.res1[k] = (u16)U.res1[k];
-      for (int k = 0; k < 4; ++k) D.res2[k] = (u8)U.res2[k];
-      dev.push_back(D);
+    const int c = cls ? (int)cls[i] : 5;
+    if (c == 5) {
+      Cls5Unit U;
+      if (!pack_cls5(units[i].B, units[i].u, U)) { ++b->n_skipped; continue; }
+      b->cls_of[i] = 5;
+      b->slot_of[i] = (int)b->units.size();
+      b->max_fmax2 = std::max(b->max_fmax2, U.fmax2);
+      b->max_lim = std::max(b->max_lim, U.lim);
+      PlainUnit P{};
+      P.b0 = U.base.l[0];
+      P.b1 = U.base.l[1];
+      P.b2 = U.base.l[2];
+      P.fmax1 = U.fmax1;
+      P.fmax2 = U.fmax2;
+      P.mod1 = U.mod1;
+      P.mod2 = U.mod2;
+      P.nres1 = U.n1;
+      P.nres2 = U.n2;
+      P.uid = (u32)i;
+      for (u32 k = 0; k < U.n1; ++k) P.res[k] = U.res1[k];
+      for (u32 k = 0; k < U.n2; ++k) P.res[U.n1 + k] = U.res2[k];
+      plain.push_back(P);
+      if (U.regular) {
+        DevUnit5 D{};
+        D.Q_lo = (u64)U.Q;
+        D.Q_hi = (u64)(U.Q >> 64);
+        D.fmax1 = U.fmax1;
+        D.fmax2 = U.fmax2;
+        D.rows = 6u * (U.fmax1 / 729u + 1u);
+        D.item_base = item;
+        item += (D.rows + kRowsPerItem - 1) / kRowsPerItem;
+        D.te = U.te;
+        D.td = U.td;
+        D.uid = (u32)i;
+        D.band = 5;
+        for (int k = 0; k < 6; ++k) D.res1[k] = (u16)U.res1[k];
+        for (int k = 0; k < 4; ++k) D.res2[k] = (u8)U.res2[k];
+        dev.push_back(D);
+      } else {
+        b->irregular.push_back((u32)b->units.size());
+      }
+      b->units.push_back(U);
+      b->uid.push_back((u32)i);
+    } else if (c >= 2 && c <= 4) {
+      Cls234Unit U;
+      if (!pack_cls234(c, units[i].B, units[i].u, U)) { ++b->n_skipped; continue; }
+      b->cls_of[i] = (u8)c;
+      b->slot_of[i] = (int)b->units3.size();
+      b->max_lim3 = std::max(b->max_lim3, U.lim);
+      if (U.regular) {
+        DevUnit3 D{};
+        D.Q_lo = (u64)U.Q;
+        D.Q_hi = (u64)(U.Q >> 64);
+        D.inv_lo = (u64)U.inv;
+        D.inv_hi = (u64)(U.inv >> 64);
+        D.fmax = U.fmax;
+        D.lim = U.lim;
+        D.rows = U.nres * (U.fmax / U.mod + 1u);
+        D.item_base = item3;
+        item3 += (D.rows + kRowsPerItem - 1) / kRowsPerItem;
+        D.mod = U.mod;
+        D.dr = U.dr;
+        D.shift = U.shift;
+        D.nres = U.nres;
+        D.uid = (u32)i;
+        D.band = (u32)c;
+        for (u32 k = 0; k < U.nres; ++k) D.res[k] = (u16)U.res[k];
+        dev3.push_back(D);
+      } else {
+        b->irregular3.push_back((u32)b->units3.size());
+      }
+      b->units3.push_back(U);
+      b->uid3.push_back((u32)i);
     } else {
-      b->irregular.push_back(idx);
+      ++b->n_skipped;
     }
-    b->units.push_back(U);
-    b->uid.push_back((u32)i);
   }
   b->n_dev = (u32)dev.size();
   b->n_items = item;
   b->n_plain = (u32)plain.size();
-  if (!dev.empty()) {
-    if (hipMalloc(&b->d_units, dev.size() * sizeof(DevUnit5)) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "units");
-    HIPCHK(ctx, hipMemcpy(b->d_units, dev.data(), dev.size() * sizeof(DevUnit5), hipMemcpyHostToDevice));
-  }
-  if (!plain.empty()) {
-    if (hipMalloc(&b->d_plain, plain.size() * sizeof(PlainUnit)) != hipSuccess) {
-      if (b->d_units) (void)hipFree(b->d_units);
-      FAIL(ctx, FC5_E_NOMEM, "plain units");
-    }
-    HIPCHK(ctx, hipMemcpy(b->d_plain, plain.data(), plain.size() * sizeof(PlainUnit), hipMemcpyHostToDevice));
-  }
-  const int rc = ensure_ndg(ctx, b->max_fmax2);
+  b->n_dev3 = (u32)dev3.size();
+  b->n_items3 = item3;
+  int rc = FC5_OK;
+  auto upload = [&](void** dst, const void* src, size_t bytes) {
+    if (!bytes || rc != FC5_OK) return;
+    if (hipMalloc(dst, bytes) != hipSuccess) { ctx->err = "unit upload"; rc = FC5_E_NOMEM; return; }
+    if (hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) { ctx->err = "unit copy"; rc = FC5_E_HIP; }
+  };
+  upload((void**)&b->d_units, dev.data(), dev.size() * sizeof(DevUnit5));
+  upload((void**)&b->d_plain, plain.data(), plain.size() * sizeof(PlainUnit));
+  upload((void**)&b->d_units3, dev3.data(), dev3.size() * sizeof(DevUnit3));
+  if (rc == FC5_OK && !b->units.empty()) rc = ensure_ndg(ctx, b->max_fmax2);
+  if (rc == FC5_OK && !b->units3.empty()) rc = ensure_c6g(ctx, b->max_lim3 + 64);
   if (rc != FC5_OK) {
     fc5_band_destroy(ctx, b.release());
     return rc;
@@ -545,13 +630,14 @@ extern "C" int fc5_band_create(fc5_ctx* ctx, const fc5_unit* units, size_t n, fc
   return FC5_OK;
 }
 
-extern "C" int fc5_band_destroy(fc5_ctx* ctx, fc5_band* band) {
-  if (!band) return FC5_OK;
-  if (ctx) (void)hipSetDevice(ctx->device);
-  if (band->d_units) (void)hipFree(band->d_units);
-  if (band->d_plain) (void)hipFree(band->d_plain);
-  delete band;
-  return FC5_OK;
+extern "C" int fc5_band_create(fc5_ctx* ctx, const fc5_unit* units, size_t n, fc5_band** out) {
+  return band_create(ctx, nullptr, units, n, out);
+}
+
+extern "C" int fc5_band_create_mixed(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units, size_t n,
+                                     fc5_band** out) {
+  if (!cls && n) return FC5_E_ARG;
+  return band_create(ctx, cls, units, n, out);
 }
 
 static PlainArgs plain_args(fc5_ctx* ctx) {
@@ -564,8 +650,7 @@ static PlainArgs plain_args(fc5_ctx* ctx) {
   const u64 m504 = 504ULL * kM42;
   p.c_m504 = (u64)(((u128)1 << 64) % m504);
   p.c_m247 = (u64)(((u128)1 << 64) % 247u);
-  // inverse of 42^6 mod 247
-  const u64 p6 = (u64)pow6_mod(42, 247);
+  const u64 p6 = (u64)pow6_mod(42, 247);  // inverse of 42^6 mod 247
   u64 inv = 1;
   for (u64 x = 1; x < 247; ++x)
     if (p6 * x % 247 == 1) { inv = x; break; }
@@ -573,7 +658,7 @@ static PlainArgs plain_args(fc5_ctx* ctx) {
   p.hits = ctx->d_hits;
   p.hit_cap = ctx->hit_cap;
   p.nhits = ctx->d_nhits;
-  p.ctr = ctx->d_ctr + 1;
+  p.ctr = ctx->d_ctr + 8 + 5;
   return p;
 }
 
@@ -616,6 +701,77 @@ static int run_plain_units(fc5_ctx* ctx, fc5_band* b, const std::vector<u32>& wh
   return rc;
 }
 
+// classes 2-4 in the reference shape: host expansion of every unit into jobs
+// (fourcore_find_v4.cu:165-205), windows (:401-425), 256 candidates per
+// synchronous k_find3 launch (:492, CHUNK :584).  failT is counted on the host.
+static int run_plain_find3(fc5_ctx* ctx, fc5_band* b, const std::vector<u32>& which, u64 fail_by_cls[6],
+                           u64 jobs_by_cls[6]) {
+  constexpr int kChunk = 256;
+  const bool all = which.empty();
+  const size_t total = all ? b->units3.size() : which.size();
+  if (!total) return FC5_OK;
+  PlainCand3* h_c = nullptr;
+  PlainCand3* d_c = nullptr;
+  if (hipHostMalloc((void**)&h_c, kChunk * sizeof(PlainCand3), hipHostMallocDefault) != hipSuccess)
+    FAIL(ctx, FC5_E_NOMEM, "pinned candidates");
+  if (hipMalloc(&d_c, kChunk * sizeof(PlainCand3)) != hipSuccess) {
+    (void)hipHostFree(h_c);
+    FAIL(ctx, FC5_E_NOMEM, "candidates");
+  }
+  PlainF3Args a{};
+  a.use_gate = ctx->use_gate;
+  a.gate = ctx->gw;
+  a.p6 = ctx->d_p6;
+  a.main = ctx->main.dev;
+  a.side = ctx->side.present ? ctx->side.dev : XorDev{};
+  a.hits = ctx->d_hits;
+  a.hit_cap = ctx->hit_cap;
+  a.nhits = ctx->d_nhits;
+  a.ctr = ctx->d_ctr + 8;
+  a.cands = d_c;
+  int rc = FC5_OK, n = 0;
+  auto flush = [&]() {
+    if (!n || rc != FC5_OK) return;
+    a.n = n;
+    if (hipMemcpyAsync(d_c, h_c, n * sizeof(PlainCand3), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        launch_find3_plain(a, ctx->stream) != 0 || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      ctx->err = "plain find3 launch";
+      rc = FC5_E_HIP;
+    }
+    n = 0;
+  };
+  std::vector<std::pair<u32, u128>> jobs;
+  for (size_t k = 0; k < total && rc == FC5_OK; ++k) {
+    const Cls234Unit& U = b->units3[all ? k : which[k]];
+    jobs.clear();
+    u64 fail = 0;
+    expand_cls234(U, jobs, fail);
+    fail_by_cls[U.cls] += fail;
+    jobs_by_cls[U.cls] += jobs.size();
+    const u32 uid = b->uid3[all ? k : which[k]];
+    for (const auto& [d, T] : jobs) {
+      u32 lo, hi;
+      find3_window(T, U.lim, lo, hi);
+      if (lo > hi) continue;
+      PlainCand3& C = h_c[n++];
+      C.q_lo = (u64)T;
+      C.lo = lo;
+      C.hi = hi;
+      C.r504 = (u32)(T % 504);
+      C.r247 = (u32)(T % 247);
+      C.uid = uid;
+      C.d = d;
+      C.band = (u32)U.cls;
+      C.pad = 0;
+      if (n == kChunk) flush();
+    }
+  }
+  flush();
+  (void)hipFree(d_c);
+  (void)hipHostFree(h_c);
+  return rc;
+}
+
 extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res) {
   if (!ctx || !b || !res) return FC5_E_ARG;
   if (mode != FC5_MODE_PLAIN && mode != FC5_MODE_BATCHED) FAIL(ctx, FC5_E_ARG, "unknown mode");
@@ -625,14 +781,16 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   int rc = ensure_hits(ctx);
   if (rc != FC5_OK) return rc;
   std::memset(res, 0, sizeof *res);
+  std::memset(ctx->class_res, 0, sizeof ctx->class_res);
   ctx->hits.clear();
   ctx->sols.clear();
-  HIPCHK(ctx, hipMemsetAsync(ctx->d_ctr, 0, 4 * sizeof(Counters), ctx->stream));
+  u64 fail_host[6] = {0}, jobs_host[6] = {0};
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_ctr, 0, 16 * sizeof(Counters), ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->d_nhits, 0, 8, ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->d_item_ctr, 0, 8, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (mode == FC5_MODE_BATCHED) {
-    if (b->n_items) {
+    if (b->n_items + b->n_items3) {
       BatchedArgs a{};
       a.units = b->d_units;
       a.n_units = b->n_dev;
@@ -649,22 +807,34 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
       a.ctr = ctx->d_ctr;
       a.item_ctr = ctx->d_item_ctr;
       a.inv729 = inv_mod_2_64(729);
+      a.units3 = b->d_units3;
+      a.n_units3 = b->n_dev3;
+      a.n_items3 = b->n_items3;
+      a.c6g = ctx->d_c6g;
       const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu();
-      const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)b->n_items));
+      const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)(b->n_items + b->n_items3)));
       if (launch_cls5_batched(a, grid, ctx->stream) != 0) FAIL(ctx, FC5_E_HIP, "batched launch");
     }
     if (!b->irregular.empty()) {
       rc = run_plain_units(ctx, b, b->irregular, false);
       if (rc != FC5_OK) return rc;
     }
+    if (!b->irregular3.empty()) {
+      rc = run_plain_find3(ctx, b, b->irregular3, fail_host, jobs_host);
+      if (rc != FC5_OK) return rc;
+    }
   } else {
-    rc = run_plain_units(ctx, b, {}, true);
+    if (b->n_plain) {
+      rc = run_plain_units(ctx, b, {}, true);
+      if (rc != FC5_OK) return rc;
+    }
+    rc = run_plain_find3(ctx, b, {}, fail_host, jobs_host);
     if (rc != FC5_OK) return rc;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-  Counters c[2];
+  Counters c[16];
   u64 nh = 0;
-  HIPCHK(ctx, hipMemcpyAsync(c, ctx->d_ctr, 2 * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(c, ctx->d_ctr, 16 * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(&nh, ctx->d_nhits, 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   float ms = 0;
@@ -673,18 +843,38 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   std::vector<HitRec> raw(stored);
   if (stored)
     HIPCHK(ctx, hipMemcpy(raw.data(), ctx->d_hits, stored * sizeof(HitRec), hipMemcpyDeviceToHost));
-  res->calls = c[0].calls + c[1].calls;
-  res->neg = c[0].neg + c[1].neg;
-  res->zero = c[0].zero + c[1].zero;
-  res->probes = c[0].probes + c[1].probes;
-  res->gated = (c[0].pos - c[0].probes) + c[1].gated;
-  res->maybe = c[0].maybe + c[1].maybe;
-  res->side_maybe = c[0].side_maybe + c[1].side_maybe;
+  for (int k = 2; k <= 5; ++k) {
+    const Counters& B = c[k];      // batched kernel: gated = positives - probes
+    const Counters& P = c[8 + k];  // plain kernels: gated counted directly
+    fc5_result& r = ctx->class_res[k];
+    r.calls = B.calls + P.calls;
+    r.neg = B.neg + P.neg;
+    r.zero = B.zero + P.zero;
+    r.probes = B.probes + P.probes;
+    r.gated = (B.pos - B.probes) + P.gated;
+    r.maybe = B.maybe + P.maybe;
+    r.side_maybe = B.side_maybe + P.side_maybe;
+    r.jobs = B.jobs + jobs_host[k];
+    r.fail_t = B.failT + fail_host[k];
+    res->calls += r.calls;
+    res->neg += r.neg;
+    res->zero += r.zero;
+    res->probes += r.probes;
+    res->gated += r.gated;
+    res->maybe += r.maybe;
+    res->side_maybe += r.side_maybe;
+    res->jobs += r.jobs;
+    res->fail_t += r.fail_t;
+  }
+  for (size_t i = 0; i < b->n_in; ++i) {
+    const int k = b->cls_of[i];
+    if (k) ctx->class_res[k].units++;
+  }
   res->n_hits = nh;
   res->hits_dropped = nh - stored;
-  res->units = b->units.size();
+  res->units = b->units.size() + b->units3.size();
   res->units_skipped = b->n_skipped;
-  res->units_irregular = b->irregular.size();
+  res->units_irregular = b->irregular.size() + b->irregular3.size();
   res->kernel_ms = ms;
   ctx->hits.resize(stored);
   for (u64 i = 0; i < stored; ++i)
@@ -696,29 +886,34 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
     return x.flags < y.flags;
   });
   if (ctx->verify && !ctx->hits.empty()) {
-    const int N = ctx->main.hdr.N ? (int)ctx->main.hdr.N : (int)b->max_lim;
+    const int N = ctx->main.hdr.N ? (int)ctx->main.hdr.N : (int)std::max(b->max_lim, b->max_lim3);
     if (ctx->pr_N != N) {
       ctx->pr.build(N);
       ctx->pr_N = N;
     }
-    // caller index -> packed unit
-    std::vector<int> pos(b->n_in, -1);
-    for (size_t i = 0; i < b->uid.size(); ++i) pos[b->uid[i]] = (int)i;
     std::vector<Solution> sols;
     for (const fc5_hit& h : ctx->hits) {
-      if (!(h.flags & 1u)) continue;
-      const Cls5Unit& U = b->units[(size_t)pos[h.unit]];
-      verify_cls5_hit(ctx->pr, U.B, U.u, U.lim, h.d, h.e, h.fp, sols);
+      if (!(h.flags & 1u) || h.unit >= b->n_in) continue;
+      const int k = b->cls_of[h.unit], slot = b->slot_of[h.unit];
+      if (k == 5) {
+        const Cls5Unit& U = b->units[(size_t)slot];
+        verify_cls5_hit(ctx->pr, U.B, U.u, U.lim, h.d, h.e, h.fp, sols);
+      } else if (k >= 2) {
+        const Cls234Unit& U = b->units3[(size_t)slot];
+        verify_find3_hit(ctx->pr, k, U.B, U.u, h.d, h.e, h.fp, sols);
+      }
       if (ctx->stop_first && !sols.empty()) break;
     }
     for (const Solution& S : sols) {
       fc5_solution o{};
       o.B = S.B;
       o.u = S.u;
+      o.cls = S.cls;
       o.d = S.d;
       o.e = S.e;
       o.a = S.a;
       o.b = S.b;
+      o.c = S.c;
       for (int k = 0; k < 5; ++k) o.terms[k] = S.terms[k];
       o.gcd = S.gcd;
       ctx->sols.push_back(o);
@@ -730,6 +925,12 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
     ctx->err = "hit buffer overflow: " + std::to_string(res->hits_dropped) + " hits dropped";
     return FC5_E_OVERFLOW;
   }
+  return FC5_OK;
+}
+
+extern "C" int fc5_last_class_result(fc5_ctx* ctx, int cls, fc5_result* out) {
+  if (!ctx || !out || cls < 2 || cls > 5) return FC5_E_ARG;
+  *out = ctx->class_res[cls];
   return FC5_OK;
 }
 

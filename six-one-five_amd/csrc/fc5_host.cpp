@@ -2,6 +2,7 @@
 #include "fc5_host.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -626,6 +627,94 @@ u64 cls5_calls(const Cls5Unit& U) {
   return count(U.res1, U.n1, U.mod1, U.fmax1) * count(U.res2, U.n2, U.mod2, U.fmax2);
 }
 
+static u128 inv_odd_2_128(u128 a) {  // a odd; Newton iteration doubles the correct bits
+  u128 x = a;
+  for (int i = 0; i < 7; ++i) x *= (u128)2 - a * x;
+  return x;
+}
+
+bool pack_cls234(int cls, u64 B, u64 u, Cls234Unit& U) {
+  U = Cls234Unit{};
+  U.cls = cls;
+  U.B = B;
+  U.u = u;
+  U.lim = (u32)((B - 1) / 42);
+  if (cls < 2 || cls > 4 || u >= B || B >= (1ULL << 32)) return false;
+  U.base = U256::pow6(B);
+  if (sub_in_place(U.base, U256::pow6(u)) || U.base.is_zero()) return false;
+  const FreeSpec s = free_d_cls234((i64)B, (i64)u, cls);
+  if (s.mod <= 0 || s.fmax < 1 || s.res.empty() || s.res.size() > 64) return false;
+  U.f = (u32)free_factor(cls);
+  U.mod = (u32)s.mod;
+  U.fmax = (u32)s.fmax;
+  U.nres = (u32)s.res.size();
+  for (u32 i = 0; i < U.nres; ++i) U.res[i] = (u32)s.res[i];
+  U.shift = (u32)__builtin_ctz(U.mod);
+  U.inv = inv_odd_2_128((u128)(U.mod >> U.shift));
+  U.dr = (u32)pow6_mod((i64)U.res[0], (i64)U.mod);
+  bool uniform = true;
+  for (u32 i = 0; i < U.nres; ++i) uniform &= pow6_mod((i64)U.res[i], (i64)U.mod) == (i64)U.dr;
+  U256 q = U.base;
+  const u64 br = divmod_small(q, kM42);
+  const u64 f6 = (u64)U.f * U.f * U.f;
+  const __int128 rem = (__int128)br - (__int128)(f6 * f6) * U.dr;
+  const bool divisible = rem == 0 || rem == -(__int128)kM42;
+  U.regular = uniform && divisible && q.fits128() && U.fmax < (1u << 21) && U.nres <= 24;
+  if (U.regular) U.Q = q.lo128() - (rem == 0 ? 0 : 1);
+  return true;
+}
+
+u32 iroot6(u128 n) {
+  const double x = (double)(u64)(n >> 64) * 18446744073709551616.0 + (double)(u64)n;
+  if (x < 1.0) return 0;
+  u64 a = (u64)std::cbrt(std::sqrt(x));
+  auto p6 = [](u64 v) { const u128 v2 = (u128)v * v; return v2 * v2 * v2; };
+  while (p6(a + 1) <= n) ++a;
+  while (a && p6(a) > n) --a;
+  return (u32)a;
+}
+
+void find3_window(u128 T, u32 lim, u32& lo, u32& hi) {
+  hi = std::min(iroot6(T), lim);
+  auto p6 = [](u64 v) { const u128 v2 = (u128)v * v; return v2 * v2 * v2; };
+  const double x = (double)(u64)(T >> 64) * 18446744073709551616.0 + (double)(u64)T;
+  u64 l = (u64)std::cbrt(std::sqrt(x / 3.0));
+  if (l < 1) l = 1;
+  while (l > 1 && p6(l - 1) * 3 >= T) --l;
+  while (l <= hi && p6(l) * 3 < T) ++l;
+  lo = (u32)l;
+}
+
+void expand_cls234(const Cls234Unit& U, std::vector<std::pair<u32, u128>>& jobs, u64& failT) {
+  for (u32 i = 0; i < U.nres; ++i)
+    for (u64 d = U.res[i]; d <= U.fmax; d += U.mod) {
+      if (!d) continue;
+      u128 T;
+      if (U.regular) {
+        const u128 d2 = (u128)d * d, d6 = d2 * d2 * d2;
+        const u128 dq = ((d6 - U.dr) >> U.shift) * U.inv;
+        if (U.Q <= dq) { ++failT; continue; }
+        T = U.Q - dq;
+      } else if (!exact_T_cls234(U.B, U.u, U.f, d, T)) {
+        ++failT;
+        continue;
+      }
+      if (T == 0) continue;
+      jobs.emplace_back((u32)d, T);
+    }
+}
+
+bool verify_cls234(u64 B, u64 u, u64 f, u64 d, u64 a, u64 b, u64 c) {
+  U256 rhs = U256::pow6(u);
+  add_in_place(rhs, U256::pow6(f * d));
+  U256 core = U256::pow6(a);
+  add_in_place(core, U256::pow6(b));
+  add_in_place(core, U256::pow6(c));
+  mul_small(core, kM42);
+  add_in_place(rhs, core);
+  return cmp(rhs, U256::pow6(B)) == 0;
+}
+
 // ---------------------------------------------------------------------------
 // pair recovery / verify
 // ---------------------------------------------------------------------------
@@ -672,11 +761,42 @@ int verify_cls5_hit(const PairRecover& pr, u64 B, u64 u, u32 lim, u32 d, u32 e, 
     i64 g = t[0];
     for (int i = 1; i < 5; ++i) g = gcd_i64(g, t[i]);
     g = gcd_i64(g, (i64)B);
-    Solution S{B, u, d, e, a, b, {t[0], t[1], t[2], t[3], t[4]}, g};
+    Solution S{B, u, 5, d, e, a, b, 0, {t[0], t[1], t[2], t[3], t[4]}, g};
     out.push_back(S);
     ++found;
   }
   return found;
 }
 
+}  // namespace fc5
+
+namespace fc5 {
+int verify_find3_hit(const PairRecover& pr, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp,
+                     std::vector<Solution>& out) {
+  const u64 f = (u64)free_factor(cls);
+  u128 T;
+  if (!exact_T_cls234(B, u, f, d, T)) return 0;
+  std::vector<std::pair<u32, u32>> pairs;
+  pr.recover(fp, pairs);
+  int found = 0;
+  for (auto [a, b] : pairs) {
+    if (c < 1 || b > c) continue;
+    U256 s = U256::pow6(a);
+    add_in_place(s, U256::pow6(b));
+    add_in_place(s, U256::pow6(c));
+    if (!s.fits128() || s.lo128() != T) continue;
+    if (!verify_cls234(B, u, f, d, a, b, c)) continue;
+    i64 t[5] = {42LL * a, 42LL * b, 42LL * c, (i64)f * (i64)d, (i64)u};
+    std::sort(t, t + 5);
+    bool bad = false;
+    for (int i = 0; i < 4; ++i) bad |= t[i] == t[i + 1];
+    if (bad || t[4] >= (i64)B || t[0] < 1) continue;
+    i64 g = t[0];
+    for (int i = 1; i < 5; ++i) g = std::gcd(g, t[i]);
+    g = std::gcd(g, (i64)B);
+    out.push_back(Solution{B, u, (u32)cls, d, 0, a, b, c, {t[0], t[1], t[2], t[3], t[4]}, g});
+    ++found;
+  }
+  return found;
+}
 }  // namespace fc5

@@ -167,12 +167,35 @@ bool pack_cls5(u64 B, u64 u, Cls5Unit& U);
 // calls of one unit in closed form: #valid e * #valid d
 u64 cls5_calls(const Cls5Unit& U);
 
+// Classes 2-4 (find3 leaf): T(d) = (B^6 - u^6 - (f d)^6) / 42^6 over the free
+// term d = res + mod*k.  With f^6 * mod = 42^6 (f, mod) = (14, 729), (21, 64),
+// (7, 46656) and d^6 mod `mod` constant over the residues, exactly
+// T(d) = Q - floor(d^6 / mod), Q = floor((B^6-u^6)/42^6) + c, c in {-1, 0}.
+struct Cls234Unit {
+  int cls = 0;
+  u64 B = 0, u = 0;
+  U256 base;
+  u128 Q = 0, inv = 1;  // inv: inverse of the odd part of mod, mod 2^128
+  u32 f = 0, lim = 0, fmax = 0, mod = 0, dr = 0, shift = 0, nres = 0;
+  u32 res[64] = {0};
+  bool regular = false;
+};
+bool pack_cls234(int cls, u64 B, u64 u, Cls234Unit& U);
+u32 iroot6(u128 n);  // largest a with a^6 <= n
+// the find3 window of a job (make_cand mode 3, fourcore_find_v4.cu:412-418):
+// hi = min(iroot6(T), lim), lo = least c >= 1 with 3 c^6 >= T; empty if lo > hi
+void find3_window(u128 T, u32 lim, u32& lo, u32& hi);
+// jobs of one unit: (d, T) for every valid d with T > 0 (exact); failT counts
+// the d whose residual is not positive or not divisible (fourcore_find_v4.cu:181-189)
+void expand_cls234(const Cls234Unit& U, std::vector<std::pair<u32, u128>>& jobs, u64& failT);
+bool verify_cls234(u64 B, u64 u, u64 f, u64 d, u64 a, u64 b, u64 c);
+
 // ---------------------------------------------------------------------------
 // pair recovery and exact verification (cls5v4:513-542, xstore:23-51)
 // ---------------------------------------------------------------------------
 struct Solution {
   u64 B, u;
-  u32 d, e, a, b;
+  u32 cls, d, e, a, b, c;
   i64 terms[5];
   i64 gcd;
 };
@@ -190,6 +213,9 @@ class PairRecover {
 // full verify chain for one cls5 maybe-hit; appends solutions
 int verify_cls5_hit(const PairRecover& pr, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp,
                     std::vector<Solution>& out);
+// find3 maybe-hit of class cls at (d, c3) (fourcore_find_v4.cu:535-551, :427-457)
+int verify_find3_hit(const PairRecover& pr, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp,
+                     std::vector<Solution>& out);
 
 // splitmix64 (Vigna), the generator for planted keys
 inline u64 splitmix64(u64& s) {

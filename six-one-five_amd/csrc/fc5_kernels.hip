@@ -68,27 +68,52 @@ __device__ __forceinline__ u128 dq_of(u32 x) {
 // ---------------------------------------------------------------------------
 // batched persistent kernel
 // ---------------------------------------------------------------------------
+constexpr u32 kFind3Bit = 0x80000000u;  // ring entry unit field: find3 row
+
+__device__ __forceinline__ u32 iroot6_u128(u128 X) {  // largest a with a^6 <= X (X < 2^126)
+  const double x = (double)(u64)(X >> 64) * 18446744073709551616.0 + (double)(u64)X;
+  u32 a = (u32)cbrt(sqrt(x));
+  auto p6 = [](u32 v) { const u64 v3 = (u64)v * v * v; return (u128)v3 * v3; };
+  while (p6(a + 1) <= X) ++a;
+  while (a > 0 && p6(a) > X) --a;
+  return a;
+}
+
 __device__ __forceinline__ void drain(const BatchedArgs& a, const u32* qu, const u32* qd,
                                       const u32* qe, u32 head, u32 n, u32 lane) {
   if (lane < n) {
     const u32 slot = (head + lane) & (kRing - 1);
-    const u32 ui = qu[slot], d = qd[slot], e = qe[slot];
-    const DevUnit5* U = a.units + ui;
-    const u64 e2 = (u64)e * e;
-    const u64 eq = (e2 * e2 * e2 - (u64)U->te) * a.inv729;  // floor(e^6/729) mod 2^64
-    const u64 d3 = (u64)d * d * d;
-    const u64 dq = (u64)(((u128)d3 * d3) >> 6);             // floor(d^6/64) mod 2^64
-    const u64 fp = U->Q_lo - eq - dq;                        // T mod 2^64
+    const u32 tag = qu[slot], d = qd[slot], e = qe[slot];
+    u64 fp;
+    u32 uid, band;
+    if (tag & kFind3Bit) {  // classes 2-4: V = Q - floor(d^6/mod) - c^6, c in e
+      const DevUnit3* U = a.units3 + (tag & ~kFind3Bit);
+      const u64 d3 = (u64)d * d * d;
+      const u64 dq = (u64)((((u128)d3 * d3) - U->dr) >> U->shift) * U->inv_lo;
+      const u64 c2 = (u64)e * e;
+      fp = U->Q_lo - dq - c2 * c2 * c2;
+      uid = U->uid;
+      band = U->band;
+    } else {  // class 5: T = Q - floor(e^6/729) - floor(d^6/64)
+      const DevUnit5* U = a.units + tag;
+      const u64 e2 = (u64)e * e;
+      const u64 eq = (e2 * e2 * e2 - (u64)U->te) * a.inv729;
+      const u64 d3 = (u64)d * d * d;
+      const u64 dq = (u64)(((u128)d3 * d3) >> 6);
+      fp = U->Q_lo - eq - dq;
+      uid = U->uid;
+      band = U->band;
+    }
     u32 flags = d_query(a.main, fp) ? 1u : 0u;
     if (a.side.present && d_query(a.side, fp)) flags |= 2u;
     if (flags) {
-      Counters* c = a.ctr + U->band;
+      Counters* c = a.ctr + band;
       if (flags & 1u) atomicAdd((unsigned long long*)&c->maybe, 1ULL);
       if (flags & 2u) atomicAdd((unsigned long long*)&c->side_maybe, 1ULL);
       const u64 s = atomicAdd((unsigned long long*)a.nhits, 1ULL);
       if (s < a.hit_cap) {
         HitRec h;
-        h.unit = U->uid;
+        h.unit = uid;
         h.d = d;
         h.e = e;
         h.flags = flags;
@@ -99,6 +124,213 @@ __device__ __forceinline__ void drain(const BatchedArgs& a, const u32* qu, const
   }
 }
 
+struct Ring {
+  u32 *qu, *qd, *qe;
+  u32 head, tail;
+};
+
+// append the lanes with `pass` set; drain full batches of 64
+__device__ __forceinline__ u32 ring_push(const BatchedArgs& a, Ring& q, bool pass, u32 tag, u32 x, u32 y,
+                                         u32 lane) {
+  const u64 m = __ballot(pass);
+  if (!m) return 0;
+  if (pass) {
+    const u32 slot = (q.tail + lanes_below(m)) & (kRing - 1);
+    q.qu[slot] = tag;
+    q.qd[slot] = x;
+    q.qe[slot] = y;
+  }
+  const u32 cnt = (u32)__popcll(m);
+  q.tail += cnt;
+  if (q.tail - q.head >= 64u) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    drain(a, q.qu, q.qd, q.qe, q.head, 64u, lane);
+    q.head += 64u;
+  }
+  return cnt;
+}
+
+__device__ __forceinline__ bool gate_pass(const u32* s_gate, u32 xg, u32 n) {
+  u32 t = xg + n;
+  t = min(t, t - kGateMod);
+  return (s_gate[t >> 5] >> (t & 31u)) & 1u;
+}
+
+__device__ __forceinline__ void flush_counters(Counters* c, u32 lane, u32 calls, u32 neg, u32 zero, u32 pos,
+                                               u32 probes, u32 jobs, u32 failT) {
+  u64 s_calls = 0, s_neg = 0, s_zero = 0, s_pos = 0, s_jobs = 0, s_fail = 0;
+#pragma unroll
+  for (int i = 0; i < kRowsPerItem; ++i) {
+    s_calls += rdl(calls, i);
+    s_neg += rdl(neg, i);
+    s_zero += rdl(zero, i);
+    s_pos += rdl(pos, i);
+    s_jobs += rdl(jobs, i);
+    s_fail += rdl(failT, i);
+  }
+  if (lane == 0) {
+    atomicAdd((unsigned long long*)&c->calls, (unsigned long long)s_calls);
+    if (s_neg) atomicAdd((unsigned long long*)&c->neg, (unsigned long long)s_neg);
+    if (s_zero) atomicAdd((unsigned long long*)&c->zero, (unsigned long long)s_zero);
+    atomicAdd((unsigned long long*)&c->pos, (unsigned long long)s_pos);
+    atomicAdd((unsigned long long*)&c->probes, (unsigned long long)probes);
+    if (s_jobs) atomicAdd((unsigned long long*)&c->jobs, (unsigned long long)s_jobs);
+    if (s_fail) atomicAdd((unsigned long long*)&c->failT, (unsigned long long)s_fail);
+  }
+}
+
+// one class-5 item: 8 e-rows of one unit, every d
+__device__ __forceinline__ void cls5_item(const BatchedArgs& a, const u32* s_gate, Ring& q, u32 item, u32 lane) {
+  const u32 K = a.ndg_K;
+  u32 lo = 0, hi = a.n_units;
+  while (hi - lo > 1) {
+    const u32 mid = (lo + hi) >> 1;
+    if (a.units[mid].item_base <= item) lo = mid;
+    else hi = mid;
+  }
+  const u32 ui = lo;
+  const DevUnit5 U = a.units[ui];
+  const u32 row0 = (item - U.item_base) * (u32)kRowsPerItem;
+  const u32 r0 = U.res2[0], r1 = U.res2[1], r2 = U.res2[2], r3 = U.res2[3];
+  const u32 minr = min(min(r0, r1), min(r2, r3));
+
+  // per-row set-up on lanes 0..7: e, T-space row value, positive cut
+  u32 e = 0, dlim = 0, xg = 0;
+  u32 c_calls = 0, c_neg = 0, c_zero = 0, c_pos = 0;
+  if (lane < (u32)kRowsPerItem) {
+    const u32 r = row0 + lane;
+    if (r < U.rows) {
+      e = (u32)a.units[ui].res1[r % 6u] + (r / 6u) * 729u;
+      if (e >= 1 && e <= U.fmax1) {
+        auto cnt = [](u32 rj, u32 lim) -> u32 { return rj <= lim ? (lim - rj) / 64u + 1u : 0u; };
+        const u32 cr = cnt(r0, U.fmax2) + cnt(r1, U.fmax2) + cnt(r2, U.fmax2) + cnt(r3, U.fmax2);
+        const u64 e3 = (u64)e * e * e;
+        const u128 eq = (((u128)e3 * e3) - U.te) / 729u;
+        const u128 Q = ((u128)U.Q_hi << 64) | U.Q_lo;
+        u32 pos = 0, zero = 0;
+        if (Q > eq) {
+          const u128 X = Q - eq;  // T = X - floor(d^6/64)
+          const double xd = ((double)(u64)(X >> 64) * 18446744073709551616.0 + (double)(u64)X) * 64.0;
+          u32 dc = (u32)cbrt(sqrt(xd));
+          while (dq_of(dc + 1) < X) ++dc;
+          while (dc > 0 && dq_of(dc) >= X) --dc;
+          dlim = min(dc, U.fmax2);
+          pos = cnt(r0, dlim) + cnt(r1, dlim) + cnt(r2, dlim) + cnt(r3, dlim);
+          const u64 xh = (u64)(X >> 64), xl = (u64)X;
+          xg = (u32)(((xh % kGateMod) * (u64)(18446744073709551615ULL % kGateMod + 1) + xl % kGateMod) % kGateMod);
+          const u32 z = dc + 1;  // the only d that can give T == 0
+          if (z <= U.fmax2 && dq_of(z) == X) {
+            const u32 zr = z & 63u;
+            zero = (zr == r0 || zr == r1 || zr == r2 || zr == r3) ? 1u : 0u;
+          }
+        } else if (Q == eq) {
+          zero = (U.fmax2 >= 1 && (r0 == 1 || r1 == 1 || r2 == 1 || r3 == 1)) ? 1u : 0u;
+        }
+        c_calls = cr;
+        c_pos = pos;
+        c_zero = zero;
+        c_neg = cr - pos - zero;
+      }
+    }
+  }
+  u32 maxdl = 0;
+#pragma unroll
+  for (int i = 0; i < kRowsPerItem; ++i) maxdl = max(maxdl, rdl(dlim, i));
+
+  // gate over the positive part of the rows; lanes run over k
+  u32 probes_item = 0;
+  for (u32 kb = 0; minr + 64u * kb <= maxdl; kb += 64u) {
+    const u32 k = kb + lane;
+    const u32 d0 = r0 + 64u * k, d1 = r1 + 64u * k, d2 = r2 + 64u * k, d3 = r3 + 64u * k;
+    const u32 n0 = d0 <= U.fmax2 ? a.ndg[r0 * K + k] : 0u;
+    const u32 n1 = d1 <= U.fmax2 ? a.ndg[r1 * K + k] : 0u;
+    const u32 n2 = d2 <= U.fmax2 ? a.ndg[r2 * K + k] : 0u;
+    const u32 n3 = d3 <= U.fmax2 ? a.ndg[r3 * K + k] : 0u;
+    const u32 dmin_blk = minr + 64u * kb;
+    for (int i = 0; i < kRowsPerItem; ++i) {
+      const u32 dl = rdl(dlim, i);
+      if (dl < dmin_blk) continue;
+      const u32 xgi = rdl(xg, i);
+      const u32 ei = rdl(e, i);
+      const u32 ds[4] = {d0, d1, d2, d3};
+      const u32 ns[4] = {n0, n1, n2, n3};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool pass = gate_pass(s_gate, xgi, ns[j]) && ds[j] <= dl;
+        probes_item += ring_push(a, q, pass, ui, ds[j], ei, lane);
+      }
+    }
+  }
+  flush_counters(a.ctr + U.band, lane, c_calls, c_neg, c_zero, c_pos, probes_item, 0u, 0u);
+}
+
+// one class 2-4 item: 8 d-rows of one unit, every c3 of each row's window
+// (fourcore_find_v4.cu:165-205 expansion, :401-425 window, :255-284 sweep)
+__device__ __forceinline__ void find3_item(const BatchedArgs& a, const u32* s_gate, Ring& q, u32 item, u32 lane) {
+  u32 lo = 0, hi = a.n_units3;
+  while (hi - lo > 1) {
+    const u32 mid = (lo + hi) >> 1;
+    if (a.units3[mid].item_base <= item) lo = mid;
+    else hi = mid;
+  }
+  const u32 ui = lo;
+  const DevUnit3* Up = a.units3 + ui;
+  const u32 row0 = (item - Up->item_base) * (u32)kRowsPerItem;
+  u32 d = 0, wlo = 1, whi = 0, xg = 0;
+  u32 c_calls = 0, c_jobs = 0, c_fail = 0;
+  if (lane < (u32)kRowsPerItem) {
+    const u32 r = row0 + lane;
+    if (r < Up->rows) {
+      d = (u32)Up->res[r % Up->nres] + (r / Up->nres) * Up->mod;
+      if (d >= 1 && d <= Up->fmax) {
+        const u64 d3 = (u64)d * d * d;
+        const u128 inv = ((u128)Up->inv_hi << 64) | Up->inv_lo;
+        const u128 dq = ((((u128)d3 * d3) - Up->dr) >> Up->shift) * inv;  // exact floor(d^6/mod)
+        const u128 Q = ((u128)Up->Q_hi << 64) | Up->Q_lo;
+        if (Q > dq) {
+          const u128 X = Q - dq;  // the job's T
+          c_jobs = 1;
+          u32 h = iroot6_u128(X);
+          if (h > Up->lim) h = Up->lim;
+          // smallest c >= 1 with 3 c^6 >= T
+          const double xd = (double)(u64)(X >> 64) * 18446744073709551616.0 + (double)(u64)X;
+          u32 l = (u32)cbrt(sqrt(xd / 3.0));
+          if (l < 1) l = 1;
+          auto p6 = [](u32 v) { const u64 v3 = (u64)v * v * v; return (u128)v3 * v3; };
+          while (l > 1 && p6(l - 1) * 3 >= X) --l;
+          while (l <= h && p6(l) * 3 < X) ++l;
+          if (l <= h) {
+            wlo = l;
+            whi = h;
+            c_calls = h - l + 1;
+          }
+          const u64 xh = (u64)(X >> 64), xl = (u64)X;
+          xg = (u32)(((xh % kGateMod) * (u64)(18446744073709551615ULL % kGateMod + 1) + xl % kGateMod) % kGateMod);
+        } else {
+          c_fail = 1;  // not positive: compute_T fails (fourcore_gmp.hpp:47)
+        }
+      }
+    }
+  }
+  u32 probes_item = 0;
+  for (int i = 0; i < kRowsPerItem; ++i) {
+    const u32 l = rdl(wlo, i), h = rdl(whi, i);
+    if (l > h) continue;
+    const u32 xgi = rdl(xg, i), di = rdl(d, i);
+    for (u32 cb = l; cb <= h; cb += 64u) {
+      const u32 c = cb + lane;
+      const bool in = c <= h;
+      const u32 n = in ? a.c6g[c] : 0u;
+      const bool pass = in && gate_pass(s_gate, xgi, n);
+      probes_item += ring_push(a, q, pass, ui | kFind3Bit, di, c, lane);
+      if (h - cb < 64u) break;  // cb + 64 could wrap past 2^32
+    }
+  }
+  flush_counters(a.ctr + Up->band, lane, c_calls, 0u, 0u, c_calls, probes_item, c_jobs, c_fail);
+}
+
 __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
   __shared__ u32 s_gate[kGateWords];
   __shared__ u32 s_qu[kWaves][kRing], s_qd[kWaves][kRing], s_qe[kWaves][kRing];
@@ -107,140 +339,21 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
 
   const u32 lane = threadIdx.x & 63u;
   const u32 w = threadIdx.x >> 6;
-  u32* qu = s_qu[w];
-  u32* qd = s_qd[w];
-  u32* qe = s_qe[w];
-  u32 head = 0, tail = 0;
-  const u32 K = a.ndg_K;
-
+  Ring q{s_qu[w], s_qd[w], s_qe[w], 0u, 0u};
+  const u32 total = a.n_items + a.n_items3;
   for (;;) {
     u32 item = 0;
     if (lane == 0) item = atomicAdd(a.item_ctr, 1u);
     item = __shfl(item, 0, 64);
-    if (item >= a.n_items) break;
-
-    // unit of this item: last unit with item_base <= item
-    u32 lo = 0, hi = a.n_units;
-    while (hi - lo > 1) {
-      const u32 mid = (lo + hi) >> 1;
-      if (a.units[mid].item_base <= item) lo = mid;
-      else hi = mid;
-    }
-    const u32 ui = lo;
-    const DevUnit5 U = a.units[ui];
-    const u32 row0 = (item - U.item_base) * (u32)kRowsPerItem;
-    const u32 r0 = U.res2[0], r1 = U.res2[1], r2 = U.res2[2], r3 = U.res2[3];
-    const u32 minr = min(min(r0, r1), min(r2, r3));
-
-    // ---- per-row set-up on lanes 0..7: e, T-space row value, positive cut
-    u32 e = 0, dlim = 0, xg = 0;
-    u32 c_calls = 0, c_neg = 0, c_zero = 0, c_pos = 0;
-    if (lane < (u32)kRowsPerItem) {
-      const u32 r = row0 + lane;
-      if (r < U.rows) {
-        e = (u32)a.units[ui].res1[r % 6u] + (r / 6u) * 729u;
-        if (e >= 1 && e <= U.fmax1) {
-          auto cnt = [](u32 rj, u32 lim) -> u32 { return rj <= lim ? (lim - rj) / 64u + 1u : 0u; };
-          const u32 cr = cnt(r0, U.fmax2) + cnt(r1, U.fmax2) + cnt(r2, U.fmax2) + cnt(r3, U.fmax2);
-          const u64 e3 = (u64)e * e * e;
-          const u128 eq = (((u128)e3 * e3) - U.te) / 729u;
-          const u128 Q = ((u128)U.Q_hi << 64) | U.Q_lo;
-          u32 pos = 0, zero = 0;
-          if (Q > eq) {
-            const u128 X = Q - eq;  // T = X - floor(d^6/64)
-            const double xd = ((double)(u64)(X >> 64) * 18446744073709551616.0 + (double)(u64)X) * 64.0;
-            u32 dc = (u32)cbrt(sqrt(xd));
-            while (dq_of(dc + 1) < X) ++dc;
-            while (dc > 0 && dq_of(dc) >= X) --dc;
-            dlim = min(dc, U.fmax2);
-            pos = cnt(r0, dlim) + cnt(r1, dlim) + cnt(r2, dlim) + cnt(r3, dlim);
-            const u64 xh = (u64)(X >> 64), xl = (u64)X;
-            xg = (u32)(((xh % kGateMod) * (u64)(18446744073709551615ULL % kGateMod + 1) +
-                        xl % kGateMod) % kGateMod);
-            const u32 z = dc + 1;  // the only d that can give T == 0
-            if (z <= U.fmax2 && dq_of(z) == X) {
-              const u32 zr = z & 63u;
-              zero = (zr == r0 || zr == r1 || zr == r2 || zr == r3) ? 1u : 0u;
-            }
-          } else if (Q == eq) {
-            zero = (U.fmax2 >= 1 && (r0 == 1 || r1 == 1 || r2 == 1 || r3 == 1)) ? 1u : 0u;
-          }
-          c_calls = cr;
-          c_pos = pos;
-          c_zero = zero;
-          c_neg = cr - pos - zero;
-        }
-      }
-    }
-    u32 maxdl = 0;
-    u64 s_calls = 0, s_neg = 0, s_zero = 0, s_pos = 0;
-#pragma unroll
-    for (int i = 0; i < kRowsPerItem; ++i) {
-      maxdl = max(maxdl, rdl(dlim, i));
-      s_calls += rdl(c_calls, i);
-      s_neg += rdl(c_neg, i);
-      s_zero += rdl(c_zero, i);
-      s_pos += rdl(c_pos, i);
-    }
-
-    // ---- gate over the positive part of the rows; lanes run over k
-    u32 probes_item = 0;
-    for (u32 kb = 0; minr + 64u * kb <= maxdl; kb += 64u) {
-      const u32 k = kb + lane;
-      const u32 d0 = r0 + 64u * k, d1 = r1 + 64u * k, d2 = r2 + 64u * k, d3 = r3 + 64u * k;
-      const u32 n0 = d0 <= U.fmax2 ? a.ndg[r0 * K + k] : 0u;
-      const u32 n1 = d1 <= U.fmax2 ? a.ndg[r1 * K + k] : 0u;
-      const u32 n2 = d2 <= U.fmax2 ? a.ndg[r2 * K + k] : 0u;
-      const u32 n3 = d3 <= U.fmax2 ? a.ndg[r3 * K + k] : 0u;
-      const u32 dmin_blk = minr + 64u * kb;
-      for (int i = 0; i < kRowsPerItem; ++i) {
-        const u32 dl = rdl(dlim, i);
-        if (dl < dmin_blk) continue;
-        const u32 xgi = rdl(xg, i);
-        const u32 ei = rdl(e, i);
-        const u32 ds[4] = {d0, d1, d2, d3};
-        const u32 ns[4] = {n0, n1, n2, n3};
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          u32 t = xgi + ns[j];
-          t = min(t, t - kGateMod);
-          const bool pass = ((s_gate[t >> 5] >> (t & 31u)) & 1u) && ds[j] <= dl;
-          const u64 m = __ballot(pass);
-          if (m) {
-            if (pass) {
-              const u32 slot = (tail + lanes_below(m)) & (kRing - 1);
-              qu[slot] = ui;
-              qd[slot] = ds[j];
-              qe[slot] = ei;
-            }
-            const u32 cnt = (u32)__popcll(m);
-            tail += cnt;
-            probes_item += cnt;
-            if (tail - head >= 64u) {
-              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-              __builtin_amdgcn_wave_barrier();
-              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-              drain(a, qu, qd, qe, head, 64u, lane);
-              head += 64u;
-            }
-          }
-        }
-      }
-    }
-    if (lane == 0) {
-      Counters* c = a.ctr + U.band;
-      atomicAdd((unsigned long long*)&c->calls, (unsigned long long)s_calls);
-      atomicAdd((unsigned long long*)&c->neg, (unsigned long long)s_neg);
-      atomicAdd((unsigned long long*)&c->zero, (unsigned long long)s_zero);
-      atomicAdd((unsigned long long*)&c->pos, (unsigned long long)s_pos);
-      atomicAdd((unsigned long long*)&c->probes, (unsigned long long)probes_item);
-    }
+    if (item >= total) break;
+    if (item < a.n_items) cls5_item(a, s_gate, q, item, lane);
+    else find3_item(a, s_gate, q, item - a.n_items, lane);
   }
-  if (tail != head) {
+  if (q.tail != q.head) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    drain(a, qu, qd, qe, head, tail - head, lane);
+    drain(a, q.qu, q.qd, q.qe, q.head, q.tail - q.head, lane);
   }
 }
 
@@ -347,6 +460,66 @@ __global__ __launch_bounds__(256) void k_cls5_plain(PlainArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// plain find3 (the reference kernel shape, fourcore_find_v4.cu:255-284):
+// block per candidate, threads stride c3 over [lo, hi]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_find3_plain(PlainF3Args a) {
+  const int ci = blockIdx.x;
+  if (ci >= a.n) return;
+  const PlainCand3 C = a.cands[ci];
+  __shared__ u16 s504[504], s247[247];
+  __shared__ u64 g504[8], g247[4];
+  for (u32 k = threadIdx.x; k < 504; k += blockDim.x) s504[k] = a.p6[k];
+  for (u32 k = threadIdx.x; k < 247; k += blockDim.x) s247[k] = a.p6[504 + k];
+  if (threadIdx.x < 8) g504[threadIdx.x] = a.gate.g504[threadIdx.x];
+  if (threadIdx.x < 4) g247[threadIdx.x] = a.gate.g247[threadIdx.x];
+  __syncthreads();
+  u64 calls = 0, gated = 0, probes = 0;
+  for (u32 c3 = C.lo + threadIdx.x; c3 <= C.hi && c3 >= C.lo; c3 += blockDim.x) {
+    ++calls;
+    if (a.use_gate) {
+      u32 r504 = C.r504 + 504u - s504[c3 % 504];
+      if (r504 >= 504u) r504 -= 504u;
+      u32 r247 = C.r247 + 247u - s247[c3 % 247];
+      if (r247 >= 247u) r247 -= 247u;
+      if (!(((g504[r504 >> 6] >> (r504 & 63)) & 1ULL) && ((g247[r247 >> 6] >> (r247 & 63)) & 1ULL))) {
+        ++gated;
+        continue;
+      }
+    }
+    ++probes;
+    const u64 c2 = (u64)c3 * c3;
+    const u64 fp = C.q_lo - c2 * c2 * c2;
+    u32 flags = d_query(a.main, fp) ? 1u : 0u;
+    if (a.side.present && d_query(a.side, fp)) flags |= 2u;
+    if (flags) {
+      Counters* c = a.ctr + C.band;
+      if (flags & 1u) atomicAdd((unsigned long long*)&c->maybe, 1ULL);
+      if (flags & 2u) atomicAdd((unsigned long long*)&c->side_maybe, 1ULL);
+      const u64 s = atomicAdd((unsigned long long*)a.nhits, 1ULL);
+      if (s < a.hit_cap) {
+        HitRec h;
+        h.unit = C.uid;
+        h.d = C.d;
+        h.e = c3;
+        h.flags = flags;
+        h.fp = fp;
+        a.hits[s] = h;
+      }
+    }
+  }
+  calls = wave_sum(calls);
+  gated = wave_sum(gated);
+  probes = wave_sum(probes);
+  if ((threadIdx.x & 63u) == 0) {
+    Counters* c = a.ctr + C.band;
+    atomicAdd((unsigned long long*)&c->calls, (unsigned long long)calls);
+    atomicAdd((unsigned long long*)&c->gated, (unsigned long long)gated);
+    atomicAdd((unsigned long long*)&c->probes, (unsigned long long)probes);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // utilities
 // ---------------------------------------------------------------------------
 __global__ void k_expand_cells(const u32* __restrict__ pw, u64 m, u32 r, u64* __restrict__ cells) {
@@ -383,6 +556,11 @@ int launch_cls5_batched(const BatchedArgs& a, int grid, hipStream_t s) {
 
 int launch_cls5_plain(const PlainArgs& a, int n_units, int ymax, hipStream_t s) {
   hipLaunchKernelGGL(k_cls5_plain, dim3(n_units, ymax), dim3(256), 0, s, a);
+  return (int)hipGetLastError();
+}
+
+int launch_find3_plain(const PlainF3Args& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_find3_plain, dim3(a.n), dim3(256), 0, s, a);
   return (int)hipGetLastError();
 }
 

@@ -58,6 +58,21 @@ struct DevUnit5 {
 };
 static_assert(sizeof(DevUnit5) == 64, "DevUnit5 is 64 bytes");
 
+// class 2-4 unit in T-space: T(d) = Q - floor(d^6/mod) for d = res[i] + mod*k,
+// mod = 729 (cls2, f=14), 64 (cls3, f=21), 46656 (cls4, f=7); f^6 * mod = 42^6
+struct DevUnit3 {
+  u64 Q_lo, Q_hi;
+  u64 inv_lo, inv_hi;   // inverse of the odd part of mod, mod 2^128 (exact quotients)
+  u32 fmax, lim;
+  u32 rows, item_base;  // rows = nres * (fmax/mod + 1); first item after the cls5 items
+  u32 mod, dr;          // dr = d^6 mod mod over the residue classes
+  u32 shift, nres;      // mod = 2^shift * odd
+  u32 uid, band;
+  u16 res[24];
+  u32 pad[2];
+};
+static_assert(sizeof(DevUnit3) == 128, "DevUnit3 is 128 bytes");
+
 // reference-shaped unit (Unit5D, fourcore_cls5_gpu_v4.cu:346-351)
 struct PlainUnit {
   u64 b0, b1, b2;
@@ -66,7 +81,7 @@ struct PlainUnit {
 };
 
 struct Counters {
-  u64 calls, neg, zero, gated, probes, maybe, side_maybe, pos;
+  u64 calls, neg, zero, gated, probes, maybe, side_maybe, pos, jobs, failT;
 };
 
 struct HitRec {
@@ -92,6 +107,32 @@ struct BatchedArgs {
   Counters* ctr;  // indexed by DevUnit5::band
   u32* item_ctr;
   u64 inv729;  // 729^-1 mod 2^64
+  // classes 2-4 (find3 rows); their items follow the n_items cls5 items
+  const DevUnit3* units3;
+  u32 n_units3, n_items3;
+  const u32* c6g;  // (124488 - c^6 mod 124488) for c = 0..lim
+};
+
+// reference-shaped find3 candidate (Cand, fourcore_find_v4.cu:102-109)
+struct PlainCand3 {
+  u64 q_lo;
+  u32 lo, hi;
+  u32 r504, r247;
+  u32 uid, d;
+  u32 band, pad;
+};
+
+struct PlainF3Args {
+  const PlainCand3* cands;
+  int n;
+  int use_gate;
+  GateWords gate;
+  const u16* p6;  // [504] x^6 mod 504, then [247] x^6 mod 247 (GateData.p6504/p6247)
+  XorDev main, side;
+  HitRec* hits;
+  u64 hit_cap;
+  u64* nhits;
+  Counters* ctr;  // indexed by band
 };
 
 struct PlainArgs {
@@ -112,6 +153,7 @@ constexpr int kRowsPerItem = 8;  // e rows per work item of the batched kernel
 // launchers (return hipError_t as int)
 int launch_cls5_batched(const BatchedArgs& a, int grid, hipStream_t s);
 int launch_cls5_plain(const PlainArgs& a, int n_units, int ymax, hipStream_t s);
+int launch_find3_plain(const PlainF3Args& a, hipStream_t s);
 int launch_expand_cells(const u32* packed_words, u64 m_cells, u32 r, u64* cells, hipStream_t s);
 int launch_query(const XorDev& x, const u64* keys, u64 n, u8* out, hipStream_t s);
 int launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s);

@@ -42,7 +42,7 @@ class Unit(C.Structure):
 class Result(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe", "n_hits",
-        "hits_dropped", "solutions", "units", "units_skipped", "units_irregular")] + \
+        "hits_dropped", "solutions", "units", "units_skipped", "units_irregular", "jobs", "fail_t")] + \
                [("kernel_ms", C.c_double), ("wall_ms", C.c_double)]
 
     def as_dict(self):
@@ -55,7 +55,7 @@ class Hit(C.Structure):
 
 class Solution(C.Structure):
     _fields_ = [("B", C.c_uint64), ("u", C.c_uint64)] + \
-               [(n, C.c_uint32) for n in ("d", "e", "a", "b")] + \
+               [(n, C.c_uint32) for n in ("cls", "d", "e", "a", "b", "c")] + \
                [("terms", C.c_int64 * 5), ("gcd", C.c_int64)]
 
 
@@ -77,7 +77,9 @@ _sig = {
     "fc5_side_build": (C.c_int, [_P, _P, C.c_size_t, C.c_int]),
     "fc5_side_clear": (C.c_int, [_P]),
     "fc5_band_create": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(_P)]),
+    "fc5_band_create_mixed": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(_P)]),
     "fc5_band_destroy": (C.c_int, [_P, _P]),
+    "fc5_last_class_result": (C.c_int, [_P, C.c_int, C.POINTER(Result)]),
     "fc5_band_run": (C.c_int, [_P, _P, C.c_int, C.POINTER(Result)]),
     "fc5_run_units": (C.c_int, [_P, _P, C.c_size_t, C.c_int, C.POINTER(Result)]),
     "fc5_last_hits": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -161,11 +163,18 @@ HIT_DTYPE = np.dtype([("unit", "<u4"), ("d", "<u4"), ("e", "<u4"), ("flags", "<u
 
 
 class Band:
-    def __init__(self, ctx, units):
+    """units: fc5_unit array; cls: optional per-unit class (2..5), default all 5"""
+
+    def __init__(self, ctx, units, cls=None):
         self.ctx = ctx
         self.units = np.ascontiguousarray(units)
         self.h = _P()
-        rc = _lib.fc5_band_create(ctx.h, _ptr(self.units), len(self.units), C.byref(self.h))
+        if cls is None:
+            rc = _lib.fc5_band_create(ctx.h, _ptr(self.units), len(self.units), C.byref(self.h))
+        else:
+            self.cls = np.ascontiguousarray(cls, dtype=np.uint32)
+            rc = _lib.fc5_band_create_mixed(ctx.h, _ptr(self.cls), _ptr(self.units), len(self.units),
+                                            C.byref(self.h))
         ctx._check(rc)
 
     def run(self, mode=MODE_BATCHED, allow_overflow=False):
@@ -173,7 +182,9 @@ class Band:
         rc = _lib.fc5_band_run(self.ctx.h, self.h, mode, C.byref(res))
         if rc not in (OK, E_OVERFLOW) or (rc == E_OVERFLOW and not allow_overflow):
             self.ctx._check(rc)
-        return RunOutput(res.as_dict(), self.ctx.last_hits(), self.ctx.last_solutions(), rc)
+        out = RunOutput(res.as_dict(), self.ctx.last_hits(), self.ctx.last_solutions(), rc)
+        out.by_class = {k: self.ctx.class_result(k) for k in (2, 3, 4, 5)}
+        return out
 
     def close(self):
         if self.h:
@@ -262,11 +273,16 @@ class Context:
         self._check(_lib.fc5_measure_copy(self.h, nbytes, C.byref(t)))
         return t.value
 
-    def band(self, units):
-        return Band(self, units)
+    def band(self, units, cls=None):
+        return Band(self, units, cls)
 
-    def run(self, units, mode=MODE_BATCHED, allow_overflow=False):
-        b = Band(self, units)
+    def class_result(self, cls):
+        r = Result()
+        self._check(_lib.fc5_last_class_result(self.h, cls, C.byref(r)))
+        return r.as_dict()
+
+    def run(self, units, mode=MODE_BATCHED, allow_overflow=False, cls=None):
+        b = Band(self, units, cls)
         try:
             return b.run(mode, allow_overflow)
         finally:
@@ -286,8 +302,8 @@ class Context:
         arr = (Solution * max(1, n.value))()
         if n.value:
             self._check(_lib.fc5_last_solutions(self.h, C.cast(arr, _P), n.value, C.byref(n)))
-        return [dict(B=s.B, u=s.u, d=s.d, e=s.e, a=s.a, b=s.b, terms=list(s.terms), gcd=s.gcd)
-                for s in arr[:n.value]]
+        return [dict(B=s.B, u=s.u, cls=s.cls, d=s.d, e=s.e, a=s.a, b=s.b, c=s.c, terms=list(s.terms),
+                     gcd=s.gcd) for s in arr[:n.value]]
 
     def close(self):
         if self.h:

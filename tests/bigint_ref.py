@@ -91,3 +91,58 @@ def xor_query(header, packed, key):
         word = int.from_bytes(packed[bit // 8: bit // 8 + 16], "little")
         got ^= (word >> (bit % 8)) & mask
     return got == (h & mask)
+
+
+F_OF = {2: 14, 3: 21, 4: 7}
+MOD_OF = {2: 729, 3: 64, 4: 46656}
+
+
+def cls234_residues(cls, B, u):
+    """free-term residues from the definition: units d mod `mod` with
+    (f d)^6 = B^6 - u^6 (mod 2^6 for cls3 and cls4, mod 3^6 for cls2 and cls4)"""
+    from math import gcd
+    f, m = F_OF[cls], MOD_OF[cls]
+    rhs = (pow(B, 6, m) - pow(u, 6, m)) % m
+    return [d for d in range(1, m) if gcd(d, m) == 1 and pow(f * d, 6, m) == rhs]
+
+
+def iroot6(n):
+    a = int(round(n ** (1.0 / 6.0))) if n > 0 else 0
+    while (a + 1) ** 6 <= n:
+        a += 1
+    while a > 0 and a ** 6 > n:
+        a -= 1
+    return a
+
+
+def find3_counts(cls, B, u, contains=None, use_gate=True):
+    f, m = F_OF[cls], MOD_OF[cls]
+    fmax = (B - 1) // f
+    lim = (B - 1) // 42
+    c = dict(jobs=0, failT=0, calls=0, gated=0, probes=0, maybe=0)
+    hits = []
+    base = B ** 6 - u ** 6
+    for r in cls234_residues(cls, B, u):
+        for d in range(r, fmax + 1, m):
+            R = base - (f * d) ** 6
+            if R <= 0 or R % M42:
+                c["failT"] += 1
+                continue
+            T = R // M42
+            c["jobs"] += 1
+            hi = min(iroot6(T), lim)
+            lo = 1
+            while lo <= hi and 3 * lo ** 6 < T:
+                lo += 1
+            for c3 in range(lo, hi + 1):
+                c["calls"] += 1
+                V = T - c3 ** 6
+                if use_gate and not (V % 504 in S504 and V % 247 in S247):
+                    c["gated"] += 1
+                    continue
+                c["probes"] += 1
+                fp = V % 2 ** 64
+                if contains is not None and contains(fp):
+                    c["maybe"] += 1
+                    hits.append((d, c3, fp))
+    return c, hits

@@ -256,3 +256,113 @@ def test_device_built_table_search_matches_oracle(ctx):
     oc, oh = _oracle_run(Bs, us, t)
     _compare(out, oc, oh)
     assert len(oh) > 0
+
+
+F3_KEYS = ("jobs", "fail_t", "calls", "gated", "probes", "maybe")
+C5_KEYS = ("calls", "neg", "zero", "gated", "probes", "maybe")
+
+
+def _oracle_mixed(cls, Bs, us, table):
+    """per-class oracle counters and the hit list with caller unit indices"""
+    out, hits = {}, []
+    idx5 = np.nonzero(cls == 5)[0]
+    if len(idx5):
+        c5, h5 = _oracle_run(Bs[idx5], us[idx5], table)
+        out[5] = c5
+        h5 = h5.copy()
+        h5["unit"] = idx5[h5["unit"]]
+        hits.append(h5)
+    idx3 = np.nonzero(cls != 5)[0]
+    if len(idx3):
+        for k in (2, 3, 4):
+            sel = idx3[cls[idx3] == k]
+            if not len(sel):
+                continue
+            c3, h3, _ = O.find3(cls[sel], Bs[sel], us[sel], table, threads=min(16, os.cpu_count() or 8))
+            out[k] = {"jobs": c3["jobs"], "fail_t": c3["failT"], "calls": c3["calls"], "gated": c3["gated"],
+                      "probes": c3["probes"], "maybe": c3["maybe"]}
+            h3 = h3.copy()
+            h3["unit"] = sel[h3["unit"]]
+            hits.append(h3)
+    allh = np.concatenate(hits) if hits else np.zeros(0, O.HIT_DTYPE)
+    return out, np.sort(allh, order=["unit", "e", "d", "flags"])
+
+
+def _compare_mixed(out, oc, oh):
+    for k, ref in oc.items():
+        got = out.by_class[k]
+        for key in (C5_KEYS if k == 5 else F3_KEYS):
+            assert got[key] == ref[key], (k, key, got[key], ref[key])
+    got = np.sort(out.hits, order=["unit", "e", "d", "flags"])
+    assert len(got) == len(oh)
+    for f in ("unit", "d", "e", "fp"):
+        assert np.array_equal(got[f], oh[f]), f
+
+
+@pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN], ids=["batched", "plain"])
+def test_mixed_classes_cfg1_match_oracle(ctx, mode):
+    """classes 2-5 of B in [100001,102000] in one band: cls2-4 run the find3
+    leaf in the same persistent kernel as cls5 (batched) or the reference
+    per-class launches (plain)"""
+    say(f"mixed classes cfg1 mode={mode}")
+    cls, Bs, us = fc5.emit_units(100001, 102000, classes=(2, 3, 4, 5))
+    t = _load(ctx, 2428, 16)
+    out = ctx.run(fc5.units_array(Bs, us), mode, cls=cls)
+    oc, oh = _oracle_mixed(cls, Bs, us, t)
+    _compare_mixed(out, oc, oh)
+    assert sum(out.by_class[k]["calls"] for k in (2, 3, 4)) > 0 and len(oh) > 0
+
+
+def test_mixed_classes_cfg3_sample_match_oracle(ctx):
+    """a unit sample of the 2.0M band, all four classes, batched and plain"""
+    say("mixed classes cfg3 sample")
+    cls, Bs, us = fc5.emit_units(2000001, 2010000, classes=(2, 3, 4, 5))
+    sel = np.concatenate([np.nonzero(cls == k)[0][::m] for k, m in ((2, 60), (3, 60), (4, 400), (5, 400))])
+    sel.sort()
+    cls, Bs, us = cls[sel], Bs[sel], us[sel]
+    t = _load(ctx, 2428, 16)
+    b = ctx.band(fc5.units_array(Bs, us), cls)
+    ob = b.run(fc5.MODE_BATCHED)
+    op = b.run(fc5.MODE_PLAIN)
+    b.close()
+    oc, oh = _oracle_mixed(cls, Bs, us, t)
+    _compare_mixed(ob, oc, oh)
+    _compare_mixed(op, oc, oh)
+
+
+def test_find3_plants_recovered(ctx):
+    """planted three-core keys V = T(d) - c3^6 in the side table are found at (d, c3)"""
+    say("find3 plants")
+    cls, Bs, us = fc5.emit_units(2000001, 2000400, classes=(2, 3, 4))
+    rng = np.random.default_rng(11)
+    pts = []
+    for i in rng.choice(len(Bs), 12, replace=False):
+        k, B, u = int(cls[i]), int(Bs[i]), int(us[i])
+        res, mod, fmax = O.free_d_cls234(B, u, k)
+        for _ in range(200):
+            d = int(res[rng.integers(len(res))] + mod * rng.integers(fmax // mod + 1))
+            T = O.T_peel1(B, u, {2: 14, 3: 21, 4: 7}[k], d) if 1 <= d <= fmax else None
+            if T:
+                break
+        if not T:
+            continue
+        hi = min(int(round(T ** (1 / 6))) + 2, (B - 1) // 42)
+        while hi ** 6 > T:
+            hi -= 1
+        lo = 1
+        while lo <= hi and 3 * lo ** 6 < T:
+            lo += 1
+        if lo > hi:
+            continue
+        c3 = int(rng.integers(lo, hi + 1))
+        pts.append((int(i), d, c3, (T - c3 ** 6) % 2 ** 64))
+    _load(ctx, 2428, 48)
+    ctx.set_option("use_gate", 0)  # plants need not pass the residue gate here
+    ctx.side_build(np.array([p[3] for p in pts], np.uint64))
+    try:
+        out = ctx.run(fc5.units_array(Bs, us), cls=cls)
+    finally:
+        ctx.side_clear()
+        ctx.set_option("use_gate", 1)
+    found = {(int(h["unit"]), int(h["d"]), int(h["e"])) for h in out.hits if h["flags"] & 2}
+    assert {(i, d, c) for i, d, c, _ in pts} <= found

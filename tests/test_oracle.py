@@ -186,3 +186,44 @@ def test_verify_rejects_non_pair_sums():
     assert O.verify_hit(B, u, (B - 1) // 42, d, e, T % 2 ** 64, 2428) == []
     # a fingerprint that is a pair sum mod 2^64 but not equal to T: rejected by the exact check
     assert O.verify_hit(B, u, (B - 1) // 42, d, e, (5 ** 6 + 7 ** 6) % 2 ** 64, 2428) == []
+
+
+def test_cls234_residues_match_reference():
+    for rec in json.load(open(os.path.join(GOLDEN, "residues.json"))):
+        if rec["cls"] == 5:
+            continue
+        res, mod, fmax = O.free_d_cls234(rec["B"], rec["u"], rec["cls"])
+        assert (res, mod, fmax) == (rec["d_res"], rec["d_mod"], rec["d_fmax"])
+        assert res == br.cls234_residues(rec["cls"], rec["B"], rec["u"])
+
+
+@pytest.mark.parametrize("cls", [2, 3, 4])
+def test_find3_counters_brute_force(cls):
+    """jobs, failT, windows, per-gate counters and maybe-hits of whole units"""
+    c_, Bs, us = O.emit_units(100001, 100400, classes=(cls,))
+    table = O.Table.build_pairs(300, 16)
+    for k in (0, len(Bs) // 2):
+        B, u = int(Bs[k]), int(us[k])
+        oc, oh, _ = O.find3([cls], [B], [u], table, threads=1)
+        ref, rhits = br.find3_counts(cls, B, u, contains=table.contains)
+        for key in ref:
+            assert oc[key] == ref[key], (cls, B, u, key)
+        assert sorted((int(h["d"]), int(h["e"]), int(h["fp"])) for h in oh) == sorted(rhits)
+
+
+def test_find3_exact_T_and_three_core_plant():
+    # the peel identity B^6 = u^6 + (f d)^6 + 42^6 T on a real unit
+    _, Bs, us = O.emit_units(100001, 100100, classes=(4,))
+    B, u = int(Bs[0]), int(us[0])
+    res, mod, fmax = O.free_d_cls234(B, u, 4)
+    d = res[0]
+    T = O.T_peel1(B, u, 7, d)
+    assert B ** 6 == u ** 6 + (7 * d) ** 6 + 42 ** 6 * T
+    # a fabricated three-core residual T = 3^6 + 4^6 + 5^6 has the window
+    # (fourcore_find_v4.cu self-test shape) containing c3 = 5 and its pair 3,4
+    T = 3 ** 6 + 4 ** 6 + 5 ** 6
+    hi = br.iroot6(T)
+    lo = 1
+    while lo <= hi and 3 * lo ** 6 < T:
+        lo += 1
+    assert lo <= 5 <= hi and (T - 5 ** 6) == 3 ** 6 + 4 ** 6
