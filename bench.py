@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--cpu-every", type=int, default=400, help="CPU oracle on every k-th unit")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-measure", action="store_true", help="skip the gather/copy microbenchmarks")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape")
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
     ap.add_argument("--load-policy", type=int, default=None, choices=[0, 1, 2, 3],
                     help="cache policy of the cell gathers (library default 1, nontemporal)")
     args = ap.parse_args()
@@ -112,6 +114,15 @@ def main():
         assert not missing, f"{len(missing)} planted keys not recovered"
         return len(plant_set & found)
 
+    tune = None
+    if not args.no_autotune:
+        t1 = time.time()
+        tune = band.autotune(sample_every=16)
+        tune["seconds"] = time.time() - t1
+        log(f"[bench] autotune: rows_per_item={tune['rows_per_item']} blocks_per_cu={tune['blocks_per_cu']} "
+            f"({tune['best_ms']:.1f} ms vs default {tune['default_ms']:.1f} ms on 1/16 of the band)")
+    if not args.no_graph:
+        ctx.set_option("use_graph", 1)
     for _ in range(args.warmup):
         check(band.run(fc5.MODE_BATCHED))
     log(f"[bench] setup {time.time() - t_setup:.1f}s; timing {args.steps} steps")
@@ -135,6 +146,7 @@ def main():
     log(f"[bench] {ms_step:.1f} ms/step, kernel {1000 * kern_s:.1f} ms, {value:.4e} points/s, "
         f"probes={r['probes']} maybe={r['maybe']} side_maybe={r['side_maybe']}")
 
+    ctx.set_option("use_graph", 0)
     plain = None
     if not args.no_plain:
         sub = units[:: args.plain_every]
@@ -211,6 +223,8 @@ def main():
                      "per_unit": f"{3 * cell_bytes} B per filter probe (3 cells x {cell_bytes} B)",
                      "transaction_GBps_at_64B": r["probes"] * 3 * 64 / kern_s / 1e9},
         "gather_ceiling": ceiling,
+        "autotune": tune,
+        "graph": not args.no_graph,
         "plain_form": plain,
         "cpu_baseline": cpu,
         "table_build_s": t_build,

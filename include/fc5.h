@@ -106,7 +106,9 @@ void fc5_ctx_destroy(fc5_ctx *ctx);
 const char *fc5_last_error(const fc5_ctx *ctx);
 /* options: "hit_cap" (default 1<<20), "use_gate" (1; 0 = --no-gate,
  * fourcore_cls5_gpu_v4.cu:634), "blocks_per_cu" (0 = occupancy limit) for the
- * persistent grid, "load_policy" (0..3, default 1: cache policy of the xor cell gathers:
+ * persistent grid, "rows_per_item" (4, 8, 16, 32; default 8: e- or d-rows per work
+ * item), "use_graph" (0; 1 replays the batched pipeline as a captured hipGraph),
+ * "load_policy" (0..3, default 1: cache policy of the xor cell gathers:
  * default, nontemporal, device-scope, system-scope), "verify" (1: pair recovery
  * + exact check of hits on the host),
  * "stop_first" (0, fourcore_cls5_gpu_v4.cu:635) */
@@ -157,6 +159,16 @@ int fc5_band_create_mixed(fc5_ctx *ctx, const uint32_t *cls, const fc5_unit *uni
 int fc5_band_destroy(fc5_ctx *ctx, fc5_band *band);
 /* run one band: all counters, hits and (if "verify") solutions */
 int fc5_band_run(fc5_ctx *ctx, fc5_band *band, int mode, fc5_result *res);
+/* autotuner (no reference counterpart; the reference fixes 16 units per launch
+ * and 256 threads, fourcore_cls5_gpu_v4.cu:468,496): times the persistent
+ * kernel on every sample_every-th unit of the band for rows per work item in
+ * {4, 8, 16, 32} and two persistent grid sizes, keeps the fastest in the
+ * context options "rows_per_item" and "blocks_per_cu", and reports it */
+typedef struct {
+  int rows_per_item, blocks_per_cu, n_tried, pad;
+  double best_ms, default_ms;
+} fc5_tune;
+int fc5_autotune(fc5_ctx *ctx, fc5_band *band, int sample_every, fc5_tune *out);
 /* one-call form: create, run, destroy (run_cls5_gpu, fourcore_cls5_gpu_v4.cu:412) */
 int fc5_run_units(fc5_ctx *ctx, const fc5_unit *units, size_t n, int mode, fc5_result *res);
 /* counters of one class (2..5) in the last run */

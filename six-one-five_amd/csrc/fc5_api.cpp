@@ -63,6 +63,16 @@ struct fc5_band {
   DevUnit3* d_units3 = nullptr;
   u32 n_dev3 = 0, n_items3 = 0;
   u32 max_lim3 = 0;
+  // inputs and host images of the device units (re-numbered per rows-per-item)
+  std::vector<fc5_unit> in_units;
+  std::vector<u32> in_cls;
+  std::vector<DevUnit5> dev_h;
+  std::vector<DevUnit3> dev3_h;
+  int rows = 0;  // rows per work item the item numbering was made for
+  // captured batched pipeline
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<u64> graph_key;
 };
 
 struct fc5_ctx {
@@ -74,6 +84,9 @@ struct fc5_ctx {
   // options
   u64 hit_cap = 1u << 20;
   int use_gate = 1, verify = 1, stop_first = 0, blocks_per_cu = 0;
+  int rows_per_item = kRowsPerItem, use_graph = 0;
+  Counters* h_ctr = nullptr;  // pinned read-back of the counters
+  u64* h_nhits = nullptr;
   u32 load_policy = 1;  // cache policy of the xor cell gathers (nontemporal: no MALL allocation)
   // tables
   DeviceTable main, side;
@@ -99,8 +112,6 @@ struct fc5_ctx {
   // last results
   std::vector<fc5_hit> hits;
   std::vector<fc5_solution> sols;
-  PairRecover pr;
-  int pr_N = -1;
 };
 
 #define FAIL(ctx, code, msg)  \
@@ -164,6 +175,8 @@ extern "C" int fc5_ctx_create(int device, fc5_ctx** out) {
       hipMalloc(&ctx->d_gate_ones, kGateWordsU * 4) != hipSuccess ||
       hipMalloc(&ctx->d_ctr, 16 * sizeof(Counters)) != hipSuccess ||
       hipMalloc(&ctx->d_p6, 751 * sizeof(u16)) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_ctr, 16 * sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_nhits, 8, hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&ctx->d_nhits, 8) != hipSuccess || hipMalloc(&ctx->d_item_ctr, 8) != hipSuccess)
     return FC5_E_NOMEM;
   std::vector<u16> p6(751);
@@ -187,6 +200,8 @@ extern "C" void fc5_ctx_destroy(fc5_ctx* ctx) {
                   (void*)ctx->d_ctr, (void*)ctx->d_nhits, (void*)ctx->d_item_ctr, (void*)ctx->d_p6,
                   (void*)ctx->d_c6g})
     if (p) (void)hipFree(p);
+  if (ctx->h_ctr) (void)hipHostFree(ctx->h_ctr);
+  if (ctx->h_nhits) (void)hipHostFree(ctx->h_nhits);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   (void)hipStreamDestroy(ctx->stream);
@@ -212,6 +227,11 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) {
     ctx->load_policy = (u32)v;
     ctx->main.dev.policy = ctx->load_policy;
     ctx->side.dev.policy = ctx->load_policy;
+  } else if (k == "rows_per_item") {
+    if (v != 4 && v != 8 && v != 16 && v != 32) FAIL(ctx, FC5_E_ARG, "rows_per_item must be 4, 8, 16 or 32");
+    ctx->rows_per_item = (int)v;
+  } else if (k == "use_graph") {
+    ctx->use_graph = v ? 1 : 0;
   } else if (k == "blocks_per_cu") {
     if (v < 0 || v > 64) FAIL(ctx, FC5_E_ARG, "blocks_per_cu out of range");
     ctx->blocks_per_cu = (int)v;
@@ -272,7 +292,6 @@ static int adopt_main(fc5_ctx* ctx, XorTable&& t) {
   ctx->host_main_valid = t.packed.size() <= kKeepHostTable;
   if (ctx->host_main_valid) ctx->host_main = std::move(t);
   else ctx->host_main = XorTable{};
-  ctx->pr_N = -1;
   return FC5_OK;
 }
 
@@ -328,8 +347,7 @@ extern "C" int fc5_table_build(fc5_ctx* ctx, int N, int r, const uint64_t* extra
     t.dev.present = t.dev.block ? 1u : 0u;
     t.dev.policy = ctx->load_policy;
     t.present = true;
-    ctx->pr_N = -1;
-    return FC5_OK;
+      return FC5_OK;
   }
   std::vector<u64> ex(extra, extra + n_extra);
   XorTable t;
@@ -509,9 +527,37 @@ static int ensure_hits(fc5_ctx* ctx) {
 extern "C" int fc5_band_destroy(fc5_ctx* ctx, fc5_band* band) {
   if (!band) return FC5_OK;
   if (ctx) (void)hipSetDevice(ctx->device);
+  if (band->graph_exec) (void)hipGraphExecDestroy(band->graph_exec);
+  if (band->graph) (void)hipGraphDestroy(band->graph);
   for (void* p : {(void*)band->d_units, (void*)band->d_plain, (void*)band->d_units3})
     if (p) (void)hipFree(p);
   delete band;
+  return FC5_OK;
+}
+
+// work items: ceil(rows / E) per unit, classes 2-4 numbered after class 5
+static void number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3) {
+  n5 = 0;
+  for (DevUnit5& D : dev) {
+    D.item_base = n5;
+    n5 += (D.rows + (u32)E - 1) / (u32)E;
+  }
+  n3 = 0;
+  for (DevUnit3& D : dev3) {
+    D.item_base = n3;
+    n3 += (D.rows + (u32)E - 1) / (u32)E;
+  }
+}
+
+static int renumber(fc5_ctx* ctx, fc5_band* b) {
+  if (b->rows == ctx->rows_per_item) return FC5_OK;
+  number_items(b->dev_h, b->dev3_h, ctx->rows_per_item, b->n_items, b->n_items3);
+  if (!b->dev_h.empty())
+    HIPCHK(ctx, hipMemcpy(b->d_units, b->dev_h.data(), b->dev_h.size() * sizeof(DevUnit5), hipMemcpyHostToDevice));
+  if (!b->dev3_h.empty())
+    HIPCHK(ctx, hipMemcpy(b->d_units3, b->dev3_h.data(), b->dev3_h.size() * sizeof(DevUnit3),
+                          hipMemcpyHostToDevice));
+  b->rows = ctx->rows_per_item;
   return FC5_OK;
 }
 
@@ -522,12 +568,13 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
   HIPCHK(ctx, hipSetDevice(ctx->device));
   auto b = std::make_unique<fc5_band>();
   b->n_in = n;
+  b->in_units.assign(units, units + n);
+  for (size_t i = 0; i < n; ++i) b->in_cls.push_back(cls ? cls[i] : 5u);
   b->cls_of.assign(n, 0);
   b->slot_of.assign(n, -1);
   std::vector<DevUnit5> dev;
   std::vector<DevUnit3> dev3;
   std::vector<PlainUnit> plain;
-  u32 item = 0, item3 = 0;
   for (size_t i = 0; i < n; ++i) {
     const int c = cls ? (int)cls[i] : 5;
     if (c == 5) {
@@ -558,8 +605,6 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
         D.fmax1 = U.fmax1;
         D.fmax2 = U.fmax2;
         D.rows = 6u * (U.fmax1 / 729u + 1u);
-        D.item_base = item;
-        item += (D.rows + kRowsPerItem - 1) / kRowsPerItem;
         D.te = U.te;
         D.td = U.td;
         D.uid = (u32)i;
@@ -587,8 +632,6 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
         D.fmax = U.fmax;
         D.lim = U.lim;
         D.rows = U.nres * (U.fmax / U.mod + 1u);
-        D.item_base = item3;
-        item3 += (D.rows + kRowsPerItem - 1) / kRowsPerItem;
         D.mod = U.mod;
         D.dr = U.dr;
         D.shift = U.shift;
@@ -607,10 +650,10 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
     }
   }
   b->n_dev = (u32)dev.size();
-  b->n_items = item;
   b->n_plain = (u32)plain.size();
   b->n_dev3 = (u32)dev3.size();
-  b->n_items3 = item3;
+  number_items(dev, dev3, ctx->rows_per_item, b->n_items, b->n_items3);
+  b->rows = ctx->rows_per_item;
   int rc = FC5_OK;
   auto upload = [&](void** dst, const void* src, size_t bytes) {
     if (!bytes || rc != FC5_OK) return;
@@ -620,6 +663,8 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
   upload((void**)&b->d_units, dev.data(), dev.size() * sizeof(DevUnit5));
   upload((void**)&b->d_plain, plain.data(), plain.size() * sizeof(PlainUnit));
   upload((void**)&b->d_units3, dev3.data(), dev3.size() * sizeof(DevUnit3));
+  b->dev_h = std::move(dev);
+  b->dev3_h = std::move(dev3);
   if (rc == FC5_OK && !b->units.empty()) rc = ensure_ndg(ctx, b->max_fmax2);
   if (rc == FC5_OK && !b->units3.empty()) rc = ensure_c6g(ctx, b->max_lim3 + 64);
   if (rc != FC5_OK) {
@@ -772,6 +817,75 @@ static int run_plain_find3(fc5_ctx* ctx, fc5_band* b, const std::vector<u32>& wh
   return rc;
 }
 
+static BatchedArgs batched_args(fc5_ctx* ctx, fc5_band* b) {
+  BatchedArgs a{};
+  a.units = b->d_units;
+  a.n_units = b->n_dev;
+  a.n_items = b->n_items;
+  a.gate = ctx->use_gate ? ctx->d_gate : ctx->d_gate_ones;
+  a.ndg = ctx->d_ndg;
+  a.ndg_K = ctx->ndg_K;
+  a.use_gate = (u32)ctx->use_gate;
+  a.main = ctx->main.dev;
+  a.side = ctx->side.present ? ctx->side.dev : XorDev{};
+  a.hits = ctx->d_hits;
+  a.hit_cap = ctx->hit_cap;
+  a.nhits = ctx->d_nhits;
+  a.ctr = ctx->d_ctr;
+  a.item_ctr = ctx->d_item_ctr;
+  a.inv729 = inv_mod_2_64(729);
+  a.units3 = b->d_units3;
+  a.n_units3 = b->n_dev3;
+  a.n_items3 = b->n_items3;
+  a.c6g = ctx->d_c6g;
+  return a;
+}
+
+// counters, hit slot and work-queue counter cleared by a fill kernel (no
+// memset nodes, so the same sequence is valid inside a captured graph)
+static int reset_counters(fc5_ctx* ctx) {
+  if (launch_fill_u64((u64*)ctx->d_ctr, 16 * sizeof(Counters) / 8, 0, ctx->stream) != 0 ||
+      launch_fill_u64(ctx->d_nhits, 1, 0, ctx->stream) != 0 ||
+      launch_fill_u64((u64*)ctx->d_item_ctr, 1, 0, ctx->stream) != 0)
+    FAIL(ctx, FC5_E_HIP, "counter reset");
+  return FC5_OK;
+}
+
+// capture reset + persistent kernel + counter read-back as one graph per band;
+// captured again whenever a baked argument (tables, options, grid) changes
+static int ensure_graph(fc5_ctx* ctx, fc5_band* b, const BatchedArgs& a, int grid) {
+  std::vector<u64> key(sizeof(BatchedArgs) / 8 + 2);
+  std::memcpy(key.data(), &a, sizeof(BatchedArgs));
+  key[key.size() - 2] = (u64)grid;
+  key[key.size() - 1] = (u64)ctx->rows_per_item;
+  if (b->graph_exec && key == b->graph_key) return FC5_OK;
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = reset_counters(ctx);
+  if (rc == FC5_OK && launch_cls5_batched(a, grid, ctx->rows_per_item, ctx->stream) != 0) rc = FC5_E_HIP;
+  if (rc == FC5_OK &&
+      (hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, 16 * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream) !=
+           hipSuccess ||
+       hipMemcpyAsync(ctx->h_nhits, ctx->d_nhits, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess))
+    rc = FC5_E_HIP;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc != FC5_OK || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    FAIL(ctx, FC5_E_HIP, "graph capture failed");
+  }
+  if (hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    FAIL(ctx, FC5_E_HIP, "graph instantiate failed");
+  }
+  b->graph = g;
+  b->graph_key = key;
+  return FC5_OK;
+}
+
 extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res) {
   if (!ctx || !b || !res) return FC5_E_ARG;
   if (mode != FC5_MODE_PLAIN && mode != FC5_MODE_BATCHED) FAIL(ctx, FC5_E_ARG, "unknown mode");
@@ -785,58 +899,49 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   ctx->hits.clear();
   ctx->sols.clear();
   u64 fail_host[6] = {0}, jobs_host[6] = {0};
-  HIPCHK(ctx, hipMemsetAsync(ctx->d_ctr, 0, 16 * sizeof(Counters), ctx->stream));
-  HIPCHK(ctx, hipMemsetAsync(ctx->d_nhits, 0, 8, ctx->stream));
-  HIPCHK(ctx, hipMemsetAsync(ctx->d_item_ctr, 0, 8, ctx->stream));
+  rc = renumber(ctx, b);
+  if (rc != FC5_OK) return rc;
+  const bool batched = mode == FC5_MODE_BATCHED;
+  const bool any_items = b->n_items + b->n_items3 > 0;
+  BatchedArgs a = batched_args(ctx, b);
+  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(ctx->rows_per_item);
+  const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)(b->n_items + b->n_items3)));
+  const bool graph = batched && any_items && ctx->use_graph && b->irregular.empty() && b->irregular3.empty();
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  if (mode == FC5_MODE_BATCHED) {
-    if (b->n_items + b->n_items3) {
-      BatchedArgs a{};
-      a.units = b->d_units;
-      a.n_units = b->n_dev;
-      a.n_items = b->n_items;
-      a.gate = ctx->use_gate ? ctx->d_gate : ctx->d_gate_ones;
-      a.ndg = ctx->d_ndg;
-      a.ndg_K = ctx->ndg_K;
-      a.use_gate = (u32)ctx->use_gate;
-      a.main = ctx->main.dev;
-      a.side = ctx->side.present ? ctx->side.dev : XorDev{};
-      a.hits = ctx->d_hits;
-      a.hit_cap = ctx->hit_cap;
-      a.nhits = ctx->d_nhits;
-      a.ctr = ctx->d_ctr;
-      a.item_ctr = ctx->d_item_ctr;
-      a.inv729 = inv_mod_2_64(729);
-      a.units3 = b->d_units3;
-      a.n_units3 = b->n_dev3;
-      a.n_items3 = b->n_items3;
-      a.c6g = ctx->d_c6g;
-      const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu();
-      const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)(b->n_items + b->n_items3)));
-      if (launch_cls5_batched(a, grid, ctx->stream) != 0) FAIL(ctx, FC5_E_HIP, "batched launch");
-    }
-    if (!b->irregular.empty()) {
-      rc = run_plain_units(ctx, b, b->irregular, false);
-      if (rc != FC5_OK) return rc;
-    }
-    if (!b->irregular3.empty()) {
-      rc = run_plain_find3(ctx, b, b->irregular3, fail_host, jobs_host);
-      if (rc != FC5_OK) return rc;
-    }
-  } else {
-    if (b->n_plain) {
-      rc = run_plain_units(ctx, b, {}, true);
-      if (rc != FC5_OK) return rc;
-    }
-    rc = run_plain_find3(ctx, b, {}, fail_host, jobs_host);
+  if (graph) {
+    rc = ensure_graph(ctx, b, a, grid);
     if (rc != FC5_OK) return rc;
+    HIPCHK(ctx, hipGraphLaunch(b->graph_exec, ctx->stream));
+  } else {
+    rc = reset_counters(ctx);
+    if (rc != FC5_OK) return rc;
+    if (batched) {
+      if (any_items && launch_cls5_batched(a, grid, ctx->rows_per_item, ctx->stream) != 0)
+        FAIL(ctx, FC5_E_HIP, "batched launch");
+      if (!b->irregular.empty()) {
+        rc = run_plain_units(ctx, b, b->irregular, false);
+        if (rc != FC5_OK) return rc;
+      }
+      if (!b->irregular3.empty()) {
+        rc = run_plain_find3(ctx, b, b->irregular3, fail_host, jobs_host);
+        if (rc != FC5_OK) return rc;
+      }
+    } else {
+      if (b->n_plain) {
+        rc = run_plain_units(ctx, b, {}, true);
+        if (rc != FC5_OK) return rc;
+      }
+      rc = run_plain_find3(ctx, b, {}, fail_host, jobs_host);
+      if (rc != FC5_OK) return rc;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, 16 * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_nhits, ctx->d_nhits, 8, hipMemcpyDeviceToHost, ctx->stream));
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-  Counters c[16];
-  u64 nh = 0;
-  HIPCHK(ctx, hipMemcpyAsync(c, ctx->d_ctr, 16 * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(&nh, ctx->d_nhits, 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  Counters c[16];
+  std::memcpy(c, ctx->h_ctr, sizeof c);
+  const u64 nh = *ctx->h_nhits;
   float ms = 0;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   const u64 stored = std::min<u64>(nh, ctx->hit_cap);
@@ -887,20 +992,16 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   });
   if (ctx->verify && !ctx->hits.empty()) {
     const int N = ctx->main.hdr.N ? (int)ctx->main.hdr.N : (int)std::max(b->max_lim, b->max_lim3);
-    if (ctx->pr_N != N) {
-      ctx->pr.build(N);
-      ctx->pr_N = N;
-    }
     std::vector<Solution> sols;
     for (const fc5_hit& h : ctx->hits) {
       if (!(h.flags & 1u) || h.unit >= b->n_in) continue;
       const int k = b->cls_of[h.unit], slot = b->slot_of[h.unit];
       if (k == 5) {
         const Cls5Unit& U = b->units[(size_t)slot];
-        verify_cls5_hit(ctx->pr, U.B, U.u, U.lim, h.d, h.e, h.fp, sols);
+        verify_cls5_hit(N, U.B, U.u, U.lim, h.d, h.e, h.fp, sols);
       } else if (k >= 2) {
         const Cls234Unit& U = b->units3[(size_t)slot];
-        verify_find3_hit(ctx->pr, k, U.B, U.u, h.d, h.e, h.fp, sols);
+        verify_find3_hit(N, k, U.B, U.u, h.d, h.e, h.fp, sols);
       }
       if (ctx->stop_first && !sols.empty()) break;
     }
@@ -925,6 +1026,55 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
     ctx->err = "hit buffer overflow: " + std::to_string(res->hits_dropped) + " hits dropped";
     return FC5_E_OVERFLOW;
   }
+  return FC5_OK;
+}
+
+extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tune* out) {
+  if (!ctx || !b || !out || sample_every < 1) return FC5_E_ARG;
+  std::vector<fc5_unit> su;
+  std::vector<u32> sc;
+  for (size_t i = 0; i < b->n_in; i += (size_t)sample_every) {
+    su.push_back(b->in_units[i]);
+    sc.push_back(b->in_cls[i]);
+  }
+  fc5_band* sb = nullptr;
+  int rc = band_create(ctx, sc.data(), su.data(), su.size(), &sb);
+  if (rc != FC5_OK) return rc;
+  const int keep_verify = ctx->verify, keep_rows = ctx->rows_per_item, keep_bpc = ctx->blocks_per_cu;
+  ctx->verify = 0;
+  std::memset(out, 0, sizeof *out);
+  out->best_ms = 1e30;
+  fc5_result r;
+  for (int E : {8, 4, 16, 32}) {
+    const int occ = batched_blocks_per_cu(E);
+    for (int bpc : {occ, std::max(1, occ / 2)}) {
+      ctx->rows_per_item = E;
+      ctx->blocks_per_cu = bpc;
+      double ms = 1e30;
+      for (int rep = 0; rep < 2 && rc == FC5_OK; ++rep) {
+        rc = fc5_band_run(ctx, sb, FC5_MODE_BATCHED, &r);
+        ms = std::min(ms, r.kernel_ms);
+      }
+      if (rc != FC5_OK) break;
+      ++out->n_tried;
+      if (E == kRowsPerItem && bpc == occ) out->default_ms = ms;
+      if (ms < out->best_ms) {
+        out->best_ms = ms;
+        out->rows_per_item = E;
+        out->blocks_per_cu = bpc;
+      }
+    }
+    if (rc != FC5_OK) break;
+  }
+  ctx->verify = keep_verify;
+  fc5_band_destroy(ctx, sb);
+  if (rc != FC5_OK) {
+    ctx->rows_per_item = keep_rows;
+    ctx->blocks_per_cu = keep_bpc;
+    return rc;
+  }
+  ctx->rows_per_item = out->rows_per_item;
+  ctx->blocks_per_cu = out->blocks_per_cu;
   return FC5_OK;
 }
 

@@ -718,40 +718,26 @@ bool verify_cls234(u64 B, u64 u, u64 f, u64 d, u64 a, u64 b, u64 c) {
 // ---------------------------------------------------------------------------
 // pair recovery / verify
 // ---------------------------------------------------------------------------
-void PairRecover::build(int N) {
-  N_ = N;
-  sorted_.resize((size_t)N);
-  for (int x = 1; x <= N; ++x) {
-    const u64 x2 = (u64)x * (u64)x;
-    sorted_[(size_t)x - 1] = {x2 * x2 * x2, (u32)x};
-  }
-  std::sort(sorted_.begin(), sorted_.end());
-}
-
-void PairRecover::recover(u64 fp, std::vector<std::pair<u32, u32>>& out) const {
+void exact_pairs(u128 T, u32 hi, std::vector<std::pair<u32, u32>>& out) {
   out.clear();
-  for (int i = 1; i <= N_; ++i) {
-    const u64 i2 = (u64)i * (u64)i;
-    const u64 need = fp - i2 * i2 * i2;
-    auto it = std::lower_bound(sorted_.begin(), sorted_.end(), std::make_pair(need, (u32)0));
-    for (; it != sorted_.end() && it->first == need; ++it)
-      if ((u32)i <= it->second) out.emplace_back((u32)i, it->second);
+  auto p6 = [](u64 x) { const u128 x2 = (u128)x * x; return x2 * x2 * x2; };
+  u64 b = std::min<u64>(hi, iroot6(T));
+  for (u64 a = 1; a <= b; ++a) {
+    const u128 pa = p6(a);
+    if (pa > T) break;
+    while (b >= a && pa + p6(b) > T) --b;
+    if (b < a) break;
+    if (pa + p6(b) == T) out.emplace_back((u32)a, (u32)b);
   }
 }
 
-int verify_cls5_hit(const PairRecover& pr, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp,
-                    std::vector<Solution>& out) {
+int verify_cls5_hit(int N, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp, std::vector<Solution>& out) {
+  u128 T;
+  if (!exact_T_cls5(B, u, d, e, T) || (u64)T != fp) return 0;
   std::vector<std::pair<u32, u32>> pairs;
-  pr.recover(fp, pairs);
+  exact_pairs(T, std::min<u32>(lim, (u32)N), pairs);
   int found = 0;
   for (auto [a, b] : pairs) {
-    if (a > lim || b > lim) continue;
-    u128 T;
-    if (!exact_T_cls5(B, u, d, e, T)) continue;
-    const U256 pa = U256::pow6(a), pb = U256::pow6(b);
-    U256 s = pa;
-    add_in_place(s, pb);
-    if (!s.fits128() || s.lo128() != T) continue;
     if (!verify_cls5(B, u, d, e, a, b)) continue;
     i64 t[5] = {42LL * a, 42LL * b, 21LL * (i64)d, 14LL * (i64)e, (i64)u};
     std::sort(t, t + 5);
@@ -761,8 +747,7 @@ int verify_cls5_hit(const PairRecover& pr, u64 B, u64 u, u32 lim, u32 d, u32 e, 
     i64 g = t[0];
     for (int i = 1; i < 5; ++i) g = gcd_i64(g, t[i]);
     g = gcd_i64(g, (i64)B);
-    Solution S{B, u, 5, d, e, a, b, 0, {t[0], t[1], t[2], t[3], t[4]}, g};
-    out.push_back(S);
+    out.push_back(Solution{B, u, 5, d, e, a, b, 0, {t[0], t[1], t[2], t[3], t[4]}, g});
     ++found;
   }
   return found;
@@ -771,20 +756,17 @@ int verify_cls5_hit(const PairRecover& pr, u64 B, u64 u, u32 lim, u32 d, u32 e, 
 }  // namespace fc5
 
 namespace fc5 {
-int verify_find3_hit(const PairRecover& pr, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp,
-                     std::vector<Solution>& out) {
+int verify_find3_hit(int N, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp, std::vector<Solution>& out) {
   const u64 f = (u64)free_factor(cls);
   u128 T;
-  if (!exact_T_cls234(B, u, f, d, T)) return 0;
+  if (c < 1 || !exact_T_cls234(B, u, f, d, T)) return 0;
+  const u64 c3 = (u64)c * c * c;
+  const u128 c6 = (u128)c3 * c3;
+  if (c6 > T || (u64)(T - c6) != fp) return 0;
   std::vector<std::pair<u32, u32>> pairs;
-  pr.recover(fp, pairs);
+  exact_pairs(T - c6, std::min<u32>(c, (u32)N), pairs);  // b <= c (fourcore_find_v4.cu:548)
   int found = 0;
   for (auto [a, b] : pairs) {
-    if (c < 1 || b > c) continue;
-    U256 s = U256::pow6(a);
-    add_in_place(s, U256::pow6(b));
-    add_in_place(s, U256::pow6(c));
-    if (!s.fits128() || s.lo128() != T) continue;
     if (!verify_cls234(B, u, f, d, a, b, c)) continue;
     i64 t[5] = {42LL * a, 42LL * b, 42LL * c, (i64)f * (i64)d, (i64)u};
     std::sort(t, t + 5);

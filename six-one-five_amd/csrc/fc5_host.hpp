@@ -199,23 +199,15 @@ struct Solution {
   i64 terms[5];
   i64 gcd;
 };
-class PairRecover {
- public:
-  void build(int N);
-  // all (a<=b<=N) with a^6 + b^6 == fp (mod 2^64), a ascending
-  void recover(u64 fp, std::vector<std::pair<u32, u32>>& out) const;
-  int N() const { return N_; }
-
- private:
-  int N_ = 0;
-  std::vector<std::pair<u64, u32>> sorted_;  // (x^6 mod 2^64, x)
-};
-// full verify chain for one cls5 maybe-hit; appends solutions
-int verify_cls5_hit(const PairRecover& pr, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp,
-                    std::vector<Solution>& out);
+// Exact pairs a <= b <= hi with a^6 + b^6 == T, by two pointers over x^6.
+// The reference recovers every pair with a^6+b^6 = fp (mod 2^64) and then keeps
+// the exact ones (fourcore_cls5_gpu_v4.cu:517-522); fp = T mod 2^64 for every
+// hit, so the exact pairs are the same set.
+void exact_pairs(u128 T, u32 hi, std::vector<std::pair<u32, u32>>& out);
+// full verify chain for one cls5 maybe-hit (table index cap N); appends solutions
+int verify_cls5_hit(int N, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp, std::vector<Solution>& out);
 // find3 maybe-hit of class cls at (d, c3) (fourcore_find_v4.cu:535-551, :427-457)
-int verify_find3_hit(const PairRecover& pr, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp,
-                     std::vector<Solution>& out);
+int verify_find3_hit(int N, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp, std::vector<Solution>& out);
 
 // splitmix64 (Vigna), the generator for planted keys
 inline u64 splitmix64(u64& s) {

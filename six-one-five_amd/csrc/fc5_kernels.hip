@@ -158,11 +158,12 @@ __device__ __forceinline__ bool gate_pass(const u32* s_gate, u32 xg, u32 n) {
   return (s_gate[t >> 5] >> (t & 31u)) & 1u;
 }
 
+template <int E>
 __device__ __forceinline__ void flush_counters(Counters* c, u32 lane, u32 calls, u32 neg, u32 zero, u32 pos,
                                                u32 probes, u32 jobs, u32 failT) {
   u64 s_calls = 0, s_neg = 0, s_zero = 0, s_pos = 0, s_jobs = 0, s_fail = 0;
 #pragma unroll
-  for (int i = 0; i < kRowsPerItem; ++i) {
+  for (int i = 0; i < E; ++i) {
     s_calls += rdl(calls, i);
     s_neg += rdl(neg, i);
     s_zero += rdl(zero, i);
@@ -181,7 +182,8 @@ __device__ __forceinline__ void flush_counters(Counters* c, u32 lane, u32 calls,
   }
 }
 
-// one class-5 item: 8 e-rows of one unit, every d
+// one class-5 item: E e-rows of one unit, every d
+template <int E>
 __device__ __forceinline__ void cls5_item(const BatchedArgs& a, const u32* s_gate, Ring& q, u32 item, u32 lane) {
   const u32 K = a.ndg_K;
   u32 lo = 0, hi = a.n_units;
@@ -192,14 +194,14 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, const u32* s_gat
   }
   const u32 ui = lo;
   const DevUnit5 U = a.units[ui];
-  const u32 row0 = (item - U.item_base) * (u32)kRowsPerItem;
+  const u32 row0 = (item - U.item_base) * (u32)E;
   const u32 r0 = U.res2[0], r1 = U.res2[1], r2 = U.res2[2], r3 = U.res2[3];
   const u32 minr = min(min(r0, r1), min(r2, r3));
 
-  // per-row set-up on lanes 0..7: e, T-space row value, positive cut
+  // per-row set-up on lanes 0..E-1: e, T-space row value, positive cut
   u32 e = 0, dlim = 0, xg = 0;
   u32 c_calls = 0, c_neg = 0, c_zero = 0, c_pos = 0;
-  if (lane < (u32)kRowsPerItem) {
+  if (lane < (u32)E) {
     const u32 r = row0 + lane;
     if (r < U.rows) {
       e = (u32)a.units[ui].res1[r % 6u] + (r / 6u) * 729u;
@@ -237,7 +239,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, const u32* s_gat
   }
   u32 maxdl = 0;
 #pragma unroll
-  for (int i = 0; i < kRowsPerItem; ++i) maxdl = max(maxdl, rdl(dlim, i));
+  for (int i = 0; i < E; ++i) maxdl = max(maxdl, rdl(dlim, i));
 
   // gate over the positive part of the rows; lanes run over k
   u32 probes_item = 0;
@@ -249,7 +251,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, const u32* s_gat
     const u32 n2 = d2 <= U.fmax2 ? a.ndg[r2 * K + k] : 0u;
     const u32 n3 = d3 <= U.fmax2 ? a.ndg[r3 * K + k] : 0u;
     const u32 dmin_blk = minr + 64u * kb;
-    for (int i = 0; i < kRowsPerItem; ++i) {
+    for (int i = 0; i < E; ++i) {
       const u32 dl = rdl(dlim, i);
       if (dl < dmin_blk) continue;
       const u32 xgi = rdl(xg, i);
@@ -263,11 +265,12 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, const u32* s_gat
       }
     }
   }
-  flush_counters(a.ctr + U.band, lane, c_calls, c_neg, c_zero, c_pos, probes_item, 0u, 0u);
+  flush_counters<E>(a.ctr + U.band, lane, c_calls, c_neg, c_zero, c_pos, probes_item, 0u, 0u);
 }
 
-// one class 2-4 item: 8 d-rows of one unit, every c3 of each row's window
+// one class 2-4 item: E d-rows of one unit, every c3 of each row's window
 // (fourcore_find_v4.cu:165-205 expansion, :401-425 window, :255-284 sweep)
+template <int E>
 __device__ __forceinline__ void find3_item(const BatchedArgs& a, const u32* s_gate, Ring& q, u32 item, u32 lane) {
   u32 lo = 0, hi = a.n_units3;
   while (hi - lo > 1) {
@@ -277,10 +280,10 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, const u32* s_ga
   }
   const u32 ui = lo;
   const DevUnit3* Up = a.units3 + ui;
-  const u32 row0 = (item - Up->item_base) * (u32)kRowsPerItem;
+  const u32 row0 = (item - Up->item_base) * (u32)E;
   u32 d = 0, wlo = 1, whi = 0, xg = 0;
   u32 c_calls = 0, c_jobs = 0, c_fail = 0;
-  if (lane < (u32)kRowsPerItem) {
+  if (lane < (u32)E) {
     const u32 r = row0 + lane;
     if (r < Up->rows) {
       d = (u32)Up->res[r % Up->nres] + (r / Up->nres) * Up->mod;
@@ -315,7 +318,7 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, const u32* s_ga
     }
   }
   u32 probes_item = 0;
-  for (int i = 0; i < kRowsPerItem; ++i) {
+  for (int i = 0; i < E; ++i) {
     const u32 l = rdl(wlo, i), h = rdl(whi, i);
     if (l > h) continue;
     const u32 xgi = rdl(xg, i), di = rdl(d, i);
@@ -328,9 +331,10 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, const u32* s_ga
       if (h - cb < 64u) break;  // cb + 64 could wrap past 2^32
     }
   }
-  flush_counters(a.ctr + Up->band, lane, c_calls, 0u, 0u, c_calls, probes_item, c_jobs, c_fail);
+  flush_counters<E>(a.ctr + Up->band, lane, c_calls, 0u, 0u, c_calls, probes_item, c_jobs, c_fail);
 }
 
+template <int E>
 __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
   __shared__ u32 s_gate[kGateWords];
   __shared__ u32 s_qu[kWaves][kRing], s_qd[kWaves][kRing], s_qe[kWaves][kRing];
@@ -346,8 +350,8 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
     if (lane == 0) item = atomicAdd(a.item_ctr, 1u);
     item = __shfl(item, 0, 64);
     if (item >= total) break;
-    if (item < a.n_items) cls5_item(a, s_gate, q, item, lane);
-    else find3_item(a, s_gate, q, item - a.n_items, lane);
+    if (item < a.n_items) cls5_item<E>(a, s_gate, q, item, lane);
+    else find3_item<E>(a, s_gate, q, item - a.n_items, lane);
   }
   if (q.tail != q.head) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -549,8 +553,14 @@ static int grid_for(u64 n, int threads) {
   return (int)(g < 65536 ? (g ? g : 1) : 65536);
 }
 
-int launch_cls5_batched(const BatchedArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_cls5_batched, dim3(grid), dim3(64 * kWaves), 0, s, a);
+int launch_cls5_batched(const BatchedArgs& a, int grid, int rows, hipStream_t s) {
+  switch (rows) {
+    case 4: hipLaunchKernelGGL(k_cls5_batched<4>, dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_cls5_batched<8>, dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 16: hipLaunchKernelGGL(k_cls5_batched<16>, dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 32: hipLaunchKernelGGL(k_cls5_batched<32>, dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    default: return (int)hipErrorInvalidValue;
+  }
   return (int)hipGetLastError();
 }
 
@@ -579,10 +589,17 @@ int launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
-int batched_blocks_per_cu() {
+int batched_blocks_per_cu(int rows) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched, 64 * kWaves, 0) != hipSuccess)
-    return 4;
+  hipError_t e = hipErrorInvalidValue;
+  switch (rows) {
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<4>, 64 * kWaves, 0); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<8>, 64 * kWaves, 0); break;
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<16>, 64 * kWaves, 0); break;
+    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<32>, 64 * kWaves, 0); break;
+    default: break;
+  }
+  if (e != hipSuccess) return 4;
   return nb > 0 ? nb : 1;
 }
 

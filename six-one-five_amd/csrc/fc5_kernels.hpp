@@ -148,16 +148,16 @@ struct PlainArgs {
   Counters* ctr;
 };
 
-constexpr int kRowsPerItem = 8;  // e rows per work item of the batched kernel
+constexpr int kRowsPerItem = 8;  // default rows (e- or d-rows) per work item of the batched kernel
 
 // launchers (return hipError_t as int)
-int launch_cls5_batched(const BatchedArgs& a, int grid, hipStream_t s);
+int launch_cls5_batched(const BatchedArgs& a, int grid, int rows, hipStream_t s);  // rows: 4, 8, 16, 32
 int launch_cls5_plain(const PlainArgs& a, int n_units, int ymax, hipStream_t s);
 int launch_find3_plain(const PlainF3Args& a, hipStream_t s);
 int launch_expand_cells(const u32* packed_words, u64 m_cells, u32 r, u64* cells, hipStream_t s);
 int launch_query(const XorDev& x, const u64* keys, u64 n, u8* out, hipStream_t s);
 int launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s);
-int batched_blocks_per_cu();
+int batched_blocks_per_cu(int rows);
 int launch_probe_only(const XorDev& x, u64 n, u64 salt, unsigned long long* sink, int grid, hipStream_t s);
 int launch_gather_raw(const u64* cells, u64 m, u64 n, u64 salt, u32 policy, unsigned long long* sink, int grid,
                       hipStream_t s);

@@ -49,6 +49,11 @@ class Result(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Tune(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("rows_per_item", "blocks_per_cu", "n_tried", "pad")] + \
+               [("best_ms", C.c_double), ("default_ms", C.c_double)]
+
+
 class Hit(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("unit", "d", "e", "flags")] + [("fp", C.c_uint64)]
 
@@ -80,6 +85,7 @@ _sig = {
     "fc5_band_create_mixed": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(_P)]),
     "fc5_band_destroy": (C.c_int, [_P, _P]),
     "fc5_last_class_result": (C.c_int, [_P, C.c_int, C.POINTER(Result)]),
+    "fc5_autotune": (C.c_int, [_P, _P, C.c_int, C.POINTER(Tune)]),
     "fc5_band_run": (C.c_int, [_P, _P, C.c_int, C.POINTER(Result)]),
     "fc5_run_units": (C.c_int, [_P, _P, C.c_size_t, C.c_int, C.POINTER(Result)]),
     "fc5_last_hits": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -185,6 +191,11 @@ class Band:
         out = RunOutput(res.as_dict(), self.ctx.last_hits(), self.ctx.last_solutions(), rc)
         out.by_class = {k: self.ctx.class_result(k) for k in (2, 3, 4, 5)}
         return out
+
+    def autotune(self, sample_every=16):
+        t = Tune()
+        self.ctx._check(_lib.fc5_autotune(self.ctx.h, self.h, sample_every, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tune._fields_ if f != "pad"}
 
     def close(self):
         if self.h:

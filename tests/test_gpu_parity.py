@@ -366,3 +366,67 @@ def test_find3_plants_recovered(ctx):
         ctx.set_option("use_gate", 1)
     found = {(int(h["unit"]), int(h["d"]), int(h["e"])) for h in out.hits if h["flags"] & 2}
     assert {(i, d, c) for i, d, c, _ in pts} <= found
+
+
+def _same_run(x, y):
+    for k in (2, 3, 4, 5):
+        for key in F3_KEYS + ("neg", "zero", "side_maybe"):
+            assert x.by_class[k][key] == y.by_class[k][key], (k, key)
+    hx = np.sort(x.hits, order=["unit", "e", "d", "flags"])
+    hy = np.sort(y.hits, order=["unit", "e", "d", "flags"])
+    assert np.array_equal(hx, hy)
+
+
+def test_graph_replays_match_eager(ctx):
+    """the batched pipeline captured as a hipGraph, replayed several times with
+    eager work between the replays, equals the eager result every time; an
+    option change forces a re-capture"""
+    say("graph replay")
+    cls, Bs, us = fc5.emit_units(100001, 102000, classes=(2, 3, 4, 5))
+    units = fc5.units_array(Bs, us)
+    _load(ctx, 2428, 16)
+    b = ctx.band(units, cls)
+    other = ctx.band(units[::3], cls[::3])
+    eager = b.run()
+    eager_other = other.run()
+    ctx.set_option("use_graph", 1)
+    try:
+        for rep in range(4):
+            _same_run(b.run(), eager)
+            ctx.set_option("use_graph", 0)
+            _same_run(other.run(), eager_other)          # eager work between replays
+            assert ctx.table_query(np.arange(1, 1000, dtype=np.uint64)).sum() >= 0
+            ctx.set_option("use_graph", 1)
+        ctx.set_option("use_gate", 0)
+        ctx.set_option("use_graph", 0)
+        eager_ng = b.run()
+        ctx.set_option("use_graph", 1)
+        _same_run(b.run(), eager_ng)                      # re-captured for the new option
+        _same_run(b.run(), eager_ng)
+    finally:
+        ctx.set_option("use_gate", 1)
+        ctx.set_option("use_graph", 0)
+        b.close()
+        other.close()
+
+
+def test_tile_shapes_and_autotune_keep_results(ctx):
+    """every rows-per-item instantiation and the tuned choice give the same
+    counters and hits"""
+    say("tile shapes / autotune")
+    cls, Bs, us = fc5.emit_units(2000001, 2000300, classes=(2, 3, 4, 5))
+    _load(ctx, 2428, 16)
+    b = ctx.band(fc5.units_array(Bs, us), cls)
+    base = b.run()
+    try:
+        for E in (4, 16, 32):
+            ctx.set_option("rows_per_item", E)
+            _same_run(b.run(), base)
+        tune = b.autotune(sample_every=3)
+        assert tune["rows_per_item"] in (4, 8, 16, 32) and tune["n_tried"] == 8
+        assert tune["best_ms"] <= tune["default_ms"]
+        _same_run(b.run(), base)
+    finally:
+        ctx.set_option("rows_per_item", 8)
+        ctx.set_option("blocks_per_cu", 0)
+        b.close()

@@ -12,3 +12,14 @@ def test_moddiv_matches_operator(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout
     assert "moddiv ok" in out.stdout
+
+
+def test_exact_pairs_and_verify_reject(tmp_path):
+    exe = str(tmp_path / "pairs_check")
+    csrc = os.path.join(PKG, "csrc")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-I", csrc,
+                    os.path.join(REPO, "tests", "native", "pairs_check.cpp"), os.path.join(csrc, "fc5_host.cpp"),
+                    "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout
+    assert "pairs ok" in out.stdout
