@@ -126,12 +126,13 @@ def main():
     for _ in range(args.warmup):
         check(band.run(fc5.MODE_BATCHED))
     log(f"[bench] setup {time.time() - t_setup:.1f}s; timing {args.steps} steps")
-    kms, outs = [], []
+    kms, wms, outs = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         out = band.run(fc5.MODE_BATCHED)
         outs.append(out)
         kms.append(out.result["kernel_ms"])
+        wms.append(out.result["wall_ms"])
     wall = time.perf_counter() - t0
     recovered = [check(o) for o in outs]
     r = outs[-1].result
@@ -143,7 +144,8 @@ def main():
     kern_s = float(np.mean(kms)) / 1000.0
     cell_bytes = math.ceil(args.r / 8)
     alg_bytes = r["probes"] * 3 * cell_bytes
-    log(f"[bench] {ms_step:.1f} ms/step, kernel {1000 * kern_s:.1f} ms, {value:.4e} points/s, "
+    log(f"[bench] {ms_step:.1f} ms/step (library {float(np.mean(wms)):.1f} ms), kernel {1000 * kern_s:.1f} ms, "
+        f"{value:.4e} points/s, "
         f"probes={r['probes']} maybe={r['maybe']} side_maybe={r['side_maybe']}")
 
     ctx.set_option("use_graph", 0)

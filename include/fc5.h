@@ -179,6 +179,62 @@ int fc5_last_solutions(fc5_ctx *ctx, fc5_solution *out, size_t cap, size_t *n);
 /* closed-form calls of a unit list (the counter `calls`, fourcore_cls5_gpu_v4.cu:383-386) */
 int fc5_units_calls(const fc5_unit *units, size_t n, uint64_t *calls);
 
+/* ---- (6,1,7) Branch-A class-5 twin (solve_617_v1.cu) ----
+ * B^6 = u^6 + w^6 + (21d)^6 + 42^6 (c1^6 + c2^6 + c3^6 + c4^6) with u = 14u'
+ * (the term not divisible by 3), w = 6v' (the term not divisible by 7) and the
+ * grid term 21d (the odd term).  Options: "use_gate" (shared; --no-gate),
+ * "twin_tri_gate" (1; 0 = --no-tri-gate), "twin_c4_div" (3 = the reference
+ * kernel's c4 lower bound 3 c4^6 >= T, solve_617_v1.cu:587; 4 = the exact
+ * bound 4 c4^6 >= T for the largest of four cores), "hit_cap", "verify". */
+
+/* candidate (GpuCand of class 5, solve_617_v1.cu:354-363): base = B^6 - u^6 - w^6
+ * (< 2^127), grid d = res[i] + mod1 * k in [1, fmax1], cores <= lim.  mod1 must be
+ * 64 and every residue must have the same sixth power mod 64 (true for the
+ * generator's candidates); lim < 65536 (B <= 2,353,973, solve_617_v1.cu:146). */
+typedef struct {
+  uint64_t B, u, w, base_lo, base_hi;
+  uint32_t lim, fmax1, mod1, nres1;
+  uint32_t res[8];
+} fc5_twin_cand;
+
+/* counters (solve_617_v1.cu:386-390): jobs = (cand, d) cells with a non-empty c4
+ * window; c4_total / c4_tri_skipped = tri_total / tri_skipped; calls = probe()
+ * invocations (pair gate passed); gated = c3 values the pair gate removed;
+ * probes = table cell reads (3 per call); maybe = table maybe-hits */
+typedef struct {
+  uint64_t jobs, c4_total, c4_tri_skipped, calls, gated, probes, maybe;
+  uint64_t n_hits, hits_dropped, solutions, cands, items;
+  double kernel_ms, wall_ms;
+} fc5_twin_result;
+
+/* a maybe-hit (Hit, solve_617_v1.cu:365: c = c3, d = c4, e = grid d) */
+typedef struct {
+  uint32_t cand, d, c4, c3;
+  uint64_t fp;
+} fc5_twin_hit;
+
+/* a verified solution "SOLUTION cls=5 B= a1..a7 (anchor=u) gcd=" (solve_617_v1.cu:1096-1108):
+ * pair (a, b) from the table, cores c3, c4, grid d; terms sorted ascending */
+typedef struct {
+  uint64_t B, u, w;
+  uint32_t d, c4, c3, a, b, pad;
+  int64_t terms[7];
+  int64_t gcd;
+} fc5_twin_solution;
+
+/* host generator gen_class_cands(cls 5) for one B (solve_617_v1.cu:807-812,885-906):
+ * u' ascending, then v' ascending.  Writes up to cap, *n gets the full count. */
+int fc5_twin_cands(uint64_t B, double u_lo, double u_hi, fc5_twin_cand *out, size_t cap, size_t *n);
+/* run candidates (gpu_run with k_find4_cls5, then expand_xor_hits + verify_hit,
+ * solve_617_v1.cu:924-1006,1019-1110) in one persistent launch */
+int fc5_twin_run(fc5_ctx *ctx, const fc5_twin_cand *cands, size_t n, fc5_twin_result *res);
+/* Branch A class 5 over B in [B_lo, B_hi] (B odd, 3 and 7 not dividing B): the
+ * campaign loop of solve_617_v1.cu:1895-1950 for classes = "5" */
+int fc5_twin_run_range(fc5_ctx *ctx, uint64_t B_lo, uint64_t B_hi, double u_lo, double u_hi,
+                       fc5_twin_result *res);
+int fc5_twin_last_hits(fc5_ctx *ctx, fc5_twin_hit *out, size_t cap, size_t *n);
+int fc5_twin_last_solutions(fc5_ctx *ctx, fc5_twin_solution *out, size_t cap, size_t *n);
+
 /* ---- measurement (no reference counterpart: the roofline microbenchmarks) ---- */
 /* n pure xor-filter queries of pseudo-random keys against the main table at
  * full occupancy (probe_ms), and 3n raw random 8-byte loads of its cells

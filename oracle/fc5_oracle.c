@@ -1083,3 +1083,263 @@ int or_verify_find3(int cls, u64 B, u64 u, u32 d, u32 c, u64 fp, int N, int64_t 
   }
   return nsol;
 }
+
+/* ------------------------------------------------------------------ */
+/* (6,1,7) Branch-A class-5 twin (solve_617_v1.cu)                     */
+/* ------------------------------------------------------------------ */
+typedef __int128 i128;
+
+static i128 ipow6_i128(int64_t x) { i128 r = 1; for (int e = 0; e < 6; ++e) r *= x; return r; }
+
+/* lift (solve_617_v1.cu:815-824): residues r mod m lifted to values <= maxv, sorted */
+static int lift_res(const int32_t *res, int nr, int64_t m, int64_t maxv, int64_t **out) {
+  int cap = 16, n = 0;
+  int64_t *v = (int64_t *)malloc(sizeof(int64_t) * (size_t)cap);
+  for (int i = 0; i < nr; ++i) {
+    int64_t x = res[i] % m;
+    if (x == 0) x = m;
+    for (; x <= maxv; x += m) {
+      if (n == cap) { cap *= 2; v = (int64_t *)realloc(v, sizeof(int64_t) * (size_t)cap); }
+      v[n++] = x;
+    }
+  }
+  qsort(v, (size_t)n, sizeof(int64_t), cmp_i64);
+  *out = v;
+  return n;
+}
+
+int or_twin_cands(int64_t B, double u_lo, double u_hi, or_twin_cand_t *out, int cap) {
+  /* solve_617_v1.cu:807-812,885-906 */
+  roots_init();
+  int64_t lo = (int64_t)(u_lo * (double)B), hi = (int64_t)(u_hi * (double)B);
+  if (lo < 1) lo = 1;
+  if (hi > B - 1) hi = B - 1;
+  const i128 B6 = ipow6_i128(B);
+  const u32 lim = (u32)((B - 1) / 42);
+  const int32_t *wr, *ur, *vr;
+  /* scaled_roots(r2, B^6, 21, 2^6): (21w')^6 = B^6 (mod 2^6) */
+  const int nw = roots_of(&g_r2, or_mod_pow6(B, M2) * mod_inv(or_mod_pow6(21, M2), M2) % M2, &wr);
+  /* classes_d_cls2: (14u')^6 = B^6 (mod 3^6) */
+  const int nu = roots_of(&g_r3, or_mod_pow6(B, M3) * mod_inv(or_mod_pow6(14, M3), M3) % M3, &ur);
+  /* scaled_roots(r7, B^6, 6, 7^6): (6v')^6 = B^6 (mod 7^6) */
+  const int nv = roots_of(&g_r7, or_mod_pow6(B, M7) * mod_inv(or_mod_pow6(6, M7), M7) % M7, &vr);
+  int64_t *uv, *vv;
+  const int nuv = lift_res(ur, nu, M3, (B - 1) / 14, &uv);
+  const int nvv = lift_res(vr, nv, M7, (B - 1) / 6, &vv);
+  int n = 0;
+  for (int i = 0; i < nuv; ++i) {
+    const int64_t u = 14 * uv[i];
+    if (u < lo || u > hi) continue;
+    const i128 u6 = ipow6_i128(u);
+    for (int j = 0; j < nvv; ++j) {
+      const i128 base = B6 - u6 - ipow6_i128(6 * vv[j]);
+      if (base <= 0) continue;
+      if (n < cap) {
+        or_twin_cand_t *C = &out[n];
+        memset(C, 0, sizeof *C);
+        C->B = (u64)B; C->u = (u64)u; C->w = (u64)(6 * vv[j]);
+        C->base_lo = (u64)base; C->base_hi = (u64)((u128)base >> 64);
+        C->lim = lim; C->mod1 = (u32)M2; C->fmax1 = (u32)((B - 1) / 21);
+        C->nres1 = (u32)(nw < 8 ? nw : 8);
+        for (int k = 0; k < nw && k < 8; ++k) C->res[k] = (u32)wr[k];
+      }
+      ++n;
+    }
+  }
+  free(uv);
+  free(vv);
+  return n;
+}
+
+void or_build_tri(u64 t504[8], u64 t247[4], int *n504, int *n247) { /* solve_617_v1.cu:334-346 */
+  const int ms[2] = {504, 247};
+  u64 *bm[2] = {t504, t247};
+  int *cnt[2] = {n504, n247};
+  memset(t504, 0, 8 * sizeof(u64));
+  memset(t247, 0, 4 * sizeof(u64));
+  for (int k = 0; k < 2; ++k) {
+    const int m = ms[k];
+    char *sp = (char *)calloc((size_t)m, 1), *p2 = (char *)calloc((size_t)m, 1),
+         *p3 = (char *)calloc((size_t)m, 1);
+    for (int i = 0; i < m; ++i) sp[mod_pow6_h((u32)i, (u32)m)] = 1;
+    for (int a = 0; a < m; ++a) if (sp[a]) for (int c = 0; c < m; ++c) if (sp[c]) p2[(a + c) % m] = 1;
+    for (int a = 0; a < m; ++a) if (p2[a]) for (int c = 0; c < m; ++c) if (sp[c]) p3[(a + c) % m] = 1;
+    int c = 0;
+    for (int x = 0; x < m; ++x) if (p3[x]) { gate_set(bm[k], x); ++c; }
+    *cnt[k] = c;
+    free(sp); free(p2); free(p3);
+  }
+}
+
+/* largest r with r^6 <= n (exact) */
+static u32 iroot6_exact(u128 n) { return iroot6_u128(n); }
+/* smallest c >= 1 with R <= k * c^6 (R > 0) */
+static u32 min_c_bound(u128 R, u64 k) {
+  const u128 need = (R + k - 1) / k; /* c^6 >= ceil(R / k) */
+  u32 c = iroot6_u128(need);
+  if (c < 1) c = 1;
+  while (pow6_u128(c) < need) ++c;
+  while (c > 1 && pow6_u128(c - 1) >= need) --c;
+  return c;
+}
+
+int or_twin_run(const or_twin_cand_t *cands, int n, const or_xor_t *f, int use_gate, int use_tri,
+                int c4_div, or_twin_counts_t *counts, or_twin_hit_t *hits, u64 cap, u64 *n_hits,
+                int threads) {
+  or_gate_t g;
+  or_build_gate(&g);
+  u64 t504[8], t247[4];
+  int n504, n247;
+  or_build_tri(t504, t247, &n504, &n247);
+  const u64 inv216 = or_inv216();
+  const u64 m504 = 504ULL * (u64)M42;
+  const u64 inv42_247 = (u64)mod_inv(or_mod_pow6(42, 247), 247);
+  const u64 blk = (f && f->hdr.m_cells) ? f->hdr.m_cells / 3 : 0;
+  u64 jobs = 0, c4t = 0, c4s = 0, calls = 0, gated = 0, probes = 0, maybe = 0, nh = 0;
+  if (threads < 1) threads = 1;
+  if (c4_div != 4) c4_div = 3;
+  /* one work row per (cand, t); rows of a cand are its blockIdx.y values */
+  u64 *row0 = (u64 *)malloc(sizeof(u64) * (size_t)(n + 1));
+  row0[0] = 0;
+  for (int ci = 0; ci < n; ++ci) {
+    const or_twin_cand_t *C = &cands[ci];
+    const u64 rows = (C->nres1 && C->mod1) ? (u64)C->nres1 * (C->fmax1 / C->mod1 + 1) : 0;
+    row0[ci + 1] = row0[ci] + rows;
+  }
+  const long total = (long)row0[n];
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads) \
+    reduction(+ : jobs, c4t, c4s, calls, gated, probes, maybe)
+  for (long row = 0; row < total; ++row) {
+    int lo_c = 0, hi_c = n;
+    while (hi_c - lo_c > 1) { const int mid = (lo_c + hi_c) / 2; if (row0[mid] <= (u64)row) lo_c = mid; else hi_c = mid; }
+    const int ci = lo_c;
+    const or_twin_cand_t *C = &cands[ci];
+    const u128 base = ((u128)C->base_hi << 64) | C->base_lo;
+    {
+      const u32 t = (u32)((u64)row - row0[ci]);      /* blockIdx.y (solve_617_v1.cu:567-571) */
+      const u32 d = C->res[t % C->nres1] + (t / C->nres1) * C->mod1;
+      if (d == 0 || d > C->fmax1) continue;
+      const u128 g6 = pow6_u128(21ULL * d);
+      if (g6 >= base) continue;                       /* R negative or zero */
+      const u128 R = base - g6;
+      u32 hi4 = iroot6_exact(R / (u128)M42);
+      if (hi4 > C->lim) hi4 = C->lim;
+      if (hi4 < 1) continue;
+      const u32 lo4 = min_c_bound(R, (u64)c4_div * (u64)M42);
+      if (lo4 > hi4) continue;
+      ++jobs;
+      const u32 Q504 = (u32)((u64)(R % (u128)m504) / (u64)M42 % 504ULL);
+      const u32 Q247 = (u32)((u64)(R % (u128)247) * inv42_247 % 247ULL);
+      for (u32 c4 = lo4; c4 <= hi4; ++c4) {
+        if (use_tri) {
+          ++c4t;
+          u32 tr504 = Q504 + 504u - g.p6504[c4 % 504];
+          if (tr504 >= 504u) tr504 -= 504u;
+          u32 tr247 = Q247 + 247u - g.p6247[c4 % 247];
+          if (tr247 >= 247u) tr247 -= 247u;
+          if (!gate_bit(t504, (int)tr504) || !gate_bit(t247, (int)tr247)) { ++c4s; continue; }
+        }
+        const u128 m4 = pow6_u128(c4) * (u128)M42;
+        if (m4 >= R) continue;
+        const u128 R4 = R - m4;
+        u32 hi3 = iroot6_exact(R4 / (u128)M42);
+        const u32 capm = C->lim < c4 ? C->lim : c4;
+        if (hi3 > capm) hi3 = capm;
+        if (hi3 < 1) continue;
+        const u32 lo3 = min_c_bound(R4, 3ULL * (u64)M42);
+        const u64 tfp = (u64)(R4 >> 6) * inv216;      /* funnel_fp: T4 mod 2^64 */
+        u32 t4504 = Q504 + 504u - g.p6504[c4 % 504];
+        if (t4504 >= 504u) t4504 -= 504u;
+        u32 t4247 = Q247 + 247u - g.p6247[c4 % 247];
+        if (t4247 >= 247u) t4247 -= 247u;
+        u32 r504 = lo3 % 504, r247 = lo3 % 247;
+        for (u32 c3 = lo3; c3 <= hi3; ++c3, r504 = r504 == 503 ? 0 : r504 + 1, r247 = r247 == 246 ? 0 : r247 + 1) {
+          if (use_gate) {
+            u32 x = t4504 + 504u - g.p6504[r504];
+            if (x >= 504u) x -= 504u;
+            u32 y = t4247 + 247u - g.p6247[r247];
+            if (y >= 247u) y -= 247u;
+            if (!(gate_bit(g.g504, (int)x) && gate_bit(g.g247, (int)y))) { ++gated; continue; }
+          }
+          ++calls;
+          probes += 3;
+          const u64 c2 = (u64)c3 * c3;
+          const u64 fp = tfp - c2 * c2 * c2;
+          if (f && device_query(f->packed, blk, f->hdr.r, f->hdr.mix_seed, fp)) {
+            ++maybe;
+            u64 slot;
+#pragma omp atomic capture
+            slot = nh++;
+            if (hits && slot < cap) {
+              hits[slot].cand = (u32)ci;
+              hits[slot].d = d;
+              hits[slot].c4 = c4;
+              hits[slot].c3 = c3;
+              hits[slot].fp = fp;
+            }
+          }
+        }
+      }
+    }
+  }
+  free(row0);
+  counts->jobs = jobs; counts->c4_total = c4t; counts->c4_tri_skipped = c4s;
+  counts->calls = calls; counts->gated = gated; counts->probes = probes; counts->maybe = maybe;
+  *n_hits = nh;
+  return 0;
+}
+
+typedef struct { u64 v; u32 x; } pw_ix_t;
+static int cmp_pw_ix(const void *a, const void *b) {
+  const pw_ix_t *p = (const pw_ix_t *)a, *q = (const pw_ix_t *)b;
+  if (p->v != q->v) return p->v < q->v ? -1 : 1;
+  return p->x < q->x ? -1 : p->x > q->x;
+}
+
+int or_twin_verify(const or_twin_cand_t *C, u32 d, u32 c4, u32 c3, u64 fp, int N, int64_t *terms_out,
+                   int64_t *gcd_out, int cap) { /* solve_617_v1.cu:1019-1110 */
+  if (N < 1) return 0;
+  /* PairRecover: every a <= b <= N with a^6 + b^6 = fp (mod 2^64) */
+  pw_ix_t *pw = (pw_ix_t *)malloc(sizeof(pw_ix_t) * (size_t)N);
+  for (int x = 1; x <= N; ++x) {
+    const u64 x2 = (u64)x * x;
+    pw[x - 1].v = x2 * x2 * x2;
+    pw[x - 1].x = (u32)x;
+  }
+  qsort(pw, (size_t)N, sizeof(pw_ix_t), cmp_pw_ix);
+  const u128 q = ((u128)C->base_hi << 64) | C->base_lo;
+  int nsol = 0;
+  for (int a = 1; a <= N; ++a) {
+    const u64 a2 = (u64)a * a;
+    const u64 need = fp - a2 * a2 * a2;
+    size_t lo = 0, hi = (size_t)N;
+    while (lo < hi) { const size_t mid = (lo + hi) / 2; if (pw[mid].v < need) lo = mid + 1; else hi = mid; }
+    for (size_t k = lo; k < (size_t)N && pw[k].v == need; ++k) {
+      const u32 b = pw[k].x;
+      if (b < (u32)a) continue;
+      const u128 g6 = pow6_u128(21ULL * d);
+      if (g6 > q) continue;
+      const u128 rhs = q - g6;
+      if (rhs % (u128)M42 != 0) continue;
+      if (pow6_u128((u64)a) + pow6_u128(b) + pow6_u128(c3) + pow6_u128(c4) != rhs / (u128)M42) continue;
+      int64_t t[7] = {42LL * a, 42LL * b, 42LL * c3, 42LL * c4, 21LL * d, (int64_t)C->u, (int64_t)C->w};
+      qsort(t, 7, sizeof(int64_t), cmp_i64);
+      int bad = 0;
+      for (int i = 0; i < 6; ++i) if (t[i] == t[i + 1]) bad = 1;
+      if (bad || t[6] >= (int64_t)C->B || t[0] < 1) continue;
+      i128 lhs = ipow6_i128((int64_t)C->B);
+      for (int i = 0; i < 7 && lhs >= 0; ++i) lhs -= ipow6_i128(t[i]);
+      if (lhs != 0) continue;
+      int64_t gg = t[0];
+      for (int i = 1; i < 7; ++i) gg = gcd64(gg, t[i]);
+      gg = gcd64(gg, (int64_t)C->B);
+      if (nsol < cap) {
+        for (int i = 0; i < 7; ++i) terms_out[nsol * 7 + i] = t[i];
+        gcd_out[nsol] = gg;
+      }
+      ++nsol;
+    }
+  }
+  free(pw);
+  return nsol;
+}

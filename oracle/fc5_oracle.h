@@ -151,6 +151,42 @@ int or_verify_hit(uint64_t B, uint64_t u, uint32_t lim, uint32_t d, uint32_t e,
                   uint64_t fp, int N, int64_t *terms_out, int64_t *gcd_out,
                   int cap);
 
+/* ---- (6,1,7) Branch-A class-5 twin: solve_617_v1.cu ----------------------
+ * B^6 = u^6 + w^6 + (21d)^6 + 42^6 (c1^6 + c2^6 + c3^6 + c4^6) with
+ * u = 14u' (the term not divisible by 3), w = 6v' (the term not divisible by 7)
+ * and the grid term 21d (the odd term), d over residues mod 64. */
+typedef struct {
+  uint64_t B, u, w;
+  uint64_t base_lo, base_hi; /* B^6 - u^6 - w^6 */
+  uint32_t lim, fmax1, mod1, nres1;
+  uint32_t res[8];
+} or_twin_cand_t;
+/* gen_class_cands cls5 (solve_617_v1.cu:885-906): candidates of B in the u band,
+ * u' ascending then v' ascending; writes up to cap, returns the total count */
+int or_twin_cands(int64_t B, double u_lo, double u_hi, or_twin_cand_t *out, int cap);
+/* triple-sum sets mod 504 and 247 (solve_617_v1.cu:334-346) */
+void or_build_tri(uint64_t t504[8], uint64_t t247[4], int *n504, int *n247);
+typedef struct {
+  uint64_t jobs;            /* (cand, d) cells with a non-empty c4 window */
+  uint64_t c4_total;        /* c4 values that reached the triple-sum gate */
+  uint64_t c4_tri_skipped;  /* c4 values the triple-sum gate removed */
+  uint64_t calls;           /* probe() invocations (pair gate passed) */
+  uint64_t gated;           /* c3 values the pair gate removed */
+  uint64_t probes;          /* table cell reads, 3 per call */
+  uint64_t maybe;           /* maybe-hits of the table */
+} or_twin_counts_t;
+typedef struct { uint32_t cand, d, c4, c3; uint64_t fp; } or_twin_hit_t;
+/* k_find4_cls5 semantics (solve_617_v1.cu:563-657) over every (cand, d, c4)
+ * once.  c4_div: 3 = the reference kernel's c4 lower bound (3 c4^6 >= T),
+ * 4 = the exact bound for the largest of four cores (4 c4^6 >= T). */
+int or_twin_run(const or_twin_cand_t *cands, int n, const or_xor_t *f, int use_gate, int use_tri,
+                int c4_div, or_twin_counts_t *counts, or_twin_hit_t *hits, uint64_t cap,
+                uint64_t *n_hits, int threads);
+/* expand_xor_hits + verify_hit cls5 (solve_617_v1.cu:1019-1110): writes up to cap
+ * solutions as 7 sorted terms + gcd; returns the count (before deduplication) */
+int or_twin_verify(const or_twin_cand_t *C, uint32_t d, uint32_t c4, uint32_t c3, uint64_t fp,
+                   int N, int64_t *terms_out, int64_t *gcd_out, int cap);
+
 /* inverse of 21^6 mod 2^64 (fourcore_cls5_gpu_v4.cu:62-66) */
 uint64_t or_inv216(void);
 /* splitmix64 step (public-domain generator by S. Vigna) used for plants */

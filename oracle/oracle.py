@@ -279,3 +279,71 @@ def verify_find3(cls, B, u, d, c, fp, N, cap=16):
     g = np.zeros(cap, np.int64)
     n = _lib.or_verify_find3(cls, B, u, d, c, fp, N, terms.ctypes.data_as(_P), g.ctypes.data_as(_P), cap)
     return [(terms[5 * i:5 * i + 5].tolist(), int(g[i])) for i in range(min(n, cap))]
+
+
+# ---- (6,1,7) Branch-A class-5 twin ---------------------------------------
+TWIN_CAND_DTYPE = np.dtype([("B", "<u8"), ("u", "<u8"), ("w", "<u8"), ("base_lo", "<u8"), ("base_hi", "<u8"),
+                            ("lim", "<u4"), ("fmax1", "<u4"), ("mod1", "<u4"), ("nres1", "<u4"),
+                            ("res", "<u4", (8,))])
+TWIN_HIT_DTYPE = np.dtype([("cand", "<u4"), ("d", "<u4"), ("c4", "<u4"), ("c3", "<u4"), ("fp", "<u8")])
+assert TWIN_CAND_DTYPE.itemsize == 88 and TWIN_HIT_DTYPE.itemsize == 24
+
+
+class TwinCounts(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "probes",
+                                          "maybe")]
+
+
+_lib.or_twin_cands.argtypes = [C.c_int64, C.c_double, C.c_double, _P, C.c_int]
+_lib.or_build_tri.argtypes = [_P, _P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+_lib.or_twin_run.argtypes = [_P, C.c_int, C.POINTER(Xor), C.c_int, C.c_int, C.c_int, C.POINTER(TwinCounts), _P,
+                             C.c_uint64, C.POINTER(C.c_uint64), C.c_int]
+_lib.or_twin_verify.argtypes = [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, _P, _P, C.c_int]
+
+
+def twin_cands(B, u_band=(0.0, 1.0)):
+    """gen_class_cands cls5 of solve_617_v1 for one B (structured array)"""
+    n = _lib.or_twin_cands(B, u_band[0], u_band[1], None, 0)
+    out = np.zeros(max(n, 1), TWIN_CAND_DTYPE)
+    _lib.or_twin_cands(B, u_band[0], u_band[1], out.ctypes.data_as(_P), n)
+    return out[:n]
+
+
+def twin_cand(B, u, w, base, lim, fmax1, res, mod1=64):
+    """one hand-made candidate (plants and fixtures)"""
+    c = np.zeros(1, TWIN_CAND_DTYPE)
+    c["B"], c["u"], c["w"] = B, u, w
+    c["base_lo"], c["base_hi"] = base & (2**64 - 1), base >> 64
+    c["lim"], c["fmax1"], c["mod1"], c["nres1"] = lim, fmax1, mod1, len(res)
+    c["res"][0, :len(res)] = res
+    return c
+
+
+def tri_sets():
+    t504 = np.zeros(8, np.uint64)
+    t247 = np.zeros(4, np.uint64)
+    a, b = C.c_int(), C.c_int()
+    _lib.or_build_tri(t504.ctypes.data_as(_P), t247.ctypes.data_as(_P), C.byref(a), C.byref(b))
+    return t504, t247, a.value, b.value
+
+
+def twin_run(cands, table, use_gate=True, use_tri=True, c4_div=3, threads=8, cap=1 << 20):
+    """k_find4_cls5 semantics: counters dict and sorted hits"""
+    cands = np.ascontiguousarray(cands, dtype=TWIN_CAND_DTYPE)
+    c = TwinCounts()
+    hits = np.zeros(cap, TWIN_HIT_DTYPE)
+    nh = C.c_uint64()
+    _lib.or_twin_run(cands.ctypes.data_as(_P), len(cands), C.byref(table.x) if table is not None else None,
+                     1 if use_gate else 0, 1 if use_tri else 0, c4_div, C.byref(c), hits.ctypes.data_as(_P),
+                     cap, C.byref(nh), threads)
+    hits = np.sort(hits[:min(nh.value, cap)], order=["cand", "d", "c4", "c3"])
+    return {f: getattr(c, f) for f, _ in TwinCounts._fields_}, hits, nh.value
+
+
+def twin_verify(cand, d, c4, c3, fp, N, cap=16):
+    cand = np.ascontiguousarray(cand, dtype=TWIN_CAND_DTYPE).reshape(1)
+    terms = np.zeros(cap * 7, np.int64)
+    g = np.zeros(cap, np.int64)
+    n = _lib.or_twin_verify(cand.ctypes.data_as(_P), d, c4, c3, fp, N, terms.ctypes.data_as(_P),
+                            g.ctypes.data_as(_P), cap)
+    return [(terms[7 * i:7 * i + 7].tolist(), int(g[i])) for i in range(min(n, cap))]

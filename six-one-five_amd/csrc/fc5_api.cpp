@@ -13,121 +13,9 @@
 #include <vector>
 
 #include "fc5_build.hpp"
-#include "fc5_host.hpp"
-#include "fc5_kernels.hpp"
+#include "fc5_ctx.hpp"
 
 using namespace fc5;
-
-namespace {
-
-constexpr u64 kGateModU = 124488;
-constexpr u32 kGateWordsU = (124488 + 31) / 32;
-
-struct DeviceTable {
-  StoreHeader hdr{};
-  u64* cells = nullptr;
-  XorDev dev{};
-  bool present = false;
-};
-
-void free_table(DeviceTable& t) {
-  if (t.cells) (void)hipFree(t.cells);
-  t = DeviceTable{};
-}
-
-u64 inv_mod_2_64(u64 a) {  // a odd; Newton iteration doubles correct bits
-  u64 x = a;
-  for (int i = 0; i < 6; ++i) x *= 2 - a * x;
-  return x;
-}
-
-}  // namespace
-
-struct fc5_band {
-  size_t n_in = 0, n_skipped = 0;
-  std::vector<u8> cls_of;           // class of each caller index (0 = skipped)
-  std::vector<int> slot_of;         // caller index -> index in units / units3
-  // class 5
-  std::vector<Cls5Unit> units;      // packed cls5 units
-  std::vector<u32> uid;             // caller index of each packed cls5 unit
-  std::vector<u32> irregular;       // cls5 units on the per-point 192-bit path
-  DevUnit5* d_units = nullptr;
-  u32 n_dev = 0, n_items = 0;
-  PlainUnit* d_plain = nullptr;     // every packed cls5 unit, reference shape
-  u32 n_plain = 0;
-  u32 max_fmax2 = 0, max_lim = 0;
-  // classes 2-4
-  std::vector<Cls234Unit> units3;
-  std::vector<u32> uid3;
-  std::vector<u32> irregular3;
-  DevUnit3* d_units3 = nullptr;
-  u32 n_dev3 = 0, n_items3 = 0;
-  u32 max_lim3 = 0;
-  // inputs and host images of the device units (re-numbered per rows-per-item)
-  std::vector<fc5_unit> in_units;
-  std::vector<u32> in_cls;
-  std::vector<DevUnit5> dev_h;
-  std::vector<DevUnit3> dev3_h;
-  int rows = 0;  // rows per work item the item numbering was made for
-  // captured batched pipeline
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  std::vector<u64> graph_key;
-};
-
-struct fc5_ctx {
-  int device = 0;
-  int num_cu = 0;
-  std::string err;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // options
-  u64 hit_cap = 1u << 20;
-  int use_gate = 1, verify = 1, stop_first = 0, blocks_per_cu = 0;
-  int rows_per_item = kRowsPerItem, use_graph = 0;
-  Counters* h_ctr = nullptr;  // pinned read-back of the counters
-  u64* h_nhits = nullptr;
-  u32 load_policy = 1;  // cache policy of the xor cell gathers (nontemporal: no MALL allocation)
-  // tables
-  DeviceTable main, side;
-  XorTable host_main;  // host copy when small enough (save, host verify)
-  bool host_main_valid = false;
-  // gate
-  u32* d_gate = nullptr;
-  u32* d_gate_ones = nullptr;
-  GateWords gw{};
-  // d table
-  u32* d_ndg = nullptr;
-  u32 ndg_K = 0, ndg_dmax = 0;
-  // run buffers
-  HitRec* d_hits = nullptr;
-  u64 d_hits_cap = 0;
-  Counters* d_ctr = nullptr;  // [cls] batched kernel, [8 + cls] plain kernels
-  u16* d_p6 = nullptr;        // x^6 mod 504 (504 entries) then mod 247 (247 entries)
-  u32* d_c6g = nullptr;       // (124488 - c^6 mod 124488) for c = 0..c6g_max
-  u32 c6g_max = 0;
-  fc5_result class_res[6]{};
-  u64* d_nhits = nullptr;
-  u32* d_item_ctr = nullptr;
-  // last results
-  std::vector<fc5_hit> hits;
-  std::vector<fc5_solution> sols;
-};
-
-#define FAIL(ctx, code, msg)  \
-  do {                        \
-    (ctx)->err = (msg);       \
-    return (code);            \
-  } while (0)
-
-#define HIPCHK(ctx, expr)                                                              \
-  do {                                                                                 \
-    hipError_t _e = (expr);                                                            \
-    if (_e != hipSuccess) {                                                            \
-      (ctx)->err = std::string("HIP error ") + hipGetErrorString(_e) + " at " #expr;   \
-      return FC5_E_HIP;                                                                \
-    }                                                                                  \
-  } while (0)
 
 static void build_gate_host(GateWords& gw, std::vector<u32>& bitmap) {
   // pair-sum sets of sixth powers mod 504 and 247, 0 included
@@ -202,6 +90,7 @@ extern "C" void fc5_ctx_destroy(fc5_ctx* ctx) {
     if (p) (void)hipFree(p);
   if (ctx->h_ctr) (void)hipHostFree(ctx->h_ctr);
   if (ctx->h_nhits) (void)hipHostFree(ctx->h_nhits);
+  free_twin(ctx);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   (void)hipStreamDestroy(ctx->stream);
@@ -232,6 +121,11 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) {
     ctx->rows_per_item = (int)v;
   } else if (k == "use_graph") {
     ctx->use_graph = v ? 1 : 0;
+  } else if (k == "twin_tri_gate") {
+    ctx->twin_tri = v ? 1 : 0;
+  } else if (k == "twin_c4_div") {
+    if (v != 3 && v != 4) FAIL(ctx, FC5_E_ARG, "twin_c4_div must be 3 or 4");
+    ctx->twin_c4_div = (int)v;
   } else if (k == "blocks_per_cu") {
     if (v < 0 || v > 64) FAIL(ctx, FC5_E_ARG, "blocks_per_cu out of range");
     ctx->blocks_per_cu = (int)v;
@@ -498,7 +392,7 @@ static int ensure_ndg(fc5_ctx* ctx, u32 dmax) {
   return FC5_OK;
 }
 
-static int ensure_c6g(fc5_ctx* ctx, u32 cmax) {
+int fc5::ensure_c6g(fc5_ctx* ctx, u32 cmax) {
   if (ctx->d_c6g && ctx->c6g_max >= cmax) return FC5_OK;
   std::vector<u32> h((size_t)cmax + 1);
   for (u32 c = 0; c <= cmax; ++c) {
@@ -514,7 +408,7 @@ static int ensure_c6g(fc5_ctx* ctx, u32 cmax) {
   return FC5_OK;
 }
 
-static int ensure_hits(fc5_ctx* ctx) {
+int fc5::ensure_hits(fc5_ctx* ctx) {
   if (ctx->d_hits && ctx->d_hits_cap == ctx->hit_cap) return FC5_OK;
   if (ctx->d_hits) (void)hipFree(ctx->d_hits);
   ctx->d_hits = nullptr;
@@ -992,19 +886,20 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   });
   if (ctx->verify && !ctx->hits.empty()) {
     const int N = ctx->main.hdr.N ? (int)ctx->main.hdr.N : (int)std::max(b->max_lim, b->max_lim3);
-    std::vector<Solution> sols;
+    std::vector<VerifyJob> jobs;
     for (const fc5_hit& h : ctx->hits) {
       if (!(h.flags & 1u) || h.unit >= b->n_in) continue;
       const int k = b->cls_of[h.unit], slot = b->slot_of[h.unit];
       if (k == 5) {
         const Cls5Unit& U = b->units[(size_t)slot];
-        verify_cls5_hit(N, U.B, U.u, U.lim, h.d, h.e, h.fp, sols);
+        jobs.push_back(VerifyJob{5, U.B, U.u, U.lim, h.d, h.e, h.fp});
       } else if (k >= 2) {
         const Cls234Unit& U = b->units3[(size_t)slot];
-        verify_find3_hit(N, k, U.B, U.u, h.d, h.e, h.fp, sols);
+        jobs.push_back(VerifyJob{k, U.B, U.u, U.lim, h.d, h.e, h.fp});
       }
-      if (ctx->stop_first && !sols.empty()) break;
     }
+    std::vector<Solution> sols;
+    verify_jobs(N, jobs, ctx->stop_first != 0, sols);
     for (const Solution& S : sols) {
       fc5_solution o{};
       o.B = S.B;

@@ -1,0 +1,144 @@
+// fc5_ctx.hpp -- internal state behind the C ABI (contexts, bands, device
+// tables) shared by the ABI translation units.  Not a public header.
+#pragma once
+
+#include "fc5.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fc5_host.hpp"
+#include "fc5_kernels.hpp"
+#include "fc5_twin.hpp"
+
+namespace fc5 {
+
+constexpr u64 kGateModU = 124488;
+constexpr u32 kGateWordsU = (124488 + 31) / 32;
+
+struct DeviceTable {
+  StoreHeader hdr{};
+  u64* cells = nullptr;
+  XorDev dev{};
+  bool present = false;
+};
+
+inline void free_table(DeviceTable& t) {
+  if (t.cells) (void)hipFree(t.cells);
+  t = DeviceTable{};
+}
+
+inline u64 inv_mod_2_64(u64 a) {  // a odd; Newton iteration doubles correct bits
+  u64 x = a;
+  for (int i = 0; i < 6; ++i) x *= 2 - a * x;
+  return x;
+}
+
+}  // namespace fc5
+
+using namespace fc5;
+
+struct fc5_band {
+  size_t n_in = 0, n_skipped = 0;
+  std::vector<u8> cls_of;           // class of each caller index (0 = skipped)
+  std::vector<int> slot_of;         // caller index -> index in units / units3
+  // class 5
+  std::vector<Cls5Unit> units;      // packed cls5 units
+  std::vector<u32> uid;             // caller index of each packed cls5 unit
+  std::vector<u32> irregular;       // cls5 units on the per-point 192-bit path
+  DevUnit5* d_units = nullptr;
+  u32 n_dev = 0, n_items = 0;
+  PlainUnit* d_plain = nullptr;     // every packed cls5 unit, reference shape
+  u32 n_plain = 0;
+  u32 max_fmax2 = 0, max_lim = 0;
+  // classes 2-4
+  std::vector<Cls234Unit> units3;
+  std::vector<u32> uid3;
+  std::vector<u32> irregular3;
+  DevUnit3* d_units3 = nullptr;
+  u32 n_dev3 = 0, n_items3 = 0;
+  u32 max_lim3 = 0;
+  // inputs and host images of the device units (re-numbered per rows-per-item)
+  std::vector<fc5_unit> in_units;
+  std::vector<u32> in_cls;
+  std::vector<DevUnit5> dev_h;
+  std::vector<DevUnit3> dev3_h;
+  int rows = 0;  // rows per work item the item numbering was made for
+  // captured batched pipeline
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<u64> graph_key;
+};
+
+struct fc5_ctx {
+  int device = 0;
+  int num_cu = 0;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // options
+  u64 hit_cap = 1u << 20;
+  int use_gate = 1, verify = 1, stop_first = 0, blocks_per_cu = 0;
+  int rows_per_item = kRowsPerItem, use_graph = 0;
+  Counters* h_ctr = nullptr;  // pinned read-back of the counters
+  u64* h_nhits = nullptr;
+  u32 load_policy = 1;  // cache policy of the xor cell gathers (nontemporal: no MALL allocation)
+  // tables
+  DeviceTable main, side;
+  XorTable host_main;  // host copy when small enough (save, host verify)
+  bool host_main_valid = false;
+  // gate
+  u32* d_gate = nullptr;
+  u32* d_gate_ones = nullptr;
+  GateWords gw{};
+  // d table
+  u32* d_ndg = nullptr;
+  u32 ndg_K = 0, ndg_dmax = 0;
+  // run buffers
+  HitRec* d_hits = nullptr;
+  u64 d_hits_cap = 0;
+  Counters* d_ctr = nullptr;  // [cls] batched kernel, [8 + cls] plain kernels
+  u16* d_p6 = nullptr;        // x^6 mod 504 (504 entries) then mod 247 (247 entries)
+  u32* d_c6g = nullptr;       // (124488 - c^6 mod 124488) for c = 0..c6g_max
+  u32 c6g_max = 0;
+  fc5_result class_res[6]{};
+  u64* d_nhits = nullptr;
+  u32* d_item_ctr = nullptr;
+  // last results
+  std::vector<fc5_hit> hits;
+  std::vector<fc5_solution> sols;
+  // (6,1,7) class-5 twin
+  int twin_tri = 1, twin_c4_div = 3;
+  u32* d_tri = nullptr;               // triple-sum sets mod 504 (words 0..15) and 247 (16..23)
+  TwinCounters* d_twin_ctr = nullptr;
+  TwinCounters* h_twin_ctr = nullptr; // pinned read-back
+  DevTwinCand* d_twin = nullptr;
+  size_t d_twin_cap = 0;
+  std::vector<fc5_twin_hit> twin_hits;
+  std::vector<fc5_twin_solution> twin_sols;
+};
+
+#define FAIL(ctx, code, msg)  \
+  do {                        \
+    (ctx)->err = (msg);       \
+    return (code);            \
+  } while (0)
+
+#define HIPCHK(ctx, expr)                                                              \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      (ctx)->err = std::string("HIP error ") + hipGetErrorString(_e) + " at " #expr;   \
+      return FC5_E_HIP;                                                                \
+    }                                                                                  \
+  } while (0)
+
+namespace fc5 {
+// shared run-buffer helpers (fc5_api.cpp)
+int ensure_c6g(fc5_ctx* ctx, u32 cmax);
+int ensure_hits(fc5_ctx* ctx);
+// twin buffers (fc5_twin_api.cpp)
+void free_twin(fc5_ctx* ctx);
+}  // namespace fc5

@@ -781,4 +781,147 @@ int verify_find3_hit(int N, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp, std::ve
   }
   return found;
 }
+
+void verify_jobs(int N, const std::vector<VerifyJob>& jobs, bool stop_first, std::vector<Solution>& out) {
+  std::vector<std::vector<Solution>> per(jobs.size());
+#pragma omp parallel for schedule(dynamic, 1) if (jobs.size() > 1)
+  for (long i = 0; i < (long)jobs.size(); ++i) {
+    const VerifyJob& j = jobs[(size_t)i];
+    if (j.cls == 5)
+      verify_cls5_hit(N, j.B, j.u, j.lim, j.d, j.e, j.fp, per[(size_t)i]);
+    else if (j.cls >= 2 && j.cls <= 4)
+      verify_find3_hit(N, j.cls, j.B, j.u, j.d, j.e, j.fp, per[(size_t)i]);
+  }
+  for (auto& v : per) {
+    out.insert(out.end(), v.begin(), v.end());
+    if (stop_first && !v.empty()) break;
+  }
+}
+}  // namespace fc5
+
+// ---------------------------------------------------------------------------
+// (6,1,7) class-5 twin
+// ---------------------------------------------------------------------------
+namespace fc5 {
+
+static u128 pow6_128(u64 x) {
+  const u128 x2 = (u128)x * x;
+  return x2 * x2 * x2;
+}
+
+static std::vector<i64> lift(const std::vector<i64>& res, i64 m, i64 maxv) {
+  std::vector<i64> v;
+  for (i64 r : res)
+    for (i64 x = (r % m) ? r % m : m; x <= maxv; x += m) v.push_back(x);
+  std::sort(v.begin(), v.end());
+  return v;
+}
+
+void twin_cands(u64 B, double u_lo, double u_hi, std::vector<TwinCandRaw>& out) {
+  const i64 Bi = (i64)B;
+  const i64 lo = std::max<i64>(1, (i64)(u_lo * (double)B));
+  const i64 hi = std::min<i64>(Bi - 1, (i64)(u_hi * (double)B));
+  // the three role terms: (21w')^6 = B^6 (2^6), (14u')^6 = B^6 (3^6), (6v')^6 = B^6 (7^6)
+  const auto wres = sixth_roots(pow6_mod(Bi, kM2) * inv_mod(pow6_mod(21, kM2), kM2) % kM2, kM2);
+  const auto ures = sixth_roots(pow6_mod(Bi, kM3) * inv_mod(pow6_mod(14, kM3), kM3) % kM3, kM3);
+  const auto vres = sixth_roots(pow6_mod(Bi, kM7) * inv_mod(pow6_mod(6, kM7), kM7) % kM7, kM7);
+  const auto uv = lift(ures, kM3, (Bi - 1) / 14);
+  const auto vv = lift(vres, kM7, (Bi - 1) / 6);
+  const U256 B6 = U256::pow6(B);
+  for (i64 up : uv) {
+    const i64 u = 14 * up;
+    if (u < lo || u > hi) continue;
+    U256 bu = B6;
+    if (sub_in_place(bu, U256::pow6((u64)u))) continue;
+    for (i64 vp : vv) {
+      U256 base = bu;
+      if (sub_in_place(base, U256::pow6((u64)(6 * vp))) || base.is_zero()) continue;
+      if (!base.fits128()) continue;  // B^6 >= 2^128: outside the reference's i128 range
+      TwinCandRaw C{};
+      C.B = B;
+      C.u = (u64)u;
+      C.w = (u64)(6 * vp);
+      C.base = base.lo128();
+      C.lim = (u32)((B - 1) / 42);
+      C.fmax1 = (u32)((B - 1) / 21);
+      C.mod1 = (u32)kM2;
+      C.nres1 = (u32)std::min<size_t>(wres.size(), 8);
+      for (u32 i = 0; i < C.nres1; ++i) C.res[i] = (u32)wres[i];
+      out.push_back(C);
+    }
+  }
+}
+
+bool pack_twin(const TwinCandRaw& in, TwinCand& C, std::string& why) {
+  C = TwinCand{};
+  if (in.mod1 != 64) { why = "grid modulus must be 64"; return false; }
+  if (in.nres1 < 1 || in.nres1 > 8) { why = "1..8 grid residues required"; return false; }
+  if (in.base >> 127) { why = "base must be below 2^127"; return false; }
+  if (in.lim >= 65536) { why = "lim must be below 65536"; return false; }
+  if (in.fmax1 >= (1u << 21)) { why = "fmax1 must be below 2^21"; return false; }
+  C.B = in.B;
+  C.u = in.u;
+  C.w = in.w;
+  C.base = in.base;
+  C.lim = in.lim;
+  C.fmax1 = in.fmax1;
+  C.nres = in.nres1;
+  for (u32 i = 0; i < in.nres1; ++i) {
+    if (in.res[i] >= 64) { why = "grid residue must be below 64"; return false; }
+    C.res[i] = in.res[i];
+    const u32 t = (u32)pow6_mod((i64)in.res[i], 64);
+    if (i == 0) C.td = t;
+    else if (t != C.td) { why = "grid residues with different sixth powers mod 64"; return false; }
+  }
+  const u64 m21 = 85766121ULL;  // 21^6
+  if ((u64)(in.base % (u128)kM42) != m21 * C.td) {
+    why = "base - (21d)^6 is not a multiple of 42^6";
+    return false;
+  }
+  C.Q = in.base / (u128)kM42;
+  return true;
+}
+
+int verify_twin_hit(int N, const TwinCand& C, u32 d, u32 c4, u32 c3, u64 fp, std::vector<TwinSolution>& out) {
+  const u128 g6 = pow6_128(21ULL * d);
+  if (g6 > C.base) return 0;
+  const u128 rhs = C.base - g6;
+  if (rhs % (u128)kM42) return 0;
+  const u128 T = rhs / (u128)kM42, a6 = pow6_128(c4), b6 = pow6_128(c3);
+  if (a6 > T || b6 > T - a6) return 0;
+  const u128 T2 = T - a6 - b6;
+  if ((u64)T2 != fp) return 0;
+  std::vector<std::pair<u32, u32>> pairs;
+  exact_pairs(T2, (u32)std::max(N, 0), pairs);
+  int found = 0;
+  for (auto [a, b] : pairs) {
+    i64 t[7] = {42LL * a, 42LL * b, 42LL * c3, 42LL * c4, 21LL * (i64)d, (i64)C.u, (i64)C.w};
+    std::sort(t, t + 7);
+    bool bad = t[0] < 1 || t[6] >= (i64)C.B;
+    for (int i = 0; i < 6; ++i) bad |= t[i] == t[i + 1];
+    if (bad) continue;
+    U256 lhs = U256::of(0);
+    for (int i = 0; i < 7; ++i) add_in_place(lhs, U256::pow6((u64)t[i]));
+    if (cmp(lhs, U256::pow6(C.B)) != 0) continue;
+    i64 g = t[0];
+    for (int i = 1; i < 7; ++i) g = std::gcd(g, t[i]);
+    g = std::gcd(g, (i64)C.B);
+    TwinSolution S{C.B, C.u, C.w, d, c4, c3, a, b, {}, g};
+    for (int i = 0; i < 7; ++i) S.terms[i] = t[i];
+    out.push_back(S);
+    ++found;
+  }
+  return found;
+}
+
+void verify_twin_hits(int N, const std::vector<TwinCand>& cands, const std::vector<TwinHitIn>& hits,
+                      std::vector<std::vector<TwinSolution>>& per) {
+  per.assign(hits.size(), {});
+#pragma omp parallel for schedule(dynamic, 1) if (hits.size() > 1)
+  for (long i = 0; i < (long)hits.size(); ++i) {
+    const TwinHitIn& h = hits[(size_t)i];
+    if (h.cand < cands.size()) verify_twin_hit(N, cands[h.cand], h.d, h.c4, h.c3, h.fp, per[(size_t)i]);
+  }
+}
+
 }  // namespace fc5

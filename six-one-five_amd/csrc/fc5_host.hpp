@@ -208,6 +208,57 @@ void exact_pairs(u128 T, u32 hi, std::vector<std::pair<u32, u32>>& out);
 int verify_cls5_hit(int N, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp, std::vector<Solution>& out);
 // find3 maybe-hit of class cls at (d, c3) (fourcore_find_v4.cu:535-551, :427-457)
 int verify_find3_hit(int N, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp, std::vector<Solution>& out);
+// One maybe-hit to verify: cls 5 uses (d, e) with cap lim; classes 2-4 use (d, c3 = e).
+struct VerifyJob {
+  int cls;
+  u64 B, u;
+  u32 lim, d, e;
+  u64 fp;
+};
+// Verifies jobs in parallel (OpenMP); solutions come out in job order.  With
+// stop_first, output ends after the first job that yields a solution.
+void verify_jobs(int N, const std::vector<VerifyJob>& jobs, bool stop_first, std::vector<Solution>& out);
+
+// ---------------------------------------------------------------------------
+// (6,1,7) class-5 twin (solve_617_v1.cu)
+// ---------------------------------------------------------------------------
+// Candidate (B, u = 14u', w = 6v'): base = B^6 - u^6 - w^6 and the grid term
+// 21d, d = res[i] + 64k <= fmax1.  Every residue has the same d^6 mod 64
+// (td), so R = base - (21d)^6 is a multiple of 42^6 exactly when
+// base = 21^6 td (mod 42^6), and then T(d) = R/42^6 = Q - floor(d^6/64) with
+// Q = floor(base/42^6).
+struct TwinCand {
+  u64 B = 0, u = 0, w = 0;
+  u128 base = 0, Q = 0;
+  u32 lim = 0, fmax1 = 0, nres = 0, td = 0;
+  u32 res[8] = {0};
+};
+struct TwinCandRaw {  // the generator's output, fields of GpuCand (solve_617_v1.cu:354-363)
+  u64 B, u, w;
+  u128 base;
+  u32 lim, fmax1, mod1, nres1;
+  u32 res[8];
+};
+// gen_class_cands cls 5 (solve_617_v1.cu:807-812,885-906)
+void twin_cands(u64 B, double u_lo, double u_hi, std::vector<TwinCandRaw>& out);
+// checks the T-space contract; false with a reason otherwise
+bool pack_twin(const TwinCandRaw& in, TwinCand& out, std::string& why);
+struct TwinSolution {
+  u64 B, u, w;
+  u32 d, c4, c3, a, b;
+  i64 terms[7];
+  i64 gcd;
+};
+// expand_xor_hits + verify_hit cls 5 (solve_617_v1.cu:1019-1110) for one hit;
+// appends every exact, distinct, in-range 7-term identity (not deduplicated)
+int verify_twin_hit(int N, const TwinCand& C, u32 d, u32 c4, u32 c3, u64 fp, std::vector<TwinSolution>& out);
+struct TwinHitIn {
+  u32 cand, d, c4, c3;
+  u64 fp;
+};
+// verify_twin_hit for every hit in parallel (OpenMP); per[i] gets hit i's solutions
+void verify_twin_hits(int N, const std::vector<TwinCand>& cands, const std::vector<TwinHitIn>& hits,
+                      std::vector<std::vector<TwinSolution>>& per);
 
 // splitmix64 (Vigna), the generator for planted keys
 inline u64 splitmix64(u64& s) {

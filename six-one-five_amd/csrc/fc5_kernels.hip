@@ -11,53 +11,13 @@
 //                 sixth power, subtract-with-borrow and modular gate.
 // k_expand_cells  widens the r-bit packed table to one u64 per cell.
 // k_query         membership queries for a key list (tests, FPR smoke).
-#include "fc5_kernels.hpp"
+#include "fc5_device.hpp"
 
 namespace fc5 {
 
-typedef unsigned __int128 u128;
-
 constexpr u64 kM42 = 5489031744ULL;
-constexpr u32 kGateMod = 124488;
-constexpr u32 kGateWords = (kGateMod + 31) / 32;
 constexpr int kWaves = 4;   // wavefronts per workgroup
 constexpr u32 kRing = 128;  // per-wave survivor ring (entries)
-
-__device__ __forceinline__ u64 d_mix64(u64 x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdULL;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ULL;
-  x ^= x >> 33;
-  return x;
-}
-
-// xor filter query on expanded cells (xor_filter.hpp:53-64,472-483)
-__device__ __forceinline__ bool d_query(const XorDev& x, u64 key) {
-  if (!x.present) return false;
-  const u64 h = d_mix64(key + x.seed);
-  const u64 h1 = d_mix64(h ^ 0x9E3779B97F4A7C15ULL);
-  const u64 h2 = d_mix64(h ^ 0xBF58476D1CE4E5B9ULL);
-  const u64 i0 = mod_fast(h, x.md);
-  const u64 i1 = mod_fast(h1, x.md) + x.block;
-  const u64 i2 = mod_fast(h2, x.md) + 2 * x.block;
-  const u64 got = load_cell_policy(x.cells + i0, x.policy) ^ load_cell_policy(x.cells + i1, x.policy) ^
-                  load_cell_policy(x.cells + i2, x.policy);
-  return got == (h & x.mask);
-}
-
-__device__ __forceinline__ u64 wave_sum(u64 v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-__device__ __forceinline__ u32 rdl(u32 v, int lane) {
-  return (u32)__builtin_amdgcn_readlane((int)v, lane);
-}
-
-__device__ __forceinline__ u32 lanes_below(u64 m) {
-  return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
-}
 
 // floor(x^6 / 64) for x < 2^21
 __device__ __forceinline__ u128 dq_of(u32 x) {
@@ -69,15 +29,6 @@ __device__ __forceinline__ u128 dq_of(u32 x) {
 // batched persistent kernel
 // ---------------------------------------------------------------------------
 constexpr u32 kFind3Bit = 0x80000000u;  // ring entry unit field: find3 row
-
-__device__ __forceinline__ u32 iroot6_u128(u128 X) {  // largest a with a^6 <= X (X < 2^126)
-  const double x = (double)(u64)(X >> 64) * 18446744073709551616.0 + (double)(u64)X;
-  u32 a = (u32)cbrt(sqrt(x));
-  auto p6 = [](u32 v) { const u64 v3 = (u64)v * v * v; return (u128)v3 * v3; };
-  while (p6(a + 1) <= X) ++a;
-  while (a > 0 && p6(a) > X) --a;
-  return a;
-}
 
 __device__ __forceinline__ void drain(const BatchedArgs& a, const u32* qu, const u32* qd,
                                       const u32* qe, u32 head, u32 n, u32 lane) {
@@ -152,12 +103,6 @@ __device__ __forceinline__ u32 ring_push(const BatchedArgs& a, Ring& q, bool pas
   return cnt;
 }
 
-__device__ __forceinline__ bool gate_pass(const u32* s_gate, u32 xg, u32 n) {
-  u32 t = xg + n;
-  t = min(t, t - kGateMod);
-  return (s_gate[t >> 5] >> (t & 31u)) & 1u;
-}
-
 template <int E>
 __device__ __forceinline__ void flush_counters(Counters* c, u32 lane, u32 calls, u32 neg, u32 zero, u32 pos,
                                                u32 probes, u32 jobs, u32 failT) {
@@ -220,8 +165,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, const u32* s_gat
           while (dc > 0 && dq_of(dc) >= X) --dc;
           dlim = min(dc, U.fmax2);
           pos = cnt(r0, dlim) + cnt(r1, dlim) + cnt(r2, dlim) + cnt(r3, dlim);
-          const u64 xh = (u64)(X >> 64), xl = (u64)X;
-          xg = (u32)(((xh % kGateMod) * (u64)(18446744073709551615ULL % kGateMod + 1) + xl % kGateMod) % kGateMod);
+          xg = mod_gate(X);
           const u32 z = dc + 1;  // the only d that can give T == 0
           if (z <= U.fmax2 && dq_of(z) == X) {
             const u32 zr = z & 63u;
@@ -297,20 +241,13 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, const u32* s_ga
           c_jobs = 1;
           u32 h = iroot6_u128(X);
           if (h > Up->lim) h = Up->lim;
-          // smallest c >= 1 with 3 c^6 >= T
-          const double xd = (double)(u64)(X >> 64) * 18446744073709551616.0 + (double)(u64)X;
-          u32 l = (u32)cbrt(sqrt(xd / 3.0));
-          if (l < 1) l = 1;
-          auto p6 = [](u32 v) { const u64 v3 = (u64)v * v * v; return (u128)v3 * v3; };
-          while (l > 1 && p6(l - 1) * 3 >= X) --l;
-          while (l <= h && p6(l) * 3 < X) ++l;
+          const u32 l = min_core(X, 3u);  // smallest c >= 1 with 3 c^6 >= T
           if (l <= h) {
             wlo = l;
             whi = h;
             c_calls = h - l + 1;
           }
-          const u64 xh = (u64)(X >> 64), xl = (u64)X;
-          xg = (u32)(((xh % kGateMod) * (u64)(18446744073709551615ULL % kGateMod + 1) + xl % kGateMod) % kGateMod);
+          xg = mod_gate(X);
         } else {
           c_fail = 1;  // not positive: compute_T fails (fourcore_gmp.hpp:47)
         }

@@ -64,6 +64,27 @@ class Solution(C.Structure):
                [("terms", C.c_int64 * 5), ("gcd", C.c_int64)]
 
 
+class TwinResult(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "probes", "maybe", "n_hits", "hits_dropped",
+        "solutions", "cands", "items")] + [("kernel_ms", C.c_double), ("wall_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class TwinSolution(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("B", "u", "w")] + \
+               [(n, C.c_uint32) for n in ("d", "c4", "c3", "a", "b", "pad")] + \
+               [("terms", C.c_int64 * 7), ("gcd", C.c_int64)]
+
+
+# fc5_twin_cand / fc5_twin_hit as numpy records (88 and 24 bytes)
+TWIN_CAND_DTYPE = np.dtype([("B", "<u8"), ("u", "<u8"), ("w", "<u8"), ("base_lo", "<u8"), ("base_hi", "<u8"),
+                            ("lim", "<u4"), ("fmax1", "<u4"), ("mod1", "<u4"), ("nres1", "<u4"),
+                            ("res", "<u4", (8,))])
+TWIN_HIT_DTYPE = np.dtype([("cand", "<u4"), ("d", "<u4"), ("c4", "<u4"), ("c3", "<u4"), ("fp", "<u8")])
+
 _P = C.c_void_p
 _sig = {
     "fc5_abi_version": (C.c_int, []),
@@ -96,6 +117,11 @@ _sig = {
     "fc5_emit_units": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double,
                                  _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_build_pairs_file": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.POINTER(StoreHeader)]),
+    "fc5_twin_cands": (C.c_int, [C.c_uint64, C.c_double, C.c_double, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fc5_twin_run": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(TwinResult)]),
+    "fc5_twin_run_range": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.POINTER(TwinResult)]),
+    "fc5_twin_last_hits": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fc5_twin_last_solutions": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
 }
 EXPORTS = tuple(_sig)
 for _name, (_res, _args) in _sig.items():
@@ -147,6 +173,18 @@ def units_calls(units):
     if rc:
         raise Fc5Error(rc, "units_calls")
     return out.value
+
+
+def twin_cands(B, u_band=(0.0, 1.0)):
+    """(6,1,7) class-5 candidates of one B (gen_class_cands, solve_617_v1.cu:885-906)."""
+    n = C.c_size_t()
+    rc = _lib.fc5_twin_cands(B, u_band[0], u_band[1], None, 0, C.byref(n))
+    if rc:
+        raise Fc5Error(rc, "fc5_twin_cands")
+    out = np.zeros(n.value, TWIN_CAND_DTYPE)
+    if n.value:
+        _lib.fc5_twin_cands(B, u_band[0], u_band[1], out.ctypes.data_as(_P), n.value, C.byref(n))
+    return out
 
 
 def build_pairs_file(N, r, path):
@@ -315,6 +353,34 @@ class Context:
             self._check(_lib.fc5_last_solutions(self.h, C.cast(arr, _P), n.value, C.byref(n)))
         return [dict(B=s.B, u=s.u, cls=s.cls, d=s.d, e=s.e, a=s.a, b=s.b, c=s.c, terms=list(s.terms),
                      gcd=s.gcd) for s in arr[:n.value]]
+
+    def twin_run(self, cands, allow_overflow=False):
+        """(6,1,7) class-5 twin over candidate records (TWIN_CAND_DTYPE)."""
+        cands = np.ascontiguousarray(cands, dtype=TWIN_CAND_DTYPE)
+        res = TwinResult()
+        rc = _lib.fc5_twin_run(self.h, _ptr(cands), len(cands), C.byref(res))
+        return self._twin_out(rc, res, allow_overflow)
+
+    def twin_run_range(self, B_lo, B_hi, u_band=(0.0, 1.0), allow_overflow=False):
+        res = TwinResult()
+        rc = _lib.fc5_twin_run_range(self.h, B_lo, B_hi, u_band[0], u_band[1], C.byref(res))
+        return self._twin_out(rc, res, allow_overflow)
+
+    def _twin_out(self, rc, res, allow_overflow):
+        if rc not in (OK, E_OVERFLOW) or (rc == E_OVERFLOW and not allow_overflow):
+            self._check(rc)
+        n = C.c_size_t()
+        self._check(_lib.fc5_twin_last_hits(self.h, None, 0, C.byref(n)))
+        hits = np.zeros(n.value, TWIN_HIT_DTYPE)
+        if n.value:
+            self._check(_lib.fc5_twin_last_hits(self.h, hits.ctypes.data_as(_P), n.value, C.byref(n)))
+        self._check(_lib.fc5_twin_last_solutions(self.h, None, 0, C.byref(n)))
+        arr = (TwinSolution * max(1, n.value))()
+        if n.value:
+            self._check(_lib.fc5_twin_last_solutions(self.h, C.cast(arr, _P), n.value, C.byref(n)))
+        sols = [dict(B=x.B, u=x.u, w=x.w, d=x.d, c4=x.c4, c3=x.c3, a=x.a, b=x.b, terms=list(x.terms), gcd=x.gcd)
+                for x in arr[:n.value]]
+        return res.as_dict(), hits, sols
 
     def close(self):
         if self.h:

@@ -146,3 +146,67 @@ def find3_counts(cls, B, u, contains=None, use_gate=True):
                     c["maybe"] += 1
                     hits.append((d, c3, fp))
     return c, hits
+
+
+# ---- (6,1,7) class-5 twin ---------------------------------------------------
+def triple_sum_set(m):
+    """residues of x^6 + y^6 + z^6 mod m over all x, y, z"""
+    p = {pow(a, 6, m) for a in range(m)}
+    p2 = {(x + y) % m for x in p for y in p}
+    return {(x + y) % m for x in p2 for y in p}
+
+
+T504 = triple_sum_set(504)
+T247 = triple_sum_set(247)
+
+
+def twin_counts(cand, contains=None, c4_div=3, use_gate=True, use_tri=True):
+    """k_find4_cls5 from its definition: every grid d, every c4 of the window
+    (each once), every c3 of the inner window; Python integers throughout."""
+    base = cand["base"]
+    lim, fmax1 = cand["lim"], cand["fmax1"]
+    c = dict(jobs=0, c4_total=0, c4_tri_skipped=0, calls=0, gated=0, probes=0, maybe=0)
+    hits = []
+    ds = sorted(d for r in cand["res"] for d in range(r, fmax1 + 1, 64) if d)
+    for d in ds:
+        R = base - (21 * d) ** 6
+        if R <= 0:
+            continue
+        assert R % M42 == 0
+        T = R // M42
+        hi4 = min(iroot6(T), lim)
+        if hi4 < 1:
+            continue
+        lo4 = 1
+        while c4_div * lo4 ** 6 < T:
+            lo4 += 1
+        if lo4 > hi4:
+            continue
+        c["jobs"] += 1
+        for c4 in range(lo4, hi4 + 1):
+            T4 = T - c4 ** 6
+            if use_tri:
+                c["c4_total"] += 1
+                if not (T4 % 504 in T504 and T4 % 247 in T247):
+                    c["c4_tri_skipped"] += 1
+                    continue
+            if T4 <= 0:
+                continue
+            hi3 = min(iroot6(T4), lim, c4)
+            if hi3 < 1:
+                continue
+            lo3 = 1
+            while 3 * lo3 ** 6 < T4:
+                lo3 += 1
+            for c3 in range(lo3, hi3 + 1):
+                V = T4 - c3 ** 6
+                if use_gate and not (V % 504 in S504 and V % 247 in S247):
+                    c["gated"] += 1
+                    continue
+                c["calls"] += 1
+                c["probes"] += 3
+                fp = V % 2 ** 64
+                if contains is not None and contains(fp):
+                    c["maybe"] += 1
+                    hits.append((d, c4, c3, fp))
+    return c, hits

@@ -8,11 +8,17 @@ Outputs (data only: inputs and expected outputs):
     units_cfg1.txt      Stage-1 "cls B u" lines, classes 2-5, B in [100001,102000]
     residues.json       cls5 / cls2-4 free-term residue specs at sampled units
     seeds.json          seed counts per class at sampled B
+    twin_cands.json     (6,1,7) class-5 candidate lists (count, SHA-256 of the
+                        lines, small lists in full) and the triple-sum sets
+    known_617_cls5.json the class-5 rows of the reference's list of known
+                        (6,1,7) solutions (617-solutions-clean.txt), with the
+                        role decomposition of each
 """
 import hashlib
 import json
 import os
 import random
+import re
 import subprocess
 import sys
 import tempfile
@@ -20,6 +26,7 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 REF = os.path.join(REPO, "oracle", "_ref")
+REF_SRC = "/root/reference"
 
 
 def run(*args):
@@ -77,7 +84,44 @@ def main():
             continue
         seeds.append({"B": B, "counts": list(map(int, run("ref_classes", "seeds", B).split()))})
     json.dump(seeds, open(os.path.join(HERE, "seeds.json"), "w"), indent=0)
+    known = known_617_cls5(os.path.join(REF_SRC, "617-solutions-clean.txt"))
+    json.dump(known, open(os.path.join(HERE, "known_617_cls5.json"), "w"), indent=0)
+
+    twin = {"tri": list(map(int, run("ref_617", "tri").split())), "cands": []}
+    Bs = sorted({k["B"] for k in known} | {1009, 2003, 5003, 10007, 100003, 1000003, 2000003, 2353001, 2353003})
+    for B, band in [(B, (0.0, 1.0)) for B in Bs] + [(400471, (0.3, 0.7)), (100003, (0.5, 1.0))]:
+        text = run("ref_617", "cands", B, band[0], band[1])
+        lines = [l for l in text.split("\n") if l]
+        rec = {"B": B, "u_band": list(band), "count": len(lines),
+               "sha256": hashlib.sha256("\n".join(lines).encode()).hexdigest()}
+        if len(lines) <= 40:
+            rec["lines"] = lines
+        twin["cands"].append(rec)
+    json.dump(twin, open(os.path.join(HERE, "twin_cands.json"), "w"), indent=0)
     print("golden fixtures written")
+
+
+def known_617_cls5(path):
+    """Class-5 rows (the odd, not-divisible-by-3 and not-divisible-by-7 terms are
+    three different terms) of the known-solution list, 2nd kind (7 does not divide B)."""
+    out = []
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        m = re.match(r"\s*(\d+)\s*=\s*([\d\s+]+)\|", line)
+        B = int(m.group(1))
+        t = sorted(int(x) for x in m.group(2).split("+"))
+        assert sum(x ** 6 for x in t) == B ** 6, line
+        if B % 7 == 0:
+            continue
+        odd = [x for x in t if x % 2]
+        n3 = [x for x in t if x % 3]
+        n7 = [x for x in t if x % 7]
+        if len(odd) != 1 or len(n3) != 1 or len(n7) != 1 or len({odd[0], n3[0], n7[0]}) != 3:
+            continue
+        cores = sorted(x // 42 for x in t if x not in (odd[0], n3[0], n7[0]))
+        out.append({"B": B, "terms": t, "d": odd[0] // 21, "u": n3[0], "w": n7[0], "cores": cores})
+    return out
 
 
 if __name__ == "__main__":

@@ -1,0 +1,169 @@
+"""GPU: the (6,1,7) class-5 twin (libfc5.so, k_twin_batched) against the oracle
+and against the reference's list of known (6,1,7) solutions.
+
+Bit-exact bar: counters (jobs, c4_total, c4_tri_skipped, calls, gated, probes,
+maybe) and the sorted maybe-hit list equal the oracle's on the same table
+bytes; every known class-5 solution is recovered, and whole-B runs report no
+primitive solution outside the list.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import fc5
+import oracle as O
+from conftest import GOLDEN
+from test_twin import KNOWN, planted
+
+pytestmark = pytest.mark.gpu
+
+COUNTERS = ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "probes", "maybe")
+THREADS = min(16, os.cpu_count() or 8)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = fc5.Context(0)
+    yield c
+    c.close()
+
+
+def device_table(ctx, N, r):
+    """device-built table, its bytes shared with the oracle"""
+    ctx.table_build(N, r, on_gpu=True)
+    h, packed = ctx.table_download()
+    return O.Table.from_bytes({f: getattr(h, f) for f, _ in O.Header._fields_}, packed)
+
+
+def compare(res, hits, oc, ohits):
+    for k in COUNTERS:
+        assert res[k] == oc[k], (k, res[k], oc[k])
+    got = np.sort(hits, order=["cand", "d", "c4", "c3"])
+    assert len(got) == len(ohits)
+    for f in ("cand", "d", "c4", "c3", "fp"):
+        assert np.array_equal(got[f], ohits[f]), f
+
+
+def set_opts(ctx, c4_div, use_gate, use_tri):
+    ctx.set_option("twin_c4_div", c4_div)
+    ctx.set_option("use_gate", int(use_gate))
+    ctx.set_option("twin_tri_gate", int(use_tri))
+
+
+@pytest.mark.parametrize("r", [16, 48])
+@pytest.mark.parametrize("c4_div,use_gate,use_tri", [(3, True, True), (4, True, True), (3, True, False),
+                                                     (3, False, True)])
+def test_twin_whole_B_matches_oracle(ctx, r, c4_div, use_gate, use_tri):
+    say(f"twin parity r={r} c4_div={c4_div} gate={use_gate} tri={use_tri}")
+    t = device_table(ctx, 1500, r)
+    cands = np.concatenate([O.twin_cands(54347), O.twin_cands(59819)])
+    set_opts(ctx, c4_div, use_gate, use_tri)
+    try:
+        res, hits, sols = ctx.twin_run(cands.view(fc5.TWIN_CAND_DTYPE))
+    finally:
+        set_opts(ctx, 3, True, True)
+    oc, oh, _ = O.twin_run(cands, t, use_gate=use_gate, use_tri=use_tri, c4_div=c4_div, threads=THREADS)
+    compare(res, hits, oc, oh)
+    assert res["cands"] == len(cands) and res["hits_dropped"] == 0
+    if r == 16:
+        assert oc["maybe"] > 0
+
+
+def test_twin_large_candidates_match_oracle(ctx):
+    """two full candidates at B = 400471 (one holds the known solution), r = 48,
+    N = 9535: 4.4e8 c3 values and a 400 MB table"""
+    say("twin parity B=400471")
+    t = device_table(ctx, 9535, 48)
+    c = O.twin_cands(400471)
+    pick = np.concatenate([c[(c["u"] == 355502) & (c["w"] == 144756)], c[len(c) // 3:len(c) // 3 + 1]])
+    res, hits, sols = ctx.twin_run(pick.view(fc5.TWIN_CAND_DTYPE))
+    oc, oh, _ = O.twin_run(pick, t, threads=THREADS)
+    compare(res, hits, oc, oh)
+    assert [s["terms"] for s in sols] == [[13860, 83202, 93240, 129696, 144756, 355502, 357609]]
+
+
+@pytest.mark.parametrize("c4_div", [3, 4])
+def test_twin_recovers_every_known_solution(ctx, c4_div):
+    """the candidate (B, u, w) of each of the 54 known class-5 solutions"""
+    say(f"twin known answers c4_div={c4_div}")
+    N = max(k["B"] for k in KNOWN) // 42
+    ctx.table_build(N, 48, on_gpu=True)
+    rows = []
+    for k in KNOWN:
+        c = fc5.twin_cands(k["B"])
+        sel = c[(c["u"] == k["u"]) & (c["w"] == k["w"])]
+        assert len(sel) == 1, k["B"]
+        rows.append(sel)
+    ctx.set_option("twin_c4_div", c4_div)
+    try:
+        res, hits, sols = ctx.twin_run(np.concatenate(rows))
+    finally:
+        ctx.set_option("twin_c4_div", 3)
+    got = sorted(tuple(s["terms"]) for s in sols if s["gcd"] == 1)
+    assert got == sorted(tuple(k["terms"]) for k in KNOWN)
+
+
+def test_twin_whole_B_ranges_find_exactly_the_known(ctx):
+    """every candidate of every Branch-A B of small ranges around known B"""
+    say("twin whole-B ranges")
+    known_B = {k["B"]: tuple(k["terms"]) for k in KNOWN}
+    ctx.table_build(130000 // 42, 48, on_gpu=True)
+    for lo, hi in [(54300, 54400), (97000, 97100), (123900, 124000)]:
+        res, hits, sols = ctx.twin_run_range(lo, hi)
+        got = {tuple(s["terms"]) for s in sols if s["gcd"] == 1}
+        want = {t for B, t in known_B.items() if lo <= B <= hi}
+        assert got == want, (lo, hi, got)
+        per_B = [fc5.twin_cands(B) for B in range(lo, hi + 1) if B % 2 and B % 3 and B % 7]
+        assert res["cands"] == sum(len(c) for c in per_B)
+
+
+def test_twin_plants_recovered(ctx):
+    """the reference's find4 plant pattern (solve_617_v1.cu:1564-1603), 256 plants"""
+    say("twin plants")
+    N = 2000
+    ctx.table_build(N, 48, on_gpu=True)
+    plants = planted(N, 256, 1617)
+    cands = np.concatenate([p[0] for p in plants]).view(fc5.TWIN_CAND_DTYPE)
+    for c4_div in (4, 3):
+        ctx.set_option("twin_c4_div", c4_div)
+        try:
+            res, hits, _ = ctx.twin_run(cands)
+        finally:
+            ctx.set_option("twin_c4_div", 3)
+        found = set()
+        for h in hits:
+            _, d, cs = plants[h["cand"]]
+            T = sum(x ** 6 for x in cs)
+            if int(h["d"]) == d and (T - int(h["c4"]) ** 6 - int(h["c3"]) ** 6) % 2 ** 64 == int(h["fp"]):
+                found.add(int(h["cand"]))
+        want = {i for i, (_, d, cs) in enumerate(plants) if c4_div * cs[3] ** 6 >= sum(x ** 6 for x in cs)}
+        assert found >= want
+        if c4_div == 4:
+            assert len(found) == 256
+
+
+def test_twin_errors(ctx):
+    ctx.table_build(500, 8, on_gpu=True)
+    c = fc5.twin_cands(54347)
+    bad = c[:1].copy()
+    bad["base_lo"] += 1  # base - (21d)^6 no longer a multiple of 42^6
+    with pytest.raises(fc5.Fc5Error) as e:
+        ctx.twin_run(bad)
+    assert e.value.code == fc5.E_ARG
+    ctx.set_option("hit_cap", 4)  # r = 8: many maybe-hits
+    try:
+        with pytest.raises(fc5.Fc5Error) as e:
+            ctx.twin_run(c)
+        assert e.value.code == fc5.E_OVERFLOW
+        res, hits, _ = ctx.twin_run(c, allow_overflow=True)
+        assert res["hits_dropped"] == res["n_hits"] - 4 and len(hits) == 4
+    finally:
+        ctx.set_option("hit_cap", 1 << 20)
