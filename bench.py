@@ -54,7 +54,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS))
+    ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS) + [617],
+                    help="1-4: class-5 peel bands; 617: the (6,1,7) class-5 twin")
+    ap.add_argument("--twin-lo", type=int, default=400471, help="617: first B")
+    ap.add_argument("--twin-hi", type=int, default=400471, help="617: last B")
+    ap.add_argument("--twin-cpu-every", type=int, default=128, help="617: CPU oracle on every k-th candidate")
     ap.add_argument("--r", type=int, default=48)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--plain-every", type=int, default=64, help="plain form on every k-th unit")
@@ -71,6 +75,8 @@ def main():
         print("bench.py measures one GPU: every box of this build has one GPU and the band "
               "shards by units with no exchange (see DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.config == 617:
+        return bench_twin(args)
     cfg = CONFIGS[args.config]
     t_setup = time.time()
     mixed = cfg["classes"] != (5,)
@@ -234,6 +240,103 @@ def main():
     print(json.dumps(line), flush=True)
     band.close()
     ctx.close()
+
+
+def bench_twin(args):
+    """(6,1,7) class-5 twin: every Branch-A candidate of B in [twin_lo, twin_hi]
+    in one persistent launch per step, table N = twin_hi // 42 at r = 48."""
+    lo, hi = args.twin_lo, args.twin_hi
+    Bs = [B for B in range(lo, hi + 1) if B % 2 and B % 3 and B % 7]
+    cands = np.concatenate([fc5.twin_cands(B) for B in Bs])
+    N = hi // 42
+    ctx = fc5.Context(0)
+    if args.load_policy is not None:
+        ctx.set_option("load_policy", args.load_policy)
+    if args.blocks_per_cu:
+        ctx.set_option("blocks_per_cu", args.blocks_per_cu)
+    t0 = time.time()
+    ctx.table_build(N, args.r, on_gpu=True)
+    hdr = ctx.table_info()
+    log(f"[bench] 617 twin: B=[{lo},{hi}] eligible_B={len(Bs)} candidates={len(cands)}; device table N={N} "
+        f"keys={hdr.n_keys} packed={fc5_packed_gb(hdr):.3f} GB built in {time.time() - t0:.1f}s")
+    known = {(400471, (13860, 83202, 93240, 129696, 144756, 355502, 357609))}
+    for _ in range(args.warmup):
+        res, hits, sols = ctx.twin_run(cands)
+    outs, kms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, hits, sols = ctx.twin_run(cands)
+        outs.append(res)
+        kms.append(res["kernel_ms"])
+    wall = time.perf_counter() - t0
+    for o in outs[1:]:
+        for k in ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "maybe"):
+            assert o[k] == outs[0][k], k
+    got = {(s["B"], tuple(s["terms"])) for s in sols if s["gcd"] == 1}
+    want = {k for k in known if lo <= k[0] <= hi}
+    assert want <= got, "known solution not recovered"
+    ms_step = 1000.0 * wall / args.steps
+    kern_s = float(np.mean(kms)) / 1000.0
+    calls_s = res["calls"] / (ms_step / 1000.0)
+    cell_bytes = math.ceil(args.r / 8)
+    alg_bytes = res["calls"] * 3 * cell_bytes
+    log(f"[bench] {ms_step:.1f} ms/step, kernel {1000 * kern_s:.1f} ms, {calls_s:.4e} probe calls/s, "
+        f"c3 values {res['calls'] + res['gated']:.4e}, solutions {sorted(got)}")
+    ceiling = None
+    if not args.no_measure:
+        n = 1 << 30
+        pms, rms = ctx.measure_gathers(n)
+        probe_rate = n / (pms / 1000.0)
+        ceiling = {"probe_only_probes_per_s": probe_rate, "raw_random_8B_gathers_per_s": 3 * n / (rms / 1000.0),
+                   "kernel_call_rate_frac_of_probe_only": res["calls"] / kern_s / probe_rate}
+        log(f"[bench] gather ceiling on this table: {probe_rate:.4e} probes/s; kernel at "
+            f"{res['calls'] / kern_s / probe_rate:.1%}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        h, packed = ctx.table_download()
+        t = O.Table.from_bytes({f: getattr(h, f) for f, _ in fc5.StoreHeader._fields_}, packed)
+        sub = cands[:: args.twin_cpu_every]
+        threads = min(16, os.cpu_count() or 1)
+        t1 = time.perf_counter()
+        oc, _, _ = O.twin_run(sub.view(O.TWIN_CAND_DTYPE), t, threads=threads)
+        dt = time.perf_counter() - t1
+        gres, _, _ = ctx.twin_run(sub)
+        for k in ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "maybe"):
+            assert gres[k] == oc[k], k
+        cpu = {"value": oc["calls"] / dt, "unit": "probe calls/s", "cores": threads, "kind": "port",
+               "sample": f"every {args.twin_cpu_every}th candidate ({len(sub)} candidates, {oc['calls']} calls, "
+                         f"{dt:.1f} s), same table bytes; counters identical to the GPU on that sample"}
+        log(f"[bench] cpu oracle {oc['calls'] / dt:.4e} calls/s on {len(sub)} candidates, {threads} threads")
+    line = {
+        "metric": "617 cls5 twin probe calls/sec, 1 GPU, bit-exact", "value": calls_s,
+        "unit": "probe calls/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (deterministic candidate enumeration of the B range)",
+        "config": {"workload": f"(6,1,7) Branch-A class-5 twin, B in [{lo},{hi}] ({len(Bs)} B, {len(cands)} "
+                               f"candidates), xor pair table N={N} r={hdr.r} (device-built)",
+                   "B_lo": lo, "B_hi": hi, "N": N, "r": int(hdr.r), "candidates": int(len(cands))},
+        "eligible_B_per_hour": len(Bs) * 3600.0 / (ms_step / 1000.0),
+        "counters": {k: res[k] for k in ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "probes",
+                                         "maybe", "solutions")},
+        "kernel_ms": float(np.mean(kms)),
+        "roofline": {"bound": "hbm", "achieved": alg_bytes / kern_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg_bytes / kern_s / 1e9 / HBM_PEAK_GBS, "traffic": twin_traffic(lo, hi),
+                     "kernel": "k_twin_batched",
+                     "per_unit": f"{3 * cell_bytes} B per probe call (3 cells x {cell_bytes} B)"},
+        "gather_ceiling": ceiling,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def twin_traffic(lo, hi):
+    """HBM bytes per launch from the committed rocprofv3 counter summary of the
+    same workload, or None"""
+    p = os.path.join(REPO, "profiles", f"traffic_twin617_B{lo}.json")
+    return json.load(open(p)).get("traffic_bytes_per_launch") if lo == hi and os.path.exists(p) else None
 
 
 def fc5_packed_gb(h):
