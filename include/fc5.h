@@ -38,7 +38,8 @@ enum {
   FC5_E_NOMEM = -7,      /* device or host allocation failed */
   FC5_E_STATE = -8,      /* no table loaded, unknown band, ... */
   FC5_E_OVERFLOW = -9,   /* hit buffer overflowed (hits were dropped) */
-  FC5_E_BUILD = -10      /* xor construction stalled at every seed */
+  FC5_E_BUILD = -10,     /* xor construction stalled at every seed */
+  FC5_E_PENDING = -11    /* serving: the band is not finished yet */
 };
 
 /* search modes of fc5_band_run */
@@ -234,6 +235,53 @@ int fc5_twin_run_range(fc5_ctx *ctx, uint64_t B_lo, uint64_t B_hi, double u_lo, 
                        fc5_twin_result *res);
 int fc5_twin_last_hits(fc5_ctx *ctx, fc5_twin_hit *out, size_t cap, size_t *n);
 int fc5_twin_last_solutions(fc5_ctx *ctx, fc5_twin_solution *out, size_t cap, size_t *n);
+
+/* ---- resident serving loop (no reference counterpart: the reference runs one
+ *      band per process, fourcore_cls5_gpu_v4.cu:596-709).  A server thread owns
+ *      the context's stream; bands of Stage-1 units (classes 2-5) are submitted
+ *      at any time, admitted as a counter slot and room in the resident unit
+ *      arenas free up, and served by continuous batching: every scheduling
+ *      round launches the persistent kernel over a slice of every admitted band,
+ *      so bands that arrive while others run share the device with them.  A
+ *      band's counters, hits and solutions equal those of fc5_band_run on the
+ *      same units.  While the server runs, only fc5_serve_* calls (and
+ *      fc5_last_error) may be used on the context; others return FC5_E_STATE.
+ *      The pair table (and side table) must be loaded before fc5_serve_start;
+ *      bands whose B exceed what the table's N covers are refused. ---- */
+typedef struct {
+  int max_bands;          /* bands admitted at once (counter slots), 1..64; default 16 */
+  int rows_per_item;      /* 4, 8, 16, 32; default: the context option */
+  uint64_t arena_units5;  /* resident class-5 unit capacity (64 B each); default 1<<23 */
+  uint64_t arena_units3;  /* resident class 2-4 unit capacity (128 B each); default 1<<22 */
+  double round_points;    /* grid points per scheduling round; default 2e10 (~30 ms) */
+  uint64_t mem_budget;    /* bytes: table + arenas + run buffers must fit; default 288e9 */
+} fc5_serve_opts;
+typedef struct {
+  double t_submit, t_admit, t_done;  /* seconds on the server's steady clock */
+  uint64_t rounds;                   /* scheduling rounds the band took part in */
+} fc5_serve_timing;
+typedef struct {
+  uint64_t submitted, admitted, done, pending, active, rounds;
+  uint64_t arena5_used, arena3_used, device_bytes;
+  double busy_ms;  /* summed event time of the rounds */
+} fc5_serve_stats;
+int fc5_serve_start(fc5_ctx *ctx, const fc5_serve_opts *opts); /* opts may be NULL */
+/* copy in and pack a band; returns its id.  cls may be NULL (all class 5) */
+int fc5_serve_submit(fc5_ctx *ctx, const uint32_t *cls, const fc5_unit *units, size_t n, uint64_t *band_id);
+/* FC5_OK and the band's result when done, FC5_E_PENDING otherwise; a finished
+ * band whose hit buffer overflowed returns FC5_E_OVERFLOW with the result */
+int fc5_serve_poll(fc5_ctx *ctx, uint64_t band_id, fc5_result *res);
+/* blocks until done or timeout_s elapses (timeout_s < 0: no limit) */
+int fc5_serve_wait(fc5_ctx *ctx, uint64_t band_id, double timeout_s, fc5_result *res);
+int fc5_serve_class_result(fc5_ctx *ctx, uint64_t band_id, int cls, fc5_result *out);
+int fc5_serve_hits(fc5_ctx *ctx, uint64_t band_id, fc5_hit *out, size_t cap, size_t *n);
+int fc5_serve_solutions(fc5_ctx *ctx, uint64_t band_id, fc5_solution *out, size_t cap, size_t *n);
+int fc5_serve_timing_of(fc5_ctx *ctx, uint64_t band_id, fc5_serve_timing *out);
+/* forget a finished band (its results are freed) */
+int fc5_serve_release(fc5_ctx *ctx, uint64_t band_id);
+int fc5_serve_stats_get(fc5_ctx *ctx, fc5_serve_stats *out);
+/* finish every submitted band, then stop the server thread */
+int fc5_serve_stop(fc5_ctx *ctx);
 
 /* ---- measurement (no reference counterpart: the roofline microbenchmarks) ---- */
 /* n pure xor-filter queries of pseudo-random keys against the main table at

@@ -80,6 +80,7 @@ extern "C" int fc5_ctx_create(int device, fc5_ctx** out) {
 
 extern "C" void fc5_ctx_destroy(fc5_ctx* ctx) {
   if (!ctx) return;
+  serve_shutdown(ctx);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   free_table(ctx->main);
@@ -101,6 +102,7 @@ extern "C" const char* fc5_last_error(const fc5_ctx* ctx) { return ctx ? ctx->er
 
 extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) {
   if (!ctx || !key) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   const std::string k = key;
   if (k == "hit_cap") {
     if (v < 1 || v > (int64_t)1 << 32) FAIL(ctx, FC5_E_ARG, "hit_cap out of range");
@@ -191,6 +193,7 @@ static int adopt_main(fc5_ctx* ctx, XorTable&& t) {
 
 extern "C" int fc5_table_load(fc5_ctx* ctx, const char* path, int expected_N) {
   if (!ctx || !path) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   XorTable t;
   std::string msg;
   const int st = load_bin(path, expected_N, t, msg);
@@ -210,6 +213,7 @@ extern "C" int fc5_table_load(fc5_ctx* ctx, const char* path, int expected_N) {
 extern "C" int fc5_table_upload(fc5_ctx* ctx, const fc5_store_header* hdr, const uint8_t* packed,
                                 size_t len) {
   if (!ctx || !hdr || (!packed && len)) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   XorTable t;
   std::memcpy(&t.hdr, hdr, sizeof t.hdr);
   if (t.hdr.magic != 0x53353136u || t.hdr.version != 2 || t.hdr.kind != 1 || t.hdr.r == 0 || t.hdr.r > 64)
@@ -221,6 +225,7 @@ extern "C" int fc5_table_upload(fc5_ctx* ctx, const fc5_store_header* hdr, const
 extern "C" int fc5_table_build(fc5_ctx* ctx, int N, int r, const uint64_t* extra, size_t n_extra,
                                int on_gpu) {
   if (!ctx || N < 1 || r < 1 || r > 64 || (!extra && n_extra)) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   if (on_gpu) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
     free_table(ctx->main);
@@ -288,6 +293,7 @@ static int table_packed(fc5_ctx* ctx, XorTable& t, bool want_bytes) {
 
 extern "C" int fc5_table_save(fc5_ctx* ctx, const char* path) {
   if (!ctx || !path) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   XorTable t;
   const int rc = table_packed(ctx, t, true);
   if (rc != FC5_OK) return rc;
@@ -298,6 +304,7 @@ extern "C" int fc5_table_save(fc5_ctx* ctx, const char* path) {
 
 extern "C" int fc5_table_download(fc5_ctx* ctx, fc5_store_header* hdr, uint8_t* packed, size_t cap, size_t* len) {
   if (!ctx || !len) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no table");
   const size_t want = packed_bytes(ctx->main.hdr.m_cells, ctx->main.hdr.r);
   *len = want;
@@ -334,6 +341,7 @@ static int device_query(fc5_ctx* ctx, const XorDev& x, const uint64_t* keys, siz
 
 extern "C" int fc5_table_query(fc5_ctx* ctx, const uint64_t* keys, size_t n, uint8_t* out) {
   if (!ctx || (!keys && n) || (!out && n)) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no table");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   return device_query(ctx, ctx->main.dev, keys, n, out);
@@ -341,6 +349,7 @@ extern "C" int fc5_table_query(fc5_ctx* ctx, const uint64_t* keys, size_t n, uin
 
 extern "C" int fc5_table_fpr_smoke(fc5_ctx* ctx, int trials, int* hits) {
   if (!ctx || !hits || trials < 0) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no table");
   std::vector<u64> keys((size_t)trials);
   u64 x = 0x123456789abcdef0ULL;
@@ -357,6 +366,7 @@ extern "C" int fc5_table_fpr_smoke(fc5_ctx* ctx, int trials, int* hits) {
 
 extern "C" int fc5_side_build(fc5_ctx* ctx, const uint64_t* keys, size_t n, int r) {
   if (!ctx || (!keys && n) || r < 1 || r > 64) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   std::vector<u64> k(keys, keys + n);
   XorTable t;
   if (!build_xor_cpu(k, (u32)r, 0x51DE51DE51DEULL, 64, t)) FAIL(ctx, FC5_E_BUILD, "side table build stalled");
@@ -365,6 +375,7 @@ extern "C" int fc5_side_build(fc5_ctx* ctx, const uint64_t* keys, size_t n, int 
 
 extern "C" int fc5_side_clear(fc5_ctx* ctx) {
   if (!ctx) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   free_table(ctx->side);
   return FC5_OK;
 }
@@ -372,7 +383,7 @@ extern "C" int fc5_side_clear(fc5_ctx* ctx) {
 // ---------------------------------------------------------------------------
 // bands
 // ---------------------------------------------------------------------------
-static int ensure_ndg(fc5_ctx* ctx, u32 dmax) {
+int fc5::ensure_ndg(fc5_ctx* ctx, u32 dmax) {
   if (ctx->d_ndg && ctx->ndg_dmax >= dmax) return FC5_OK;
   const u32 K = dmax / 64 + 1;
   std::vector<u32> h((size_t)64 * K);
@@ -423,14 +434,60 @@ extern "C" int fc5_band_destroy(fc5_ctx* ctx, fc5_band* band) {
   if (ctx) (void)hipSetDevice(ctx->device);
   if (band->graph_exec) (void)hipGraphExecDestroy(band->graph_exec);
   if (band->graph) (void)hipGraphDestroy(band->graph);
-  for (void* p : {(void*)band->d_units, (void*)band->d_plain, (void*)band->d_units3})
+  for (void* p : {(void*)band->d_units, (void*)band->d_plain, (void*)band->d_units3, (void*)band->d_segs})
     if (p) (void)hipFree(p);
   delete band;
   return FC5_OK;
 }
 
-// work items: ceil(rows / E) per unit, classes 2-4 numbered after class 5
-static void number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3) {
+DevUnit5 fc5::dev_unit5(const Cls5Unit& U, u32 uid, u32 band) {
+  DevUnit5 D{};
+  D.Q_lo = (u64)U.Q;
+  D.Q_hi = (u64)(U.Q >> 64);
+  D.fmax1 = U.fmax1;
+  D.fmax2 = U.fmax2;
+  D.rows = 6u * (U.fmax1 / 729u + 1u);
+  D.te = U.te;
+  D.td = U.td;
+  D.uid = uid;
+  D.band = band;
+  for (int k = 0; k < 6; ++k) D.res1[k] = (u16)U.res1[k];
+  for (int k = 0; k < 4; ++k) D.res2[k] = (u8)U.res2[k];
+  return D;
+}
+
+DevUnit3 fc5::dev_unit3(const Cls234Unit& U, u32 uid, u32 band) {
+  DevUnit3 D{};
+  D.Q_lo = (u64)U.Q;
+  D.Q_hi = (u64)(U.Q >> 64);
+  D.inv_lo = (u64)U.inv;
+  D.inv_hi = (u64)(U.inv >> 64);
+  D.fmax = U.fmax;
+  D.lim = U.lim;
+  D.rows = U.nres * (U.fmax / U.mod + 1u);
+  D.mod = U.mod;
+  D.dr = U.dr;
+  D.shift = U.shift;
+  D.nres = U.nres;
+  D.uid = uid;
+  D.band = band;
+  for (u32 k = 0; k < U.nres; ++k) D.res[k] = (u16)U.res[k];
+  return D;
+}
+
+// the band's launch segments: its class-5 items, then its class 2-4 items
+static int upload_segments(fc5_ctx* ctx, fc5_band* b) {
+  Segment seg[2];
+  u32 n = 0;
+  if (b->n_items) seg[n++] = Segment{0, 0, 0, b->n_dev, 0, 0};
+  if (b->n_items3) seg[n++] = Segment{b->n_items, 0, 0, b->n_dev3, 1, 0};
+  if (!b->d_segs && hipMalloc(&b->d_segs, 2 * sizeof(Segment)) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "segments");
+  if (n) HIPCHK(ctx, hipMemcpy(b->d_segs, seg, n * sizeof(Segment), hipMemcpyHostToDevice));
+  b->n_segs = n;
+  return FC5_OK;
+}
+
+void fc5::number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3) {
   n5 = 0;
   for (DevUnit5& D : dev) {
     D.item_base = n5;
@@ -452,11 +509,12 @@ static int renumber(fc5_ctx* ctx, fc5_band* b) {
     HIPCHK(ctx, hipMemcpy(b->d_units3, b->dev3_h.data(), b->dev3_h.size() * sizeof(DevUnit3),
                           hipMemcpyHostToDevice));
   b->rows = ctx->rows_per_item;
-  return FC5_OK;
+  return upload_segments(ctx, b);
 }
 
 static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units, size_t n, fc5_band** out) {
   if (!ctx || !out || (!units && n)) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   if (n > 0x7fffffffull) FAIL(ctx, FC5_E_ARG, "too many units in one band");
   *out = nullptr;
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -493,19 +551,7 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
       for (u32 k = 0; k < U.n2; ++k) P.res[U.n1 + k] = U.res2[k];
       plain.push_back(P);
       if (U.regular) {
-        DevUnit5 D{};
-        D.Q_lo = (u64)U.Q;
-        D.Q_hi = (u64)(U.Q >> 64);
-        D.fmax1 = U.fmax1;
-        D.fmax2 = U.fmax2;
-        D.rows = 6u * (U.fmax1 / 729u + 1u);
-        D.te = U.te;
-        D.td = U.td;
-        D.uid = (u32)i;
-        D.band = 5;
-        for (int k = 0; k < 6; ++k) D.res1[k] = (u16)U.res1[k];
-        for (int k = 0; k < 4; ++k) D.res2[k] = (u8)U.res2[k];
-        dev.push_back(D);
+        dev.push_back(dev_unit5(U, (u32)i, 5));
       } else {
         b->irregular.push_back((u32)b->units.size());
       }
@@ -518,22 +564,7 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
       b->slot_of[i] = (int)b->units3.size();
       b->max_lim3 = std::max(b->max_lim3, U.lim);
       if (U.regular) {
-        DevUnit3 D{};
-        D.Q_lo = (u64)U.Q;
-        D.Q_hi = (u64)(U.Q >> 64);
-        D.inv_lo = (u64)U.inv;
-        D.inv_hi = (u64)(U.inv >> 64);
-        D.fmax = U.fmax;
-        D.lim = U.lim;
-        D.rows = U.nres * (U.fmax / U.mod + 1u);
-        D.mod = U.mod;
-        D.dr = U.dr;
-        D.shift = U.shift;
-        D.nres = U.nres;
-        D.uid = (u32)i;
-        D.band = (u32)c;
-        for (u32 k = 0; k < U.nres; ++k) D.res[k] = (u16)U.res[k];
-        dev3.push_back(D);
+        dev3.push_back(dev_unit3(U, (u32)i, (u32)c));
       } else {
         b->irregular3.push_back((u32)b->units3.size());
       }
@@ -559,6 +590,7 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
   upload((void**)&b->d_units3, dev3.data(), dev3.size() * sizeof(DevUnit3));
   b->dev_h = std::move(dev);
   b->dev3_h = std::move(dev3);
+  if (rc == FC5_OK) rc = upload_segments(ctx, b.get());
   if (rc == FC5_OK && !b->units.empty()) rc = ensure_ndg(ctx, b->max_fmax2);
   if (rc == FC5_OK && !b->units3.empty()) rc = ensure_c6g(ctx, b->max_lim3 + 64);
   if (rc != FC5_OK) {
@@ -714,8 +746,9 @@ static int run_plain_find3(fc5_ctx* ctx, fc5_band* b, const std::vector<u32>& wh
 static BatchedArgs batched_args(fc5_ctx* ctx, fc5_band* b) {
   BatchedArgs a{};
   a.units = b->d_units;
-  a.n_units = b->n_dev;
-  a.n_items = b->n_items;
+  a.segs = b->d_segs;
+  a.n_segs = b->n_segs;
+  a.total = b->n_items + b->n_items3;
   a.gate = ctx->use_gate ? ctx->d_gate : ctx->d_gate_ones;
   a.ndg = ctx->d_ndg;
   a.ndg_K = ctx->ndg_K;
@@ -729,8 +762,6 @@ static BatchedArgs batched_args(fc5_ctx* ctx, fc5_band* b) {
   a.item_ctr = ctx->d_item_ctr;
   a.inv729 = inv_mod_2_64(729);
   a.units3 = b->d_units3;
-  a.n_units3 = b->n_dev3;
-  a.n_items3 = b->n_items3;
   a.c6g = ctx->d_c6g;
   return a;
 }
@@ -782,6 +813,7 @@ static int ensure_graph(fc5_ctx* ctx, fc5_band* b, const BatchedArgs& a, int gri
 
 extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res) {
   if (!ctx || !b || !res) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   if (mode != FC5_MODE_PLAIN && mode != FC5_MODE_BATCHED) FAIL(ctx, FC5_E_ARG, "unknown mode");
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no pair table loaded");
   const auto t0 = std::chrono::steady_clock::now();
@@ -877,7 +909,7 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   res->kernel_ms = ms;
   ctx->hits.resize(stored);
   for (u64 i = 0; i < stored; ++i)
-    ctx->hits[i] = fc5_hit{raw[i].unit, raw[i].d, raw[i].e, raw[i].flags, raw[i].fp};
+    ctx->hits[i] = fc5_hit{raw[i].unit, raw[i].d, raw[i].e, raw[i].flags & 0xffu, raw[i].fp};
   std::sort(ctx->hits.begin(), ctx->hits.end(), [](const fc5_hit& x, const fc5_hit& y) {
     if (x.unit != y.unit) return x.unit < y.unit;
     if (x.e != y.e) return x.e < y.e;
@@ -926,6 +958,7 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
 
 extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tune* out) {
   if (!ctx || !b || !out || sample_every < 1) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   std::vector<fc5_unit> su;
   std::vector<u32> sc;
   for (size_t i = 0; i < b->n_in; i += (size_t)sample_every) {
@@ -1050,6 +1083,7 @@ extern "C" int fc5_build_pairs_file(int N, int r, const char* path, fc5_store_he
 // ---------------------------------------------------------------------------
 extern "C" int fc5_measure_gathers(fc5_ctx* ctx, uint64_t n, double* probe_ms, double* raw_ms) {
   if (!ctx || !probe_ms || !raw_ms || n == 0) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   const u32 policy = ctx->main.dev.policy;
   if (!ctx->main.present || !ctx->main.cells) FAIL(ctx, FC5_E_STATE, "no device table");
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1079,6 +1113,7 @@ extern "C" int fc5_measure_gathers(fc5_ctx* ctx, uint64_t n, double* probe_ms, d
 
 extern "C" int fc5_measure_copy(fc5_ctx* ctx, uint64_t bytes, double* ms) {
   if (!ctx || !ms || bytes < 8) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const u64 n = bytes / 8;
   u64 *src = nullptr, *dst = nullptr;

@@ -59,6 +59,8 @@ struct fc5_band {
   std::vector<u32> irregular3;
   DevUnit3* d_units3 = nullptr;
   u32 n_dev3 = 0, n_items3 = 0;
+  Segment* d_segs = nullptr;        // launch segments over the band's two unit arrays
+  u32 n_segs = 0;
   u32 max_lim3 = 0;
   // inputs and host images of the device units (re-numbered per rows-per-item)
   std::vector<fc5_unit> in_units;
@@ -118,7 +120,17 @@ struct fc5_ctx {
   size_t d_twin_cap = 0;
   std::vector<fc5_twin_hit> twin_hits;
   std::vector<fc5_twin_solution> twin_sols;
+  // resident serving loop (fc5_serve.cpp); while set, only fc5_serve_* may run
+  struct fc5_server* server = nullptr;
 };
+
+#define NOT_SERVING(ctx)                                                              \
+  do {                                                                                \
+    if ((ctx)->server) {                                                              \
+      (ctx)->err = "the context is serving: only fc5_serve_* calls are allowed";      \
+      return FC5_E_STATE;                                                             \
+    }                                                                                 \
+  } while (0)
 
 #define FAIL(ctx, code, msg)  \
   do {                        \
@@ -138,7 +150,14 @@ struct fc5_ctx {
 namespace fc5 {
 // shared run-buffer helpers (fc5_api.cpp)
 int ensure_c6g(fc5_ctx* ctx, u32 cmax);
+int ensure_ndg(fc5_ctx* ctx, u32 dmax);
 int ensure_hits(fc5_ctx* ctx);
+DevUnit5 dev_unit5(const Cls5Unit& U, u32 uid, u32 band);
+DevUnit3 dev_unit3(const Cls234Unit& U, u32 uid, u32 band);
+// work items: ceil(rows / E) per unit, numbered per arena
+void number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3);
+// serving loop (fc5_serve.cpp): stops and frees the server of a context
+void serve_shutdown(fc5_ctx* ctx);
 // twin buffers (fc5_twin_api.cpp)
 void free_twin(fc5_ctx* ctx);
 }  // namespace fc5

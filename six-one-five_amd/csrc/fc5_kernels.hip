@@ -1,7 +1,9 @@
 // fc5_kernels.hip -- class-5 free-peel kernels for gfx950 (MI355X).
 //
 // k_cls5_batched  persistent kernel fed by a device work queue (one item =
-//                 one unit x 8 e-rows x all d).  Per point it tests the
+//                 one unit x E e-rows x all d, or E d-rows of a class 2-4
+//                 unit; the launch's items are segments of unit arenas, so
+//                 one launch can serve several bands).  Per point it tests the
 //                 sixth-power residue gate in T-space with 32-bit VALU work
 //                 and an LDS bitmap; survivors are compacted per wavefront
 //                 (ballot + mbcnt into an LDS ring) and probed 64 at a time:
@@ -67,7 +69,7 @@ __device__ __forceinline__ void drain(const BatchedArgs& a, const u32* qu, const
         h.unit = uid;
         h.d = d;
         h.e = e;
-        h.flags = flags;
+        h.flags = flags | (band << 8);
         h.fp = fp;
         a.hits[s] = h;
       }
@@ -129,9 +131,10 @@ __device__ __forceinline__ void flush_counters(Counters* c, u32 lane, u32 calls,
 
 // one class-5 item: E e-rows of one unit, every d
 template <int E>
-__device__ __forceinline__ void cls5_item(const BatchedArgs& a, const u32* s_gate, Ring& q, u32 item, u32 lane) {
+__device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_units, const u32* s_gate, Ring& q,
+                                          u32 item, u32 lane) {
   const u32 K = a.ndg_K;
-  u32 lo = 0, hi = a.n_units;
+  u32 lo = u0, hi = u0 + n_units;
   while (hi - lo > 1) {
     const u32 mid = (lo + hi) >> 1;
     if (a.units[mid].item_base <= item) lo = mid;
@@ -215,8 +218,9 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, const u32* s_gat
 // one class 2-4 item: E d-rows of one unit, every c3 of each row's window
 // (fourcore_find_v4.cu:165-205 expansion, :401-425 window, :255-284 sweep)
 template <int E>
-__device__ __forceinline__ void find3_item(const BatchedArgs& a, const u32* s_gate, Ring& q, u32 item, u32 lane) {
-  u32 lo = 0, hi = a.n_units3;
+__device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_units, const u32* s_gate, Ring& q,
+                                           u32 item, u32 lane) {
+  u32 lo = u0, hi = u0 + n_units;
   while (hi - lo > 1) {
     const u32 mid = (lo + hi) >> 1;
     if (a.units3[mid].item_base <= item) lo = mid;
@@ -281,14 +285,21 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
   const u32 lane = threadIdx.x & 63u;
   const u32 w = threadIdx.x >> 6;
   Ring q{s_qu[w], s_qd[w], s_qe[w], 0u, 0u};
-  const u32 total = a.n_items + a.n_items3;
   for (;;) {
     u32 item = 0;
     if (lane == 0) item = atomicAdd(a.item_ctr, 1u);
     item = __shfl(item, 0, 64);
-    if (item >= total) break;
-    if (item < a.n_items) cls5_item<E>(a, s_gate, q, item, lane);
-    else find3_item<E>(a, s_gate, q, item - a.n_items, lane);
+    if (item >= a.total) break;
+    u32 lo = 0, hi = a.n_segs;  // the segment holding this launch item
+    while (hi - lo > 1) {
+      const u32 mid = (lo + hi) >> 1;
+      if (a.segs[mid].g0 <= item) lo = mid;
+      else hi = mid;
+    }
+    const Segment S = a.segs[lo];
+    const u32 local = S.item0 + (item - S.g0);
+    if (S.kind == 0) cls5_item<E>(a, S.u0, S.n_units, s_gate, q, local, lane);
+    else find3_item<E>(a, S.u0, S.n_units, s_gate, q, local, lane);
   }
   if (q.tail != q.head) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

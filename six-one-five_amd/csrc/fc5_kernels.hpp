@@ -64,7 +64,7 @@ struct DevUnit3 {
   u64 Q_lo, Q_hi;
   u64 inv_lo, inv_hi;   // inverse of the odd part of mod, mod 2^128 (exact quotients)
   u32 fmax, lim;
-  u32 rows, item_base;  // rows = nres * (fmax/mod + 1); first item after the cls5 items
+  u32 rows, item_base;  // rows = nres * (fmax/mod + 1); first work item (numbered per arena)
   u32 mod, dr;          // dr = d^6 mod mod over the residue classes
   u32 shift, nres;      // mod = 2^shift * odd
   u32 uid, band;
@@ -85,7 +85,7 @@ struct Counters {
 };
 
 struct HitRec {
-  u32 unit, d, e, flags;
+  u32 unit, d, e, flags;  // flags: bit0 main table, bit1 side table; batched kernel: counter slot << 8
   u64 fp;
 };
 
@@ -94,9 +94,17 @@ struct GateWords {  // reference bitmaps (GateData.g504/g247, fourcore_cls5_gpu_
   u64 g247[4];
 };
 
+// A run of consecutive work items of one launch: items g0.. map to the
+// unit-local items item0.. of the units [u0, u0 + n_units) of one arena
+// (kind 0: class-5 units, kind 1: class 2-4 units).
+struct Segment {
+  u32 g0, item0, u0, n_units, kind, pad;
+};
+
 struct BatchedArgs {
-  const DevUnit5* units;
-  u32 n_units, n_items;
+  const DevUnit5* units;  // class-5 unit arena (ring entries index it)
+  const Segment* segs;
+  u32 n_segs, total;      // segments and work items of this launch
   const u32* gate;  // 124488-bit bitmap
   const u32* ndg;   // [64][ndg_K]
   u32 ndg_K, use_gate;
@@ -107,9 +115,7 @@ struct BatchedArgs {
   Counters* ctr;  // indexed by DevUnit5::band
   u32* item_ctr;
   u64 inv729;  // 729^-1 mod 2^64
-  // classes 2-4 (find3 rows); their items follow the n_items cls5 items
-  const DevUnit3* units3;
-  u32 n_units3, n_items3;
+  const DevUnit3* units3;  // class 2-4 unit arena
   const u32* c6g;  // (124488 - c^6 mod 124488) for c = 0..lim
 };
 

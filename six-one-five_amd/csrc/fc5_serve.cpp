@@ -1,0 +1,675 @@
+// fc5_serve.cpp -- resident serving loop of include/fc5.h (fc5_serve_*).
+//
+// One server thread per context owns the context's stream.  Submitted bands
+// are packed on the caller's thread and queued; the server admits them in
+// arrival order when a counter slot is free and their unit arrays fit into the
+// resident unit arenas (first-fit ranges of two device arrays sized from the
+// memory budget).  Every scheduling round builds a segment list that gives
+// each admitted band an equal share of the round's grid points (its next work
+// items), launches the persistent kernel once over all of them, reads back the
+// per-slot counters and the round's hits, and finalizes the bands whose last
+// items ran.  Bands that arrive while others run join at the next round and
+// share the device with them (continuous batching).  Counters are sums over
+// work items and hits are sorted, so a band's result does not depend on what
+// it shared rounds with.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "fc5_ctx.hpp"
+
+using namespace fc5;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr u32 kSlotStride = 8;  // counter slots per band: index slot * 8 + class
+
+// first-fit ranges over [0, cap)
+struct Arena {
+  u64 cap = 0, used = 0;
+  std::map<u64, u64> free_;  // offset -> length
+  void init(u64 c) {
+    cap = c;
+    used = 0;
+    free_.clear();
+    if (c) free_[0] = c;
+  }
+  bool alloc(u64 n, u64& off) {
+    off = 0;
+    if (!n) return true;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      off = it->first;
+      const u64 rest = it->second - n;
+      free_.erase(it);
+      if (rest) free_[off + n] = rest;
+      used += n;
+      return true;
+    }
+    return false;
+  }
+  void release(u64 off, u64 n) {
+    if (!n) return;
+    used -= n;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+enum BandState { kPending, kActive, kDone };
+
+struct SBand {
+  u64 id = 0;
+  BandState state = kPending;
+  int rc = FC5_OK;
+  size_t n_in = 0, n_skipped = 0;
+  std::vector<u8> cls_of;    // class of each caller index (0 = skipped)
+  std::vector<int> slot_of;  // caller index -> index in units5 / units3
+  std::vector<Cls5Unit> units5;
+  std::vector<Cls234Unit> units3;
+  std::vector<DevUnit5> dev5;
+  std::vector<DevUnit3> dev3;
+  u32 items5 = 0, items3 = 0, next5 = 0, next3 = 0;
+  double cost5 = 1, cost3 = 1;  // estimated grid points (c3 values) per work item
+  u64 off5 = 0, off3 = 0;
+  int slot = -1;
+  bool overflow = false;
+  std::vector<fc5_hit> hits;
+  std::vector<fc5_solution> sols;
+  fc5_result res{};
+  fc5_result cls_res[6]{};
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  u64 rounds = 0;
+  double kernel_ms = 0;
+  bool finished_items() const { return next5 >= items5 && next3 >= items3; }
+};
+
+}  // namespace
+
+struct fc5_server {
+  fc5_ctx* ctx = nullptr;
+  fc5_serve_opts opt{};
+  int rows = kRowsPerItem;
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stopping = false;
+  int fail_rc = FC5_OK;
+  u64 next_id = 1;
+  std::deque<u64> pending;
+  std::map<u64, std::unique_ptr<SBand>> bands;
+  std::vector<SBand*> slots;  // admitted band per counter slot
+  Arena a5, a3;
+  DevUnit5* d5 = nullptr;
+  DevUnit3* d3 = nullptr;
+  Segment* d_segs = nullptr;
+  Segment* h_segs = nullptr;  // pinned
+  Counters* d_ctr = nullptr;
+  Counters* h_ctr = nullptr;  // pinned
+  HitRec* h_hits = nullptr;   // pinned
+  u64* h_nhits = nullptr;     // pinned
+  u32 max_fmax2 = 0, max_lim = 0;
+  u64 submitted = 0, admitted = 0, done = 0, rounds = 0, device_bytes = 0;
+  double busy_ms = 0;
+  Clock::time_point t0 = Clock::now();
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+void free_server_buffers(fc5_server* S) {
+  for (void* p : {(void*)S->d5, (void*)S->d3, (void*)S->d_segs, (void*)S->d_ctr})
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)S->h_segs, (void*)S->h_ctr, (void*)S->h_hits, (void*)S->h_nhits})
+    if (p) (void)hipHostFree(p);
+  S->d5 = nullptr;
+  S->d3 = nullptr;
+  S->d_segs = nullptr;
+  S->d_ctr = nullptr;
+  S->h_segs = nullptr;
+  S->h_ctr = nullptr;
+  S->h_hits = nullptr;
+  S->h_nhits = nullptr;
+}
+
+// the band's result from its counter slots (same reduction as fc5_band_run)
+void collect(fc5_server* S, SBand& B) {
+  const Counters* c = S->h_ctr + (size_t)B.slot * kSlotStride;
+  std::memset(&B.res, 0, sizeof B.res);
+  std::memset(B.cls_res, 0, sizeof B.cls_res);
+  for (int k = 2; k <= 5; ++k) {
+    const Counters& C = c[k];
+    fc5_result& r = B.cls_res[k];
+    r.calls = C.calls;
+    r.neg = C.neg;
+    r.zero = C.zero;
+    r.probes = C.probes;
+    r.gated = C.pos - C.probes;
+    r.maybe = C.maybe;
+    r.side_maybe = C.side_maybe;
+    r.jobs = C.jobs;
+    r.fail_t = C.failT;
+    B.res.calls += r.calls;
+    B.res.neg += r.neg;
+    B.res.zero += r.zero;
+    B.res.probes += r.probes;
+    B.res.gated += r.gated;
+    B.res.maybe += r.maybe;
+    B.res.side_maybe += r.side_maybe;
+    B.res.jobs += r.jobs;
+    B.res.fail_t += r.fail_t;
+  }
+  for (size_t i = 0; i < B.n_in; ++i)
+    if (B.cls_of[i]) B.cls_res[B.cls_of[i]].units++;
+}
+
+void finalize(fc5_server* S, SBand& B) {
+  fc5_ctx* ctx = S->ctx;
+  if (B.slot >= 0) collect(S, B);
+  std::sort(B.hits.begin(), B.hits.end(), [](const fc5_hit& x, const fc5_hit& y) {
+    if (x.unit != y.unit) return x.unit < y.unit;
+    if (x.e != y.e) return x.e < y.e;
+    if (x.d != y.d) return x.d < y.d;
+    return x.flags < y.flags;
+  });
+  if (ctx->verify && !B.hits.empty()) {
+    const int N = (int)ctx->main.hdr.N;
+    std::vector<VerifyJob> jobs;
+    for (const fc5_hit& h : B.hits) {
+      if (!(h.flags & 1u) || h.unit >= B.n_in) continue;
+      const int k = B.cls_of[h.unit], sl = B.slot_of[h.unit];
+      if (k == 5) {
+        const Cls5Unit& U = B.units5[(size_t)sl];
+        jobs.push_back(VerifyJob{5, U.B, U.u, U.lim, h.d, h.e, h.fp});
+      } else if (k >= 2) {
+        const Cls234Unit& U = B.units3[(size_t)sl];
+        jobs.push_back(VerifyJob{k, U.B, U.u, U.lim, h.d, h.e, h.fp});
+      }
+    }
+    std::vector<Solution> sols;
+    verify_jobs(N, jobs, false, sols);
+    for (const Solution& x : sols) {
+      fc5_solution o{};
+      o.B = x.B;
+      o.u = x.u;
+      o.cls = x.cls;
+      o.d = x.d;
+      o.e = x.e;
+      o.a = x.a;
+      o.b = x.b;
+      o.c = x.c;
+      for (int k = 0; k < 5; ++k) o.terms[k] = x.terms[k];
+      o.gcd = x.gcd;
+      B.sols.push_back(o);
+    }
+  }
+  B.res.n_hits = B.hits.size();
+  B.res.hits_dropped = B.overflow ? 1 : 0;
+  B.res.solutions = B.sols.size();
+  B.res.units = B.units5.size() + B.units3.size();
+  B.res.units_skipped = B.n_skipped;
+  B.res.kernel_ms = B.kernel_ms;
+  B.t_done = S->now();
+  B.res.wall_ms = 1000.0 * (B.t_done - B.t_submit);
+  if (B.overflow) B.rc = FC5_E_OVERFLOW;
+  B.state = kDone;
+  // the unit arrays are no longer needed
+  B.dev5 = {};
+  B.dev3 = {};
+}
+
+// admission in arrival order (under the lock); device uploads happen here,
+// between rounds, on the server thread
+int admit_locked(fc5_server* S) {
+  fc5_ctx* ctx = S->ctx;
+  while (!S->pending.empty()) {
+    SBand& B = *S->bands.at(S->pending.front());
+    if (B.items5 + B.items3 == 0) {  // nothing to run
+      S->pending.pop_front();
+      B.t_admit = S->now();
+      finalize(S, B);
+      S->done++;
+      S->cv_done.notify_all();
+      continue;
+    }
+    int slot = -1;
+    for (size_t i = 0; i < S->slots.size(); ++i)
+      if (!S->slots[i]) { slot = (int)i; break; }
+    if (slot < 0) return FC5_OK;
+    u64 off5 = 0, off3 = 0;
+    if (!S->a5.alloc(B.dev5.size(), off5)) return FC5_OK;
+    if (!S->a3.alloc(B.dev3.size(), off3)) {
+      S->a5.release(off5, B.dev5.size());
+      return FC5_OK;
+    }
+    S->pending.pop_front();
+    B.slot = slot;
+    B.off5 = off5;
+    B.off3 = off3;
+    for (DevUnit5& D : B.dev5) D.band = (u32)slot * kSlotStride + 5;
+    for (DevUnit3& D : B.dev3) D.band = (u32)slot * kSlotStride + (D.band % kSlotStride);
+    if (!B.dev5.empty())
+      HIPCHK(ctx, hipMemcpy(S->d5 + off5, B.dev5.data(), B.dev5.size() * sizeof(DevUnit5), hipMemcpyHostToDevice));
+    if (!B.dev3.empty())
+      HIPCHK(ctx, hipMemcpy(S->d3 + off3, B.dev3.data(), B.dev3.size() * sizeof(DevUnit3), hipMemcpyHostToDevice));
+    if (launch_fill_u64(reinterpret_cast<u64*>(S->d_ctr + (size_t)slot * kSlotStride),
+                        kSlotStride * sizeof(Counters) / 8, 0, ctx->stream) != 0)
+      FAIL(ctx, FC5_E_HIP, "counter reset");
+    S->slots[(size_t)slot] = &B;
+    B.state = kActive;
+    B.t_admit = S->now();
+    S->admitted++;
+  }
+  return FC5_OK;
+}
+
+// one scheduling round over the admitted bands
+int run_round(fc5_server* S, std::vector<SBand*>& active) {
+  fc5_ctx* ctx = S->ctx;
+  const double share = S->opt.round_points / (double)std::max<size_t>(1, active.size());
+  u32 n_segs = 0, total = 0;
+  std::vector<std::pair<SBand*, std::pair<u32, u32>>> took;  // band, items of each kind
+  for (SBand* B : active) {
+    const u32 r5 = B->items5 - B->next5, r3 = B->items3 - B->next3;
+    u32 n5 = 0, n3 = 0;
+    double left = share;
+    if (r5) {
+      n5 = (u32)std::min<double>(r5, std::max(1.0, left / B->cost5));
+      left -= n5 * B->cost5;
+    }
+    if (r3 && (left > 0 || !n5)) n3 = (u32)std::min<double>(r3, std::max(1.0, left / B->cost3));
+    if (n5) {
+      S->h_segs[n_segs++] = Segment{total, B->next5, (u32)B->off5, (u32)B->dev5.size(), 0, 0};
+      total += n5;
+    }
+    if (n3) {
+      S->h_segs[n_segs++] = Segment{total, B->next3, (u32)B->off3, (u32)B->dev3.size(), 1, 0};
+      total += n3;
+    }
+    took.push_back({B, {n5, n3}});
+  }
+  BatchedArgs a{};
+  a.units = S->d5;
+  a.units3 = S->d3;
+  a.segs = S->d_segs;
+  a.n_segs = n_segs;
+  a.total = total;
+  a.gate = ctx->use_gate ? ctx->d_gate : ctx->d_gate_ones;
+  a.ndg = ctx->d_ndg;
+  a.ndg_K = ctx->ndg_K;
+  a.use_gate = (u32)ctx->use_gate;
+  a.main = ctx->main.dev;
+  a.side = ctx->side.present ? ctx->side.dev : XorDev{};
+  a.hits = ctx->d_hits;
+  a.hit_cap = ctx->hit_cap;
+  a.nhits = ctx->d_nhits;
+  a.ctr = S->d_ctr;
+  a.item_ctr = ctx->d_item_ctr;
+  a.inv729 = inv_mod_2_64(729);
+  a.c6g = ctx->d_c6g;
+  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(S->rows);
+  const int grid = (int)std::max<u64>(1, std::min<u64>((u64)ctx->num_cu * (u64)bpc, total));
+  HIPCHK(ctx, hipMemcpyAsync(S->d_segs, S->h_segs, n_segs * sizeof(Segment), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (launch_fill_u64(ctx->d_nhits, 1, 0, ctx->stream) != 0 ||
+      launch_fill_u64(reinterpret_cast<u64*>(ctx->d_item_ctr), 1, 0, ctx->stream) != 0)
+    FAIL(ctx, FC5_E_HIP, "counter reset");
+  if (launch_cls5_batched(a, grid, S->rows, ctx->stream) != 0) FAIL(ctx, FC5_E_HIP, "serving launch");
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  const size_t nslots = S->slots.size() * kSlotStride;
+  HIPCHK(ctx, hipMemcpyAsync(S->h_ctr, S->d_ctr, nslots * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(S->h_nhits, ctx->d_nhits, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  const u64 nh = *S->h_nhits, stored = std::min<u64>(nh, ctx->hit_cap);
+  if (stored)
+    HIPCHK(ctx, hipMemcpy(S->h_hits, ctx->d_hits, stored * sizeof(HitRec), hipMemcpyDeviceToHost));
+  for (u64 i = 0; i < stored; ++i) {
+    const HitRec& h = S->h_hits[i];
+    const u32 slot = (h.flags >> 8) / kSlotStride;
+    if (slot < S->slots.size() && S->slots[slot])
+      S->slots[slot]->hits.push_back(fc5_hit{h.unit, h.d, h.e, h.flags & 0xffu, h.fp});
+  }
+  S->busy_ms += ms;
+  S->rounds++;
+  for (auto& [B, n] : took) {
+    B->next5 += n.first;
+    B->next3 += n.second;
+    B->rounds++;
+    B->kernel_ms += ms;
+    if (nh > stored) B->overflow = true;
+    // running estimates of the work per item from the band's own counters
+    const Counters* c = S->h_ctr + (size_t)B->slot * kSlotStride;
+    if (B->next5) B->cost5 = std::max(1.0, (double)c[5].calls / B->next5);
+    const double c3 = (double)(c[2].calls + c[3].calls + c[4].calls);
+    if (B->next3 && c3 > 0) B->cost3 = std::max(1.0, c3 / B->next3);
+  }
+  return FC5_OK;
+}
+
+void serve_loop(fc5_server* S) {
+  fc5_ctx* ctx = S->ctx;
+  (void)hipSetDevice(ctx->device);
+  for (;;) {
+    std::vector<SBand*> active;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      int rc = admit_locked(S);
+      if (rc == FC5_OK) {
+        for (SBand* B : S->slots)
+          if (B) active.push_back(B);
+        if (active.empty()) {
+          if (S->stopping && S->pending.empty()) return;
+          S->cv_work.wait(lk);
+          continue;
+        }
+      } else {
+        S->fail_rc = rc;
+      }
+      if (S->fail_rc != FC5_OK) {  // a device error ends the server: fail every open band
+        for (auto& [id, B] : S->bands)
+          if (B->state != kDone) {
+            B->rc = S->fail_rc;
+            B->state = kDone;
+          }
+        S->pending.clear();
+        S->cv_done.notify_all();
+        return;
+      }
+    }
+    const int rc = run_round(S, active);
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (rc != FC5_OK) {
+      S->fail_rc = rc;
+      continue;
+    }
+    for (SBand* B : active) {
+      if (!B->finished_items()) continue;
+      finalize(S, *B);
+      S->a5.release(B->off5, B->units5.size());
+      S->a3.release(B->off3, B->units3.size());
+      S->slots[(size_t)B->slot] = nullptr;
+      S->done++;
+    }
+    S->cv_done.notify_all();
+  }
+}
+
+SBand* find_band(fc5_server* S, u64 id) {
+  auto it = S->bands.find(id);
+  return it == S->bands.end() ? nullptr : it->second.get();
+}
+
+}  // namespace
+
+void fc5::serve_shutdown(fc5_ctx* ctx) {
+  fc5_server* S = ctx->server;
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stopping = true;
+  }
+  S->cv_work.notify_all();
+  if (S->th.joinable()) S->th.join();
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  free_server_buffers(S);
+  delete S;
+  ctx->server = nullptr;
+}
+
+extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) {
+  if (!ctx) return FC5_E_ARG;
+  NOT_SERVING(ctx);
+  if (!ctx->main.present || !ctx->main.hdr.N) FAIL(ctx, FC5_E_STATE, "serving needs a pair table with its N");
+  fc5_serve_opts o{16, ctx->rows_per_item, 1ull << 23, 1ull << 22, 2e10, 288000000000ull};
+  if (opts) {
+    if (opts->max_bands) o.max_bands = opts->max_bands;
+    if (opts->rows_per_item) o.rows_per_item = opts->rows_per_item;
+    if (opts->arena_units5) o.arena_units5 = opts->arena_units5;
+    if (opts->arena_units3) o.arena_units3 = opts->arena_units3;
+    if (opts->round_points > 0) o.round_points = opts->round_points;
+    if (opts->mem_budget) o.mem_budget = opts->mem_budget;
+  }
+  if (o.max_bands < 1 || o.max_bands > 64) FAIL(ctx, FC5_E_ARG, "max_bands must be 1..64");
+  if (o.rows_per_item != 4 && o.rows_per_item != 8 && o.rows_per_item != 16 && o.rows_per_item != 32)
+    FAIL(ctx, FC5_E_ARG, "rows_per_item must be 4, 8, 16 or 32");
+  if (o.arena_units5 >= (1ull << 31) || o.arena_units3 >= (1ull << 31)) FAIL(ctx, FC5_E_ARG, "arena too large");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  // every band the table can serve: B <= 42 (N + 1) gives fmax2 <= 2N + 2 and lim <= N + 1
+  const u32 N = ctx->main.hdr.N;
+  int rc = ensure_ndg(ctx, 2 * N + 2);
+  if (rc == FC5_OK) rc = ensure_c6g(ctx, N + 64);
+  if (rc == FC5_OK) rc = ensure_hits(ctx);
+  if (rc != FC5_OK) return rc;
+  const u64 table_bytes = ctx->main.hdr.m_cells * 8 + ctx->side.hdr.m_cells * 8;
+  const u64 arena_bytes = o.arena_units5 * sizeof(DevUnit5) + o.arena_units3 * sizeof(DevUnit3);
+  const u64 run_bytes = ctx->hit_cap * sizeof(HitRec) + (u64)o.max_bands * kSlotStride * sizeof(Counters);
+  if (table_bytes + arena_bytes + run_bytes > o.mem_budget)
+    FAIL(ctx, FC5_E_NOMEM, "table + unit arenas + run buffers exceed the memory budget");
+  auto S = std::make_unique<fc5_server>();
+  S->ctx = ctx;
+  S->opt = o;
+  S->rows = o.rows_per_item;
+  S->slots.assign((size_t)o.max_bands, nullptr);
+  S->a5.init(o.arena_units5);
+  S->a3.init(o.arena_units3);
+  S->max_fmax2 = 2 * N + 2;
+  S->max_lim = N + 1;
+  S->device_bytes = table_bytes + arena_bytes + run_bytes;
+  const size_t nslots = (size_t)o.max_bands * kSlotStride;
+  if (hipMalloc(&S->d5, o.arena_units5 * sizeof(DevUnit5)) != hipSuccess ||
+      hipMalloc(&S->d3, o.arena_units3 * sizeof(DevUnit3)) != hipSuccess ||
+      hipMalloc(&S->d_segs, 2 * (size_t)o.max_bands * sizeof(Segment)) != hipSuccess ||
+      hipMalloc(&S->d_ctr, nslots * sizeof(Counters)) != hipSuccess ||
+      hipHostMalloc((void**)&S->h_segs, 2 * (size_t)o.max_bands * sizeof(Segment), hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&S->h_ctr, nslots * sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&S->h_hits, ctx->hit_cap * sizeof(HitRec), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&S->h_nhits, 8, hipHostMallocDefault) != hipSuccess) {
+    free_server_buffers(S.get());
+    FAIL(ctx, FC5_E_NOMEM, "serving buffers");
+  }
+  if (launch_fill_u64(reinterpret_cast<u64*>(S->d_ctr), nslots * sizeof(Counters) / 8, 0, ctx->stream) != 0 ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    free_server_buffers(S.get());
+    FAIL(ctx, FC5_E_HIP, "serving counter reset");
+  }
+  ctx->server = S.release();
+  ctx->server->th = std::thread(serve_loop, ctx->server);
+  return FC5_OK;
+}
+
+extern "C" int fc5_serve_submit(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units, size_t n,
+                                uint64_t* band_id) {
+  if (!ctx || !band_id || (!units && n)) return FC5_E_ARG;
+  fc5_server* S = ctx->server;
+  if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
+  if (n > 0x7fffffffull) FAIL(ctx, FC5_E_ARG, "too many units in one band");
+  auto B = std::make_unique<SBand>();
+  B->t_submit = S->now();
+  B->n_in = n;
+  B->cls_of.assign(n, 0);
+  B->slot_of.assign(n, -1);
+  for (size_t i = 0; i < n; ++i) {
+    const int c = cls ? (int)cls[i] : 5;
+    if (c == 5) {
+      Cls5Unit U;
+      if (!pack_cls5(units[i].B, units[i].u, U)) { ++B->n_skipped; continue; }
+      if (!U.regular) FAIL(ctx, FC5_E_ARG, "unit " + std::to_string(i) + " is not a Stage-1 class-5 unit");
+      if (U.fmax2 > S->max_fmax2) FAIL(ctx, FC5_E_ARG, "band needs a pair table with a larger N");
+      B->cls_of[i] = 5;
+      B->slot_of[i] = (int)B->units5.size();
+      B->dev5.push_back(dev_unit5(U, (u32)i, 5));
+      B->units5.push_back(U);
+    } else if (c >= 2 && c <= 4) {
+      Cls234Unit U;
+      if (!pack_cls234(c, units[i].B, units[i].u, U)) { ++B->n_skipped; continue; }
+      if (!U.regular) FAIL(ctx, FC5_E_ARG, "unit " + std::to_string(i) + " is not a Stage-1 class 2-4 unit");
+      if (U.lim > S->max_lim) FAIL(ctx, FC5_E_ARG, "band needs a pair table with a larger N");
+      B->cls_of[i] = (u8)c;
+      B->slot_of[i] = (int)B->units3.size();
+      B->dev3.push_back(dev_unit3(U, (u32)i, (u32)c));
+      B->units3.push_back(U);
+    } else {
+      ++B->n_skipped;
+    }
+  }
+  if (B->dev5.size() > S->a5.cap || B->dev3.size() > S->a3.cap)
+    FAIL(ctx, FC5_E_NOMEM, "band larger than the resident unit arenas");
+  number_items(B->dev5, B->dev3, S->rows, B->items5, B->items3);
+  u64 calls5 = 0, rows3 = 0, lim3 = 0;
+  for (const Cls5Unit& U : B->units5) calls5 += cls5_calls(U);
+  for (const DevUnit3& D : B->dev3) {
+    rows3 += D.rows;
+    lim3 += D.lim;
+  }
+  if (B->items5) B->cost5 = std::max(1.0, (double)calls5 / B->items5);
+  // a d-row sweeps about (1 - 3^(-1/6)) * T^(1/6) ~ lim / 6 values of c3
+  if (B->items3) B->cost3 = std::max(1.0, (double)rows3 / B->items3 * (double)lim3 / B->dev3.size() / 6.0);
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->stopping) FAIL(ctx, FC5_E_STATE, "server is stopping");
+  B->id = S->next_id++;
+  *band_id = B->id;
+  S->pending.push_back(B->id);
+  S->bands[B->id] = std::move(B);
+  S->submitted++;
+  S->cv_work.notify_all();
+  return FC5_OK;
+}
+
+static int band_result(fc5_ctx* ctx, SBand* B, fc5_result* res) {
+  if (B->state != kDone) return FC5_E_PENDING;
+  if (res) *res = B->res;
+  if (B->rc == FC5_E_OVERFLOW) ctx->err = "hit buffer overflow in a round of this band";
+  else if (B->rc != FC5_OK) ctx->err = "the server failed while this band was open";
+  return B->rc;
+}
+
+extern "C" int fc5_serve_poll(fc5_ctx* ctx, uint64_t band_id, fc5_result* res) {
+  if (!ctx) return FC5_E_ARG;
+  fc5_server* S = ctx->server;
+  if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
+  std::lock_guard<std::mutex> lk(S->mu);
+  SBand* B = find_band(S, band_id);
+  if (!B) FAIL(ctx, FC5_E_STATE, "unknown band");
+  return band_result(ctx, B, res);
+}
+
+extern "C" int fc5_serve_wait(fc5_ctx* ctx, uint64_t band_id, double timeout_s, fc5_result* res) {
+  if (!ctx) return FC5_E_ARG;
+  fc5_server* S = ctx->server;
+  if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
+  std::unique_lock<std::mutex> lk(S->mu);
+  SBand* B = find_band(S, band_id);
+  if (!B) FAIL(ctx, FC5_E_STATE, "unknown band");
+  auto ready = [&] { return B->state == kDone; };
+  if (timeout_s < 0) S->cv_done.wait(lk, ready);
+  else S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready);
+  return band_result(ctx, B, res);
+}
+
+extern "C" int fc5_serve_class_result(fc5_ctx* ctx, uint64_t band_id, int cls, fc5_result* out) {
+  if (!ctx || !out || cls < 2 || cls > 5) return FC5_E_ARG;
+  fc5_server* S = ctx->server;
+  if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
+  std::lock_guard<std::mutex> lk(S->mu);
+  SBand* B = find_band(S, band_id);
+  if (!B) FAIL(ctx, FC5_E_STATE, "unknown band");
+  if (B->state != kDone) return FC5_E_PENDING;
+  *out = B->cls_res[cls];
+  return FC5_OK;
+}
+
+extern "C" int fc5_serve_hits(fc5_ctx* ctx, uint64_t band_id, fc5_hit* out, size_t cap, size_t* n) {
+  if (!ctx || !n) return FC5_E_ARG;
+  fc5_server* S = ctx->server;
+  if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
+  std::lock_guard<std::mutex> lk(S->mu);
+  SBand* B = find_band(S, band_id);
+  if (!B) FAIL(ctx, FC5_E_STATE, "unknown band");
+  if (B->state != kDone) return FC5_E_PENDING;
+  *n = B->hits.size();
+  if (out) std::copy_n(B->hits.begin(), std::min(cap, B->hits.size()), out);
+  return FC5_OK;
+}
+
+extern "C" int fc5_serve_solutions(fc5_ctx* ctx, uint64_t band_id, fc5_solution* out, size_t cap, size_t* n) {
+  if (!ctx || !n) return FC5_E_ARG;
+  fc5_server* S = ctx->server;
+  if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
+  std::lock_guard<std::mutex> lk(S->mu);
+  SBand* B = find_band(S, band_id);
+  if (!B) FAIL(ctx, FC5_E_STATE, "unknown band");
+  if (B->state != kDone) return FC5_E_PENDING;
+  *n = B->sols.size();
+  if (out) std::copy_n(B->sols.begin(), std::min(cap, B->sols.size()), out);
+  return FC5_OK;
+}
+
+extern "C" int fc5_serve_timing_of(fc5_ctx* ctx, uint64_t band_id, fc5_serve_timing* out) {
+  if (!ctx || !out) return FC5_E_ARG;
+  fc5_server* S = ctx->server;
+  if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
+  std::lock_guard<std::mutex> lk(S->mu);
+  SBand* B = find_band(S, band_id);
+  if (!B) FAIL(ctx, FC5_E_STATE, "unknown band");
+  *out = fc5_serve_timing{B->t_submit, B->t_admit, B->t_done, B->rounds};
+  return B->state == kDone ? FC5_OK : FC5_E_PENDING;
+}
+
+extern "C" int fc5_serve_release(fc5_ctx* ctx, uint64_t band_id) {
+  if (!ctx) return FC5_E_ARG;
+  fc5_server* S = ctx->server;
+  if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
+  std::lock_guard<std::mutex> lk(S->mu);
+  SBand* B = find_band(S, band_id);
+  if (!B) FAIL(ctx, FC5_E_STATE, "unknown band");
+  if (B->state != kDone) FAIL(ctx, FC5_E_PENDING, "band not finished");
+  S->bands.erase(band_id);
+  return FC5_OK;
+}
+
+extern "C" int fc5_serve_stats_get(fc5_ctx* ctx, fc5_serve_stats* out) {
+  if (!ctx || !out) return FC5_E_ARG;
+  fc5_server* S = ctx->server;
+  if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
+  std::lock_guard<std::mutex> lk(S->mu);
+  u64 active = 0;
+  for (SBand* B : S->slots) active += B != nullptr;
+  *out = fc5_serve_stats{S->submitted, S->admitted, S->done, (u64)S->pending.size(), active, S->rounds,
+                         S->a5.used, S->a3.used, S->device_bytes, S->busy_ms};
+  return FC5_OK;
+}
+
+extern "C" int fc5_serve_stop(fc5_ctx* ctx) {
+  if (!ctx) return FC5_E_ARG;
+  if (!ctx->server) FAIL(ctx, FC5_E_STATE, "no server running");
+  const int rc = ctx->server->fail_rc;
+  serve_shutdown(ctx);
+  return rc;
+}

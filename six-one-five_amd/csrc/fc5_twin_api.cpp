@@ -249,6 +249,7 @@ extern "C" int fc5_twin_cands(uint64_t B, double u_lo, double u_hi, fc5_twin_can
 
 extern "C" int fc5_twin_run(fc5_ctx* ctx, const fc5_twin_cand* cands, size_t n, fc5_twin_result* res) {
   if (!ctx || !res || (!cands && n)) return FC5_E_ARG;
+  NOT_SERVING(ctx);
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no pair table loaded");
   const auto t0 = std::chrono::steady_clock::now();
   HIPCHK(ctx, hipSetDevice(ctx->device));

@@ -24,8 +24,8 @@ if not os.path.exists(LIB_PATH):
 
 _lib = C.CDLL(LIB_PATH)
 
-OK, E_ARG, E_IO, E_FORMAT, E_NMISMATCH, E_FPR, E_HIP, E_NOMEM, E_STATE, E_OVERFLOW, E_BUILD = (
-    0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10)
+OK, E_ARG, E_IO, E_FORMAT, E_NMISMATCH, E_FPR, E_HIP, E_NOMEM, E_STATE, E_OVERFLOW, E_BUILD, E_PENDING = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10, -11)
 MODE_PLAIN, MODE_BATCHED = 0, 1
 
 
@@ -79,6 +79,20 @@ class TwinSolution(C.Structure):
                [("terms", C.c_int64 * 7), ("gcd", C.c_int64)]
 
 
+class ServeOpts(C.Structure):
+    _fields_ = [("max_bands", C.c_int), ("rows_per_item", C.c_int), ("arena_units5", C.c_uint64),
+                ("arena_units3", C.c_uint64), ("round_points", C.c_double), ("mem_budget", C.c_uint64)]
+
+
+class ServeTiming(C.Structure):
+    _fields_ = [("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double), ("rounds", C.c_uint64)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "admitted", "done", "pending", "active", "rounds",
+                                          "arena5_used", "arena3_used", "device_bytes")] + [("busy_ms", C.c_double)]
+
+
 # fc5_twin_cand / fc5_twin_hit as numpy records (88 and 24 bytes)
 TWIN_CAND_DTYPE = np.dtype([("B", "<u8"), ("u", "<u8"), ("w", "<u8"), ("base_lo", "<u8"), ("base_hi", "<u8"),
                             ("lim", "<u4"), ("fmax1", "<u4"), ("mod1", "<u4"), ("nres1", "<u4"),
@@ -117,6 +131,17 @@ _sig = {
     "fc5_emit_units": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double,
                                  _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_build_pairs_file": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.POINTER(StoreHeader)]),
+    "fc5_serve_start": (C.c_int, [_P, C.POINTER(ServeOpts)]),
+    "fc5_serve_submit": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_uint64)]),
+    "fc5_serve_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(Result)]),
+    "fc5_serve_wait": (C.c_int, [_P, C.c_uint64, C.c_double, C.POINTER(Result)]),
+    "fc5_serve_class_result": (C.c_int, [_P, C.c_uint64, C.c_int, C.POINTER(Result)]),
+    "fc5_serve_hits": (C.c_int, [_P, C.c_uint64, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fc5_serve_solutions": (C.c_int, [_P, C.c_uint64, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fc5_serve_timing_of": (C.c_int, [_P, C.c_uint64, C.POINTER(ServeTiming)]),
+    "fc5_serve_release": (C.c_int, [_P, C.c_uint64]),
+    "fc5_serve_stats_get": (C.c_int, [_P, C.POINTER(ServeStats)]),
+    "fc5_serve_stop": (C.c_int, [_P]),
     "fc5_twin_cands": (C.c_int, [C.c_uint64, C.c_double, C.c_double, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_twin_run": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(TwinResult)]),
     "fc5_twin_run_range": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.POINTER(TwinResult)]),
@@ -353,6 +378,81 @@ class Context:
             self._check(_lib.fc5_last_solutions(self.h, C.cast(arr, _P), n.value, C.byref(n)))
         return [dict(B=s.B, u=s.u, cls=s.cls, d=s.d, e=s.e, a=s.a, b=s.b, c=s.c, terms=list(s.terms),
                      gcd=s.gcd) for s in arr[:n.value]]
+
+    # ---- resident serving loop ----
+    def serve_start(self, **opts):
+        o = ServeOpts()
+        for k, v in opts.items():
+            setattr(o, k, v)
+        self._check(_lib.fc5_serve_start(self.h, C.byref(o)))
+
+    def serve_submit(self, units, cls=None):
+        units = np.ascontiguousarray(units)
+        cl = None if cls is None else np.ascontiguousarray(cls, dtype=np.uint32)
+        bid = C.c_uint64()
+        self._check(_lib.fc5_serve_submit(self.h, _ptr(cl), _ptr(units), len(units), C.byref(bid)))
+        return bid.value
+
+    def serve_poll(self, band_id):
+        """True when the band is finished"""
+        rc = _lib.fc5_serve_poll(self.h, band_id, None)
+        if rc == E_PENDING:
+            return False
+        if rc not in (OK, E_OVERFLOW):
+            self._check(rc)
+        return True
+
+    def serve_wait(self, band_id, timeout=-1.0):
+        rc = _lib.fc5_serve_wait(self.h, band_id, timeout, None)
+        if rc == E_PENDING:
+            return False
+        if rc not in (OK, E_OVERFLOW):
+            self._check(rc)
+        return True
+
+    def serve_result(self, band_id, allow_overflow=False):
+        """RunOutput of a finished band (result, hits, solutions, by_class)"""
+        res = Result()
+        rc = _lib.fc5_serve_poll(self.h, band_id, C.byref(res))
+        if rc not in (OK, E_OVERFLOW) or (rc == E_OVERFLOW and not allow_overflow):
+            self._check(rc)
+        n = C.c_size_t()
+        self._check(_lib.fc5_serve_hits(self.h, band_id, None, 0, C.byref(n)))
+        hits = np.zeros(n.value, HIT_DTYPE)
+        if n.value:
+            self._check(_lib.fc5_serve_hits(self.h, band_id, hits.ctypes.data_as(_P), n.value, C.byref(n)))
+        self._check(_lib.fc5_serve_solutions(self.h, band_id, None, 0, C.byref(n)))
+        arr = (Solution * max(1, n.value))()
+        if n.value:
+            self._check(_lib.fc5_serve_solutions(self.h, band_id, C.cast(arr, _P), n.value, C.byref(n)))
+        sols = [dict(B=x.B, u=x.u, cls=x.cls, d=x.d, e=x.e, a=x.a, b=x.b, c=x.c, terms=list(x.terms), gcd=x.gcd)
+                for x in arr[:n.value]]
+        out = RunOutput(res.as_dict(), hits, sols, rc)
+        by = {}
+        for k in (2, 3, 4, 5):
+            r = Result()
+            self._check(_lib.fc5_serve_class_result(self.h, band_id, k, C.byref(r)))
+            by[k] = r.as_dict()
+        out.by_class = by
+        return out
+
+    def serve_timing(self, band_id):
+        t = ServeTiming()
+        rc = _lib.fc5_serve_timing_of(self.h, band_id, C.byref(t))
+        if rc not in (OK, E_PENDING):
+            self._check(rc)
+        return {f: getattr(t, f) for f, _ in ServeTiming._fields_}
+
+    def serve_release(self, band_id):
+        self._check(_lib.fc5_serve_release(self.h, band_id))
+
+    def serve_stats(self):
+        st = ServeStats()
+        self._check(_lib.fc5_serve_stats_get(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServeStats._fields_}
+
+    def serve_stop(self):
+        self._check(_lib.fc5_serve_stop(self.h))
 
     def twin_run(self, cands, allow_overflow=False):
         """(6,1,7) class-5 twin over candidate records (TWIN_CAND_DTYPE)."""
