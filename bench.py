@@ -54,8 +54,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS) + [617],
-                    help="1-4: class-5 peel bands; 617: the (6,1,7) class-5 twin")
+    ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS) + [5, 617],
+                    help="1-4: class-5 peel bands; 5: serving (Poisson band arrivals); 617: the (6,1,7) twin")
+    ap.add_argument("--serve-requests", type=int, default=200, help="5: number of band requests")
+    ap.add_argument("--serve-load", type=float, default=0.8, help="5: offered load as a fraction of capacity")
+    ap.add_argument("--serve-calibrate", type=int, default=24, help="5: bands of the closed-loop capacity run")
+    ap.add_argument("--serve-check", type=int, default=8, help="5: bands re-run alone and compared")
     ap.add_argument("--twin-lo", type=int, default=400471, help="617: first B")
     ap.add_argument("--twin-hi", type=int, default=400471, help="617: last B")
     ap.add_argument("--twin-cpu-every", type=int, default=128, help="617: CPU oracle on every k-th candidate")
@@ -77,6 +81,8 @@ def main():
         sys.exit(2)
     if args.config == 617:
         return bench_twin(args)
+    if args.config == 5:
+        return bench_serve(args)
     cfg = CONFIGS[args.config]
     t_setup = time.time()
     mixed = cfg["classes"] != (5,)
@@ -327,6 +333,136 @@ def bench_twin(args):
                      "per_unit": f"{3 * cell_bytes} B per probe call (3 cells x {cell_bytes} B)"},
         "gather_ceiling": ceiling,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def serve_requests(n, seed):
+    """Config 5: n band requests, widths log-uniform on [500, 20000] integers,
+    starts uniform on [1.0M, 3.5M], classes 4 and 5 (seeded, deterministic)."""
+    import random
+    rnd = random.Random(seed)
+    out = []
+    for _ in range(n):
+        w = int(math.exp(rnd.uniform(math.log(500), math.log(20000))))
+        lo = rnd.randint(1000000, 3500000)
+        out.append((lo, lo + w - 1))
+    return out
+
+
+def bench_serve(args):
+    """Resident serving loop: Poisson arrivals of Stage-1 band requests at a
+    fixed fraction of the measured capacity, one shared device table."""
+    import random
+    seed = 0x615 << 8 | 5
+    reqs = serve_requests(args.serve_requests, seed)
+    t0 = time.time()
+    bands = []
+    for lo, hi in reqs:
+        cls, Bs, us = fc5.emit_units(lo, hi, classes=(4, 5))
+        units = fc5.units_array(Bs, us)
+        bands.append((lo, hi, cls, units, fc5.units_calls(units[cls == 5])))
+    N = max(83809, max(hi for _, hi in reqs) // 42 + 1)  # the table for B up to 3.52M: 4.32e9 cells > 2^32
+    log(f"[bench] serving: {len(reqs)} requests, {sum(len(b[3]) for b in bands)} units, "
+        f"B in [{min(r[0] for r in reqs)},{max(r[1] for r in reqs)}], units emitted in {time.time() - t0:.1f}s")
+    ctx = fc5.Context(0)
+    if args.load_policy is not None:
+        ctx.set_option("load_policy", args.load_policy)
+    t0 = time.time()
+    ctx.table_build(N, args.r, on_gpu=True)
+    hdr = ctx.table_info()
+    log(f"[bench] device table N={N} cells={hdr.m_cells} (> 2^32: {hdr.m_cells > 2 ** 32}) "
+        f"packed={fc5_packed_gb(hdr):.2f} GB built in {time.time() - t0:.1f}s")
+    # results alone, for the identity check of a sample of bands
+    rnd = random.Random(seed)
+    check = sorted(rnd.sample(range(len(bands)), min(args.serve_check, len(bands))))
+    alone = {i: ctx.run(bands[i][3], fc5.MODE_BATCHED, cls=bands[i][2]) for i in check}
+    ctx.set_option("rows_per_item", 16)
+    ctx.serve_start(max_bands=16)
+
+    def run(order, times):
+        """submit band i at times[i] seconds after start; returns per-band timings"""
+        start = time.perf_counter()
+        ids = {}
+        pending = list(zip(times, order))
+        while pending:
+            dt = pending[0][0] - (time.perf_counter() - start)
+            if dt > 0:
+                time.sleep(min(dt, 0.05))
+                continue
+            _, i = pending.pop(0)
+            ids[i] = ctx.serve_submit(bands[i][3], bands[i][2])
+            if len(ids) % 25 == 0:
+                st = ctx.serve_stats()
+                log(f"[bench]   t={time.perf_counter() - start:.1f}s submitted {st['submitted']} done {st['done']} "
+                    f"active {st['active']} pending {st['pending']}")
+        for i in order:
+            if not ctx.serve_wait(ids[i], 900.0):
+                raise SystemExit(f"band {i} not finished after 900 s")
+        return ids, time.perf_counter() - start
+
+    # closed-loop capacity: a sample of bands submitted at once
+    cal = list(range(min(args.serve_calibrate, len(bands))))
+    ids, span = run(cal, [0.0] * len(cal))
+    cal_points = sum(ctx.serve_result(ids[i]).result["calls"] for i in cal)
+    for i in cal:
+        ctx.serve_release(ids[i])
+    cap_points = cal_points / span
+    mean_points = float(np.mean([b[4] for b in bands]))  # class-5 closed form per request
+    cal_c5 = sum(bands[i][4] for i in cal)
+    # capacity in requests/s, scaled from the calibration's class-5 share of its points
+    rate = args.serve_load * cap_points * (cal_c5 / cal_points) / mean_points
+    log(f"[bench] capacity {cap_points:.4e} grid points/s on {len(cal)} bands at once; "
+        f"arrival rate {rate:.3f} requests/s (load {args.serve_load})")
+    rnd = random.Random(seed + 1)
+    times, t = [], 0.0
+    for _ in bands:
+        t += rnd.expovariate(rate)
+        times.append(t)
+    st0 = ctx.serve_stats()
+    ids, span = run(list(range(len(bands))), times)
+    st1 = ctx.serve_stats()
+    lat, points, probes, n_hits = [], 0, 0, 0
+    for i in range(len(bands)):
+        out = ctx.serve_result(ids[i])
+        tm = ctx.serve_timing(ids[i])
+        lat.append(tm["t_done"] - tm["t_submit"])
+        points += out.result["calls"]
+        probes += out.result["probes"]
+        n_hits += out.result["n_hits"]
+        assert out.by_class[5]["calls"] == bands[i][4], "closed-form class-5 calls"
+        r = out.result
+        assert r["calls"] == r["neg"] + r["zero"] + r["gated"] + r["probes"] + \
+            sum(out.by_class[k]["calls"] - out.by_class[k]["gated"] - out.by_class[k]["probes"] for k in (2, 3, 4))
+        if i in alone:
+            a = alone[i]
+            for k in (2, 3, 4, 5):
+                for f in ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe", "jobs", "fail_t"):
+                    assert out.by_class[k][f] == a.by_class[k][f], (i, k, f)
+            assert np.array_equal(out.hits, a.hits) and out.solutions == a.solutions
+    ctx.serve_stop()
+    lat = np.array(lat)
+    busy_ms = st1["busy_ms"] - st0["busy_ms"]
+    n_B = sum(eligible_count(lo, hi) for lo, hi in reqs)
+    log(f"[bench] served {len(bands)} bands in {span:.1f}s: p50 {np.percentile(lat, 50):.2f}s "
+        f"p99 {np.percentile(lat, 99):.2f}s, {points / span:.4e} points/s, device busy {busy_ms / 1000 / span:.1%}")
+    line = {
+        "metric": "serving: band latency p50/p99 + throughput, Poisson arrivals, 1 GPU, bit-exact",
+        "value": points / span, "unit": "(d,e) grid points + c3 values / s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": 1000.0 * span, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (seeded request stream; deterministic Stage-1 enumeration per band)",
+        "config": {"workload": f"config 5: {len(bands)} band requests (widths log-uniform [500,20000], starts "
+                               f"uniform [1.0M,3.5M], classes 4+5), Poisson arrivals at {args.serve_load} of the "
+                               f"measured capacity, one device table N={N} r={hdr.r} ({hdr.m_cells} cells)",
+                   "requests": len(bands), "load": args.serve_load, "N": N, "r": int(hdr.r)},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p90": float(np.percentile(lat, 90)),
+                      "p99": float(np.percentile(lat, 99)), "max": float(lat.max()), "mean": float(lat.mean())},
+        "arrival_rate_per_s": rate, "capacity_points_per_s": cap_points,
+        "eligible_B_per_hour": n_B * 3600.0 / span, "filter_probes_per_s": probes / span,
+        "device_busy_frac": busy_ms / 1000.0 / span, "rounds": st1["rounds"] - st0["rounds"],
+        "bands_checked_alone": len(alone), "n_hits": n_hits,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -167,3 +167,23 @@ def test_twin_errors(ctx):
         assert res["hits_dropped"] == res["n_hits"] - 4 and len(hits) == 4
     finally:
         ctx.set_option("hit_cap", 1 << 20)
+
+
+def test_cli_617_finds_and_checks_known(tmp_path):
+    """the drop-in CLI over [54000, 64000]: SOLUTION lines of the known class-5
+    rows, --check-known against a list in the reference's file format"""
+    import subprocess
+    from conftest import BIN
+    known = tmp_path / "known.txt"
+    known.write_text("# known list\n" + "".join(
+        f"{k['B']} = {' + '.join(str(x) for x in sorted(k['terms'], reverse=True))}  | 2nd kind | ref\n"
+        for k in KNOWN))
+    r = subprocess.run([os.path.join(BIN, "solve_617_v1"), "54000", "64000", "5", "--branch-a-only", "--quiet",
+                        "--check-known", "--known-file", str(known), "--counters"],
+                       capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr[-2000:]
+    sols = [l for l in r.stdout.splitlines() if l.startswith("SOLUTION") and "primitive" in l and "NON" not in l]
+    want = {k["B"] for k in KNOWN if 54000 <= k["B"] <= 64000}
+    assert {int(l.split("B=")[1].split()[0]) for l in sols} == want
+    assert "[check-known]" in r.stderr and "PASS" in r.stderr
+    assert any(l.startswith("COUNTERS cands=") for l in r.stdout.splitlines())

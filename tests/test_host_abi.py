@@ -109,3 +109,14 @@ def test_cli_cls5_host_paths(tmp_path):
     fpr = _run("fourcore_cls5_gpu_v4", "--check-table", "--N", 300, "--load-table", q, check=False)
     assert fpr.returncode == 1 and "FPR smoke too high" in fpr.stderr
     assert _run("fourcore_cls5_gpu_v4", "--units", "/nonexistent.buc", check=False).returncode == 1
+
+
+def test_cli_617_argument_refusals():
+    """solve_617_v1 runs Branch A class 5 only; everything else is refused
+    before any device use (exit 1, as the reference does for bad arguments)"""
+    for args in [("1000", "2000", "1,5", "--branch-a-only"), ("1000", "2000", "5"),
+                 ("1000", "2400000", "5", "--branch-a-only"), ("1000", "2000", "5", "--branch-a-only", "--xcheck"),
+                 ("1000", "2000", "5", "--branch-a-only", "--bogus"), ("100", "200", "5", "--branch-a-only")]:
+        r = _run("solve_617_v1", *args, check=False)
+        assert r.returncode == 1, args
+    assert "usage:" in _run("solve_617_v1", check=False).stderr
