@@ -180,6 +180,15 @@ int fc5_last_solutions(fc5_ctx *ctx, fc5_solution *out, size_t cap, size_t *n);
 /* closed-form calls of a unit list (the counter `calls`, fourcore_cls5_gpu_v4.cu:383-386) */
 int fc5_units_calls(const fc5_unit *units, size_t n, uint64_t *calls);
 
+/* exact check of filter survivors on the device (replaces PairRecover + the
+ * exact compare of the host verify, fourcore_xor_store.hpp:23-51,
+ * fourcore_cls5_gpu_v4.cu:517-522): for each 128-bit target T_i (targets[2i] low,
+ * targets[2i+1] high word) every pair a <= b <= hi[i] with a^6 + b^6 = T_i.  out
+ * gets (i, a, b) triples (up to cap, sorted), *n_out the total.  Every run's
+ * hit verification goes through this search. */
+int fc5_pair_search(fc5_ctx *ctx, const uint64_t *targets, const uint32_t *hi, size_t n, uint32_t *out,
+                    size_t cap, size_t *n_out);
+
 /* ---- (6,1,7) Branch-A class-5 twin (solve_617_v1.cu) ----
  * B^6 = u^6 + w^6 + (21d)^6 + 42^6 (c1^6 + c2^6 + c3^6 + c4^6) with u = 14u'
  * (the term not divisible by 3), w = 6v' (the term not divisible by 7) and the

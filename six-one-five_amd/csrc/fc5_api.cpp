@@ -475,6 +475,85 @@ DevUnit3 fc5::dev_unit3(const Cls234Unit& U, u32 uid, u32 band) {
   return D;
 }
 
+int fc5::device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
+                      std::vector<std::vector<std::pair<u32, u32>>>& pairs) {
+  pairs.assign(targets.size(), {});
+  std::vector<PairJob> jobs;
+  std::vector<size_t> which;
+  for (size_t i = 0; i < targets.size(); ++i)
+    if (targets[i].ok) {
+      jobs.push_back(PairJob{(u64)targets[i].T, (u64)(targets[i].T >> 64), targets[i].hi, 0});
+      which.push_back(i);
+    }
+  if (jobs.empty()) return FC5_OK;
+  if (jobs.size() > 0xffffffffull) FAIL(ctx, FC5_E_ARG, "too many hits to verify");
+  u32 cap = (u32)std::min<size_t>(std::max<size_t>(1024, 4 * jobs.size()), 1u << 30);
+  PairJob* d_jobs = nullptr;
+  PairHit* d_out = nullptr;
+  u64* d_count = nullptr;
+  auto release = [&] {
+    for (void* p : {(void*)d_jobs, (void*)d_out, (void*)d_count})
+      if (p) (void)hipFree(p);
+    d_jobs = nullptr;
+    d_out = nullptr;
+    d_count = nullptr;
+  };
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (hipMalloc(&d_jobs, jobs.size() * sizeof(PairJob)) != hipSuccess ||
+        hipMalloc(&d_out, (size_t)cap * sizeof(PairHit)) != hipSuccess || hipMalloc(&d_count, 8) != hipSuccess) {
+      release();
+      FAIL(ctx, FC5_E_NOMEM, "pair search buffers");
+    }
+    u64 count = 0;
+    if (hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_fill_u64(d_count, 1, 0, ctx->stream) != 0 ||
+        launch_pair_search(d_jobs, (u32)jobs.size(), d_out, cap, reinterpret_cast<u32*>(d_count), ctx->stream) != 0 ||
+        hipMemcpyAsync(&count, d_count, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      release();
+      FAIL(ctx, FC5_E_HIP, "pair search");
+    }
+    if (count > cap) {  // more pairs than room: once more with room for all
+      release();
+      cap = (u32)count;
+      continue;
+    }
+    std::vector<PairHit> out((size_t)count);
+    const hipError_t e =
+        count ? hipMemcpy(out.data(), d_out, count * sizeof(PairHit), hipMemcpyDeviceToHost) : hipSuccess;
+    release();
+    if (e != hipSuccess) FAIL(ctx, FC5_E_HIP, "pair search read-back");
+    for (const PairHit& h : out) pairs[which[h.job]].emplace_back(h.a, h.b);
+    for (auto& v : pairs) std::sort(v.begin(), v.end());
+    return FC5_OK;
+  }
+  FAIL(ctx, FC5_E_HIP, "pair search overflowed twice");
+}
+
+extern "C" int fc5_pair_search(fc5_ctx* ctx, const uint64_t* targets, const uint32_t* hi, size_t n, uint32_t* out,
+                               size_t cap, size_t* n_out) {
+  if (!ctx || !n_out || (n && (!targets || !hi)) || (cap && !out)) return FC5_E_ARG;
+  NOT_SERVING(ctx);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::vector<PairTarget> t(n);
+  for (size_t i = 0; i < n; ++i) t[i] = PairTarget{((u128)targets[2 * i + 1] << 64) | targets[2 * i], hi[i], true};
+  std::vector<std::vector<std::pair<u32, u32>>> pairs;
+  const int rc = device_pairs(ctx, t, pairs);
+  if (rc != FC5_OK) return rc;
+  size_t k = 0;
+  for (size_t i = 0; i < n; ++i)
+    for (auto [a, b] : pairs[i]) {
+      if (k < cap) {
+        out[3 * k] = (uint32_t)i;
+        out[3 * k + 1] = a;
+        out[3 * k + 2] = b;
+      }
+      ++k;
+    }
+  *n_out = k;
+  return FC5_OK;
+}
+
 // the band's launch segments: its class-5 items, then its class 2-4 items
 static int upload_segments(fc5_ctx* ctx, fc5_band* b) {
   Segment seg[2];
@@ -930,8 +1009,13 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
         jobs.push_back(VerifyJob{k, U.B, U.u, U.lim, h.d, h.e, h.fp});
       }
     }
+    std::vector<PairTarget> targets;
+    verify_targets(N, jobs, targets);
+    std::vector<std::vector<std::pair<u32, u32>>> pairs;
+    rc = device_pairs(ctx, targets, pairs);
+    if (rc != FC5_OK) return rc;
     std::vector<Solution> sols;
-    verify_jobs(N, jobs, ctx->stop_first != 0, sols);
+    verify_finish_all(jobs, pairs, ctx->stop_first != 0, sols);
     for (const Solution& S : sols) {
       fc5_solution o{};
       o.B = S.B;

@@ -158,6 +158,10 @@ DevUnit3 dev_unit3(const Cls234Unit& U, u32 uid, u32 band);
 void number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3);
 // serving loop (fc5_serve.cpp): stops and frees the server of a context
 void serve_shutdown(fc5_ctx* ctx);
+// exact check of filter survivors on the device (k_pair_search): pairs[i] gets
+// every a <= b <= targets[i].hi with a^6 + b^6 = targets[i].T, sorted
+int device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
+                 std::vector<std::vector<std::pair<u32, u32>>>& pairs);
 // twin buffers (fc5_twin_api.cpp)
 void free_twin(fc5_ctx* ctx);
 }  // namespace fc5

@@ -731,55 +731,95 @@ void exact_pairs(u128 T, u32 hi, std::vector<std::pair<u32, u32>>& out) {
   }
 }
 
-int verify_cls5_hit(int N, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp, std::vector<Solution>& out) {
+// the exact pair target of a class-5 hit: T itself, pairs up to min(lim, N)
+static PairTarget cls5_target(int N, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp) {
+  PairTarget t{};
   u128 T;
-  if (!exact_T_cls5(B, u, d, e, T) || (u64)T != fp) return 0;
+  if (!exact_T_cls5(B, u, d, e, T) || (u64)T != fp) return t;
+  t.T = T;
+  t.hi = std::min<u32>(lim, (u32)std::max(N, 0));
+  t.ok = true;
+  return t;
+}
+
+static int cls5_finish(u64 B, u64 u, u32 d, u32 e, u32 a, u32 b, std::vector<Solution>& out) {
+  if (!verify_cls5(B, u, d, e, a, b)) return 0;
+  i64 t[5] = {42LL * a, 42LL * b, 21LL * (i64)d, 14LL * (i64)e, (i64)u};
+  std::sort(t, t + 5);
+  bool bad = false;
+  for (int i = 0; i < 4; ++i) bad |= t[i] == t[i + 1];
+  if (bad || t[4] >= (i64)B) return 0;
+  i64 g = t[0];
+  for (int i = 1; i < 5; ++i) g = gcd_i64(g, t[i]);
+  g = gcd_i64(g, (i64)B);
+  out.push_back(Solution{B, u, 5, d, e, a, b, 0, {t[0], t[1], t[2], t[3], t[4]}, g});
+  return 1;
+}
+
+int verify_cls5_hit(int N, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp, std::vector<Solution>& out) {
+  const PairTarget t = cls5_target(N, B, u, lim, d, e, fp);
+  if (!t.ok) return 0;
   std::vector<std::pair<u32, u32>> pairs;
-  exact_pairs(T, std::min<u32>(lim, (u32)N), pairs);
+  exact_pairs(t.T, t.hi, pairs);
   int found = 0;
-  for (auto [a, b] : pairs) {
-    if (!verify_cls5(B, u, d, e, a, b)) continue;
-    i64 t[5] = {42LL * a, 42LL * b, 21LL * (i64)d, 14LL * (i64)e, (i64)u};
-    std::sort(t, t + 5);
-    bool bad = false;
-    for (int i = 0; i < 4; ++i) bad |= t[i] == t[i + 1];
-    if (bad || t[4] >= (i64)B) continue;
-    i64 g = t[0];
-    for (int i = 1; i < 5; ++i) g = gcd_i64(g, t[i]);
-    g = gcd_i64(g, (i64)B);
-    out.push_back(Solution{B, u, 5, d, e, a, b, 0, {t[0], t[1], t[2], t[3], t[4]}, g});
-    ++found;
-  }
+  for (auto [a, b] : pairs) found += cls5_finish(B, u, d, e, a, b, out);
   return found;
 }
 
 }  // namespace fc5
 
 namespace fc5 {
-int verify_find3_hit(int N, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp, std::vector<Solution>& out) {
+// the exact pair target of a class 2-4 hit at (d, c3): T - c3^6, pairs b <= min(c3, N)
+// (fourcore_find_v4.cu:548)
+static PairTarget find3_target(int N, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp) {
+  PairTarget t{};
   const u64 f = (u64)free_factor(cls);
   u128 T;
-  if (c < 1 || !exact_T_cls234(B, u, f, d, T)) return 0;
+  if (c < 1 || !exact_T_cls234(B, u, f, d, T)) return t;
   const u64 c3 = (u64)c * c * c;
   const u128 c6 = (u128)c3 * c3;
-  if (c6 > T || (u64)(T - c6) != fp) return 0;
+  if (c6 > T || (u64)(T - c6) != fp) return t;
+  t.T = T - c6;
+  t.hi = std::min<u32>(c, (u32)std::max(N, 0));
+  t.ok = true;
+  return t;
+}
+
+static int find3_finish(int cls, u64 B, u64 u, u32 d, u32 c, u32 a, u32 b, std::vector<Solution>& out) {
+  const u64 f = (u64)free_factor(cls);
+  if (!verify_cls234(B, u, f, d, a, b, c)) return 0;
+  i64 t[5] = {42LL * a, 42LL * b, 42LL * c, (i64)f * (i64)d, (i64)u};
+  std::sort(t, t + 5);
+  bool bad = false;
+  for (int i = 0; i < 4; ++i) bad |= t[i] == t[i + 1];
+  if (bad || t[4] >= (i64)B || t[0] < 1) return 0;
+  i64 g = t[0];
+  for (int i = 1; i < 5; ++i) g = std::gcd(g, t[i]);
+  g = std::gcd(g, (i64)B);
+  out.push_back(Solution{B, u, (u32)cls, d, 0, a, b, c, {t[0], t[1], t[2], t[3], t[4]}, g});
+  return 1;
+}
+
+int verify_find3_hit(int N, int cls, u64 B, u64 u, u32 d, u32 c, u64 fp, std::vector<Solution>& out) {
+  const PairTarget t = find3_target(N, cls, B, u, d, c, fp);
+  if (!t.ok) return 0;
   std::vector<std::pair<u32, u32>> pairs;
-  exact_pairs(T - c6, std::min<u32>(c, (u32)N), pairs);  // b <= c (fourcore_find_v4.cu:548)
+  exact_pairs(t.T, t.hi, pairs);
   int found = 0;
-  for (auto [a, b] : pairs) {
-    if (!verify_cls234(B, u, f, d, a, b, c)) continue;
-    i64 t[5] = {42LL * a, 42LL * b, 42LL * c, (i64)f * (i64)d, (i64)u};
-    std::sort(t, t + 5);
-    bool bad = false;
-    for (int i = 0; i < 4; ++i) bad |= t[i] == t[i + 1];
-    if (bad || t[4] >= (i64)B || t[0] < 1) continue;
-    i64 g = t[0];
-    for (int i = 1; i < 5; ++i) g = std::gcd(g, t[i]);
-    g = std::gcd(g, (i64)B);
-    out.push_back(Solution{B, u, (u32)cls, d, 0, a, b, c, {t[0], t[1], t[2], t[3], t[4]}, g});
-    ++found;
-  }
+  for (auto [a, b] : pairs) found += find3_finish(cls, B, u, d, c, a, b, out);
   return found;
+}
+
+PairTarget verify_target(int N, const VerifyJob& j) {
+  if (j.cls == 5) return cls5_target(N, j.B, j.u, j.lim, j.d, j.e, j.fp);
+  if (j.cls >= 2 && j.cls <= 4) return find3_target(N, j.cls, j.B, j.u, j.d, j.e, j.fp);
+  return PairTarget{};
+}
+
+int verify_finish(const VerifyJob& j, u32 a, u32 b, std::vector<Solution>& out) {
+  if (j.cls == 5) return cls5_finish(j.B, j.u, j.d, j.e, a, b, out);
+  if (j.cls >= 2 && j.cls <= 4) return find3_finish(j.cls, j.B, j.u, j.d, j.e, a, b, out);
+  return 0;
 }
 
 void verify_jobs(int N, const std::vector<VerifyJob>& jobs, bool stop_first, std::vector<Solution>& out) {
@@ -882,35 +922,47 @@ bool pack_twin(const TwinCandRaw& in, TwinCand& C, std::string& why) {
   return true;
 }
 
-int verify_twin_hit(int N, const TwinCand& C, u32 d, u32 c4, u32 c3, u64 fp, std::vector<TwinSolution>& out) {
+PairTarget twin_target(int N, const TwinCand& C, u32 d, u32 c4, u32 c3, u64 fp) {
+  PairTarget t{};
   const u128 g6 = pow6_128(21ULL * d);
-  if (g6 > C.base) return 0;
+  if (g6 > C.base) return t;
   const u128 rhs = C.base - g6;
-  if (rhs % (u128)kM42) return 0;
+  if (rhs % (u128)kM42) return t;
   const u128 T = rhs / (u128)kM42, a6 = pow6_128(c4), b6 = pow6_128(c3);
-  if (a6 > T || b6 > T - a6) return 0;
+  if (a6 > T || b6 > T - a6) return t;
   const u128 T2 = T - a6 - b6;
-  if ((u64)T2 != fp) return 0;
+  if ((u64)T2 != fp) return t;
+  t.T = T2;
+  t.hi = (u32)std::max(N, 0);
+  t.ok = true;
+  return t;
+}
+
+int twin_finish(const TwinCand& C, u32 d, u32 c4, u32 c3, u32 a, u32 b, std::vector<TwinSolution>& out) {
+  i64 t[7] = {42LL * a, 42LL * b, 42LL * c3, 42LL * c4, 21LL * (i64)d, (i64)C.u, (i64)C.w};
+  std::sort(t, t + 7);
+  bool bad = t[0] < 1 || t[6] >= (i64)C.B;
+  for (int i = 0; i < 6; ++i) bad |= t[i] == t[i + 1];
+  if (bad) return 0;
+  U256 lhs = U256::of(0);
+  for (int i = 0; i < 7; ++i) add_in_place(lhs, U256::pow6((u64)t[i]));
+  if (cmp(lhs, U256::pow6(C.B)) != 0) return 0;
+  i64 g = t[0];
+  for (int i = 1; i < 7; ++i) g = std::gcd(g, t[i]);
+  g = std::gcd(g, (i64)C.B);
+  TwinSolution S{C.B, C.u, C.w, d, c4, c3, a, b, {}, g};
+  for (int i = 0; i < 7; ++i) S.terms[i] = t[i];
+  out.push_back(S);
+  return 1;
+}
+
+int verify_twin_hit(int N, const TwinCand& C, u32 d, u32 c4, u32 c3, u64 fp, std::vector<TwinSolution>& out) {
+  const PairTarget t = twin_target(N, C, d, c4, c3, fp);
+  if (!t.ok) return 0;
   std::vector<std::pair<u32, u32>> pairs;
-  exact_pairs(T2, (u32)std::max(N, 0), pairs);
+  exact_pairs(t.T, t.hi, pairs);
   int found = 0;
-  for (auto [a, b] : pairs) {
-    i64 t[7] = {42LL * a, 42LL * b, 42LL * c3, 42LL * c4, 21LL * (i64)d, (i64)C.u, (i64)C.w};
-    std::sort(t, t + 7);
-    bool bad = t[0] < 1 || t[6] >= (i64)C.B;
-    for (int i = 0; i < 6; ++i) bad |= t[i] == t[i + 1];
-    if (bad) continue;
-    U256 lhs = U256::of(0);
-    for (int i = 0; i < 7; ++i) add_in_place(lhs, U256::pow6((u64)t[i]));
-    if (cmp(lhs, U256::pow6(C.B)) != 0) continue;
-    i64 g = t[0];
-    for (int i = 1; i < 7; ++i) g = std::gcd(g, t[i]);
-    g = std::gcd(g, (i64)C.B);
-    TwinSolution S{C.B, C.u, C.w, d, c4, c3, a, b, {}, g};
-    for (int i = 0; i < 7; ++i) S.terms[i] = t[i];
-    out.push_back(S);
-    ++found;
-  }
+  for (auto [a, b] : pairs) found += twin_finish(C, d, c4, c3, a, b, out);
   return found;
 }
 
@@ -921,6 +973,31 @@ void verify_twin_hits(int N, const std::vector<TwinCand>& cands, const std::vect
   for (long i = 0; i < (long)hits.size(); ++i) {
     const TwinHitIn& h = hits[(size_t)i];
     if (h.cand < cands.size()) verify_twin_hit(N, cands[h.cand], h.d, h.c4, h.c3, h.fp, per[(size_t)i]);
+  }
+}
+
+void verify_targets(int N, const std::vector<VerifyJob>& jobs, std::vector<PairTarget>& out) {
+  out.assign(jobs.size(), PairTarget{});
+#pragma omp parallel for schedule(dynamic, 64) if (jobs.size() > 256)
+  for (long i = 0; i < (long)jobs.size(); ++i) out[(size_t)i] = verify_target(N, jobs[(size_t)i]);
+}
+
+void verify_finish_all(const std::vector<VerifyJob>& jobs, const std::vector<std::vector<std::pair<u32, u32>>>& pairs,
+                       bool stop_first, std::vector<Solution>& out) {
+  for (size_t i = 0; i < jobs.size() && i < pairs.size(); ++i) {
+    int found = 0;
+    for (auto [a, b] : pairs[i]) found += verify_finish(jobs[i], a, b, out);
+    if (stop_first && found) break;
+  }
+}
+
+void twin_targets(int N, const std::vector<TwinCand>& cands, const std::vector<TwinHitIn>& hits,
+                  std::vector<PairTarget>& out) {
+  out.assign(hits.size(), PairTarget{});
+#pragma omp parallel for schedule(dynamic, 64) if (hits.size() > 256)
+  for (long i = 0; i < (long)hits.size(); ++i) {
+    const TwinHitIn& h = hits[(size_t)i];
+    if (h.cand < cands.size()) out[(size_t)i] = twin_target(N, cands[h.cand], h.d, h.c4, h.c3, h.fp);
   }
 }
 

@@ -218,6 +218,21 @@ struct VerifyJob {
 // Verifies jobs in parallel (OpenMP); solutions come out in job order.  With
 // stop_first, output ends after the first job that yields a solution.
 void verify_jobs(int N, const std::vector<VerifyJob>& jobs, bool stop_first, std::vector<Solution>& out);
+// The verification in two halves around a pair search (done on the device by
+// the library): the exact target T of the hit's pair part and the bound on
+// its terms, then the identity and term checks of one found pair (a <= b).
+struct PairTarget {
+  u128 T;
+  u32 hi;
+  bool ok;  // false: the hit's fingerprint does not match an exact, positive T
+};
+PairTarget verify_target(int N, const VerifyJob& j);
+int verify_finish(const VerifyJob& j, u32 a, u32 b, std::vector<Solution>& out);
+// verify_target for every job (OpenMP)
+void verify_targets(int N, const std::vector<VerifyJob>& jobs, std::vector<PairTarget>& out);
+// verify_finish over every job's pairs in job order (stop_first as in verify_jobs)
+void verify_finish_all(const std::vector<VerifyJob>& jobs, const std::vector<std::vector<std::pair<u32, u32>>>& pairs,
+                       bool stop_first, std::vector<Solution>& out);
 
 // ---------------------------------------------------------------------------
 // (6,1,7) class-5 twin (solve_617_v1.cu)
@@ -252,6 +267,8 @@ struct TwinSolution {
 // expand_xor_hits + verify_hit cls 5 (solve_617_v1.cu:1019-1110) for one hit;
 // appends every exact, distinct, in-range 7-term identity (not deduplicated)
 int verify_twin_hit(int N, const TwinCand& C, u32 d, u32 c4, u32 c3, u64 fp, std::vector<TwinSolution>& out);
+PairTarget twin_target(int N, const TwinCand& C, u32 d, u32 c4, u32 c3, u64 fp);
+int twin_finish(const TwinCand& C, u32 d, u32 c4, u32 c3, u32 a, u32 b, std::vector<TwinSolution>& out);
 struct TwinHitIn {
   u32 cand, d, c4, c3;
   u64 fp;
@@ -259,6 +276,9 @@ struct TwinHitIn {
 // verify_twin_hit for every hit in parallel (OpenMP); per[i] gets hit i's solutions
 void verify_twin_hits(int N, const std::vector<TwinCand>& cands, const std::vector<TwinHitIn>& hits,
                       std::vector<std::vector<TwinSolution>>& per);
+// twin_target for every hit (OpenMP)
+void twin_targets(int N, const std::vector<TwinCand>& cands, const std::vector<TwinHitIn>& hits,
+                  std::vector<PairTarget>& out);
 
 // splitmix64 (Vigna), the generator for planted keys
 inline u64 splitmix64(u64& s) {

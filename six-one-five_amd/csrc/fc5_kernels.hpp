@@ -154,6 +154,15 @@ struct PlainArgs {
   Counters* ctr;
 };
 
+// exact pair search (fc5_verify.hip): pairs a <= b <= hi with a^6 + b^6 = T
+struct PairJob {
+  u64 T_lo, T_hi;
+  u32 hi, pad;
+};
+struct PairHit {
+  u32 job, a, b, pad;
+};
+
 constexpr int kRowsPerItem = 8;  // default rows (e- or d-rows) per work item of the batched kernel
 
 // launchers (return hipError_t as int)
@@ -168,5 +177,6 @@ int launch_probe_only(const XorDev& x, u64 n, u64 salt, unsigned long long* sink
 int launch_gather_raw(const u64* cells, u64 m, u64 n, u64 salt, u32 policy, unsigned long long* sink, int grid,
                       hipStream_t s);
 int launch_copy_u64(const u64* src, u64* dst, u64 n, int grid, hipStream_t s);
+int launch_pair_search(const PairJob* jobs, u32 n, PairHit* out, u32 cap, u32* count, hipStream_t s);
 
 }  // namespace fc5

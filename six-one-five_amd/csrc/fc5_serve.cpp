@@ -209,8 +209,12 @@ void finalize(fc5_server* S, SBand& B) {
         jobs.push_back(VerifyJob{k, U.B, U.u, U.lim, h.d, h.e, h.fp});
       }
     }
+    std::vector<PairTarget> targets;
+    verify_targets(N, jobs, targets);
+    std::vector<std::vector<std::pair<u32, u32>>> pairs;
     std::vector<Solution> sols;
-    verify_jobs(N, jobs, false, sols);
+    if (device_pairs(ctx, targets, pairs) == FC5_OK) verify_finish_all(jobs, pairs, false, sols);
+    else B.rc = FC5_E_HIP;
     for (const Solution& x : sols) {
       fc5_solution o{};
       o.B = x.B;

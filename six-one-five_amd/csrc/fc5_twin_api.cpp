@@ -186,8 +186,14 @@ int twin_verify(fc5_ctx* ctx, const std::vector<TwinCand>& all) {
   std::vector<TwinHitIn> in;
   in.reserve(ctx->twin_hits.size());
   for (const fc5_twin_hit& h : ctx->twin_hits) in.push_back(TwinHitIn{h.cand, h.d, h.c4, h.c3, h.fp});
-  std::vector<std::vector<TwinSolution>> per;
-  verify_twin_hits(N, all, in, per);
+  std::vector<PairTarget> targets;
+  twin_targets(N, all, in, targets);
+  std::vector<std::vector<std::pair<u32, u32>>> pairs;
+  const int rc = device_pairs(ctx, targets, pairs);
+  if (rc != FC5_OK) return rc;
+  std::vector<std::vector<TwinSolution>> per(in.size());
+  for (size_t i = 0; i < in.size(); ++i)
+    for (auto [a, b] : pairs[i]) twin_finish(all[in[i].cand], in[i].d, in[i].c4, in[i].c3, a, b, per[i]);
   std::set<std::array<i64, 7>> reported;  // distinct tuples only (solve_617_v1.cu:1096)
   for (const auto& v : per) {
     for (const TwinSolution& S : v) {

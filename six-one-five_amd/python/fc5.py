@@ -142,6 +142,7 @@ _sig = {
     "fc5_serve_release": (C.c_int, [_P, C.c_uint64]),
     "fc5_serve_stats_get": (C.c_int, [_P, C.POINTER(ServeStats)]),
     "fc5_serve_stop": (C.c_int, [_P]),
+    "fc5_pair_search": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_twin_cands": (C.c_int, [C.c_uint64, C.c_double, C.c_double, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_twin_run": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(TwinResult)]),
     "fc5_twin_run_range": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.POINTER(TwinResult)]),
@@ -378,6 +379,19 @@ class Context:
             self._check(_lib.fc5_last_solutions(self.h, C.cast(arr, _P), n.value, C.byref(n)))
         return [dict(B=s.B, u=s.u, cls=s.cls, d=s.d, e=s.e, a=s.a, b=s.b, c=s.c, terms=list(s.terms),
                      gcd=s.gcd) for s in arr[:n.value]]
+
+    def pair_search(self, targets, hi):
+        """device exact check: list of (i, a, b) with a <= b <= hi[i], a^6 + b^6 = targets[i]"""
+        t = np.zeros(2 * len(targets), np.uint64)
+        for i, v in enumerate(targets):
+            t[2 * i], t[2 * i + 1] = v & (2 ** 64 - 1), v >> 64
+        h = np.ascontiguousarray(hi, dtype=np.uint32)
+        n = C.c_size_t()
+        self._check(_lib.fc5_pair_search(self.h, _ptr(t), _ptr(h), len(targets), None, 0, C.byref(n)))
+        out = np.zeros(3 * max(1, n.value), np.uint32)
+        self._check(_lib.fc5_pair_search(self.h, _ptr(t), _ptr(h), len(targets), out.ctypes.data_as(_P), n.value,
+                                         C.byref(n)))
+        return [tuple(int(x) for x in out[3 * k:3 * k + 3]) for k in range(n.value)]
 
     # ---- resident serving loop ----
     def serve_start(self, **opts):

@@ -430,3 +430,45 @@ def test_tile_shapes_and_autotune_keep_results(ctx):
         ctx.set_option("rows_per_item", 8)
         ctx.set_option("blocks_per_cu", 0)
         b.close()
+
+
+def test_device_pair_search_exact(ctx):
+    """fc5_pair_search against Python big-integer brute force: real pair sums
+    (a < b, a = b, 1^6 + b^6), near misses, bounds that cut the larger term,
+    zero; every reported pair is exact, also for targets near 2^127"""
+    import random
+    say("device pair search")
+    rnd = random.Random(6162)
+    targets, his = [], []
+    for bits in (8, 12, 16):
+        for _ in range(60):
+            a, b = sorted(rnd.randrange(1, 2 ** bits) for _ in range(2))
+            T = a ** 6 + b ** 6
+            for t, h in ((T, b), (T, b - 1), (T + 1, b), (T - 1, b), (2 * b ** 6, b), (1 + b ** 6, b)):
+                targets.append(t)
+                his.append(max(h, 0))
+    targets += [0, 2, 3]
+    his += [10, 1, 5]
+    got = ctx.pair_search(targets, his)
+    want = [(i, a, b) for i, (T, h) in enumerate(zip(targets, his)) for a, b in brute_pairs(T, h)]
+    assert got == want
+    assert len(want) >= 500
+    big = [2 ** 127 - 1, 2 * (2 ** 21 - 1) ** 6, (2 ** 21 - 5) ** 6 + 77 ** 6]
+    got = ctx.pair_search(big, [2 ** 21] * 3)
+    for i, a, b in got:
+        assert a <= b and a ** 6 + b ** 6 == big[i]
+    assert (1, 2 ** 21 - 1, 2 ** 21 - 1) in got and (2, 77, 2 ** 21 - 5) in got
+
+
+def brute_pairs(T, hi):
+    """all a <= b <= hi with a^6 + b^6 = T, by the larger term b"""
+    import bigint_ref
+    out = []
+    b = min(hi, bigint_ref.iroot6(T))
+    while b >= 1 and 2 * b ** 6 >= T:
+        r = T - b ** 6
+        a = bigint_ref.iroot6(r)
+        if 1 <= a <= b and a ** 6 == r:
+            out.append((a, b))
+        b -= 1
+    return sorted(out)
