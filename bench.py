@@ -63,6 +63,7 @@ def main():
     ap.add_argument("--twin-lo", type=int, default=400471, help="617: first B")
     ap.add_argument("--twin-hi", type=int, default=400471, help="617: last B")
     ap.add_argument("--twin-cpu-every", type=int, default=128, help="617: CPU oracle on every k-th candidate")
+    ap.add_argument("--twin-plain-every", type=int, default=4, help="617: plain form on every k-th candidate")
     ap.add_argument("--r", type=int, default=48)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--plain-every", type=int, default=64, help="plain form on every k-th unit")
@@ -73,7 +74,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
     ap.add_argument("--load-policy", type=int, default=None, choices=[0, 1, 2, 3],
-                    help="cache policy of the cell gathers (library default 1, nontemporal)")
+                    help="cache policy of the cell gathers (library default: by table size)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box of this build has one GPU and the band "
@@ -297,6 +298,25 @@ def bench_twin(args):
                    "kernel_call_rate_frac_of_probe_only": res["calls"] / kern_s / probe_rate}
         log(f"[bench] gather ceiling on this table: {probe_rate:.4e} probes/s; kernel at "
             f"{res['calls'] / kern_s / probe_rate:.1%}")
+    plain = None
+    if not args.no_plain:
+        sub = cands[:: args.twin_plain_every]
+        gb, _, _ = ctx.twin_run(sub)
+        ctx.set_option("twin_plain", 1)
+        t1 = time.perf_counter()
+        gp, _, _ = ctx.twin_run(sub)
+        tp = time.perf_counter() - t1
+        ctx.set_option("twin_plain", 0)
+        t1 = time.perf_counter()
+        gb2, _, _ = ctx.twin_run(sub)
+        tb = time.perf_counter() - t1
+        for k in ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "maybe"):
+            assert gp[k] == gb[k] == gb2[k], k
+        plain = {"subset": f"every {args.twin_plain_every}th candidate ({len(sub)} candidates)",
+                 "calls_per_s": gp["calls"] / tp, "batched_calls_per_s_same_subset": gb2["calls"] / tb,
+                 "speedup_batched_vs_plain": (gb2["calls"] / tb) / (gp["calls"] / tp), "counters_identical": True}
+        log(f"[bench] plain form {gp['calls'] / tp:.4e} calls/s on the subset; batched {gb2['calls'] / tb:.4e}; "
+            f"batched/plain = {plain['speedup_batched_vs_plain']:.1f}x")
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -332,6 +352,7 @@ def bench_twin(args):
                      "kernel": "k_twin_batched",
                      "per_unit": f"{3 * cell_bytes} B per probe call (3 cells x {cell_bytes} B)"},
         "gather_ceiling": ceiling,
+        "plain_form": plain,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -109,9 +109,12 @@ const char *fc5_last_error(const fc5_ctx *ctx);
  * fourcore_cls5_gpu_v4.cu:634), "blocks_per_cu" (0 = occupancy limit) for the
  * persistent grid, "rows_per_item" (4, 8, 16, 32; default 8: e- or d-rows per work
  * item), "use_graph" (0; 1 replays the batched pipeline as a captured hipGraph),
- * "load_policy" (0..3, default 1: cache policy of the xor cell gathers:
- * default, nontemporal, device-scope, system-scope), "verify" (1: pair recovery
- * + exact check of hits on the host),
+ * "load_policy" (-1..3, default -1: cache policy of the xor cell gathers:
+ * -1 by table size (plain loads up to 1 GiB of cells, non-temporal above),
+ * 0 default, 1 nontemporal, 2 device-scope, 3 system-scope), "verify" (1: exact
+ * check of hits: device pair search + host identity check),
+ * "twin_*" (the (6,1,7) twin options below; "twin_plain" 1 runs its
+ * reference-shaped kernel),
  * "stop_first" (0, fourcore_cls5_gpu_v4.cu:635) */
 int fc5_set_option(fc5_ctx *ctx, const char *key, int64_t value);
 

@@ -114,10 +114,10 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) {
   } else if (k == "stop_first") {
     ctx->stop_first = v ? 1 : 0;
   } else if (k == "load_policy") {
-    if (v < 0 || v > 3) FAIL(ctx, FC5_E_ARG, "load_policy must be 0..3");
-    ctx->load_policy = (u32)v;
-    ctx->main.dev.policy = ctx->load_policy;
-    ctx->side.dev.policy = ctx->load_policy;
+    if (v < -1 || v > 3) FAIL(ctx, FC5_E_ARG, "load_policy must be -1..3");
+    ctx->load_policy = (int)v;
+    ctx->main.dev.policy = table_policy(ctx->load_policy, ctx->main.hdr.m_cells);
+    ctx->side.dev.policy = table_policy(ctx->load_policy, ctx->side.hdr.m_cells);
   } else if (k == "rows_per_item") {
     if (v != 4 && v != 8 && v != 16 && v != 32) FAIL(ctx, FC5_E_ARG, "rows_per_item must be 4, 8, 16 or 32");
     ctx->rows_per_item = (int)v;
@@ -125,6 +125,8 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) {
     ctx->use_graph = v ? 1 : 0;
   } else if (k == "twin_tri_gate") {
     ctx->twin_tri = v ? 1 : 0;
+  } else if (k == "twin_plain") {
+    ctx->twin_plain = v ? 1 : 0;
   } else if (k == "twin_c4_div") {
     if (v != 3 && v != 4) FAIL(ctx, FC5_E_ARG, "twin_c4_div must be 3 or 4");
     ctx->twin_c4_div = (int)v;
@@ -175,7 +177,7 @@ static int upload_table(fc5_ctx* ctx, const StoreHeader& h, const u8* packed, si
   dst.dev.seed = h.mix_seed;
   dst.dev.md = make_moddiv(dst.dev.block);
   dst.dev.present = dst.dev.block ? 1u : 0u;
-  dst.dev.policy = ctx->load_policy;
+  dst.dev.policy = table_policy(ctx->load_policy, h.m_cells);
   dst.present = true;
   return FC5_OK;
 }
@@ -244,7 +246,7 @@ extern "C" int fc5_table_build(fc5_ctx* ctx, int N, int r, const uint64_t* extra
     t.dev.seed = g.hdr.mix_seed;
     t.dev.md = make_moddiv(t.dev.block);
     t.dev.present = t.dev.block ? 1u : 0u;
-    t.dev.policy = ctx->load_policy;
+    t.dev.policy = table_policy(ctx->load_policy, g.hdr.m_cells);
     t.present = true;
       return FC5_OK;
   }

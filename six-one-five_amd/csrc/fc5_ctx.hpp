@@ -30,6 +30,14 @@ inline void free_table(DeviceTable& t) {
   t = DeviceTable{};
 }
 
+// Cell-gather cache policy of a table: the option when set, else plain loads
+// for tables of up to 1 GiB of cells (they partly stay in L2 and the MALL) and
+// non-temporal loads above (lines are not reused; measured 1-12% faster).
+inline u32 table_policy(int option, u64 m_cells) {
+  if (option >= 0) return (u32)option;
+  return m_cells * 8 <= (1ull << 30) ? 0u : 1u;
+}
+
 inline u64 inv_mod_2_64(u64 a) {  // a odd; Newton iteration doubles correct bits
   u64 x = a;
   for (int i = 0; i < 6; ++i) x *= 2 - a * x;
@@ -86,7 +94,7 @@ struct fc5_ctx {
   int rows_per_item = kRowsPerItem, use_graph = 0;
   Counters* h_ctr = nullptr;  // pinned read-back of the counters
   u64* h_nhits = nullptr;
-  u32 load_policy = 1;  // cache policy of the xor cell gathers (nontemporal: no MALL allocation)
+  int load_policy = -1;  // cache policy of the xor cell gathers; -1: by table size (table_policy)
   // tables
   DeviceTable main, side;
   XorTable host_main;  // host copy when small enough (save, host verify)
@@ -112,7 +120,7 @@ struct fc5_ctx {
   std::vector<fc5_hit> hits;
   std::vector<fc5_solution> sols;
   // (6,1,7) class-5 twin
-  int twin_tri = 1, twin_c4_div = 3;
+  int twin_tri = 1, twin_c4_div = 3, twin_plain = 0;
   u32* d_tri = nullptr;               // triple-sum sets mod 504 (words 0..15) and 247 (16..23)
   TwinCounters* d_twin_ctr = nullptr;
   TwinCounters* h_twin_ctr = nullptr; // pinned read-back

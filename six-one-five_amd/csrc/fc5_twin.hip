@@ -1,5 +1,10 @@
 // fc5_twin.hip -- the (6,1,7) class-5 twin search on gfx950 (MI355X).
 //
+// k_twin_plain is the reference kernel shape for comparison: a block per
+// (candidate, d, c4 tile of 2048), thread 0 computes the windows in R-space
+// (double estimate + exact 128-bit fix-ups), the wavefronts stride c4, the
+// lanes stride c3 and probe inline.
+//
 // k_twin_batched is a persistent kernel over a device work queue; one work
 // item is one (candidate, d) cell of the reference's grid
 // (k_find4_cls5, solve_617_v1.cu:563-657).  For its T it takes 64 values of
@@ -176,7 +181,171 @@ __global__ __launch_bounds__(256) void k_twin_batched(TwinArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// plain form: the reference kernel shape (k_find4_cls5, solve_617_v1.cu:563-657)
+// ---------------------------------------------------------------------------
+constexpr u64 kM42 = 5489031744ULL;
+
+// s * c^6 > R (s * c^6 past 2^128 counts as greater)
+__device__ __forceinline__ bool scaled_gt(u128 R, u32 c, u64 s) {
+  u128 v;
+  if (__builtin_mul_overflow(p6_u128(c), (u128)s, &v)) return true;
+  return v > R;
+}
+
+// largest r with s * r^6 <= R, from a double estimate and exact fix-ups
+__device__ u32 root_fix(u128 R, u64 s) {
+  const double x = ((double)(u64)(R >> 64) * 18446744073709551616.0 + (double)(u64)R) / (double)s;
+  u32 r = (u32)pow(x, 1.0 / 6.0) + 2;
+  while (r > 0 && scaled_gt(R, r, s)) --r;
+  while (!scaled_gt(R, r + 1, s)) ++r;
+  return r;
+}
+
+// least c >= 1 with R <= s * c^6
+__device__ u32 min_fix(u128 R, u32 start, u64 s) {
+  u32 c = start < 1 ? 1 : start;
+  while (!scaled_gt(R, c, s) && (u128)s * p6_u128(c) != R) ++c;
+  while (c > 1 && (scaled_gt(R, c - 1, s) || (u128)s * p6_u128(c - 1) == R)) --c;
+  return c;
+}
+
+// (hi:lo) mod m with c = 2^64 mod m, m < 2^42
+__device__ __forceinline__ u64 mod128(u128 R, u64 m, u64 c) {
+  const u64 hi = (u64)(R >> 64), lo = (u64)R;
+  const u64 r = hi % m;
+  const u128 rc = (u128)r * c;
+  u64 t = (u64)((u128)(u64)(rc >> 64) * c % m);
+  t += (u64)((u64)rc % m);
+  if (t >= m) t -= m;
+  t += lo % m;
+  if (t >= m) t -= m;
+  return t;
+}
+
+__global__ __launch_bounds__(256) void k_twin_plain(PlainTwinArgs a) {
+  const u32 ci = blockIdx.x;
+  if (ci >= a.n_cands) return;
+  const PlainTwinCand& C = a.cands[ci];
+  const u32 kmax = C.fmax1 / 64u + 1u;
+  const u32 t = blockIdx.y;
+  if (t >= C.nres * kmax) return;
+  const u32 d = C.res[t % C.nres] + (t / C.nres) * 64u;
+  if (d == 0 || d > C.fmax1) return;
+  __shared__ u64 s_rlo, s_rhi;
+  __shared__ u32 s_lo4, s_hi4, s_q504, s_q247;
+  __shared__ int s_active;
+  __shared__ u32 s_tri[24];
+  __shared__ u16 s_p504[504], s_p247[247];
+  if (threadIdx.x == 0) {
+    s_active = 0;
+    const u128 base = ((u128)C.b_hi << 64) | C.b_lo;
+    const u128 g6 = p6_u128(21u * d);
+    if (base > g6) {
+      const u128 R = base - g6;
+      u32 hi4 = root_fix(R, kM42);
+      if (hi4 > C.lim) hi4 = C.lim;
+      if (hi4 >= 1) {
+        const u32 lo4 = min_fix(R, (u32)(hi4 * 0.8), (u64)a.c4_div * kM42);
+        if (lo4 <= hi4) {
+          s_rlo = (u64)R;
+          s_rhi = (u64)(R >> 64);
+          s_lo4 = lo4;
+          s_hi4 = hi4;
+          s_q504 = (u32)(mod128(R, 504ULL * kM42, a.c_m504) / kM42 % 504ULL);
+          s_q247 = (u32)(mod128(R, 247ULL, a.c_m247) * a.inv42_247 % 247ULL);
+          s_active = 1;
+          // a block per (cand, d, tile): count the cell once, in tile 0
+          if (blockIdx.z == 0) atomicAdd((unsigned long long*)&a.ctr->jobs, 1ULL);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (!s_active) return;
+  for (u32 i = threadIdx.x; i < 504; i += blockDim.x) s_p504[i] = a.p6[i];
+  for (u32 i = threadIdx.x; i < 247; i += blockDim.x) s_p247[i] = a.p6[504 + i];
+  if (threadIdx.x < 24) s_tri[threadIdx.x] = a.tri[threadIdx.x];
+  __syncthreads();
+  const u128 R = ((u128)s_rhi << 64) | s_rlo;
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  const u32 t0 = s_lo4 + blockIdx.z * 2048u;
+  const u32 t1 = min(s_hi4, s_lo4 + blockIdx.z * 2048u + 2047u);
+  u64 n_c4 = 0, n_skip = 0, n_calls = 0, n_gated = 0;
+  for (u32 c4 = t0 + wave; c4 <= t1; c4 += waves) {
+    u32 t4504 = s_q504 + 504u - s_p504[c4 % 504u];
+    if (t4504 >= 504u) t4504 -= 504u;
+    u32 t4247 = s_q247 + 247u - s_p247[c4 % 247u];
+    if (t4247 >= 247u) t4247 -= 247u;
+    if (a.use_tri) {
+      ++n_c4;
+      if (!((s_tri[t4504 >> 5] >> (t4504 & 31u)) & 1u) || !((s_tri[16 + (t4247 >> 5)] >> (t4247 & 31u)) & 1u)) {
+        ++n_skip;
+        continue;
+      }
+    }
+    u128 m4;
+    if (__builtin_mul_overflow(p6_u128(c4), (u128)kM42, &m4) || m4 >= R) continue;
+    const u128 R4 = R - m4;
+    u32 hi3 = root_fix(R4, kM42);
+    hi3 = min(hi3, min(C.lim, c4));
+    if (hi3 < 1) continue;
+    const u32 lo3 = min_fix(R4, (u32)(hi3 * 0.83), 3ULL * kM42);
+    const u64 tfp = (((u64)R4 >> 6) | ((u64)(R4 >> 64) << 58)) * a.inv216;  // T4 mod 2^64
+    u32 r504 = (lo3 + lane) % 504u, r247 = (lo3 + lane) % 247u;
+    for (u32 c3 = lo3 + lane; c3 <= hi3; c3 += 64u) {
+      bool pass = true;
+      if (a.use_gate) {
+        u32 x = t4504 + 504u - s_p504[r504];
+        if (x >= 504u) x -= 504u;
+        u32 y = t4247 + 247u - s_p247[r247];
+        if (y >= 247u) y -= 247u;
+        pass = ((a.gate.g504[x >> 6] >> (x & 63u)) & 1ULL) && ((a.gate.g247[y >> 6] >> (y & 63u)) & 1ULL);
+        r504 += 64u;
+        if (r504 >= 504u) r504 -= 504u;
+        r247 += 64u;
+        if (r247 >= 247u) r247 -= 247u;
+      }
+      if (!pass) {
+        ++n_gated;
+        continue;
+      }
+      ++n_calls;
+      const u64 c2 = (u64)c3 * c3;
+      const u64 fp = tfp - c2 * c2 * c2;
+      if (d_query(a.main, fp)) {
+        atomicAdd((unsigned long long*)&a.ctr->maybe, 1ULL);
+        const u64 s = atomicAdd((unsigned long long*)a.nhits, 1ULL);
+        if (s < a.hit_cap) {
+          TwinHit h;
+          h.cand = C.uid;
+          h.d = d;
+          h.c4 = c4;
+          h.c3 = c3;
+          h.fp = fp;
+          a.hits[s] = h;
+        }
+      }
+      if (hi3 - c3 < 64u) break;
+    }
+  }
+  n_calls = wave_sum(n_calls);
+  n_gated = wave_sum(n_gated);
+  if (lane == 0) {
+    TwinCounters* c = a.ctr;
+    if (n_c4) atomicAdd((unsigned long long*)&c->c4_total, (unsigned long long)n_c4);
+    if (n_skip) atomicAdd((unsigned long long*)&c->c4_tri_skipped, (unsigned long long)n_skip);
+    if (n_calls) atomicAdd((unsigned long long*)&c->calls, (unsigned long long)n_calls);
+    if (n_gated) atomicAdd((unsigned long long*)&c->gated, (unsigned long long)n_gated);
+  }
+}
+
 }  // namespace
+
+int launch_twin_plain(const PlainTwinArgs& a, u32 ymax, u32 zmax, hipStream_t s) {
+  hipLaunchKernelGGL(k_twin_plain, dim3(a.n_cands, ymax, zmax), dim3(256), 0, s, a);
+  return (int)hipGetLastError();
+}
 
 int launch_twin_batched(const TwinArgs& a, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_twin_batched, dim3(grid), dim3(64 * kTwWaves), 0, s, a);

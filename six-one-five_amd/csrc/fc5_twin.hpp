@@ -50,7 +50,31 @@ struct TwinArgs {
   u32* item_ctr;
 };
 
+// reference-shaped candidate (GpuCand class 5, solve_617_v1.cu:354-363)
+struct PlainTwinCand {
+  u64 b_lo, b_hi;  // base = B^6 - u^6 - w^6
+  u32 lim, fmax1, nres, uid;
+  u32 res[8];
+};
+static_assert(sizeof(PlainTwinCand) == 64, "PlainTwinCand is 64 bytes");
+
+struct PlainTwinArgs {
+  const PlainTwinCand* cands;
+  u32 n_cands, use_gate, use_tri, c4_div;
+  GateWords gate;   // pair-sum sets mod 504 and 247
+  const u32* tri;   // as TwinArgs::tri
+  const u16* p6;    // x^6 mod 504 (504 entries) then mod 247 (247 entries)
+  u64 inv216, c_m504, c_m247, inv42_247;
+  XorDev main;
+  TwinHit* hits;
+  u64 hit_cap;
+  u64* nhits;
+  TwinCounters* ctr;
+};
+
 int launch_twin_batched(const TwinArgs& a, int grid, hipStream_t s);
 int twin_blocks_per_cu();
+// the reference launch shape: grid (candidate, d-row, c4 tile of 2048)
+int launch_twin_plain(const PlainTwinArgs& a, u32 ymax, u32 zmax, hipStream_t s);
 
 }  // namespace fc5

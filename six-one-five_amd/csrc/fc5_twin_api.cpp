@@ -15,6 +15,12 @@ using namespace fc5;
 
 namespace {
 
+u64 inv_mod_small(u64 a, u64 m) {  // a coprime to m, m small
+  for (u64 x = 1; x < m; ++x)
+    if (a * x % m == 1) return x;
+  return 0;
+}
+
 // triple-sum sets of sixth powers mod 504 and 247, 0 included
 // (build_tri_set, solve_617_v1.cu:334-346): words 0..15 and 16..23
 std::vector<u32> tri_words() {
@@ -138,7 +144,53 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
       launch_fill_u64(ctx->d_nhits, 1, 0, ctx->stream) != 0 ||
       launch_fill_u64(reinterpret_cast<u64*>(ctx->d_item_ctr), 1, 0, ctx->stream) != 0)
     FAIL(ctx, FC5_E_HIP, "counter reset");
-  if (items) {
+  std::vector<PlainTwinCand> plain;
+  PlainTwinCand* d_plain = nullptr;
+  if (items && ctx->twin_plain) {
+    // the reference launch shape over the same candidates
+    u32 ymax = 1, lmax = 0;
+    plain.resize(cands.size());
+    for (size_t i = 0; i < cands.size(); ++i) {
+      const TwinCand& C = cands[i];
+      PlainTwinCand& P = plain[i];
+      std::memset(&P, 0, sizeof P);
+      P.b_lo = (u64)C.base;
+      P.b_hi = (u64)(C.base >> 64);
+      P.lim = C.lim;
+      P.fmax1 = C.fmax1;
+      P.nres = C.nres;
+      P.uid = uid[i];
+      for (u32 k = 0; k < C.nres; ++k) P.res[k] = C.res[k];
+      ymax = std::max(ymax, C.nres * (C.fmax1 / 64 + 1));
+      lmax = std::max(lmax, C.lim);
+    }
+    if (hipMalloc(&d_plain, plain.size() * sizeof(PlainTwinCand)) != hipSuccess)
+      FAIL(ctx, FC5_E_NOMEM, "plain twin candidates");
+    PlainTwinArgs p{};
+    p.cands = d_plain;
+    p.n_cands = (u32)plain.size();
+    p.use_gate = (u32)ctx->use_gate;
+    p.use_tri = (u32)ctx->twin_tri;
+    p.c4_div = (u32)ctx->twin_c4_div;
+    p.gate = ctx->gw;
+    p.tri = ctx->d_tri;
+    p.p6 = ctx->d_p6;
+    p.inv216 = inv_mod_2_64(85766121ULL);
+    p.c_m504 = (u64)(((u128)1 << 64) % (u128)(504ULL * kM42));
+    p.c_m247 = (u64)(((u128)1 << 64) % (u128)247ULL);
+    p.inv42_247 = (u64)inv_mod_small((u64)pow6_mod(42, 247), 247);
+    p.main = ctx->main.dev;
+    p.hits = a.hits;
+    p.hit_cap = a.hit_cap;
+    p.nhits = a.nhits;
+    p.ctr = a.ctr;
+    const u32 zmax = lmax / 2048 + 1;
+    if (hipMemcpy(d_plain, plain.data(), plain.size() * sizeof(PlainTwinCand), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_twin_plain(p, ymax, zmax, ctx->stream) != 0) {
+      (void)hipFree(d_plain);
+      FAIL(ctx, FC5_E_HIP, "plain twin launch");
+    }
+  } else if (items) {
     const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : twin_blocks_per_cu();
     const u64 want = (items + 3) / 4;  // 4 waves per workgroup
     const int grid = (int)std::max<u64>(1, std::min<u64>((u64)ctx->num_cu * (u64)bpc, want));
@@ -148,7 +200,9 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
                              ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->h_nhits, ctx->d_nhits, 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const hipError_t sync = hipStreamSynchronize(ctx->stream);
+  if (d_plain) (void)hipFree(d_plain);
+  HIPCHK(ctx, sync);
   float ms = 0;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   const TwinCounters c = *ctx->h_twin_ctr;

@@ -77,6 +77,24 @@ def test_twin_whole_B_matches_oracle(ctx, r, c4_div, use_gate, use_tri):
         assert oc["maybe"] > 0
 
 
+@pytest.mark.parametrize("c4_div,use_gate,use_tri", [(3, True, True), (4, True, True), (3, False, False)])
+def test_twin_plain_form_matches_oracle(ctx, c4_div, use_gate, use_tri):
+    """the reference-shaped kernel (block per candidate x d x c4 tile) over the same
+    candidates and table bytes"""
+    say(f"twin plain form c4_div={c4_div} gate={use_gate} tri={use_tri}")
+    t = device_table(ctx, 1500, 16)
+    cands = np.concatenate([O.twin_cands(54347), O.twin_cands(59819)])
+    set_opts(ctx, c4_div, use_gate, use_tri)
+    ctx.set_option("twin_plain", 1)
+    try:
+        res, hits, sols = ctx.twin_run(cands.view(fc5.TWIN_CAND_DTYPE))
+    finally:
+        ctx.set_option("twin_plain", 0)
+        set_opts(ctx, 3, True, True)
+    oc, oh, _ = O.twin_run(cands, t, use_gate=use_gate, use_tri=use_tri, c4_div=c4_div, threads=THREADS)
+    compare(res, hits, oc, oh)
+
+
 def test_twin_large_candidates_match_oracle(ctx):
     """two full candidates at B = 400471 (one holds the known solution), r = 48,
     N = 9535: 4.4e8 c3 values and a 400 MB table"""
