@@ -6,6 +6,8 @@
 //   fourcore_hunt_v3 --density --lo A --hi B [--classes LIST] [--u-band lo,hi]
 //   fourcore_hunt_v3 --lo A --hi B [--classes LIST] [--u-band lo,hi]
 //                    [--emit-units FILE] [--expand --emit FILE] [--max-expand N]
+// A FILE ending in .bbj is written in the binary .bbj layout (fourcore_job_bin.hpp):
+// unit records for --emit-units, expanded jobs (free terms + exact T) for --emit.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -151,14 +153,20 @@ int main(int argc, char** argv) {
   }
   FILE* fu = nullptr;
   FILE* ft = nullptr;
-  if (!emit_units.empty() && !(fu = fopen(emit_units.c_str(), "w"))) { perror(emit_units.c_str()); return 1; }
+  BbjWriter bu, bt;  // .bbj outputs (binary unit records / binary jobs)
+  const bool u_bin = !emit_units.empty() && is_bbj_path(emit_units);
+  const bool t_bin = do_expand && !emit_t.empty() && is_bbj_path(emit_t);
+  if (!emit_units.empty()) {
+    if (u_bin ? !bu.open(emit_units) : !(fu = fopen(emit_units.c_str(), "w"))) { perror(emit_units.c_str()); return 1; }
+  }
   if (do_expand) {
     if (emit_t.empty()) { fprintf(stderr, "--expand requires --emit FILE\n"); return 1; }
-    if (!(ft = fopen(emit_t.c_str(), "w"))) { perror(emit_t.c_str()); return 1; }
+    if (t_bin ? !bt.open(emit_t) : !(ft = fopen(emit_t.c_str(), "w"))) { perror(emit_t.c_str()); return 1; }
   } else if (!emit_t.empty()) {
     fprintf(stderr, "[warn] --emit without --expand ignored (use --emit-units or --expand)\n");
   }
-  if (!fu && !ft) {
+  const bool want_units = fu || u_bin, want_jobs = ft || t_bin;
+  if (!want_units && !want_jobs) {
     fprintf(stderr, "nothing to do: pass --emit-units and/or --expand --emit, or --density\n");
     usage();
     return 1;
@@ -175,12 +183,15 @@ int main(int argc, char** argv) {
       for (i64 u : unit_candidates((i64)B, cls, ulo, uhi)) {
         ++nUnits;
         if (fu) fprintf(fu, "%d %llu %lld\n", cls, (unsigned long long)B, (long long)u);
-        if (!ft) continue;
+        if (u_bin) bu.add(cls, B, (u64)u, 0, 0, 0);
+        if (!want_jobs) continue;
         auto emit = [&](u64 f1, u64 f2, u128 T) {
           if (max_expand && nJobs >= max_expand) { cap = true; return; }
-          fprintf(ft, "%d %llu %lld %llu %llu %llu %llu\n", cls, (unsigned long long)B, (long long)u,
-                  (unsigned long long)f1, (unsigned long long)f2, (unsigned long long)(u64)T,
-                  (unsigned long long)(u64)(T >> 64));
+          if (t_bin) bt.add(cls, B, (u64)u, f1, f2, T);
+          else
+            fprintf(ft, "%d %llu %lld %llu %llu %llu %llu\n", cls, (unsigned long long)B, (long long)u,
+                    (unsigned long long)f1, (unsigned long long)f2, (unsigned long long)(u64)T,
+                    (unsigned long long)(u64)(T >> 64));
           ++nJobs;
         };
         if (cls == 1) {
@@ -222,5 +233,9 @@ int main(int argc, char** argv) {
           cap ? " (hit --max-expand)" : "");
   if (fu) fclose(fu);
   if (ft) fclose(ft);
+  if ((u_bin && !bu.close()) || (t_bin && !bt.close())) {
+    fprintf(stderr, "error writing the .bbj output\n");
+    return 1;
+  }
   return 0;
 }

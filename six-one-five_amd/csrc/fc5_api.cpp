@@ -608,11 +608,17 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
   std::vector<DevUnit5> dev;
   std::vector<DevUnit3> dev3;
   std::vector<PlainUnit> plain;
+  std::vector<Cls5Unit> P5;
+  std::vector<Cls234Unit> P3;
+  std::vector<char> ok5, ok3;
+  std::vector<int> slot;
+  static_assert(sizeof(fc5_unit) == 16, "fc5_unit is two u64");
+  pack_units(cls, reinterpret_cast<const u64*>(units), n, P5, ok5, P3, ok3, slot);
   for (size_t i = 0; i < n; ++i) {
     const int c = cls ? (int)cls[i] : 5;
     if (c == 5) {
-      Cls5Unit U;
-      if (!pack_cls5(units[i].B, units[i].u, U)) { ++b->n_skipped; continue; }
+      if (!ok5[(size_t)slot[i]]) { ++b->n_skipped; continue; }
+      const Cls5Unit& U = P5[(size_t)slot[i]];
       b->cls_of[i] = 5;
       b->slot_of[i] = (int)b->units.size();
       b->max_fmax2 = std::max(b->max_fmax2, U.fmax2);
@@ -639,8 +645,8 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
       b->units.push_back(U);
       b->uid.push_back((u32)i);
     } else if (c >= 2 && c <= 4) {
-      Cls234Unit U;
-      if (!pack_cls234(c, units[i].B, units[i].u, U)) { ++b->n_skipped; continue; }
+      if (!ok3[(size_t)slot[i]]) { ++b->n_skipped; continue; }
+      const Cls234Unit& U = P3[(size_t)slot[i]];
       b->cls_of[i] = (u8)c;
       b->slot_of[i] = (int)b->units3.size();
       b->max_lim3 = std::max(b->max_lim3, U.lim);

@@ -2,6 +2,7 @@
 #include "fc5_host.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -287,7 +288,103 @@ bool verify_cls5(u64 B, u64 u, u64 d, u64 e, u64 a, u64 b) {
 // ---------------------------------------------------------------------------
 // files
 // ---------------------------------------------------------------------------
+#pragma pack(push, 1)
+struct BbjHeader {
+  u32 magic = 0x4a424346u;  // 'FCJB'
+  u32 version = 2;
+  u64 n_jobs = 0;
+  u32 rec_bytes = 36;
+  u32 min_B = 0, max_B = 0;
+  u32 pad = 0;
+};
+struct BbjRec {
+  u8 cls = 0;
+  u8 pad[3] = {0, 0, 0};
+  u32 B = 0, u = 0, free1 = 0, free2 = 0;
+  u64 T_lo = 0, T_hi = 0;
+};
+#pragma pack(pop)
+static_assert(sizeof(BbjHeader) == 32 && sizeof(BbjRec) == 36, ".bbj layout");
+
+bool is_bbj_path(const std::string& path) {
+  if (path.size() < 4) return false;
+  std::string e = path.substr(path.size() - 4);
+  for (char& c : e) c = (char)std::tolower((unsigned char)c);
+  return e == ".bbj";
+}
+
+bool BbjWriter::open(const std::string& path) {
+  f_ = std::fopen(path.c_str(), "wb");
+  if (!f_) return false;
+  BbjHeader h;
+  return std::fwrite(&h, sizeof h, 1, f_) == 1;
+}
+
+bool BbjWriter::add(int cls, u64 B, u64 u, u64 free1, u64 free2, u128 T) {
+  if (!f_ || B > 0xffffffffull || u > 0xffffffffull || free1 > 0xffffffffull || free2 > 0xffffffffull) return false;
+  BbjRec r;
+  r.cls = (u8)cls;
+  r.B = (u32)B;
+  r.u = (u32)u;
+  r.free1 = (u32)free1;
+  r.free2 = (u32)free2;
+  r.T_lo = (u64)T;
+  r.T_hi = (u64)(T >> 64);
+  ++n_;
+  min_B_ = std::min(min_B_, (u32)B);
+  max_B_ = std::max(max_B_, (u32)B);
+  return std::fwrite(&r, sizeof r, 1, f_) == 1;
+}
+
+bool BbjWriter::close() {
+  if (!f_) return false;
+  BbjHeader h;
+  h.n_jobs = n_;
+  h.min_B = n_ ? min_B_ : 0;
+  h.max_B = max_B_;
+  const bool ok = std::fseek(f_, 0, SEEK_SET) == 0 && std::fwrite(&h, sizeof h, 1, f_) == 1;
+  const bool closed = std::fclose(f_) == 0;
+  f_ = nullptr;
+  return ok && closed;
+}
+
+BbjWriter::~BbjWriter() {
+  if (f_) std::fclose(f_);
+}
+
+bool read_units_bbj(const std::string& path, unsigned cls_mask, std::vector<BucLine>& out, std::string& msg) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) { msg = "cannot open " + path; return false; }
+  BbjHeader h;
+  if (std::fread(&h, sizeof h, 1, f) != 1 || h.magic != 0x4a424346u || h.version != 2 || h.rec_bytes != 36) {
+    std::fclose(f);
+    msg = "bad .bbj header in " + path;
+    return false;
+  }
+  BbjRec r;
+  u64 n = 0;
+  while (std::fread(&r, sizeof r, 1, f) == 1) {
+    ++n;
+    if (r.free1 || r.free2 || r.T_lo || r.T_hi) {
+      std::fclose(f);
+      msg = path + " holds expanded jobs (free terms set), not Stage-1 units";
+      return false;
+    }
+    if (r.cls < 1 || r.cls > 5 || !(cls_mask & (1u << r.cls))) continue;
+    out.push_back({(int)r.cls, (u64)r.B, (u64)r.u});
+  }
+  std::fclose(f);
+  if (n != h.n_jobs) { msg = "short .bbj file " + path; return false; }
+  return true;
+}
+
 bool read_buc(const std::string& path, unsigned cls_mask, std::vector<BucLine>& out) {
+  if (is_bbj_path(path)) {
+    std::string msg;
+    if (read_units_bbj(path, cls_mask, out, msg)) return true;
+    std::fprintf(stderr, "%s\n", msg.c_str());
+    return false;
+  }
   FILE* f = std::fopen(path.c_str(), "r");
   if (!f) return false;
   char buf[256];
@@ -998,6 +1095,30 @@ void twin_targets(int N, const std::vector<TwinCand>& cands, const std::vector<T
   for (long i = 0; i < (long)hits.size(); ++i) {
     const TwinHitIn& h = hits[(size_t)i];
     if (h.cand < cands.size()) out[(size_t)i] = twin_target(N, cands[h.cand], h.d, h.c4, h.c3, h.fp);
+  }
+}
+
+void pack_units(const u32* cls, const u64* bu, size_t n, std::vector<Cls5Unit>& u5, std::vector<char>& ok5,
+                std::vector<Cls234Unit>& u3, std::vector<char>& ok3, std::vector<int>& slot) {
+  slot.assign(n, -1);
+  size_t n5 = 0, n3 = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const u32 c = cls ? cls[i] : 5u;
+    if (c == 5) slot[i] = (int)n5++;
+    else if (c >= 2 && c <= 4) slot[i] = (int)n3++;
+  }
+  u5.assign(n5, Cls5Unit{});
+  ok5.assign(n5, 0);
+  u3.assign(n3, Cls234Unit{});
+  ok3.assign(n3, 0);
+#pragma omp parallel for schedule(dynamic, 256) if (n > 4096)
+  for (long i = 0; i < (long)n; ++i) {
+    const int s = slot[(size_t)i];
+    if (s < 0) continue;
+    const u32 c = cls ? cls[i] : 5u;
+    const u64 B = bu[2 * i], u = bu[2 * i + 1];
+    if (c == 5) ok5[(size_t)s] = pack_cls5(B, u, u5[(size_t)s]) ? 1 : 0;
+    else ok3[(size_t)s] = pack_cls234((int)c, B, u, u3[(size_t)s]) ? 1 : 0;
   }
 }
 

@@ -19,6 +19,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -84,7 +85,30 @@ bool verify_cls5(u64 B, u64 u, u64 d, u64 e, u64 a, u64 b);
 // ---------------------------------------------------------------------------
 struct BucLine { int cls; u64 B, u; };
 // Lines "cls B u"; non-matching lines are skipped; cls_mask bit c keeps class c.
+// A path ending in .bbj is read as a binary file of unit records (below).
 bool read_buc(const std::string& path, unsigned cls_mask, std::vector<BucLine>& out);
+
+// Binary job/unit files in the .bbj layout (fourcore_job_bin.hpp:22-96): a
+// 32-byte header {magic 'FCJB', version 2, n_jobs, rec_bytes 36, min_B, max_B}
+// and 36-byte records {cls, pad[3], B, u, free1, free2, T_lo, T_hi}.  A unit
+// record has free1 = free2 = 0 and T = 0; an expanded job carries its free
+// terms and exact T.
+bool is_bbj_path(const std::string& path);
+class BbjWriter {
+ public:
+  bool open(const std::string& path);
+  bool add(int cls, u64 B, u64 u, u64 free1, u64 free2, u128 T);
+  bool close();  // patches the header with the count and B range
+  ~BbjWriter();
+
+ private:
+  std::FILE* f_ = nullptr;
+  u64 n_ = 0;
+  u32 min_B_ = 0xffffffffu, max_B_ = 0;
+};
+// unit records of a .bbj file; records with free terms (expanded jobs) are
+// refused with a message
+bool read_units_bbj(const std::string& path, unsigned cls_mask, std::vector<BucLine>& out, std::string& msg);
 
 #pragma pack(push, 1)
 struct StoreHeader {
@@ -181,6 +205,11 @@ struct Cls234Unit {
   bool regular = false;
 };
 bool pack_cls234(int cls, u64 B, u64 u, Cls234Unit& U);
+// Packs a unit list in parallel (OpenMP).  bu holds (B, u) pairs; cls may be
+// null (all class 5).  slot[i] indexes u5 (class 5) or u3 (classes 2-4), -1
+// for other classes; ok5/ok3 say whether the pack succeeded.
+void pack_units(const u32* cls, const u64* bu, size_t n, std::vector<Cls5Unit>& u5, std::vector<char>& ok5,
+                std::vector<Cls234Unit>& u3, std::vector<char>& ok3, std::vector<int>& slot);
 u32 iroot6(u128 n);  // largest a with a^6 <= n
 // the find3 window of a job (make_cand mode 3, fourcore_find_v4.cu:412-418):
 // hi = min(iroot6(T), lim), lo = least c >= 1 with 3 c^6 >= T; empty if lo > hi

@@ -519,11 +519,16 @@ extern "C" int fc5_serve_submit(fc5_ctx* ctx, const uint32_t* cls, const fc5_uni
   B->n_in = n;
   B->cls_of.assign(n, 0);
   B->slot_of.assign(n, -1);
+  std::vector<Cls5Unit> P5;
+  std::vector<Cls234Unit> P3;
+  std::vector<char> ok5, ok3;
+  std::vector<int> slot;
+  pack_units(cls, reinterpret_cast<const u64*>(units), n, P5, ok5, P3, ok3, slot);
   for (size_t i = 0; i < n; ++i) {
     const int c = cls ? (int)cls[i] : 5;
     if (c == 5) {
-      Cls5Unit U;
-      if (!pack_cls5(units[i].B, units[i].u, U)) { ++B->n_skipped; continue; }
+      if (!ok5[(size_t)slot[i]]) { ++B->n_skipped; continue; }
+      const Cls5Unit& U = P5[(size_t)slot[i]];
       if (!U.regular) FAIL(ctx, FC5_E_ARG, "unit " + std::to_string(i) + " is not a Stage-1 class-5 unit");
       if (U.fmax2 > S->max_fmax2) FAIL(ctx, FC5_E_ARG, "band needs a pair table with a larger N");
       B->cls_of[i] = 5;
@@ -531,8 +536,8 @@ extern "C" int fc5_serve_submit(fc5_ctx* ctx, const uint32_t* cls, const fc5_uni
       B->dev5.push_back(dev_unit5(U, (u32)i, 5));
       B->units5.push_back(U);
     } else if (c >= 2 && c <= 4) {
-      Cls234Unit U;
-      if (!pack_cls234(c, units[i].B, units[i].u, U)) { ++B->n_skipped; continue; }
+      if (!ok3[(size_t)slot[i]]) { ++B->n_skipped; continue; }
+      const Cls234Unit& U = P3[(size_t)slot[i]];
       if (!U.regular) FAIL(ctx, FC5_E_ARG, "unit " + std::to_string(i) + " is not a Stage-1 class 2-4 unit");
       if (U.lim > S->max_lim) FAIL(ctx, FC5_E_ARG, "band needs a pair table with a larger N");
       B->cls_of[i] = (u8)c;

@@ -472,3 +472,20 @@ def brute_pairs(T, hi):
             out.append((a, b))
         b -= 1
     return sorted(out)
+
+
+def test_cli_cls5_units_bbj_equals_buc(tmp_path):
+    """the cls5 CLI reads Stage-1 units from a .bbj file as from a .buc file"""
+    import subprocess
+    from conftest import BIN
+    say("cls5 CLI .bbj vs .buc")
+    outs = []
+    for ext in ("buc", "bbj"):
+        p = str(tmp_path / f"u.{ext}")
+        subprocess.run([os.path.join(BIN, "fourcore_hunt_v3"), "--lo", "100001", "--hi", "101000", "--classes", "5",
+                        "--emit-units", p], check=True, capture_output=True)
+        r = subprocess.run([os.path.join(BIN, "fourcore_cls5_gpu_v4"), "--units", p, "--N", "2428", "--r", "16",
+                            "--quiet", "--counters"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr[-1000:]
+        outs.append([l for l in r.stdout.splitlines() if l.startswith("COUNTERS") or l.startswith("----")])
+    assert outs[0] == outs[1] and outs[0]

@@ -120,3 +120,31 @@ def test_cli_617_argument_refusals():
         r = _run("solve_617_v1", *args, check=False)
         assert r.returncode == 1, args
     assert "usage:" in _run("solve_617_v1", check=False).stderr
+
+
+def _bbj(path):
+    import struct
+    b = open(path, "rb").read()
+    magic, version, n, rec, lo, hi, pad = struct.unpack("<IIQIIII", b[:32])
+    assert (magic, version, rec) == (0x4A424346, 2, 36) and len(b) == 32 + 36 * n
+    recs = [struct.unpack("<B3xIIIIQQ", b[32 + 36 * i:68 + 36 * i]) for i in range(n)]
+    return (n, lo, hi), recs
+
+
+def test_cli_hunt_bbj_units_and_jobs(tmp_path):
+    """.bbj outputs (fourcore_job_bin.hpp layout) carry the same units and jobs
+    as the text outputs"""
+    p = str(tmp_path / "u.bbj")
+    _run("fourcore_hunt_v3", "--lo", 100001, "--hi", 102000, "--classes", "2,3,4,5", "--emit-units", p)
+    (n, lo, hi), recs = _bbj(p)
+    lines = [f"{c} {B} {u}" for c, B, u, f1, f2, tl, th in recs]
+    assert lines == open(os.path.join(GOLDEN, "units_cfg1.txt")).read().split("\n")[:-1]
+    assert all(f1 == f2 == tl == th == 0 for _, _, _, f1, f2, tl, th in recs)
+    assert (lo, hi) == (min(r[1] for r in recs), max(r[1] for r in recs))
+    tb, tt = str(tmp_path / "j.bbj"), str(tmp_path / "j.but")
+    _run("fourcore_hunt_v3", "--lo", 100001, "--hi", 100300, "--classes", "4,5", "--expand", "--emit", tb)
+    _run("fourcore_hunt_v3", "--lo", 100001, "--hi", 100300, "--classes", "4,5", "--expand", "--emit", tt)
+    _, jobs = _bbj(tb)
+    text = [tuple(map(int, l.split())) for l in open(tt).read().split("\n") if l]
+    assert [(c, B, u, f1, f2, tl, th) for c, B, u, f1, f2, tl, th in jobs] == text
+    assert len(text) > 1000
