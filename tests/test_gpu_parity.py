@@ -476,6 +476,7 @@ def brute_pairs(T, hi):
 
 def test_cli_cls5_units_bbj_equals_buc(tmp_path):
     """the cls5 CLI reads Stage-1 units from a .bbj file as from a .buc file"""
+    import re
     import subprocess
     from conftest import BIN
     say("cls5 CLI .bbj vs .buc")
@@ -487,5 +488,6 @@ def test_cli_cls5_units_bbj_equals_buc(tmp_path):
         r = subprocess.run([os.path.join(BIN, "fourcore_cls5_gpu_v4"), "--units", p, "--N", "2428", "--r", "16",
                             "--quiet", "--counters"], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr[-1000:]
-        outs.append([l for l in r.stdout.splitlines() if l.startswith("COUNTERS") or l.startswith("----")])
+        lines = [l for l in r.stdout.splitlines() if l.startswith("COUNTERS") or l.startswith("----")]
+        outs.append([re.sub(r"(kernel_ms|time)=[0-9.]+s?", "", l) for l in lines])
     assert outs[0] == outs[1] and outs[0]
