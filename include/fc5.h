@@ -289,6 +289,10 @@ int fc5_serve_class_result(fc5_ctx *ctx, uint64_t band_id, int cls, fc5_result *
 int fc5_serve_hits(fc5_ctx *ctx, uint64_t band_id, fc5_hit *out, size_t cap, size_t *n);
 int fc5_serve_solutions(fc5_ctx *ctx, uint64_t band_id, fc5_solution *out, size_t cap, size_t *n);
 int fc5_serve_timing_of(fc5_ctx *ctx, uint64_t band_id, fc5_serve_timing *out);
+/* submit the Stage-1 units of classes cls_mask (bits 2..5) for B in [lo, hi],
+ * emitted on the device (on a stream of their own, beside the server's rounds) */
+int fc5_serve_submit_range(fc5_ctx *ctx, uint64_t lo, uint64_t hi, unsigned cls_mask, double lo_frac,
+                           double hi_frac, uint64_t *band_id);
 /* forget a finished band (its results are freed) */
 int fc5_serve_release(fc5_ctx *ctx, uint64_t band_id);
 int fc5_serve_stats_get(fc5_ctx *ctx, fc5_serve_stats *out);
@@ -309,6 +313,16 @@ int fc5_measure_copy(fc5_ctx *ctx, uint64_t bytes, double *ms);
  * Writes up to cap triples; *n gets the full count. */
 int fc5_emit_units(uint64_t lo, uint64_t hi, unsigned cls_mask, double lo_frac, double hi_frac,
                    uint32_t *out_cls, uint64_t *out_B, uint64_t *out_u, size_t cap, size_t *n);
+/* ---- Stage-1 on the device (no reference counterpart: the reference emits
+ *      units on the host, fourcore_hunt_v3.cpp:371-381) ---- */
+/* the same units, in the same order, as fc5_emit_units for classes 2-5
+ * (cls_mask bits 2..5; class 1 stays on the host) */
+int fc5_emit_units_device(fc5_ctx *ctx, uint64_t lo, uint64_t hi, unsigned cls_mask, double lo_frac, double hi_frac,
+                          uint32_t *out_cls, uint64_t *out_B, uint64_t *out_u, size_t cap, size_t *n);
+/* a band straight from a B range: Stage-1 on the device, then fc5_band_create_mixed */
+int fc5_band_create_range(fc5_ctx *ctx, uint64_t lo, uint64_t hi, unsigned cls_mask, double lo_frac, double hi_frac,
+                          fc5_band **out);
+/* ---- host helpers, continued ---- */
 /* host-only xor pair-table build + save (xor_build_save.cpp:21-57) */
 int fc5_build_pairs_file(int N, int r, const char *path, fc5_store_header *hdr_out);
 

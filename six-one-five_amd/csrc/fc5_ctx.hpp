@@ -166,6 +166,10 @@ DevUnit3 dev_unit3(const Cls234Unit& U, u32 uid, u32 band);
 void number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3);
 // serving loop (fc5_serve.cpp): stops and frees the server of a context
 void serve_shutdown(fc5_ctx* ctx);
+// Stage-1 units of classes in mask (bits 2..5) for B in [lo, hi] on the device,
+// in the host emitter's order, on stream s (fc5_stage1_api.cpp)
+int stage1_emit(fc5_ctx* ctx, hipStream_t s, u64 lo, u64 hi, unsigned mask, double lo_frac, double hi_frac,
+                std::vector<u32>& cls, std::vector<fc5_unit>& units);
 // exact check of filter survivors on the device (k_pair_search): pairs[i] gets
 // every a <= b <= targets[i].hi with a^6 + b^6 = targets[i].T, sorted
 int device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,

@@ -163,6 +163,21 @@ struct PairHit {
   u32 job, a, b, pad;
 };
 
+// Stage-1 emission on the device (fc5_stage1.hip), classes 2-5
+struct Stage1Args {
+  u64 lo, count;  // B = lo .. lo + count - 1
+  double lo_frac, hi_frac;
+  u32 cls_mask, pad;
+  const u64* z2;  // the 4 odd sixth roots of unity mod 2^6
+  const u64* z3;  // the 6 mod 3^6
+  const u64* z7;  // the 6 mod 7^6
+  u64 e2_14, e7_14, e3_21, e7_21;  // CRT coefficients for 14^6 = 2^6 7^6 and 21^6 = 3^6 7^6
+  u64* counts;           // pass 0: units per B
+  const u64* offsets;    // pass 1: exclusive prefix of counts
+  u32* out_cls;
+  u64 *out_B, *out_u;
+};
+
 constexpr int kRowsPerItem = 8;  // default rows (e- or d-rows) per work item of the batched kernel
 
 // launchers (return hipError_t as int)
@@ -177,6 +192,9 @@ int launch_probe_only(const XorDev& x, u64 n, u64 salt, unsigned long long* sink
 int launch_gather_raw(const u64* cells, u64 m, u64 n, u64 salt, u32 policy, unsigned long long* sink, int grid,
                       hipStream_t s);
 int launch_copy_u64(const u64* src, u64* dst, u64 n, int grid, hipStream_t s);
+int launch_stage1(const Stage1Args& a, int pass, hipStream_t s);
+// exclusive prefix sum of n u64 (temp == nullptr: query temp_bytes)
+int stage1_scan(const u64* counts_in, u64* offsets_out, u64 n, void* temp, size_t& temp_bytes, hipStream_t s);
 int launch_pair_search(const PairJob* jobs, u32 n, PairHit* out, u32 cap, u32* count, hipStream_t s);
 
 }  // namespace fc5

@@ -132,6 +132,8 @@ struct fc5_server {
   HitRec* h_hits = nullptr;   // pinned
   u64* h_nhits = nullptr;     // pinned
   u32 max_fmax2 = 0, max_lim = 0;
+  hipStream_t side = nullptr;  // Stage-1 emission of submitted ranges
+  std::mutex side_mu;
   u64 submitted = 0, admitted = 0, done = 0, rounds = 0, device_bytes = 0;
   double busy_ms = 0;
   Clock::time_point t0 = Clock::now();
@@ -441,6 +443,7 @@ void fc5::serve_shutdown(fc5_ctx* ctx) {
   if (S->th.joinable()) S->th.join();
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  if (S->side) (void)hipStreamDestroy(S->side);
   free_server_buffers(S);
   delete S;
   ctx->server = nullptr;
@@ -499,7 +502,8 @@ extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) {
     FAIL(ctx, FC5_E_NOMEM, "serving buffers");
   }
   if (launch_fill_u64(reinterpret_cast<u64*>(S->d_ctr), nslots * sizeof(Counters) / 8, 0, ctx->stream) != 0 ||
-      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      hipStreamSynchronize(ctx->stream) != hipSuccess ||
+      hipStreamCreateWithFlags(&S->side, hipStreamNonBlocking) != hipSuccess) {
     free_server_buffers(S.get());
     FAIL(ctx, FC5_E_HIP, "serving counter reset");
   }
@@ -569,6 +573,21 @@ extern "C" int fc5_serve_submit(fc5_ctx* ctx, const uint32_t* cls, const fc5_uni
   S->submitted++;
   S->cv_work.notify_all();
   return FC5_OK;
+}
+
+extern "C" int fc5_serve_submit_range(fc5_ctx* ctx, uint64_t lo, uint64_t hi, unsigned cls_mask, double lo_frac,
+                                      double hi_frac, uint64_t* band_id) {
+  if (!ctx || !band_id) return FC5_E_ARG;
+  fc5_server* S = ctx->server;
+  if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
+  std::vector<u32> cls;
+  std::vector<fc5_unit> units;
+  {
+    std::lock_guard<std::mutex> lk(S->side_mu);
+    const int rc = stage1_emit(ctx, S->side, lo, hi, cls_mask, lo_frac, hi_frac, cls, units);
+    if (rc != FC5_OK) return rc;
+  }
+  return fc5_serve_submit(ctx, cls.data(), units.data(), units.size(), band_id);
 }
 
 static int band_result(fc5_ctx* ctx, SBand* B, fc5_result* res) {

@@ -143,6 +143,12 @@ _sig = {
     "fc5_serve_stats_get": (C.c_int, [_P, C.POINTER(ServeStats)]),
     "fc5_serve_stop": (C.c_int, [_P]),
     "fc5_pair_search": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fc5_emit_units_device": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double, _P, _P, _P,
+                                        C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fc5_band_create_range": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double,
+                                        C.POINTER(_P)]),
+    "fc5_serve_submit_range": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double,
+                                         C.POINTER(C.c_uint64)]),
     "fc5_twin_cands": (C.c_int, [C.c_uint64, C.c_double, C.c_double, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_twin_run": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(TwinResult)]),
     "fc5_twin_run_range": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.POINTER(TwinResult)]),
@@ -246,6 +252,16 @@ class Band:
             rc = _lib.fc5_band_create_mixed(ctx.h, _ptr(self.cls), _ptr(self.units), len(self.units),
                                             C.byref(self.h))
         ctx._check(rc)
+
+    @classmethod
+    def from_range(cls_, ctx, lo, hi, classes=(5,), u_band=(0.0, 1.0)):
+        """a band straight from a B range (Stage-1 on the device)"""
+        b = cls_.__new__(cls_)
+        b.ctx = ctx
+        b.h = _P()
+        mask = sum(1 << c for c in classes)
+        ctx._check(_lib.fc5_band_create_range(ctx.h, lo, hi, mask, u_band[0], u_band[1], C.byref(b.h)))
+        return b
 
     def run(self, mode=MODE_BATCHED, allow_overflow=False):
         res = Result()
@@ -406,6 +422,26 @@ class Context:
         bid = C.c_uint64()
         self._check(_lib.fc5_serve_submit(self.h, _ptr(cl), _ptr(units), len(units), C.byref(bid)))
         return bid.value
+
+    def serve_submit_range(self, lo, hi, classes=(4, 5), u_band=(0.0, 1.0)):
+        mask = sum(1 << c for c in classes)
+        bid = C.c_uint64()
+        self._check(_lib.fc5_serve_submit_range(self.h, lo, hi, mask, u_band[0], u_band[1], C.byref(bid)))
+        return bid.value
+
+    def emit_units_device(self, lo, hi, classes=(5,), u_band=(0.0, 1.0)):
+        """Stage-1 on the device: (cls, B, u) arrays, as emit_units"""
+        mask = sum(1 << c for c in classes)
+        n = C.c_size_t()
+        self._check(_lib.fc5_emit_units_device(self.h, lo, hi, mask, u_band[0], u_band[1], None, None, None, 0,
+                                               C.byref(n)))
+        cl = np.zeros(n.value, np.uint32)
+        Bs = np.zeros(n.value, np.uint64)
+        us = np.zeros(n.value, np.uint64)
+        if n.value:
+            self._check(_lib.fc5_emit_units_device(self.h, lo, hi, mask, u_band[0], u_band[1], _ptr(cl), _ptr(Bs),
+                                                   _ptr(us), n.value, C.byref(n)))
+        return cl, Bs, us
 
     def serve_poll(self, band_id):
         """True when the band is finished"""

@@ -491,3 +491,33 @@ def test_cli_cls5_units_bbj_equals_buc(tmp_path):
         lines = [l for l in r.stdout.splitlines() if l.startswith("COUNTERS") or l.startswith("----")]
         outs.append([re.sub(r"(kernel_ms|time)=[0-9.]+s?", "", l) for l in lines])
     assert outs[0] == outs[1] and outs[0]
+
+
+@pytest.mark.parametrize("lo,hi,classes,band", [
+    (100001, 102000, (2, 3, 4, 5), (0.0, 1.0)),
+    (1, 3000, (2, 3, 4, 5), (0.0, 1.0)),
+    (1000001, 1004000, (2, 3, 4, 5), (0.0, 1.0)),
+    (2000001, 2002000, (4, 5), (0.2, 0.9)),
+    (3500001, 3503000, (2, 3, 4, 5), (0.5, 1.0)),
+    (9999001, 10000000, (2, 3, 5), (0.0, 1.0)),
+])
+def test_stage1_on_device_matches_host(ctx, lo, hi, classes, band):
+    """Stage-1 emission on the device: the host emitter's units in its order"""
+    say(f"device Stage-1 [{lo},{hi}] classes {classes} band {band}")
+    dc, dB, du = ctx.emit_units_device(lo, hi, classes, band)
+    hc, hB, hu = fc5.emit_units(lo, hi, classes=classes, u_band=band)
+    assert len(hc) > 0
+    assert np.array_equal(dc, hc) and np.array_equal(dB, hB) and np.array_equal(du, hu)
+
+
+def test_band_from_range_equals_band_from_units(ctx):
+    say("band from a B range")
+    _load(ctx, 24285, 16)
+    cls, Bs, us = fc5.emit_units(1000001, 1002000, classes=(4, 5))
+    a = ctx.run(fc5.units_array(Bs, us), fc5.MODE_BATCHED, cls=cls)
+    b = fc5.Band.from_range(ctx, 1000001, 1002000, classes=(4, 5))
+    out = b.run(fc5.MODE_BATCHED)
+    b.close()
+    for k in COUNTERS:
+        assert out.result[k] == a.result[k], k
+    assert np.array_equal(out.hits, a.hits) and out.result["units"] == a.result["units"]

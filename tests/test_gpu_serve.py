@@ -107,6 +107,25 @@ def test_serving_admission_queues_by_arena_space(ctx):
         ctx.serve_stop()
 
 
+def test_serving_ranges_emitted_on_device(ctx):
+    """fc5_serve_submit_range: Stage-1 on a side stream while the server runs"""
+    say("serving: submitted B ranges")
+    ranges = [(1000001, 1001500), (1100001, 1100900), (1150001, 1152500), (1199001, 1200000)]
+    alone = []
+    for lo, hi in ranges:
+        c, Bs, us = fc5.emit_units(lo, hi, classes=(4, 5))
+        alone.append(ctx.run(fc5.units_array(Bs, us), fc5.MODE_BATCHED, cls=c))
+    ctx.serve_start(max_bands=3, round_points=1e9)
+    try:
+        ids = [ctx.serve_submit_range(lo, hi, classes=(4, 5)) for lo, hi in ranges]
+        for i in ids:
+            assert ctx.serve_wait(i, 120.0)
+        for i, a in zip(ids, alone):
+            same(a, ctx.serve_result(i))
+    finally:
+        ctx.serve_stop()
+
+
 def test_serving_refusals(ctx):
     with pytest.raises(fc5.Fc5Error) as e:
         ctx.serve_submit(fc5.units_array([1000003], [6]))
