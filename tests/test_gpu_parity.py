@@ -512,7 +512,7 @@ def test_stage1_on_device_matches_host(ctx, lo, hi, classes, band):
 
 def test_band_from_range_equals_band_from_units(ctx):
     say("band from a B range")
-    _load(ctx, 24285, 16)
+    ctx.table_build(24285, 16, on_gpu=True)
     cls, Bs, us = fc5.emit_units(1000001, 1002000, classes=(4, 5))
     a = ctx.run(fc5.units_array(Bs, us), fc5.MODE_BATCHED, cls=cls)
     b = fc5.Band.from_range(ctx, 1000001, 1002000, classes=(4, 5))
