@@ -45,6 +45,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """Model name and logical CPU count of this host (reported beside the CPU baseline)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
 def eligible_count(lo, hi):
     return sum(1 for B in range(lo, hi + 1) if B % 2 and B % 3 and B % 7)
 
@@ -331,7 +345,7 @@ def bench_twin(args):
         gres, _, _ = ctx.twin_run(sub)
         for k in ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "maybe"):
             assert gres[k] == oc[k], k
-        cpu = {"value": oc["calls"] / dt, "unit": "probe calls/s", "cores": threads, "kind": "port",
+        cpu = {"value": oc["calls"] / dt, "unit": "probe calls/s", "cores": threads, "kind": "port", "host": host_cpu(),
                "sample": f"every {args.twin_cpu_every}th candidate ({len(sub)} candidates, {oc['calls']} calls, "
                          f"{dt:.1f} s), same table bytes; counters identical to the GPU on that sample"}
         log(f"[bench] cpu oracle {oc['calls'] / dt:.4e} calls/s on {len(sub)} candidates, {threads} threads")
@@ -531,7 +545,7 @@ def cpu_baseline(ctx, units, ucls, every, cfg, r, pkeys):
         c = dict(c)
         c["calls"] += c3["calls"]
     log(f"[bench] cpu oracle {c['calls'] / dt:.4e} points/s on {len(sub)} units, {threads} threads, {dt:.1f}s")
-    return {"value": c["calls"] / dt, "unit": "(d,e) grid points/s", "cores": threads, "kind": "port",
+    return {"value": c["calls"] / dt, "unit": "(d,e) grid points/s", "cores": threads, "kind": "port", "host": host_cpu(),
             "sample": f"every {every}th unit of the band ({len(sub)} units, {c['calls']} grid points, "
                       f"{dt:.1f} s), same table bytes; counters identical to the GPU on that sample"}
 
