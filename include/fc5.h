@@ -78,7 +78,8 @@ typedef struct {
 } fc5_result;
 
 /* a filter hit (fourcore_cls5_gpu_v4.cu:140).  unit indexes the band's units
- * as passed in; flags bit0 = main table maybe, bit1 = side (planted) table maybe.
+ * as passed in; flags bit0 = main table maybe, bit1 = side (planted) table maybe,
+ * bit2 = FC5_HIT_EXTRA (exact check found the target among the exact table's extra values).
  * Class 5: d, e are the free terms.  Classes 2-4: d is the free term and e holds
  * the third core c3 (Hit.c of fourcore_find_v4.cu:99). */
 typedef struct {
@@ -112,7 +113,8 @@ const char *fc5_last_error(const fc5_ctx *ctx);
  * "load_policy" (-1..3, default -1: cache policy of the xor cell gathers:
  * -1 by table size (plain loads up to 1 GiB of cells, non-temporal above),
  * 0 default, 1 nontemporal, 2 device-scope, 3 system-scope), "verify" (1: exact
- * check of hits: device pair search + host identity check),
+ * check of hits: device pair search + host identity check), "verify_table" (0;
+ * 1 answers the exact check from the exact sorted pair table below),
  * "twin_*" (the (6,1,7) twin options below; "twin_plain" 1 runs its
  * reference-shaped kernel),
  * "stop_first" (0, fourcore_cls5_gpu_v4.cu:635) */
@@ -192,6 +194,31 @@ int fc5_units_calls(const fc5_unit *units, size_t n, uint64_t *calls);
 int fc5_pair_search(fc5_ctx *ctx, const uint64_t *targets, const uint32_t *hi, size_t n, uint32_t *out,
                     size_t cap, size_t *n_out);
 
+/* ---- exact sorted pair table (the sorted pair index QuadSumIndex,
+ *      quad_sum.hpp:8-45: build = every pair sum sorted, equal_sum_range = the
+ *      pairs of one sum), resident on the device ----
+ * Entries: every a^6 + b^6 with 1 <= a <= b <= N plus n_extra extra 128-bit
+ * values (extra[2i] low, extra[2i+1] high word; planted exact targets), sorted
+ * ascending.  Each entry is one 128-bit word value * 2^s + a, s = bit length of
+ * N, a = 0 for an extra value; it must fit 128 bits (N <= 262143 when no extra
+ * value exceeds 2 N^6).  16 bytes per entry: N = 76428 -> 2.92e9 entries, 46.7 GB
+ * (the build holds twice that while it sorts).  With the option "verify_table"
+ * = 1 the exact check of every run's maybe-hits looks each target up here
+ * instead of running the pair search (same pairs); a hit whose target is an
+ * extra value gets FC5_HIT_EXTRA in its flags.  The table's N must be at least
+ * the pair bound of every hit (the main table's N). */
+#define FC5_HIT_EXTRA 4u
+int fc5_exact_build(fc5_ctx *ctx, int N, const uint64_t *extra, size_t n_extra);
+int fc5_exact_clear(fc5_ctx *ctx);
+/* entries, N and shift s of the exact table (all 0 when there is none) */
+int fc5_exact_info(const fc5_ctx *ctx, uint64_t *n_entries, int *N, uint32_t *shift);
+/* copy entries [first, first + n) out as (low, high) word pairs */
+int fc5_exact_read(fc5_ctx *ctx, uint64_t first, size_t n, uint64_t *out);
+/* equal range of each target T_i: (i, a, b) triples for the pairs with b <= hi[i],
+ * (i, 0, 0) for an extra value equal to T_i; layout and cap as fc5_pair_search */
+int fc5_exact_lookup(fc5_ctx *ctx, const uint64_t *targets, const uint32_t *hi, size_t n, uint32_t *out,
+                     size_t cap, size_t *n_out);
+
 /* ---- (6,1,7) Branch-A class-5 twin (solve_617_v1.cu) ----
  * B^6 = u^6 + w^6 + (21d)^6 + 42^6 (c1^6 + c2^6 + c3^6 + c4^6) with u = 14u'
  * (the term not divisible by 3), w = 6v' (the term not divisible by 7) and the
@@ -258,7 +285,8 @@ int fc5_twin_last_solutions(fc5_ctx *ctx, fc5_twin_solution *out, size_t cap, si
  *      band's counters, hits and solutions equal those of fc5_band_run on the
  *      same units.  While the server runs, only fc5_serve_* calls (and
  *      fc5_last_error) may be used on the context; others return FC5_E_STATE.
- *      The pair table (and side table) must be loaded before fc5_serve_start;
+ *      The pair table (and side table; with "verify_table" the exact table) must
+ *      be loaded before fc5_serve_start;
  *      bands whose B exceed what the table's N covers are refused. ---- */
 typedef struct {
   int max_bands;          /* bands admitted at once (counter slots), 1..64; default 16 */

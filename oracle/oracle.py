@@ -347,3 +347,39 @@ def twin_verify(cand, d, c4, c3, fp, N, cap=16):
     n = _lib.or_twin_verify(cand.ctypes.data_as(_P), d, c4, c3, fp, N, terms.ctypes.data_as(_P),
                             g.ctypes.data_as(_P), cap)
     return [(terms[7 * i:7 * i + 7].tolist(), int(g[i])) for i in range(min(n, cap))]
+
+
+# ---------------------------------------------------------------------------
+# exact sorted pair table (QuadSumIndex, quad_sum.hpp:8-45), Python integers
+# ---------------------------------------------------------------------------
+def exact_table(N, extra=()):
+    """Every a^6 + b^6 with 1 <= a <= b <= N sorted by sum (QuadSumIndex::build,
+    quad_sum.hpp:15-30), in the entry form value * 2^s + a (s = bit length of N;
+    a = 0 for an extra value).  Returns (entries, s).  Small N only."""
+    s = N.bit_length()
+    p = [a ** 6 for a in range(N + 1)]
+    ent = [((p[a] + p[b]) << s) | a for a in range(1, N + 1) for b in range(a, N + 1)]
+    ent += [int(v) << s for v in extra]
+    ent.sort()
+    return ent, s
+
+
+def exact_decode(entry, s):
+    """(value, a, b) of one entry; b = 0 for an extra value"""
+    v, a = entry >> s, entry & ((1 << s) - 1)
+    if not a:
+        return v, 0, 0
+    r = v - a ** 6
+    b = round(r ** (1.0 / 6.0))
+    while b ** 6 > r:
+        b -= 1
+    while (b + 1) ** 6 <= r:
+        b += 1
+    assert b ** 6 == r
+    return v, a, b
+
+
+def exact_range(entries, s, T):
+    """equal_sum_range (quad_sum.hpp:37-45) over the entry form: [lo, hi)"""
+    import bisect
+    return bisect.bisect_left(entries, T << s), bisect.bisect_left(entries, (T + 1) << s)

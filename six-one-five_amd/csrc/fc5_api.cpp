@@ -92,6 +92,7 @@ extern "C" void fc5_ctx_destroy(fc5_ctx* ctx) {
   if (ctx->h_ctr) (void)hipHostFree(ctx->h_ctr);
   if (ctx->h_nhits) (void)hipHostFree(ctx->h_nhits);
   free_twin(ctx);
+  free_exact(ctx);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   (void)hipStreamDestroy(ctx->stream);
@@ -111,6 +112,8 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) {
     ctx->use_gate = v ? 1 : 0;
   } else if (k == "verify") {
     ctx->verify = v ? 1 : 0;
+  } else if (k == "verify_table") {
+    ctx->verify_table = v ? 1 : 0;
   } else if (k == "stop_first") {
     ctx->stop_first = v ? 1 : 0;
   } else if (k == "load_policy") {
@@ -477,16 +480,9 @@ DevUnit3 fc5::dev_unit3(const Cls234Unit& U, u32 uid, u32 band) {
   return D;
 }
 
-int fc5::device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
-                      std::vector<std::vector<std::pair<u32, u32>>>& pairs) {
-  pairs.assign(targets.size(), {});
-  std::vector<PairJob> jobs;
-  std::vector<size_t> which;
-  for (size_t i = 0; i < targets.size(); ++i)
-    if (targets[i].ok) {
-      jobs.push_back(PairJob{(u64)targets[i].T, (u64)(targets[i].T >> 64), targets[i].hi, 0});
-      which.push_back(i);
-    }
+int fc5::run_pair_jobs(fc5_ctx* ctx, const std::vector<PairJob>& jobs, const PairLaunch& launch,
+                       std::vector<PairHit>& out) {
+  out.clear();
   if (jobs.empty()) return FC5_OK;
   if (jobs.size() > 0xffffffffull) FAIL(ctx, FC5_E_ARG, "too many hits to verify");
   u32 cap = (u32)std::min<size_t>(std::max<size_t>(1024, 4 * jobs.size()), 1u << 30);
@@ -509,7 +505,7 @@ int fc5::device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
     u64 count = 0;
     if (hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice) != hipSuccess ||
         launch_fill_u64(d_count, 1, 0, ctx->stream) != 0 ||
-        launch_pair_search(d_jobs, (u32)jobs.size(), d_out, cap, reinterpret_cast<u32*>(d_count), ctx->stream) != 0 ||
+        launch(d_jobs, (u32)jobs.size(), d_out, cap, reinterpret_cast<u32*>(d_count), ctx->stream) != 0 ||
         hipMemcpyAsync(&count, d_count, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
       release();
@@ -520,16 +516,34 @@ int fc5::device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
       cap = (u32)count;
       continue;
     }
-    std::vector<PairHit> out((size_t)count);
+    out.resize((size_t)count);
     const hipError_t e =
         count ? hipMemcpy(out.data(), d_out, count * sizeof(PairHit), hipMemcpyDeviceToHost) : hipSuccess;
     release();
     if (e != hipSuccess) FAIL(ctx, FC5_E_HIP, "pair search read-back");
-    for (const PairHit& h : out) pairs[which[h.job]].emplace_back(h.a, h.b);
-    for (auto& v : pairs) std::sort(v.begin(), v.end());
     return FC5_OK;
   }
   FAIL(ctx, FC5_E_HIP, "pair search overflowed twice");
+}
+
+int fc5::device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
+                      std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra) {
+  if (ctx->verify_table) return exact_table_pairs(ctx, targets, pairs, extra);
+  if (extra) extra->assign(targets.size(), 0);
+  pairs.assign(targets.size(), {});
+  std::vector<PairJob> jobs;
+  std::vector<size_t> which;
+  for (size_t i = 0; i < targets.size(); ++i)
+    if (targets[i].ok) {
+      jobs.push_back(PairJob{(u64)targets[i].T, (u64)(targets[i].T >> 64), targets[i].hi, 0});
+      which.push_back(i);
+    }
+  std::vector<PairHit> out;
+  const int rc = run_pair_jobs(ctx, jobs, launch_pair_search, out);
+  if (rc != FC5_OK) return rc;
+  for (const PairHit& h : out) pairs[which[h.job]].emplace_back(h.a, h.b);
+  for (auto& v : pairs) std::sort(v.begin(), v.end());
+  return FC5_OK;
 }
 
 extern "C" int fc5_pair_search(fc5_ctx* ctx, const uint64_t* targets, const uint32_t* hi, size_t n, uint32_t* out,
@@ -1006,7 +1020,9 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   if (ctx->verify && !ctx->hits.empty()) {
     const int N = ctx->main.hdr.N ? (int)ctx->main.hdr.N : (int)std::max(b->max_lim, b->max_lim3);
     std::vector<VerifyJob> jobs;
-    for (const fc5_hit& h : ctx->hits) {
+    std::vector<size_t> hit_of;
+    for (size_t i = 0; i < ctx->hits.size(); ++i) {
+      const fc5_hit& h = ctx->hits[i];
       if (!(h.flags & 1u) || h.unit >= b->n_in) continue;
       const int k = b->cls_of[h.unit], slot = b->slot_of[h.unit];
       if (k == 5) {
@@ -1015,13 +1031,19 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
       } else if (k >= 2) {
         const Cls234Unit& U = b->units3[(size_t)slot];
         jobs.push_back(VerifyJob{k, U.B, U.u, U.lim, h.d, h.e, h.fp});
+      } else {
+        continue;
       }
+      hit_of.push_back(i);
     }
     std::vector<PairTarget> targets;
     verify_targets(N, jobs, targets);
     std::vector<std::vector<std::pair<u32, u32>>> pairs;
-    rc = device_pairs(ctx, targets, pairs);
+    std::vector<u8> extra;
+    rc = device_pairs(ctx, targets, pairs, &extra);
     if (rc != FC5_OK) return rc;
+    for (size_t i = 0; i < extra.size(); ++i)
+      if (extra[i]) ctx->hits[hit_of[i]].flags |= FC5_HIT_EXTRA;
     std::vector<Solution> sols;
     verify_finish_all(jobs, pairs, ctx->stop_first != 0, sols);
     for (const Solution& S : sols) {

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -128,6 +129,12 @@ struct fc5_ctx {
   size_t d_twin_cap = 0;
   std::vector<fc5_twin_hit> twin_hits;
   std::vector<fc5_twin_solution> twin_sols;
+  // exact sorted pair table (fc5_exact_api.cpp): n_exact entries value * 2^exact_shift + a
+  u128* d_exact = nullptr;
+  u64 n_exact = 0;
+  int exact_N = 0;
+  u32 exact_shift = 0;
+  int verify_table = 0;  // option: answer the exact check from d_exact
   // resident serving loop (fc5_serve.cpp); while set, only fc5_serve_* may run
   struct fc5_server* server = nullptr;
 };
@@ -170,10 +177,21 @@ void serve_shutdown(fc5_ctx* ctx);
 // in the host emitter's order, on stream s (fc5_stage1_api.cpp)
 int stage1_emit(fc5_ctx* ctx, hipStream_t s, u64 lo, u64 hi, unsigned mask, double lo_frac, double hi_frac,
                 std::vector<u32>& cls, std::vector<fc5_unit>& units);
-// exact check of filter survivors on the device (k_pair_search): pairs[i] gets
-// every a <= b <= targets[i].hi with a^6 + b^6 = targets[i].T, sorted
+// exact check of filter survivors on the device: pairs[i] gets every
+// a <= b <= targets[i].hi with a^6 + b^6 = targets[i].T, sorted -- by the pair
+// search (k_pair_search), or from the exact sorted table when the option
+// verify_table is set, which also marks in extra (if given) the targets equal
+// to one of its extra values
 int device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
-                 std::vector<std::vector<std::pair<u32, u32>>>& pairs);
+                 std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra = nullptr);
+// pair jobs on the device through launch (the pair search or the exact-table
+// lookup), into an output buffer that grows once when the first pass overflows
+using PairLaunch = std::function<int(const PairJob*, u32, PairHit*, u32, u32*, hipStream_t)>;
+int run_pair_jobs(fc5_ctx* ctx, const std::vector<PairJob>& jobs, const PairLaunch& launch, std::vector<PairHit>& out);
+// the exact-table half of device_pairs (fc5_exact_api.cpp)
+int exact_table_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
+                      std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra);
+void free_exact(fc5_ctx* ctx);
 // twin buffers (fc5_twin_api.cpp)
 void free_twin(fc5_ctx* ctx);
 }  // namespace fc5

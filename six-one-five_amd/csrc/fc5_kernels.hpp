@@ -24,6 +24,7 @@ using u8 = std::uint8_t;
 using u16 = std::uint16_t;
 using u32 = std::uint32_t;
 using u64 = std::uint64_t;
+using u128 = unsigned __int128;
 
 
 struct XorDev {
@@ -196,5 +197,13 @@ int launch_stage1(const Stage1Args& a, int pass, hipStream_t s);
 // exclusive prefix sum of n u64 (temp == nullptr: query temp_bytes)
 int stage1_scan(const u64* counts_in, u64* offsets_out, u64 n, void* temp, size_t& temp_bytes, hipStream_t s);
 int launch_pair_search(const PairJob* jobs, u32 n, PairHit* out, u32 cap, u32* count, hipStream_t s);
+// exact sorted pair table (fc5_exact.hip): rows of entries (a^6 + b^6) * 2^s + a
+// for 1 <= a <= b <= N; radix sort of n entries on bits [0, end_bit) (temp ==
+// nullptr: query temp_bytes); per-target equal ranges (PairHit.pad = 1: extra value)
+int launch_exact_rows(u128* keys, u32 N, u32 s, hipStream_t st);
+int exact_sort(u128* keys, u128* alt, u64 n, u32 end_bit, void* temp, size_t& temp_bytes, hipStream_t st,
+               bool& sorted_in_alt);
+int launch_exact_lookup(const u128* keys, u64 n, u32 s, const PairJob* jobs, u32 nj, PairHit* out, u32 cap,
+                        u32* count, hipStream_t st);
 
 }  // namespace fc5

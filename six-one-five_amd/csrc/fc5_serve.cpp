@@ -200,7 +200,9 @@ void finalize(fc5_server* S, SBand& B) {
   if (ctx->verify && !B.hits.empty()) {
     const int N = (int)ctx->main.hdr.N;
     std::vector<VerifyJob> jobs;
-    for (const fc5_hit& h : B.hits) {
+    std::vector<size_t> hit_of;
+    for (size_t i = 0; i < B.hits.size(); ++i) {
+      const fc5_hit& h = B.hits[i];
       if (!(h.flags & 1u) || h.unit >= B.n_in) continue;
       const int k = B.cls_of[h.unit], sl = B.slot_of[h.unit];
       if (k == 5) {
@@ -209,14 +211,24 @@ void finalize(fc5_server* S, SBand& B) {
       } else if (k >= 2) {
         const Cls234Unit& U = B.units3[(size_t)sl];
         jobs.push_back(VerifyJob{k, U.B, U.u, U.lim, h.d, h.e, h.fp});
+      } else {
+        continue;
       }
+      hit_of.push_back(i);
     }
     std::vector<PairTarget> targets;
     verify_targets(N, jobs, targets);
     std::vector<std::vector<std::pair<u32, u32>>> pairs;
+    std::vector<u8> extra;
     std::vector<Solution> sols;
-    if (device_pairs(ctx, targets, pairs) == FC5_OK) verify_finish_all(jobs, pairs, false, sols);
-    else B.rc = FC5_E_HIP;
+    const int prc = device_pairs(ctx, targets, pairs, &extra);
+    if (prc == FC5_OK) {
+      verify_finish_all(jobs, pairs, false, sols);
+      for (size_t i = 0; i < extra.size(); ++i)
+        if (extra[i]) B.hits[hit_of[i]].flags |= FC5_HIT_EXTRA;
+    } else {
+      B.rc = prc;
+    }
     for (const Solution& x : sols) {
       fc5_solution o{};
       o.B = x.B;
@@ -453,6 +465,8 @@ extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) {
   if (!ctx) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (!ctx->main.present || !ctx->main.hdr.N) FAIL(ctx, FC5_E_STATE, "serving needs a pair table with its N");
+  if (ctx->verify && ctx->verify_table && (!ctx->d_exact || ctx->exact_N < (int)ctx->main.hdr.N))
+    FAIL(ctx, FC5_E_STATE, "verify_table needs an exact table with N at least the pair table's N");
   fc5_serve_opts o{16, ctx->rows_per_item, 1ull << 23, 1ull << 22, 2e10, 288000000000ull};
   if (opts) {
     if (opts->max_bands) o.max_bands = opts->max_bands;
@@ -473,7 +487,7 @@ extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) {
   if (rc == FC5_OK) rc = ensure_c6g(ctx, N + 64);
   if (rc == FC5_OK) rc = ensure_hits(ctx);
   if (rc != FC5_OK) return rc;
-  const u64 table_bytes = ctx->main.hdr.m_cells * 8 + ctx->side.hdr.m_cells * 8;
+  const u64 table_bytes = ctx->main.hdr.m_cells * 8 + ctx->side.hdr.m_cells * 8 + ctx->n_exact * sizeof(u128);
   const u64 arena_bytes = o.arena_units5 * sizeof(DevUnit5) + o.arena_units3 * sizeof(DevUnit3);
   const u64 run_bytes = ctx->hit_cap * sizeof(HitRec) + (u64)o.max_bands * kSlotStride * sizeof(Counters);
   if (table_bytes + arena_bytes + run_bytes > o.mem_budget)

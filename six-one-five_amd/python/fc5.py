@@ -143,6 +143,11 @@ _sig = {
     "fc5_serve_stats_get": (C.c_int, [_P, C.POINTER(ServeStats)]),
     "fc5_serve_stop": (C.c_int, [_P]),
     "fc5_pair_search": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fc5_exact_build": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
+    "fc5_exact_clear": (C.c_int, [_P]),
+    "fc5_exact_info": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_uint32)]),
+    "fc5_exact_read": (C.c_int, [_P, C.c_uint64, C.c_size_t, _P]),
+    "fc5_exact_lookup": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_emit_units_device": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double, _P, _P, _P,
                                         C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_band_create_range": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double,
@@ -236,6 +241,7 @@ class RunOutput:
 
 
 HIT_DTYPE = np.dtype([("unit", "<u4"), ("d", "<u4"), ("e", "<u4"), ("flags", "<u4"), ("fp", "<u8")])
+HIT_EXTRA = 4  # hit flag: the exact check found the target among the exact table's extra values
 
 
 class Band:
@@ -407,6 +413,45 @@ class Context:
         out = np.zeros(3 * max(1, n.value), np.uint32)
         self._check(_lib.fc5_pair_search(self.h, _ptr(t), _ptr(h), len(targets), out.ctypes.data_as(_P), n.value,
                                          C.byref(n)))
+        return [tuple(int(x) for x in out[3 * k:3 * k + 3]) for k in range(n.value)]
+
+    # ---- exact sorted pair table ----
+    @staticmethod
+    def _words(values):
+        t = np.zeros(2 * max(1, len(values)), np.uint64)
+        for i, v in enumerate(values):
+            t[2 * i], t[2 * i + 1] = int(v) & _M64, int(v) >> 64
+        return t
+
+    def exact_build(self, N, extra=()):
+        """every a^6 + b^6 (1 <= a <= b <= N) and the extra 128-bit values, sorted on the device"""
+        t = self._words(list(extra))
+        self._check(_lib.fc5_exact_build(self.h, N, _ptr(t) if len(extra) else None, len(extra)))
+
+    def exact_clear(self):
+        self._check(_lib.fc5_exact_clear(self.h))
+
+    def exact_info(self):
+        """(entries, N, shift s): entry = value * 2^s + a, a = 0 for an extra value"""
+        n, N, s = C.c_uint64(), C.c_int(), C.c_uint32()
+        self._check(_lib.fc5_exact_info(self.h, C.byref(n), C.byref(N), C.byref(s)))
+        return n.value, N.value, s.value
+
+    def exact_read(self, first, n):
+        """entries [first, first + n) as Python ints"""
+        out = np.zeros(2 * max(1, n), np.uint64)
+        self._check(_lib.fc5_exact_read(self.h, first, n, out.ctypes.data_as(_P)))
+        return [int(out[2 * i]) | (int(out[2 * i + 1]) << 64) for i in range(n)]
+
+    def exact_lookup(self, targets, hi):
+        """equal ranges: list of (i, a, b) with b <= hi[i], (i, 0, 0) for an extra value"""
+        t = self._words(list(targets))
+        h = np.ascontiguousarray(hi, dtype=np.uint32)
+        n = C.c_size_t()
+        self._check(_lib.fc5_exact_lookup(self.h, _ptr(t), _ptr(h), len(targets), None, 0, C.byref(n)))
+        out = np.zeros(3 * max(1, n.value), np.uint32)
+        self._check(_lib.fc5_exact_lookup(self.h, _ptr(t), _ptr(h), len(targets), out.ctypes.data_as(_P), n.value,
+                                          C.byref(n)))
         return [tuple(int(x) for x in out[3 * k:3 * k + 3]) for k in range(n.value)]
 
     # ---- resident serving loop ----
@@ -587,12 +632,13 @@ def _pair_sum_set(m):
 _S504, _S247 = _pair_sum_set(504), _pair_sum_set(247)
 
 
-def plant_points(Bs, us, n, seed):
+def plant_points(Bs, us, n, seed, exact=False):
     """n distinct planted grid points (unit index, e, d, key) drawn with splitmix64:
     the unit uniformly over the band, then (e, d) uniformly over the unit's
     valid values, redrawn while the residual R is not positive or T = R/42^6
     fails the sixth-power residue gate (a real solution always passes it: T is
-    a pair sum), at most 2000 draws per unit; key = T mod 2^64."""
+    a pair sum), at most 2000 draws per unit; key = T mod 2^64.  exact=True
+    appends T itself (unit index, e, d, key, T) for the exact table."""
     s = seed & _M64
     out, seen = [], set()
     while len(out) < n:
@@ -617,5 +663,5 @@ def plant_points(Bs, us, n, seed):
         if (i, e, d) in seen:
             continue
         seen.add((i, e, d))
-        out.append((i, e, d, T & _M64))
+        out.append((i, e, d, T & _M64, T) if exact else (i, e, d, T & _M64))
     return out

@@ -13,6 +13,10 @@ Outputs (data only: inputs and expected outputs):
     known_617_cls5.json the class-5 rows of the reference's list of known
                         (6,1,7) solutions (617-solutions-clean.txt), with the
                         role decomposition of each
+    exact_table.json    the reference's sorted pair index (QuadSumIndex) for
+                        small N: entry count, SHA-256 of its "sum i j" lines,
+                        the lines in full for N = 40, and equal_sum_range of
+                        sampled targets
 """
 import hashlib
 import json
@@ -98,6 +102,19 @@ def main():
             rec["lines"] = lines
         twin["cands"].append(rec)
     json.dump(twin, open(os.path.join(HERE, "twin_cands.json"), "w"), indent=0)
+
+    exact = []
+    for N in (40, 300, 1000):
+        lines = [l for l in run("ref_quad", "table", N).split("\n") if l]
+        rec = {"N": N, "count": len(lines), "sha256": hashlib.sha256("\n".join(lines).encode()).hexdigest()}
+        if N == 40:
+            rec["lines"] = lines
+        if N == 300:
+            sums = [int(l.split()[0]) for l in lines]
+            targets = [sums[0], sums[-1]] + rnd.sample(sums, 24) + [x + 1 for x in rnd.sample(sums, 8)] + [1, 3]
+            rec["ranges"] = [list(map(int, l.split())) for l in run("ref_quad", "range", N, *targets).split("\n") if l]
+        exact.append(rec)
+    json.dump(exact, open(os.path.join(HERE, "exact_table.json"), "w"), indent=0)
     print("golden fixtures written")
 
 
