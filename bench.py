@@ -33,7 +33,8 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     1: dict(lo=100001, hi=102000, N=2428, plants=0, plant_mode=None, seed=0x615 << 8 | 1, classes=(5,)),
-    2: dict(lo=1000001, hi=1020000, N=24285, plants=64, plant_mode="main", seed=0x615 << 8 | 2, classes=(5,)),
+    2: dict(lo=1000001, hi=1020000, N=24285, plants=64, plant_mode="main", seed=0x615 << 8 | 2, classes=(5,),
+            exact=True),
     3: dict(lo=2000001, hi=2010000, N=47857, plants=0, plant_mode=None, seed=0x615 << 8 | 3,
             classes=(2, 3, 4, 5)),
     4: dict(lo=3200001, hi=3210000, N=76428, plants=256, plant_mode="side", seed=0x615 << 8 | 4, classes=(5,)),
@@ -79,6 +80,9 @@ def main():
     ap.add_argument("--twin-cpu-every", type=int, default=128, help="617: CPU oracle on every k-th candidate")
     ap.add_argument("--twin-plain-every", type=int, default=4, help="617: plain form on every k-th candidate")
     ap.add_argument("--r", type=int, default=48)
+    ap.add_argument("--exact-table", choices=["auto", "on", "off"], default="auto",
+                    help="exact sorted pair table beside the xor filter, planted targets in both, hits "
+                         "checked by lookup (auto: on for config 2)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--plain-every", type=int, default=64, help="plain form on every k-th unit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -113,8 +117,9 @@ def main():
         ctx.set_option("blocks_per_cu", args.blocks_per_cu)
     if args.load_policy is not None:
         ctx.set_option("load_policy", args.load_policy)
-    plants = fc5.plant_points(Bs, us, cfg["plants"], cfg["seed"]) if cfg["plants"] else []
-    pkeys = np.array([k for *_, k in plants], np.uint64)
+    use_exact = args.exact_table == "on" or (args.exact_table == "auto" and cfg.get("exact", False))
+    plants = fc5.plant_points(Bs, us, cfg["plants"], cfg["seed"], exact=True) if cfg["plants"] else []
+    pkeys = np.array([p[3] for p in plants], np.uint64)
     t0 = time.time()
     ctx.table_build(cfg["N"], args.r, extra=pkeys if cfg["plant_mode"] == "main" else None, on_gpu=True)
     hdr = ctx.table_info()
@@ -126,8 +131,27 @@ def main():
         raise SystemExit(f"FPR smoke {fpr} > 16: refuse")
     if cfg["plant_mode"] == "side":
         ctx.side_build(pkeys, args.r)
+    exact = None
+    if use_exact:
+        t1 = time.time()
+        ctx.exact_build(cfg["N"], [p[4] for p in plants] if cfg["plant_mode"] == "main" else [])
+        n_ex, _, s_ex = ctx.exact_info()
+        exact = {"entries": n_ex, "GB": n_ex * 16 / 1e9, "build_s": time.time() - t1,
+                 "verify": "maybe-hits looked up in the exact table (equal range)"}
+        # spot checks: random pair sums are found, and windows across the table are strictly increasing
+        rng = np.random.default_rng(cfg["seed"])
+        ab = [tuple(sorted(map(int, rng.integers(1, cfg["N"] + 1, 2)))) for _ in range(256)]
+        got = ctx.exact_lookup([x ** 6 + y ** 6 for x, y in ab], [cfg["N"]] * len(ab))
+        assert sorted(got) == sorted((i, x, y) for i, (x, y) in enumerate(ab)), "exact table lookup"
+        wl = min(1024, n_ex)
+        for off in np.linspace(0, n_ex - wl, 16).astype(np.int64):
+            w = ctx.exact_read(int(off), wl)
+            assert all(w[i] < w[i + 1] for i in range(wl - 1)), f"exact table order at {off}"
+        exact["spot_checks"] = f"256 random pair sums found; 16 windows of {wl} entries strictly increasing"
+        ctx.set_option("verify_table", 1)
+        log(f"[bench] exact sorted pair table: {n_ex} entries, {exact['GB']:.2f} GB, built in {exact['build_s']:.1f}s")
     band = ctx.band(units, ucls if mixed else None)
-    plant_set = {(i, e, d) for i, e, d, _ in plants}
+    plant_set = {(p[0], p[1], p[2]) for p in plants}
     flag = 2 if cfg["plant_mode"] == "side" else 1
 
     def check(out):
@@ -139,6 +163,10 @@ def main():
         found = {(int(h["unit"]), int(h["e"]), int(h["d"])) for h in out.hits if h["flags"] & flag}
         missing = plant_set - found
         assert not missing, f"{len(missing)} planted keys not recovered"
+        if use_exact and cfg["plant_mode"] == "main":  # and confirmed by the exact table
+            exact_found = {(int(h["unit"]), int(h["e"]), int(h["d"])) for h in out.hits
+                           if h["flags"] & fc5.HIT_EXTRA}
+            assert exact_found == plant_set, "planted targets not confirmed by the exact table"
         return len(plant_set & found)
 
     tune = None
@@ -176,6 +204,7 @@ def main():
         f"probes={r['probes']} maybe={r['maybe']} side_maybe={r['side_maybe']}")
 
     ctx.set_option("use_graph", 0)
+    ctx.set_option("verify_table", 0)
     plain = None
     if not args.no_plain:
         sub = units[:: args.plain_every]
@@ -257,6 +286,7 @@ def main():
         "plain_form": plain,
         "cpu_baseline": cpu,
         "table_build_s": t_build,
+        "exact_table": exact,
     }
     print(json.dumps(line), flush=True)
     band.close()
