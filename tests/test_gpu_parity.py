@@ -521,3 +521,57 @@ def test_band_from_range_equals_band_from_units(ctx):
     for k in COUNTERS:
         assert out.result[k] == a.result[k], k
     assert np.array_equal(out.hits, a.hits) and out.result["units"] == a.result["units"]
+
+
+def _ragged_units(seed):
+    """.buc lines beyond the Stage-1 list, as the reference reads them
+    (fourcore_cls5_gpu_v4.cu:155-184): real units; (B, u) with an even u that is
+    no seed lift (the residual is not a multiple of 42^6, so these run on the
+    per-point 192-bit path), before and after the 2^128 wall; and lines that the
+    reference's packing refuses (u >= B, odd u, u = 0, ...)"""
+    import random
+
+    rnd = random.Random(seed)
+    Bs, us = [], []
+    for lo in (100001, 1000001, 3200001):
+        _, B1, u1 = fc5.emit_units(lo, lo + 60)
+        Bs += [int(b) for b in B1[:4]]
+        us += [int(u) for u in u1[:4]]
+        for b in rnd.sample(sorted({int(x) for x in B1}), 3):
+            for _ in range(3):
+                Bs.append(b)
+                us.append(2 * rnd.randint(b // 4, (b - 1) // 2))
+    Bs += [100003, 100003, 100003, 100003, 1000003, 1000003]
+    us += [100003, 100005, 7, 0, 999, 1000002]
+    order = list(range(len(Bs)))
+    rnd.shuffle(order)
+    return [Bs[i] for i in order], [us[i] for i in order]
+
+
+@pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN], ids=["batched", "plain"])
+def test_ragged_and_irregular_units_match_oracle(ctx, mode):
+    say(f"ragged unit list (regular, irregular, unpackable), mode={mode}")
+    Bs, us = _ragged_units(17)
+    arr, ok = O.pack_units(Bs, us)
+    assert 0 < ok.sum() < len(Bs)
+    t = _load(ctx, 2428, 16)
+    out = ctx.run(fc5.units_array(Bs, us), mode)
+    assert out.result["units_skipped"] == int((~ok).sum())
+    assert out.result["units_irregular"] > 0
+    keep = np.flatnonzero(ok)
+    sub = (O.Unit5 * len(keep))(*[arr[int(i)] for i in keep])
+    oc, oh = O.peel(sub, len(keep), t, threads=min(16, os.cpu_count() or 8))[:2]
+    oh = oh.copy()
+    oh["unit"] = keep[oh["unit"]].astype(np.uint32)
+    oh = np.sort(oh, order=["unit", "e", "d", "flags"])
+    _compare(out, oc, oh)
+    assert len(oh) > 0
+
+
+@pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN], ids=["batched", "plain"])
+def test_empty_band(ctx, mode):
+    say(f"empty band, mode={mode}")
+    _load(ctx, 300, 48)
+    out = ctx.run(fc5.units_array([], []), mode)
+    assert all(out.result[k] == 0 for k in COUNTERS + ("n_hits", "units", "units_skipped"))
+    assert len(out.hits) == 0 and out.solutions == []
