@@ -306,7 +306,10 @@ typedef struct {
   double busy_ms;  /* summed event time of the rounds */
 } fc5_serve_stats;
 int fc5_serve_start(fc5_ctx *ctx, const fc5_serve_opts *opts); /* opts may be NULL */
-/* copy in and pack a band; returns its id.  cls may be NULL (all class 5) */
+/* copy in and pack a band; returns its id.  cls may be NULL (all class 5).
+ * Lines of classes 2-5 must be Stage-1 units (a line that only the per-point
+ * paths of fc5_band_run can take is refused with FC5_E_ARG); lines of other
+ * classes and lines the packing refuses are skipped and counted */
 int fc5_serve_submit(fc5_ctx *ctx, const uint32_t *cls, const fc5_unit *units, size_t n, uint64_t *band_id);
 /* FC5_OK and the band's result when done, FC5_E_PENDING otherwise; a finished
  * band whose hit buffer overflowed returns FC5_E_OVERFLOW with the result */

@@ -736,9 +736,7 @@ bool pack_cls234(int cls, u64 B, u64 u, Cls234Unit& U) {
   U.B = B;
   U.u = u;
   U.lim = (u32)((B - 1) / 42);
-  if (cls < 2 || cls > 4 || u >= B || B >= (1ULL << 32)) return false;
-  U.base = U256::pow6(B);
-  if (sub_in_place(U.base, U256::pow6(u)) || U.base.is_zero()) return false;
+  if (cls < 2 || cls > 4 || B >= (1ULL << 32)) return false;
   const FreeSpec s = free_d_cls234((i64)B, (i64)u, cls);
   if (s.mod <= 0 || s.fmax < 1 || s.res.empty() || s.res.size() > 64) return false;
   U.f = (u32)free_factor(cls);
@@ -746,6 +744,13 @@ bool pack_cls234(int cls, u64 B, u64 u, Cls234Unit& U) {
   U.fmax = (u32)s.fmax;
   U.nres = (u32)s.res.size();
   for (u32 i = 0; i < U.nres; ++i) U.res[i] = (u32)s.res[i];
+  // B^6 - u^6 <= 0: the reference still walks the free terms and counts every
+  // one as failT (fourcore_find_v4.cu:179-185); the per-term path does the same
+  U.base = U256::pow6(B);
+  if (u >= B || sub_in_place(U.base, U256::pow6(u)) || U.base.is_zero()) {
+    U.base = U256{};
+    return true;
+  }
   U.shift = (u32)__builtin_ctz(U.mod);
   U.inv = inv_odd_2_128((u128)(U.mod >> U.shift));
   U.dr = (u32)pow6_mod((i64)U.res[0], (i64)U.mod);

@@ -575,3 +575,42 @@ def test_empty_band(ctx, mode):
     out = ctx.run(fc5.units_array([], []), mode)
     assert all(out.result[k] == 0 for k in COUNTERS + ("n_hits", "units", "units_skipped"))
     assert len(out.hits) == 0 and out.solutions == []
+
+
+@pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN], ids=["batched", "plain"])
+def test_mixed_ragged_lines_match_oracle(ctx, mode):
+    """.buc lines of every class in one band: Stage-1 units of classes 2-5;
+    class 2-4 lines with other u (most free terms give a residual that is not a
+    multiple of 42^6 and count as failT, fourcore_find_v4.cu:179-185); lines with
+    u >= B (every free term fails); classes 1 and 6 (skipped)"""
+    import random
+
+    say(f"mixed ragged lines mode={mode}")
+    rnd = random.Random(23)
+    cls, Bs, us = fc5.emit_units(100001, 100600, classes=(2, 3, 4, 5))
+    cls, Bs, us = [int(x) for x in cls], [int(x) for x in Bs], [int(x) for x in us]
+    n_stage1 = len(cls)
+    for _ in range(30):
+        i = rnd.randrange(n_stage1)
+        cls.append(rnd.choice((2, 3, 4)))
+        Bs.append(Bs[i])
+        us.append(rnd.randrange(1, Bs[i]))
+    for k in (2, 3, 4):
+        cls += [k, k]
+        Bs += [100003, 100003]
+        us += [100003 + k, 100003]
+    cls += [1, 6, 1]
+    Bs += [100003, 100003, 1000003]
+    us += [6, 6, 12]
+    order = list(range(len(cls)))
+    rnd.shuffle(order)
+    cls = np.array([cls[i] for i in order], np.int64)
+    Bs = np.array([Bs[i] for i in order], np.int64)
+    us = np.array([us[i] for i in order], np.int64)
+    t = _load(ctx, 2428, 16)
+    out = ctx.run(fc5.units_array(Bs, us), mode, cls=cls)
+    oc, oh = _oracle_mixed(cls, Bs, us, t)
+    _compare_mixed(out, oc, oh)
+    assert out.result["units_skipped"] >= 3
+    assert out.result["units_irregular"] > 0
+    assert sum(oc[k]["fail_t"] for k in (2, 3, 4)) > 0 and len(oh) > 0
