@@ -87,6 +87,7 @@ def main():
     ap.add_argument("--plain-every", type=int, default=64, help="plain form on every k-th unit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-every", type=int, default=400, help="CPU oracle on every k-th unit")
+    ap.add_argument("--cpu-offset", type=int, default=0, help="first unit of the CPU oracle sample")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-measure", action="store_true", help="skip the gather/copy microbenchmarks")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape")
@@ -252,7 +253,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(ctx, units, ucls, args.cpu_every, cfg, r, pkeys)
+        cpu = cpu_baseline(ctx, units, ucls, args.cpu_every, cfg, r, pkeys, args.cpu_offset)
 
     line = {
         "metric": METRIC, "value": value, "unit": "(d,e) grid points/s", "n_gpus": 1,
@@ -544,27 +545,49 @@ def fc5_packed_gb(h):
     return (h.m_cells * h.r + 7) // 8 / 1e9
 
 
-def cpu_baseline(ctx, units, ucls, every, cfg, r, pkeys):
+def cpu_baseline(ctx, units, ucls, every, cfg, r, pkeys, offset=0):
     """The oracle (CPU restatement of the reference kernels) timed on every
-    k-th unit of the same band, over the same table bytes, on this host."""
+    k-th unit of the same band, over the same table bytes, on this host; its
+    counters and its maybe-hit list are checked equal to the GPU's on that sample."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
 
     hdr, packed = ctx.table_download()
     t = O.Table.from_bytes({f: getattr(hdr, f) for f, _ in fc5.StoreHeader._fields_}, packed)
     side = O.Table.build_keys(pkeys, hdr.r) if cfg["plant_mode"] == "side" and len(pkeys) else None
-    sub, scls = units[::every], ucls[::every]
+    sub, scls = units[offset::every], ucls[offset::every]
     threads = min(16, os.cpu_count() or 1)
-    s5 = sub[scls == 5]
-    arr, ok = O.pack_units(s5["B"], s5["u"])
-    t0 = time.perf_counter()
-    c, _, _ = O.peel(arr, len(s5), t, side=side, threads=threads)
+    i5 = np.flatnonzero(scls == 5)
+    arr, ok = O.pack_units(sub["B"][i5], sub["u"][i5])
+    assert ok.all()
+    c, hits = {}, []
+    t0 = t_log = time.perf_counter()
+    chunk = 2048
+    for lo in range(0, len(i5), chunk):  # in chunks, so that a long sample reports progress
+        n = min(chunk, len(i5) - lo)
+        cc, hh, _ = O.peel((O.Unit5 * n)(*arr[lo:lo + n]), n, t, side=side, threads=threads)
+        for k, v in cc.items():
+            c[k] = c.get(k, 0) + v
+        hh = hh.copy()
+        hh["unit"] = i5[lo + hh["unit"].astype(np.int64)]
+        hits.append(hh)
+        if time.perf_counter() - t_log > 20:
+            t_log = time.perf_counter()
+            log(f"[bench]   cpu oracle: {lo + n}/{len(i5)} class-5 units, {t_log - t0:.0f}s")
     c3 = None
-    if (scls != 5).any():
-        m = scls != 5
-        c3, _, _ = O.find3(scls[m], sub["B"][m], sub["u"][m], t, threads=threads)
+    m3 = np.flatnonzero(scls != 5)
+    for lo in range(0, len(m3), chunk):
+        m = m3[lo:lo + chunk]
+        cc, h3, _ = O.find3(scls[m], sub["B"][m], sub["u"][m], t, threads=threads)
+        c3 = cc if c3 is None else {k: c3[k] + v for k, v in cc.items()}
+        h3 = h3.copy()
+        h3["unit"] = m[h3["unit"].astype(np.int64)]
+        hits.append(h3)
+        if time.perf_counter() - t_log > 20:
+            t_log = time.perf_counter()
+            log(f"[bench]   cpu oracle: {lo + len(m)}/{len(m3)} class 2-4 units, {t_log - t0:.0f}s")
     dt = time.perf_counter() - t0
-    # the same subset on the GPU, for a like-for-like counter check
+    # the same subset on the GPU, for a like-for-like check
     out = ctx.run(sub, cls=scls)
     for k in ("calls", "neg", "zero", "gated", "probes", "maybe"):
         assert out.by_class[5][k] == c[k], (k, out.by_class[5][k], c[k])
@@ -572,12 +595,17 @@ def cpu_baseline(ctx, units, ucls, every, cfg, r, pkeys):
         for k, ok_ in (("calls", "calls"), ("gated", "gated"), ("probes", "probes"), ("jobs", "jobs"),
                        ("fail_t", "failT")):
             assert sum(out.by_class[cl][k] for cl in (2, 3, 4)) == c3[ok_], k
-        c = dict(c)
         c["calls"] += c3["calls"]
-    log(f"[bench] cpu oracle {c['calls'] / dt:.4e} points/s on {len(sub)} units, {threads} threads, {dt:.1f}s")
+    oh = np.sort(np.concatenate(hits) if hits else np.zeros(0, O.HIT_DTYPE), order=["unit", "e", "d", "flags"])
+    gh = np.sort(out.hits, order=["unit", "e", "d", "flags"])
+    assert len(gh) == len(oh) and all(np.array_equal(gh[f], oh[f]) for f in ("unit", "d", "e", "flags", "fp")), \
+        "maybe-hit lists differ"
+    log(f"[bench] cpu oracle {c['calls'] / dt:.4e} points/s on {len(sub)} units, {threads} threads, {dt:.1f}s; "
+        f"counters and {len(oh)} maybe-hits identical to the GPU")
     return {"value": c["calls"] / dt, "unit": "(d,e) grid points/s", "cores": threads, "kind": "port", "host": host_cpu(),
-            "sample": f"every {every}th unit of the band ({len(sub)} units, {c['calls']} grid points, "
-                      f"{dt:.1f} s), same table bytes; counters identical to the GPU on that sample"}
+            "sample": f"every {every}th unit of the band from unit {offset} ({len(sub)} units, {c['calls']} grid points, "
+                      f"{dt:.1f} s), same table bytes; counters and the {len(oh)}-entry maybe-hit list "
+                      f"identical to the GPU on that sample"}
 
 
 if __name__ == "__main__":
