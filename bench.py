@@ -603,7 +603,8 @@ def cpu_baseline(ctx, units, ucls, every, cfg, r, pkeys, offset=0):
     log(f"[bench] cpu oracle {c['calls'] / dt:.4e} points/s on {len(sub)} units, {threads} threads, {dt:.1f}s; "
         f"counters and {len(oh)} maybe-hits identical to the GPU")
     return {"value": c["calls"] / dt, "unit": "(d,e) grid points/s", "cores": threads, "kind": "port", "host": host_cpu(),
-            "sample": f"every {every}th unit of the band from unit {offset} ({len(sub)} units, {c['calls']} grid points, "
+            "sample": ("all units of the band" if every == 1 else f"every {every}-th unit of the band from unit {offset}")
+                      + f" ({len(sub)} units, {c['calls']} grid points, "
                       f"{dt:.1f} s), same table bytes; counters and the {len(oh)}-entry maybe-hit list "
                       f"identical to the GPU on that sample"}
 
