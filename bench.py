@@ -370,16 +370,31 @@ def bench_twin(args):
         t = O.Table.from_bytes({f: getattr(h, f) for f, _ in fc5.StoreHeader._fields_}, packed)
         sub = cands[:: args.twin_cpu_every]
         threads = min(16, os.cpu_count() or 1)
-        t1 = time.perf_counter()
-        oc, _, _ = O.twin_run(sub.view(O.TWIN_CAND_DTYPE), t, threads=threads)
+        oc, ohits = {}, []
+        t1 = t_log = time.perf_counter()
+        for k0 in range(0, len(sub), 32):  # in chunks, so that a long sample reports progress
+            cc, hh, _ = O.twin_run(sub[k0:k0 + 32].view(O.TWIN_CAND_DTYPE), t, threads=threads)
+            oc = {k: oc.get(k, 0) + v for k, v in cc.items()}
+            hh = hh.copy()
+            hh["cand"] += k0
+            ohits.append(hh)
+            if time.perf_counter() - t_log > 20:
+                t_log = time.perf_counter()
+                log(f"[bench]   cpu oracle: {min(k0 + 32, len(sub))}/{len(sub)} candidates, {t_log - t1:.0f}s")
         dt = time.perf_counter() - t1
-        gres, _, _ = ctx.twin_run(sub)
+        gres, ghits, _ = ctx.twin_run(sub)
         for k in ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "maybe"):
             assert gres[k] == oc[k], k
+        ohits = np.sort(np.concatenate(ohits), order=["cand", "d", "c4", "c3"])
+        ghits = np.sort(ghits, order=["cand", "d", "c4", "c3"])
+        assert len(ghits) == len(ohits) and all(np.array_equal(ghits[f], ohits[f])
+                                                for f in ("cand", "d", "c4", "c3", "fp")), "maybe-hit lists differ"
+        what = "all candidates" if args.twin_cpu_every == 1 else f"every {args.twin_cpu_every}-th candidate"
         cpu = {"value": oc["calls"] / dt, "unit": "probe calls/s", "cores": threads, "kind": "port", "host": host_cpu(),
-               "sample": f"every {args.twin_cpu_every}th candidate ({len(sub)} candidates, {oc['calls']} calls, "
-                         f"{dt:.1f} s), same table bytes; counters identical to the GPU on that sample"}
-        log(f"[bench] cpu oracle {oc['calls'] / dt:.4e} calls/s on {len(sub)} candidates, {threads} threads")
+               "sample": f"{what} ({len(sub)} candidates, {oc['calls']} calls, {dt:.1f} s), same table bytes; "
+                         f"counters and the {len(ohits)}-entry maybe-hit list identical to the GPU on that sample"}
+        log(f"[bench] cpu oracle {oc['calls'] / dt:.4e} calls/s on {len(sub)} candidates, {threads} threads; "
+            f"counters and {len(ohits)} maybe-hits identical to the GPU")
     line = {
         "metric": "617 cls5 twin probe calls/sec, 1 GPU, bit-exact", "value": calls_s,
         "unit": "probe calls/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
