@@ -96,8 +96,9 @@ def main():
                     help="cache policy of the cell gathers (library default: by table size)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every box of this build has one GPU and the band "
-              "shards by units with no exchange (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: a band shards over GPUs by units, each GPU holding its own "
+              "copy of the read-only pair table, with only counters and hit records gathered at the end; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.config == 617:
         return bench_twin(args)

@@ -7,7 +7,7 @@
  * Function -> reference lines it restates -> what pins it (tests/):
  *   class contracts  fourcore_classes_v3.hpp:37-232
  *       pinned by: the reference header compiled in oracle/_ref (units and
- *       residues for config 1 and sampled B, committed as golden vectors);
+ *       residues for B in [100001,102000] and sampled B, committed as golden vectors);
  *       seed counts 144/24/36/6/6 at B=100003 (fourcore_hunt_v3.cpp:49-67);
  *       published count units=5765343 on [4.5M,5.0M] (README.md:146);
  *       brute force of the residue definitions on small B.

@@ -5,7 +5,7 @@ Run in the build container only (the reference is not present elsewhere):
     python tests/golden/make_golden.py
 Outputs (data only: inputs and expected outputs):
     xor_tables.json     SHA-256 and header of reference xor_build_save tables
-    units_cfg1.txt      Stage-1 "cls B u" lines, classes 2-5, B in [100001,102000]
+    units_100001_102000.txt      Stage-1 "cls B u" lines, classes 2-5, B in [100001,102000]
     residues.json       cls5 / cls2-4 free-term residue specs at sampled units
     seeds.json          seed counts per class at sampled B
     twin_cands.json     (6,1,7) class-5 candidate lists (count, SHA-256 of the
@@ -52,7 +52,7 @@ def main():
                            "header_hex": hdr.hex()})
     json.dump(tables, open(os.path.join(HERE, "xor_tables.json"), "w"), indent=1)
 
-    open(os.path.join(HERE, "units_cfg1.txt"), "w").write(run("ref_classes", "units", 100001, 102000, "2345"))
+    open(os.path.join(HERE, "units_100001_102000.txt"), "w").write(run("ref_classes", "units", 100001, 102000, "2345"))
 
     rnd = random.Random(615)
     res = []

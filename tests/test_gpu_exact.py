@@ -137,7 +137,7 @@ def test_large_table_sorted_and_complete(ctx):
 
 
 def test_band_verify_table_same_results_and_plants_flagged(ctx):
-    """cfg1 band with 64 planted keys in the xor table and their exact targets
+    """B in [100001,102000]: 64 planted keys in the xor table and their exact targets
     as extra values: verify by lookup = verify by pair search, plus the flag"""
     say("band run: verify_table vs pair search, 64 plants")
     _, Bs, us = fc5.emit_units(100001, 102000)

@@ -65,8 +65,8 @@ def _compare(out, oc, ohits):
 
 @pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN], ids=["batched", "plain"])
 @pytest.mark.parametrize("r", [48, 16])
-def test_cfg1_full_matches_oracle(ctx, mode, r):
-    say(f"cfg1 full r={r} mode={mode}")
+def test_band_100001_102000_full_matches_oracle(ctx, mode, r):
+    say(f"B in [100001,102000] full r={r} mode={mode}")
     Bs, us = _units(100001, 102000)
     t = _load(ctx, 2428, r)
     out = ctx.run(fc5.units_array(Bs, us), mode)
@@ -78,8 +78,8 @@ def test_cfg1_full_matches_oracle(ctx, mode, r):
     assert out.solutions == []
 
 
-def test_cfg1_no_gate(ctx):
-    say("cfg1 no-gate")
+def test_band_100001_102000_no_gate(ctx):
+    say("B in [100001,102000] no-gate")
     Bs, us = _units(100001, 100600)
     t = _load(ctx, 2428, 16)
     ctx.set_option("use_gate", 0)
@@ -109,7 +109,7 @@ def test_sampled_units_match_oracle(ctx, lo, hi, every):
 
 
 def test_plants_in_main_table_recovered(ctx):
-    say("plants in main table, cfg1")
+    say("plants in main table, B in [100001,102000]")
     Bs, us = _units(100001, 102000)
     pts = fc5.plant_points(Bs, us, 64, 0x615 << 8 | 2)
     keys = np.array([k for *_, k in pts], np.uint64)
@@ -186,10 +186,10 @@ def test_table_load_errors(ctx, tmp_path):
     assert e.value.code == fc5.E_IO
 
 
-def test_cfg2_full_calls_and_64_plants(ctx):
+def test_band_1000001_1020000_full_calls_and_64_plants(ctx):
     """the full 1.0M band (4.99e4 units): closed-form calls, counters conserved,
     every planted key recovered, plain and batched agree"""
-    say("cfg2 full band")
+    say("B in [1000001,1020000] full band")
     Bs, us = _units(1000001, 1020000)
     units = fc5.units_array(Bs, us)
     pts = fc5.plant_points(Bs, us, 64, 0x615 << 8 | 2)
@@ -300,11 +300,11 @@ def _compare_mixed(out, oc, oh):
 
 
 @pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN], ids=["batched", "plain"])
-def test_mixed_classes_cfg1_match_oracle(ctx, mode):
+def test_mixed_classes_band_100001_102000_match_oracle(ctx, mode):
     """classes 2-5 of B in [100001,102000] in one band: cls2-4 run the find3
     leaf in the same persistent kernel as cls5 (batched) or the reference
     per-class launches (plain)"""
-    say(f"mixed classes cfg1 mode={mode}")
+    say(f"mixed classes B in [100001,102000] mode={mode}")
     cls, Bs, us = fc5.emit_units(100001, 102000, classes=(2, 3, 4, 5))
     t = _load(ctx, 2428, 16)
     out = ctx.run(fc5.units_array(Bs, us), mode, cls=cls)
@@ -313,9 +313,9 @@ def test_mixed_classes_cfg1_match_oracle(ctx, mode):
     assert sum(out.by_class[k]["calls"] for k in (2, 3, 4)) > 0 and len(oh) > 0
 
 
-def test_mixed_classes_cfg3_sample_match_oracle(ctx):
+def test_mixed_classes_band_2000001_2010000_sample_match_oracle(ctx):
     """a unit sample of the 2.0M band, all four classes, batched and plain"""
-    say("mixed classes cfg3 sample")
+    say("mixed classes B in [2000001,2010000] sample")
     cls, Bs, us = fc5.emit_units(2000001, 2010000, classes=(2, 3, 4, 5))
     sel = np.concatenate([np.nonzero(cls == k)[0][::m] for k, m in ((2, 60), (3, 60), (4, 400), (5, 400))])
     sel.sort()

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 def test_emit_units_matches_reference_golden():
     cls, Bs, us = fc5.emit_units(100001, 102000, classes=(2, 3, 4, 5))
     mine = "".join(f"{c} {b} {u}\n" for c, b, u in zip(cls, Bs, us))
-    assert mine == open(os.path.join(GOLDEN, "units_cfg1.txt")).read()
+    assert mine == open(os.path.join(GOLDEN, "units_100001_102000.txt")).read()
 
 
 def test_emit_units_matches_oracle_other_ranges():
@@ -91,7 +91,7 @@ def test_cli_hunt(tmp_path):
     assert "SELFTEST PASS" in _run("fourcore_hunt_v3", "--selftest").stdout
     p = str(tmp_path / "u.buc")
     _run("fourcore_hunt_v3", "--lo", 100001, "--hi", 102000, "--classes", "2,3,4,5", "--emit-units", p)
-    assert open(p).read() == open(os.path.join(GOLDEN, "units_cfg1.txt")).read()
+    assert open(p).read() == open(os.path.join(GOLDEN, "units_100001_102000.txt")).read()
     d = _run("fourcore_hunt_v3", "--density", "--lo", 100001, "--hi", 102000, "--classes", "5").stdout
     assert "cls5  units=615  expanded_jobs~10973276" in d
 
@@ -138,7 +138,7 @@ def test_cli_hunt_bbj_units_and_jobs(tmp_path):
     _run("fourcore_hunt_v3", "--lo", 100001, "--hi", 102000, "--classes", "2,3,4,5", "--emit-units", p)
     (n, lo, hi), recs = _bbj(p)
     lines = [f"{c} {B} {u}" for c, B, u, f1, f2, tl, th in recs]
-    assert lines == open(os.path.join(GOLDEN, "units_cfg1.txt")).read().split("\n")[:-1]
+    assert lines == open(os.path.join(GOLDEN, "units_100001_102000.txt")).read().split("\n")[:-1]
     assert all(f1 == f2 == tl == th == 0 for _, _, _, f1, f2, tl, th in recs)
     assert (lo, hi) == (min(r[1] for r in recs), max(r[1] for r in recs))
     tb, tt = str(tmp_path / "j.bbj"), str(tmp_path / "j.but")

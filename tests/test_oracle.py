@@ -4,7 +4,7 @@ against independent mathematics.  CPU only.
 Reference vectors used (tests/golden/, written by make_golden.py from the
 reference's own sources compiled in oracle/_ref):
   xor_tables.json  xor_build_save output bytes (SHA-256) for several (N, r)
-  units_cfg1.txt   fourcore_classes_v3.hpp units for B in [100001,102000]
+  units_100001_102000.txt   fourcore_classes_v3.hpp units for B in [100001,102000]
   residues.json    free-term residue specs at sampled units up to B=3.52M
   seeds.json       seed counts per class
 plus the reference's published Stage-1 count (README.md:146) and self-test
@@ -65,10 +65,10 @@ def test_fpr_smoke_threshold():
     assert O.Table.build_pairs(300, 8).fpr_smoke() > 16        # refused by the searcher
 
 
-def test_units_cfg1_match_reference():
+def test_units_100001_102000_match_reference():
     cls, Bs, us = O.emit_units(100001, 102000, classes=(2, 3, 4, 5))
     mine = "".join(f"{c} {b} {u}\n" for c, b, u in zip(cls, Bs, us))
-    assert mine == open(os.path.join(GOLDEN, "units_cfg1.txt")).read()
+    assert mine == open(os.path.join(GOLDEN, "units_100001_102000.txt")).read()
 
 
 def test_residues_match_reference():
@@ -166,7 +166,7 @@ def test_peel_counters_brute_force(B, u, fm):
     assert c2["gated"] == 0 and c2["probes"] == c["probes"] + c["gated"]
 
 
-def test_calls_closed_form_cfg1():
+def test_calls_closed_form_100001_102000():
     _, Bs, us = O.emit_units(100001, 102000, classes=(5,))
     arr, ok = O.pack_units(Bs, us)
     assert ok.all()
