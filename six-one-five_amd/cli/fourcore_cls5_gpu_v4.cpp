@@ -7,7 +7,9 @@
 //                        [--no-gate] [--stop-first] [--device K] [--load-table F]
 //                        [--save-table F] [--allow-rebuild]
 //   fourcore_cls5_gpu_v4 --check-table --N K --load-table F [--r BITS] [--save-table F]
-// Additions: --mode plain|batched (default batched), --hit-cap N, --counters
+// Additions: --mode plain|batched (default batched), --hit-cap N, --counters,
+// --exact-table (check filter survivors against the exact sorted pair table
+// of all a^6 + b^6, a <= b <= N, built on the device)
 // (one machine-readable counters line on stdout).
 #include <algorithm>
 #include <chrono>
@@ -31,7 +33,7 @@ static void usage() {
          "                            [--stop-first] [--device K]\n"
          "                            [--load-table F] [--save-table F]\n"
          "                            [--allow-rebuild] [--mode plain|batched]\n"
-         "                            [--hit-cap N] [--counters]\n"
+         "                            [--hit-cap N] [--counters] [--exact-table]\n"
          "   or: fourcore_cls5_gpu_v4 --check-table --N K --load-table F\n"
          "                            [--r BITS] [--save-table F] [--allow-rebuild]\n");
 }
@@ -84,7 +86,7 @@ int main(int argc, char** argv) {
   int device = 0, N_override = 0, xor_r = 48;
   long long hit_cap = 1 << 20;
   bool do_host = false, quiet = true, use_gate = true, stop_first = false;
-  bool allow_rebuild = false, check_table = false, counters = false;
+  bool allow_rebuild = false, check_table = false, counters = false, exact_table = false;
   for (int i = 1; i < argc; ++i) {
     const std::string s = argv[i];
     auto next = [&]() { return (i + 1 < argc) ? std::string(argv[++i]) : std::string(); };
@@ -105,6 +107,7 @@ int main(int argc, char** argv) {
     else if (s == "--mode") mode_s = next();
     else if (s == "--hit-cap") hit_cap = std::atoll(next().c_str());
     else if (s == "--counters") counters = true;
+    else if (s == "--exact-table") exact_table = true;
     else if (s == "-h" || s == "--help") { usage(); return 0; }
     else { fprintf(stderr, "unknown %s\n", s.c_str()); usage(); return 1; }
   }
@@ -200,6 +203,19 @@ int main(int argc, char** argv) {
     return 1;
   }
   XorTable().packed.swap(t.packed);  // host copy no longer needed
+  if (exact_table) {
+    const auto tx = std::chrono::steady_clock::now();
+    uint64_t n_ex = 0;
+    if (fc5_exact_build(ctx, N, nullptr, 0) != FC5_OK || fc5_exact_info(ctx, &n_ex, nullptr, nullptr) != FC5_OK) {
+      fprintf(stderr, "[cls5-gpu] FATAL: %s\n", fc5_last_error(ctx));
+      fc5_ctx_destroy(ctx);
+      return 1;
+    }
+    fc5_set_option(ctx, "verify_table", 1);
+    fprintf(stderr, "[cls5-gpu] exact sorted pair table: %llu entries (%.2f GB) built in %.1fs\n",
+            (unsigned long long)n_ex, n_ex * 16 / 1e9,
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - tx).count());
+  }
   const auto t0 = std::chrono::steady_clock::now();
   fc5_result r{};
   const int rc = fc5_run_units(ctx, units.data(), units.size(), mode, &r);

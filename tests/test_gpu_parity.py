@@ -474,23 +474,26 @@ def brute_pairs(T, hi):
     return sorted(out)
 
 
-def test_cli_cls5_units_bbj_equals_buc(tmp_path):
-    """the cls5 CLI reads Stage-1 units from a .bbj file as from a .buc file"""
+def test_cli_cls5_units_bbj_buc_and_exact_table(tmp_path):
+    """the cls5 CLI reads Stage-1 units from a .bbj file as from a .buc file, and
+    gives the same output when it checks survivors against the exact sorted table"""
     import re
     import subprocess
     from conftest import BIN
-    say("cls5 CLI .bbj vs .buc")
+    say("cls5 CLI .bbj vs .buc, --exact-table")
     outs = []
-    for ext in ("buc", "bbj"):
+    for ext, flags in (("buc", []), ("bbj", []), ("buc", ["--exact-table"])):
         p = str(tmp_path / f"u.{ext}")
         subprocess.run([os.path.join(BIN, "fourcore_hunt_v3"), "--lo", "100001", "--hi", "101000", "--classes", "5",
                         "--emit-units", p], check=True, capture_output=True)
         r = subprocess.run([os.path.join(BIN, "fourcore_cls5_gpu_v4"), "--units", p, "--N", "2428", "--r", "16",
-                            "--quiet", "--counters"], capture_output=True, text=True, timeout=120)
+                            "--quiet", "--counters"] + flags, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr[-1000:]
         lines = [l for l in r.stdout.splitlines() if l.startswith("COUNTERS") or l.startswith("----")]
         outs.append([re.sub(r"(kernel_ms|time)=[0-9.]+s?", "", l) for l in lines])
-    assert outs[0] == outs[1] and outs[0]
+        if flags:
+            assert "exact sorted pair table: 2948806 entries" in r.stderr
+    assert outs[0] == outs[1] == outs[2] and outs[0]
 
 
 @pytest.mark.parametrize("lo,hi,classes,band", [
