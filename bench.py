@@ -281,7 +281,10 @@ def main():
                      "kernel": "k_cls5_batched",
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "per_unit": f"{3 * cell_bytes} B per filter probe (3 cells x {cell_bytes} B)",
-                     "transaction_GBps_at_64B": r["probes"] * 3 * 64 / kern_s / 1e9},
+                     "transaction_GBps_at_64B": r["probes"] * 3 * 64 / kern_s / 1e9,
+                     # measured HBM traffic (128-byte lines per random gather) over the same launch time
+                     "traffic_GBps": traffic / kern_s / 1e9 if traffic else None,
+                     "traffic_frac": traffic / kern_s / 1e9 / HBM_PEAK_GBS if traffic else None},
         "gather_ceiling": ceiling,
         "autotune": tune,
         "graph": not args.no_graph,
@@ -396,6 +399,7 @@ def bench_twin(args):
                          f"counters and the {len(ohits)}-entry maybe-hit list identical to the GPU on that sample"}
         log(f"[bench] cpu oracle {oc['calls'] / dt:.4e} calls/s on {len(sub)} candidates, {threads} threads; "
             f"counters and {len(ohits)} maybe-hits identical to the GPU")
+    traffic = twin_traffic(lo, hi)
     line = {
         "metric": "617 cls5 twin probe calls/sec, 1 GPU, bit-exact", "value": calls_s,
         "unit": "probe calls/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -409,9 +413,11 @@ def bench_twin(args):
                                          "maybe", "solutions")},
         "kernel_ms": float(np.mean(kms)),
         "roofline": {"bound": "hbm", "achieved": alg_bytes / kern_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": alg_bytes / kern_s / 1e9 / HBM_PEAK_GBS, "traffic": twin_traffic(lo, hi),
+                     "frac": alg_bytes / kern_s / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_twin_batched",
-                     "per_unit": f"{3 * cell_bytes} B per probe call (3 cells x {cell_bytes} B)"},
+                     "per_unit": f"{3 * cell_bytes} B per probe call (3 cells x {cell_bytes} B)",
+                     "traffic_GBps": traffic / kern_s / 1e9 if traffic else None,
+                     "traffic_frac": traffic / kern_s / 1e9 / HBM_PEAK_GBS if traffic else None},
         "gather_ceiling": ceiling,
         "plain_form": plain,
         "cpu_baseline": cpu,
