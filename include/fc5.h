@@ -105,6 +105,8 @@ int fc5_abi_version(void);
  *      fourcore_cls5_gpu_v4.cu:414-466,673) ---- */
 int fc5_ctx_create(int device, fc5_ctx **out);
 void fc5_ctx_destroy(fc5_ctx *ctx);
+/* text of the context's last error; it stays valid until the calling thread's
+ * next fc5_last_error call (safe while a server thread runs) */
 const char *fc5_last_error(const fc5_ctx *ctx);
 /* options: "hit_cap" (default 1<<20), "use_gate" (1; 0 = --no-gate,
  * fourcore_cls5_gpu_v4.cu:634), "blocks_per_cu" (0 = occupancy limit) for the

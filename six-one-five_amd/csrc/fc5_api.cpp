@@ -99,7 +99,12 @@ extern "C" void fc5_ctx_destroy(fc5_ctx* ctx) {
   delete ctx;
 }
 
-extern "C" const char* fc5_last_error(const fc5_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+extern "C" const char* fc5_last_error(const fc5_ctx* ctx) {
+  if (!ctx) return "null context";
+  thread_local std::string text;  // valid until this thread's next call
+  text = ctx->err.get();
+  return text.c_str();
+}
 
 extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) {
   if (!ctx || !key) return FC5_E_ARG;

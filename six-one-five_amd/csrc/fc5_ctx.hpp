@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -38,6 +39,26 @@ inline u32 table_policy(int option, u64 m_cells) {
   if (option >= 0) return (u32)option;
   return m_cells * 8 <= (1ull << 30) ? 0u : 1u;
 }
+
+// The last error text of a context.  API calls write it on the caller's
+// thread and a serving context's server thread writes it too, while
+// fc5_last_error may read it from any thread: every access takes the lock.
+class ErrText {
+ public:
+  ErrText& operator=(std::string s) {
+    std::lock_guard<std::mutex> lk(mu_);
+    s_ = std::move(s);
+    return *this;
+  }
+  std::string get() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return s_;
+  }
+
+ private:
+  mutable std::mutex mu_;
+  std::string s_;
+};
 
 inline u64 inv_mod_2_64(u64 a) {  // a odd; Newton iteration doubles correct bits
   u64 x = a;
@@ -86,7 +107,7 @@ struct fc5_band {
 struct fc5_ctx {
   int device = 0;
   int num_cu = 0;
-  std::string err;
+  ErrText err;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // options
