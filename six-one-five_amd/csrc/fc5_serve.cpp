@@ -188,6 +188,9 @@ void collect(fc5_server* S, SBand& B) {
     if (B.cls_of[i]) B.cls_res[B.cls_of[i]].units++;
 }
 
+// counters, verification and solutions of a band whose items have all run.
+// Runs on the server thread; the caller publishes the band (state kDone)
+// under the lock afterwards, and nothing else touches an open band's results.
 void finalize(fc5_server* S, SBand& B) {
   fc5_ctx* ctx = S->ctx;
   if (B.slot >= 0) collect(S, B);
@@ -253,7 +256,6 @@ void finalize(fc5_server* S, SBand& B) {
   B.t_done = S->now();
   B.res.wall_ms = 1000.0 * (B.t_done - B.t_submit);
   if (B.overflow) B.rc = FC5_E_OVERFLOW;
-  B.state = kDone;
   // the unit arrays are no longer needed
   B.dev5 = {};
   B.dev3 = {};
@@ -269,6 +271,7 @@ int admit_locked(fc5_server* S) {
       S->pending.pop_front();
       B.t_admit = S->now();
       finalize(S, B);
+      B.state = kDone;
       S->done++;
       S->cv_done.notify_all();
       continue;
@@ -420,14 +423,22 @@ void serve_loop(fc5_server* S) {
       }
     }
     const int rc = run_round(S, active);
-    std::lock_guard<std::mutex> lk(S->mu);
     if (rc != FC5_OK) {
+      std::lock_guard<std::mutex> lk(S->mu);
       S->fail_rc = rc;
       continue;
     }
-    for (SBand* B : active) {
-      if (!B->finished_items()) continue;
-      finalize(S, *B);
+    // finished bands are verified outside the lock (device pair search, host
+    // identity checks), so that submitters and pollers are not held up
+    std::vector<SBand*> fin;
+    for (SBand* B : active)
+      if (B->finished_items()) {
+        finalize(S, *B);
+        fin.push_back(B);
+      }
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (SBand* B : fin) {
+      B->state = kDone;
       S->a5.release(B->off5, B->units5.size());
       S->a3.release(B->off3, B->units3.size());
       S->slots[(size_t)B->slot] = nullptr;
