@@ -71,6 +71,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS) + [5, 617],
                     help="1-4: class-5 peel bands; 5: serving (Poisson band arrivals); 617: the (6,1,7) twin")
+    ap.add_argument("--band", type=int, nargs=2, metavar=("B_LO", "B_HI"), default=None,
+                    help="1-4: run the config's classes over this B range instead (table N = B_HI // 42, "
+                         "no planted keys), e.g. a band whose reference run time is published")
+    ap.add_argument("--table-n", type=int, default=0,
+                    help="with --band: pair table N (default B_HI // 42), e.g. to run a long band as pieces "
+                         "over the table of the whole band")
     ap.add_argument("--serve-requests", type=int, default=200, help="5: number of band requests")
     ap.add_argument("--serve-load", type=float, default=0.8, help="5: offered load as a fraction of capacity")
     ap.add_argument("--serve-calibrate", type=int, default=24, help="5: bands of the closed-loop capacity run")
@@ -104,14 +110,21 @@ def main():
         return bench_twin(args)
     if args.config == 5:
         return bench_serve(args)
-    cfg = CONFIGS[args.config]
+    cfg = dict(CONFIGS[args.config])
+    if args.band:
+        lo, hi = args.band
+        if not 1 <= lo <= hi:
+            raise SystemExit("--band needs 1 <= B_LO <= B_HI")
+        if args.table_n and args.table_n < hi // 42:
+            raise SystemExit("--table-n must be at least B_HI // 42")
+        cfg.update(lo=lo, hi=hi, N=args.table_n or hi // 42, plants=0, plant_mode=None, exact=False)
     t_setup = time.time()
     mixed = cfg["classes"] != (5,)
     ucls, Bs, us = fc5.emit_units(cfg["lo"], cfg["hi"], classes=cfg["classes"])
     units = fc5.units_array(Bs, us)
     n_B = eligible_count(cfg["lo"], cfg["hi"])
     calls_closed = fc5.units_calls(units[ucls == 5]) if mixed else fc5.units_calls(units)
-    log(f"[bench] config {args.config}: B=[{cfg['lo']},{cfg['hi']}] eligible_B={n_B} units={len(units)} "
+    log(f"[bench] {'band' if args.band else f'config {args.config}'}: B=[{cfg['lo']},{cfg['hi']}] eligible_B={n_B} units={len(units)} "
         f"calls={calls_closed:.4e}")
 
     ctx = fc5.Context(0)
@@ -152,7 +165,9 @@ def main():
         exact["spot_checks"] = f"256 random pair sums found; 16 windows of {wl} entries strictly increasing"
         ctx.set_option("verify_table", 1)
         log(f"[bench] exact sorted pair table: {n_ex} entries, {exact['GB']:.2f} GB, built in {exact['build_s']:.1f}s")
+    t1 = time.time()
     band = ctx.band(units, ucls if mixed else None)
+    log(f"[bench] band packed and uploaded in {time.time() - t1:.1f}s")
     plant_set = {(p[0], p[1], p[2]) for p in plants}
     flag = 2 if cfg["plant_mode"] == "side" else 1
 
@@ -174,10 +189,13 @@ def main():
     tune = None
     if not args.no_autotune:
         t1 = time.time()
-        tune = band.autotune(sample_every=16)
+        # trials on every k-th unit: 1/16 of the band, or fewer units for bands far above 2e12 grid points
+        every = max(16, int(calls_closed / 2e12))
+        tune = band.autotune(sample_every=every)
         tune["seconds"] = time.time() - t1
+        tune["sample_every"] = every
         log(f"[bench] autotune: rows_per_item={tune['rows_per_item']} blocks_per_cu={tune['blocks_per_cu']} "
-            f"({tune['best_ms']:.1f} ms vs default {tune['default_ms']:.1f} ms on 1/16 of the band)")
+            f"({tune['best_ms']:.1f} ms vs default {tune['default_ms']:.1f} ms on 1/{every} of the band)")
     if not args.no_graph:
         ctx.set_option("use_graph", 1)
     for _ in range(args.warmup):
@@ -249,7 +267,7 @@ def main():
 
     traffic = None
     tpath = os.path.join(REPO, "profiles", f"traffic_cls5_batched_config{args.config}.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and not args.band:
         traffic = json.load(open(tpath)).get("traffic_bytes_per_launch")
 
     cpu = None
@@ -261,7 +279,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (deterministic Stage-1 enumeration of the band; seeded planted keys)",
-        "config": {"workload": f"config {args.config}: batched persistent cls5 peel, eligible B in "
+        "config": {"workload": ("band" if args.band else f"config {args.config}") + ": batched persistent cls5 peel, eligible B in "
                                f"[{cfg['lo']},{cfg['hi']}] ({n_B} B, {len(units)} units, "
                                f"{r['calls']} grid points), xor pair table N={hdr.N} r={hdr.r} "
                                f"({fc5_packed_gb(hdr):.2f} GB packed, device-built), "
