@@ -7,16 +7,23 @@
 //
 //   1. keys a^6+b^6 (mod 2^64), 1<=a<=b<=N, generated in registers (one
 //      workgroup per row a) and deduplicated in an open-addressing hash set
-//      (atomicCAS, linear probing; key 0 kept in a flag).  Duplicates are
-//      systematic: a^6 = 0 (mod 2^64) whenever 2^11 | a.
+//      of 1.25 slots per key (atomicCAS, linear probing from a multiply-shift
+//      position; key 0 kept in a flag).  Duplicates are systematic:
+//      a^6 = 0 (mod 2^64) whenever 2^11 | a.
 //   2. incidence: per unique key, atomic xor of its hash and atomic count on
-//      its three cells.
+//      its three cells.  Counts are 16-bit, two per 32-bit word (a count
+//      reaching 2^16 fails the build, as the reference's u16 counts do).
 //   3. peel in rounds.  claim: every cell of the frontier with count 1 reads
 //      its key and keeps it iff it is the key's first cell with count 1 (the
 //      counts are not written during a claim pass, so each key is claimed
 //      once); apply: the claimed keys are removed from their other cells,
-//      whose counts dropping to 1 form the next frontier.  The claimed cell
-//      keeps the key's hash.
+//      whose counts dropping to 1 form the next frontier, written over the
+//      one just claimed (one frontier buffer of m/2 cells; a larger frontier
+//      fails the build).  The claimed cell keeps the key's hash.
+//
+// Device working set: 8 B x 1.25 per key (hash set) + 8 B per key (peel
+// order) + 8 + 2 + 4 B per cell (xor sets, counts, frontier) -- about 25 B per
+// key, 250 GB at N = 119,047 (7.09e9 keys).
 //   4. assign in reverse round order, in place: cell = fp(h) ^ other cells.
 //      Keys claimed in the same round never share cells, so a round is one
 //      parallel pass.
@@ -73,29 +80,40 @@ __global__ void k_p6(u64* p6, u32 N) {
   }
 }
 
-__device__ __forceinline__ void set_insert(u64* slots, u64 smask, u32* zero_flag, u64 key) {
+// the set holds more slots than keys, so every probe sequence ends
+__device__ __forceinline__ void set_insert(u64* slots, u64 nslots, u32* zero_flag, u64 key) {
   if (key == 0) {
     atomicOr(zero_flag, 1u);
     return;
   }
-  u64 pos = b_mix64(key ^ 0x2545F4914F6CDD1DULL) & smask;
+  u64 pos = __umul64hi(b_mix64(key ^ 0x2545F4914F6CDD1DULL), nslots);
   for (;;) {
     const u64 prev = atomicCAS((ull*)&slots[pos], 0ULL, (ull)key);
     if (prev == 0 || prev == key) return;
-    pos = (pos + 1) & smask;
+    if (++pos == nslots) pos = 0;
   }
 }
 
-__global__ void k_insert_pairs(const u64* p6, u32 N, u64* slots, u64 smask, u32* zero_flag) {
+__global__ void k_insert_pairs(const u64* p6, u32 N, u64* slots, u64 nslots, u32* zero_flag) {
   for (u32 a = blockIdx.x + 1; a <= N; a += gridDim.x) {
     const u64 pa = p6[a];
-    for (u32 b = a + threadIdx.x; b <= N; b += blockDim.x) set_insert(slots, smask, zero_flag, pa + p6[b]);
+    for (u32 b = a + threadIdx.x; b <= N; b += blockDim.x) set_insert(slots, nslots, zero_flag, pa + p6[b]);
   }
 }
 
-__global__ void k_insert_keys(const u64* keys, u64 n, u64* slots, u64 smask, u32* zero_flag) {
+__global__ void k_insert_keys(const u64* keys, u64 n, u64* slots, u64 nslots, u32* zero_flag) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
-    set_insert(slots, smask, zero_flag, keys[i]);
+    set_insert(slots, nslots, zero_flag, keys[i]);
+}
+
+// 16-bit cell counts, two per 32-bit word
+__device__ __forceinline__ u32 cnt_get(const u32* counts, u64 i) {
+  return (counts[i >> 1] >> ((u32)(i & 1) * 16u)) & 0xffffu;
+}
+__device__ __forceinline__ u32 cnt_add(u32* counts, u64 i, int delta) {  // returns the old count
+  const u32 sh = (u32)(i & 1) * 16u;
+  const u32 old = delta > 0 ? atomicAdd(&counts[i >> 1], 1u << sh) : atomicSub(&counts[i >> 1], 1u << sh);
+  return (old >> sh) & 0xffffu;
 }
 
 __global__ void k_count_slots(const u64* slots, u64 n, ull* count) {
@@ -113,7 +131,7 @@ __device__ __forceinline__ void add_key(u64 key, u64 seed, u64 block, const ModD
   b_cells(h, block, md, c);
   for (int t = 0; t < 3; ++t) {
     atomicXor((ull*)&setxor[c[t]], (ull)h);
-    if (atomicAdd(&counts[c[t]], 1u) >= 0xffffu) atomicOr(overflow, 1u);
+    if (cnt_add(counts, c[t], 1) >= 0xffffu) atomicOr(overflow, 1u);
   }
 }
 
@@ -133,7 +151,7 @@ __global__ void k_frontier_init(const u32* counts, u64 m, u64* flog, ull* flog_n
   const u64 trips = (m + stride - 1) / stride;
   for (u64 t = 0; t < trips; ++t) {
     const u64 i = t * stride + (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool one = i < m && counts[i] == 1;
+    const bool one = i < m && cnt_get(counts, i) == 1;
     const u64 slot = wave_append(flog_n, one);
     if (one && slot < flog_cap) flog[slot] = i;
   }
@@ -149,10 +167,10 @@ __global__ void k_claim(const u64* front, u64 nf, const u32* counts, const u64* 
     u64 i = 0;
     if (k < nf) {
       i = front[k];
-      if (counts[i] == 1) {
+      if (cnt_get(counts, i) == 1) {
         u64 c[3];
         b_cells(setxor[i], block, md, c);
-        const u64 first = counts[c[0]] == 1 ? c[0] : (counts[c[1]] == 1 ? c[1] : c[2]);
+        const u64 first = cnt_get(counts, c[0]) == 1 ? c[0] : (cnt_get(counts, c[1]) == 1 ? c[1] : c[2]);
         take = first == i;
       }
     }
@@ -176,9 +194,9 @@ __global__ void k_apply(const u64* peel, u64 np, u32* counts, u64* setxor, u64 b
       for (int q = 0; q < 3; ++q) {
         if (c[q] == i) continue;
         atomicXor((ull*)&setxor[c[q]], (ull)h);
-        nf[q] = atomicSub(&counts[c[q]], 1u) == 2u;
+        nf[q] = cnt_add(counts, c[q], -1) == 2u;
       }
-      counts[i] = 0;
+      (void)cnt_add(counts, i, -1);  // the claimed cell held this key only: 1 -> 0
     }
     for (int q = 0; q < 3; ++q) {
       const u64 slot = wave_append(flog_n, nf[q]);
@@ -246,9 +264,9 @@ int gpu_build_pairs(int N, u32 r, const u64* extra, size_t n_extra, hipStream_t 
   ull* ctr = nullptr;                       // [0] unique, [1] flog_n, [2] peel_n
   const u64 mask = r >= 64 ? ~0ULL : ((1ULL << r) - 1);
   const u64 npairs = (u64)N * (u64)(N + 1) / 2;
-  u64 nslots = 1024;
-  while ((double)nslots < 1.25 * (double)(npairs + n_extra)) nslots <<= 1;
-  u64 n_unique = 0, m = 0, block = 0;
+  // 1.25 slots per key (rounded up to 64): linear probing stays short
+  const u64 nslots = std::max<u64>(1024, ((npairs + n_extra) + (npairs + n_extra) / 4 + 63) / 64 * 64);
+  u64 n_unique = 0, m = 0, block = 0, cwords = 0, fcap = 0;
   ModDiv md{};
   bool ok = false;
   ull h_ctr[3];
@@ -262,13 +280,12 @@ int gpu_build_pairs(int N, u32 r, const u64* extra, size_t n_extra, hipStream_t 
   BCHK(hipMemsetAsync(flags, 0, 8, s));
   BCHK(hipMemsetAsync(ctr, 0, 3 * sizeof(ull), s));
   hipLaunchKernelGGL(k_p6, dim3(grid_of((u64)N + 1)), dim3(256), 0, s, p6, (u32)N);
-  hipLaunchKernelGGL(k_insert_pairs, dim3(std::min(N, 65536)), dim3(256), 0, s, p6, (u32)N, slots, nslots - 1,
-                     flags);
+  hipLaunchKernelGGL(k_insert_pairs, dim3(std::min(N, 65536)), dim3(256), 0, s, p6, (u32)N, slots, nslots, flags);
   if (n_extra) {
     BCHK(hipMalloc(&d_extra, n_extra * 8));
     BCHK(hipMemcpyAsync(d_extra, extra, n_extra * 8, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_insert_keys, dim3(grid_of(n_extra)), dim3(256), 0, s, d_extra, (u64)n_extra, slots,
-                       nslots - 1, flags);
+                       nslots, flags);
   }
   hipLaunchKernelGGL(k_count_slots, dim3(grid_of(nslots)), dim3(256), 0, s, slots, nslots, ctr);
   BCHK(hipGetLastError());
@@ -282,29 +299,33 @@ int gpu_build_pairs(int N, u32 r, const u64* extra, size_t n_extra, hipStream_t 
   m = capacity_rule(n_unique);
   block = m / 3;
   md = make_moddiv(block);
+  cwords = (m + 1) / 2;             // two 16-bit counts per word
+  fcap = std::max<u64>(m / 2, 1024);  // one frontier at a time
   BCHK(hipMalloc(&setxor, m * 8));
-  BCHK(hipMalloc(&counts, m * 4));
-  BCHK(hipMalloc(&flog, m * 8));
+  BCHK(hipMalloc(&counts, cwords * 4));
+  BCHK(hipMalloc(&flog, fcap * 8));
   BCHK(hipMalloc(&peel, std::max<u64>(n_unique, 1) * 8));
   for (int outer = 0; outer < 4 && !ok; ++outer) {
     const u64 seed = 0x615615615615615ULL + (u64)outer * 0x9E3779B97F4A7C15ULL;
     BCHK(hipMemsetAsync(setxor, 0, m * 8, s));
-    BCHK(hipMemsetAsync(counts, 0, m * 4, s));
+    BCHK(hipMemsetAsync(counts, 0, cwords * 4, s));
     BCHK(hipMemsetAsync(ctr, 0, 3 * sizeof(ull), s));
     BCHK(hipMemsetAsync(flags + 1, 0, 4, s));
     hipLaunchKernelGGL(k_incidence, dim3(grid_of(nslots)), dim3(256), 0, s, slots, nslots, flags, seed, block, md,
                        setxor, counts, flags + 1);
-    hipLaunchKernelGGL(k_frontier_init, dim3(grid_of(m)), dim3(256), 0, s, counts, m, flog, ctr + 1, m);
+    hipLaunchKernelGGL(k_frontier_init, dim3(grid_of(m)), dim3(256), 0, s, counts, m, flog, ctr + 1, fcap);
     BCHK(hipGetLastError());
     std::vector<u64> round_start;
-    u64 f_begin = 0, f_end = 0, p_done = 0;
+    u64 f_n = 0, p_done = 0;
     BCHK(hipMemcpyAsync(h_ctr, ctr, 3 * sizeof(ull), hipMemcpyDeviceToHost, s));
     BCHK(hipStreamSynchronize(s));
-    f_end = h_ctr[1];
-    if (f_end > m) { err = "frontier overflow"; goto fail; }
-    while (f_begin < f_end) {
-      hipLaunchKernelGGL(k_claim, dim3(grid_of(f_end - f_begin)), dim3(256), 0, s, flog + f_begin, f_end - f_begin,
-                         counts, setxor, block, md, peel, ctr + 2, n_unique);
+    f_n = h_ctr[1];
+    if (f_n > fcap) { err = "frontier overflow"; goto fail; }
+    while (f_n) {
+      hipLaunchKernelGGL(k_claim, dim3(grid_of(f_n)), dim3(256), 0, s, flog, f_n, counts, setxor, block, md, peel,
+                         ctr + 2, n_unique);
+      // the claimed frontier is dead: the next one is written over it
+      BCHK(hipMemsetAsync(ctr + 1, 0, sizeof(ull), s));
       BCHK(hipMemcpyAsync(h_ctr, ctr, 3 * sizeof(ull), hipMemcpyDeviceToHost, s));
       BCHK(hipStreamSynchronize(s));
       const u64 p_end = h_ctr[2];
@@ -312,14 +333,13 @@ int gpu_build_pairs(int N, u32 r, const u64* extra, size_t n_extra, hipStream_t 
       round_start.push_back(p_done);
       if (p_end > p_done)
         hipLaunchKernelGGL(k_apply, dim3(grid_of(p_end - p_done)), dim3(256), 0, s, peel + p_done, p_end - p_done,
-                           counts, setxor, block, md, flog, ctr + 1, m);
+                           counts, setxor, block, md, flog, ctr + 1, fcap);
       BCHK(hipGetLastError());
       BCHK(hipMemcpyAsync(h_ctr, ctr, 3 * sizeof(ull), hipMemcpyDeviceToHost, s));
       BCHK(hipStreamSynchronize(s));
       p_done = p_end;
-      f_begin = f_end;
-      f_end = h_ctr[1];
-      if (f_end > m) { err = "frontier overflow"; goto fail; }
+      f_n = h_ctr[1];
+      if (f_n > fcap) { err = "frontier overflow"; goto fail; }
     }
     BCHK(hipMemcpyAsync(h_flags, flags, 8, hipMemcpyDeviceToHost, s));
     BCHK(hipStreamSynchronize(s));
