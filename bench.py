@@ -144,6 +144,14 @@ def main():
     fpr = ctx.fpr_smoke()
     if fpr > 16:
         raise SystemExit(f"FPR smoke {fpr} > 16: refuse")
+    # no false negatives: 2^20 random pair keys a^6 + b^6 (mod 2^64), a <= b <= N, are all present
+    rng = np.random.default_rng(cfg["seed"] ^ 0xF00D)
+    ab = np.sort(rng.integers(1, cfg["N"] + 1, size=(1 << 20, 2), dtype=np.uint64), axis=1)
+    p6 = ab * ab * ab
+    p6 = p6 * p6
+    present = ctx.table_query(p6[:, 0] + p6[:, 1])
+    assert present.all(), f"{int((~present.astype(bool)).sum())} pair keys missing from the table"
+    log(f"[bench] table check: FPR smoke {fpr}/200000, 2^20 random pair keys all present")
     if cfg["plant_mode"] == "side":
         ctx.side_build(pkeys, args.r)
     exact = None
