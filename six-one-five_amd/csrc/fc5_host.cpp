@@ -420,6 +420,20 @@ int load_bin(const std::string& path, int expected_N, XorTable& t, std::string& 
     msg = "N mismatch: file N=" + std::to_string(h.N) + ", need " + std::to_string(expected_N);
     return kLoadNMismatch;
   }
+  // the header's cell count must match the bytes that follow before anything
+  // is allocated for them (a damaged header would otherwise ask for any size)
+  long here = std::ftell(f), end = -1;
+  if (here >= 0 && std::fseek(f, 0, SEEK_END) == 0) end = std::ftell(f);
+  if (end < 0 || std::fseek(f, here, SEEK_SET) != 0) {
+    std::fclose(f);
+    msg = "cannot size " + path;
+    return kLoadShort;
+  }
+  if (h.m_cells > (~0ULL - 7) / 64 || packed_bytes(h.m_cells, h.r) > (u64)(end - here)) {
+    std::fclose(f);
+    msg = "short read " + path;
+    return kLoadShort;
+  }
   const size_t want = packed_bytes(h.m_cells, h.r);
   t.hdr = h;
   t.packed.assign(want, 0);

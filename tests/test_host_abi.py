@@ -109,6 +109,13 @@ def test_cli_cls5_host_paths(tmp_path):
     fpr = _run("fourcore_cls5_gpu_v4", "--check-table", "--N", 300, "--load-table", q, check=False)
     assert fpr.returncode == 1 and "FPR smoke too high" in fpr.stderr
     assert _run("fourcore_cls5_gpu_v4", "--units", "/nonexistent.buc", check=False).returncode == 1
+    # a damaged header (cell count far beyond the file) is refused before anything is allocated
+    raw = bytearray(open(p, "rb").read())
+    raw[32:40] = (0x1FFFFFFFFFFFFFFF).to_bytes(8, "little")  # m_cells
+    d = str(tmp_path / "damaged.bin")
+    open(d, "wb").write(bytes(raw))
+    dmg = _run("fourcore_cls5_gpu_v4", "--check-table", "--N", 300, "--load-table", d, check=False)
+    assert dmg.returncode == 1 and "short read" in dmg.stderr
 
 
 def test_cli_617_argument_refusals():
