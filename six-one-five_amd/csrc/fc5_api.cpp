@@ -42,7 +42,7 @@ static void build_gate_host(GateWords& gw, std::vector<u32>& bitmap) {
 
 extern "C" int fc5_abi_version(void) { return FC5_ABI_VERSION; }
 
-extern "C" int fc5_ctx_create(int device, fc5_ctx** out) {
+extern "C" int fc5_ctx_create(int device, fc5_ctx** out) try {
   if (!out) return FC5_E_ARG;
   *out = nullptr;
   auto ctx = std::make_unique<fc5_ctx>();
@@ -76,6 +76,8 @@ extern "C" int fc5_ctx_create(int device, fc5_ctx** out) {
     return FC5_E_HIP;
   *out = ctx.release();
   return FC5_OK;
+} catch (...) {
+  return abi_exception(nullptr);
 }
 
 extern "C" void fc5_ctx_destroy(fc5_ctx* ctx) {
@@ -106,7 +108,7 @@ extern "C" const char* fc5_last_error(const fc5_ctx* ctx) {
   return text.c_str();
 }
 
-extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) {
+extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) try {
   if (!ctx || !key) return FC5_E_ARG;
   NOT_SERVING(ctx);
   const std::string k = key;
@@ -145,6 +147,8 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) {
     FAIL(ctx, FC5_E_ARG, "unknown option " + k);
   }
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 // ---------------------------------------------------------------------------
@@ -201,7 +205,7 @@ static int adopt_main(fc5_ctx* ctx, XorTable&& t) {
   return FC5_OK;
 }
 
-extern "C" int fc5_table_load(fc5_ctx* ctx, const char* path, int expected_N) {
+extern "C" int fc5_table_load(fc5_ctx* ctx, const char* path, int expected_N) try {
   if (!ctx || !path) return FC5_E_ARG;
   NOT_SERVING(ctx);
   XorTable t;
@@ -218,10 +222,12 @@ extern "C" int fc5_table_load(fc5_ctx* ctx, const char* path, int expected_N) {
   if (fpr < 0 || fpr > 16)
     FAIL(ctx, FC5_E_FPR, "xor FPR smoke too high (" + std::to_string(fpr) + ") - refuse to search");
   return adopt_main(ctx, std::move(t));
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 extern "C" int fc5_table_upload(fc5_ctx* ctx, const fc5_store_header* hdr, const uint8_t* packed,
-                                size_t len) {
+                                size_t len) try {
   if (!ctx || !hdr || (!packed && len)) return FC5_E_ARG;
   NOT_SERVING(ctx);
   XorTable t;
@@ -230,10 +236,12 @@ extern "C" int fc5_table_upload(fc5_ctx* ctx, const fc5_store_header* hdr, const
     FAIL(ctx, FC5_E_FORMAT, "bad header");
   t.packed.assign(packed, packed + len);
   return adopt_main(ctx, std::move(t));
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 extern "C" int fc5_table_build(fc5_ctx* ctx, int N, int r, const uint64_t* extra, size_t n_extra,
-                               int on_gpu) {
+                               int on_gpu) try {
   if (!ctx || N < 1 || r < 1 || r > 64 || (!extra && n_extra)) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (on_gpu) {
@@ -263,6 +271,8 @@ extern "C" int fc5_table_build(fc5_ctx* ctx, int N, int r, const uint64_t* extra
   std::string msg;
   if (!build_pairs_cpu(N, (u32)r, ex, t, msg)) FAIL(ctx, FC5_E_BUILD, msg);
   return adopt_main(ctx, std::move(t));
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 extern "C" int fc5_table_info(const fc5_ctx* ctx, fc5_store_header* out) {
@@ -301,7 +311,7 @@ static int table_packed(fc5_ctx* ctx, XorTable& t, bool want_bytes) {
   return rc;
 }
 
-extern "C" int fc5_table_save(fc5_ctx* ctx, const char* path) {
+extern "C" int fc5_table_save(fc5_ctx* ctx, const char* path) try {
   if (!ctx || !path) return FC5_E_ARG;
   NOT_SERVING(ctx);
   XorTable t;
@@ -310,9 +320,11 @@ extern "C" int fc5_table_save(fc5_ctx* ctx, const char* path) {
   std::string msg;
   if (!save_bin(path, t, msg)) FAIL(ctx, FC5_E_IO, msg);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_table_download(fc5_ctx* ctx, fc5_store_header* hdr, uint8_t* packed, size_t cap, size_t* len) {
+extern "C" int fc5_table_download(fc5_ctx* ctx, fc5_store_header* hdr, uint8_t* packed, size_t cap, size_t* len) try {
   if (!ctx || !len) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no table");
@@ -326,6 +338,8 @@ extern "C" int fc5_table_download(fc5_ctx* ctx, fc5_store_header* hdr, uint8_t* 
   if (rc != FC5_OK) return rc;
   std::memcpy(packed, t.packed.data(), want);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 static int device_query(fc5_ctx* ctx, const XorDev& x, const uint64_t* keys, size_t n, uint8_t* out) {
@@ -349,15 +363,17 @@ static int device_query(fc5_ctx* ctx, const XorDev& x, const uint64_t* keys, siz
   return rc;
 }
 
-extern "C" int fc5_table_query(fc5_ctx* ctx, const uint64_t* keys, size_t n, uint8_t* out) {
+extern "C" int fc5_table_query(fc5_ctx* ctx, const uint64_t* keys, size_t n, uint8_t* out) try {
   if (!ctx || (!keys && n) || (!out && n)) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no table");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   return device_query(ctx, ctx->main.dev, keys, n, out);
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_table_fpr_smoke(fc5_ctx* ctx, int trials, int* hits) {
+extern "C" int fc5_table_fpr_smoke(fc5_ctx* ctx, int trials, int* hits) try {
   if (!ctx || !hits || trials < 0) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no table");
@@ -372,22 +388,28 @@ extern "C" int fc5_table_fpr_smoke(fc5_ctx* ctx, int trials, int* hits) {
   for (u8 v : out) h += v;
   *hits = h;
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_side_build(fc5_ctx* ctx, const uint64_t* keys, size_t n, int r) {
+extern "C" int fc5_side_build(fc5_ctx* ctx, const uint64_t* keys, size_t n, int r) try {
   if (!ctx || (!keys && n) || r < 1 || r > 64) return FC5_E_ARG;
   NOT_SERVING(ctx);
   std::vector<u64> k(keys, keys + n);
   XorTable t;
   if (!build_xor_cpu(k, (u32)r, 0x51DE51DE51DEULL, 64, t)) FAIL(ctx, FC5_E_BUILD, "side table build stalled");
   return upload_table(ctx, t.hdr, t.packed.data(), t.packed.size(), ctx->side);
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_side_clear(fc5_ctx* ctx) {
+extern "C" int fc5_side_clear(fc5_ctx* ctx) try {
   if (!ctx) return FC5_E_ARG;
   NOT_SERVING(ctx);
   free_table(ctx->side);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 // ---------------------------------------------------------------------------
@@ -439,7 +461,7 @@ int fc5::ensure_hits(fc5_ctx* ctx) {
   return FC5_OK;
 }
 
-extern "C" int fc5_band_destroy(fc5_ctx* ctx, fc5_band* band) {
+extern "C" int fc5_band_destroy(fc5_ctx* ctx, fc5_band* band) try {
   if (!band) return FC5_OK;
   if (ctx) (void)hipSetDevice(ctx->device);
   if (band->graph_exec) (void)hipGraphExecDestroy(band->graph_exec);
@@ -448,6 +470,8 @@ extern "C" int fc5_band_destroy(fc5_ctx* ctx, fc5_band* band) {
     if (p) (void)hipFree(p);
   delete band;
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 DevUnit5 fc5::dev_unit5(const Cls5Unit& U, u32 uid, u32 band) {
@@ -552,7 +576,7 @@ int fc5::device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
 }
 
 extern "C" int fc5_pair_search(fc5_ctx* ctx, const uint64_t* targets, const uint32_t* hi, size_t n, uint32_t* out,
-                               size_t cap, size_t* n_out) {
+                               size_t cap, size_t* n_out) try {
   if (!ctx || !n_out || (n && (!targets || !hi)) || (cap && !out)) return FC5_E_ARG;
   NOT_SERVING(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -573,6 +597,8 @@ extern "C" int fc5_pair_search(fc5_ctx* ctx, const uint64_t* targets, const uint
     }
   *n_out = k;
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 // the band's launch segments: its class-5 items, then its class 2-4 items
@@ -707,14 +733,18 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
   return FC5_OK;
 }
 
-extern "C" int fc5_band_create(fc5_ctx* ctx, const fc5_unit* units, size_t n, fc5_band** out) {
+extern "C" int fc5_band_create(fc5_ctx* ctx, const fc5_unit* units, size_t n, fc5_band** out) try {
   return band_create(ctx, nullptr, units, n, out);
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 extern "C" int fc5_band_create_mixed(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units, size_t n,
-                                     fc5_band** out) {
+                                     fc5_band** out) try {
   if (!cls && n) return FC5_E_ARG;
   return band_create(ctx, cls, units, n, out);
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 static PlainArgs plain_args(fc5_ctx* ctx) {
@@ -917,7 +947,7 @@ static int ensure_graph(fc5_ctx* ctx, fc5_band* b, const BatchedArgs& a, int gri
   return FC5_OK;
 }
 
-extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res) {
+extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res) try {
   if (!ctx || !b || !res) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (mode != FC5_MODE_PLAIN && mode != FC5_MODE_BATCHED) FAIL(ctx, FC5_E_ARG, "unknown mode");
@@ -1073,9 +1103,11 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
     return FC5_E_OVERFLOW;
   }
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tune* out) {
+extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tune* out) try {
   if (!ctx || !b || !out || sample_every < 1) return FC5_E_ARG;
   NOT_SERVING(ctx);
   std::vector<fc5_unit> su;
@@ -1123,38 +1155,48 @@ extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tun
   ctx->rows_per_item = out->rows_per_item;
   ctx->blocks_per_cu = out->blocks_per_cu;
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_last_class_result(fc5_ctx* ctx, int cls, fc5_result* out) {
+extern "C" int fc5_last_class_result(fc5_ctx* ctx, int cls, fc5_result* out) try {
   if (!ctx || !out || cls < 2 || cls > 5) return FC5_E_ARG;
   *out = ctx->class_res[cls];
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_run_units(fc5_ctx* ctx, const fc5_unit* units, size_t n, int mode, fc5_result* res) {
+extern "C" int fc5_run_units(fc5_ctx* ctx, const fc5_unit* units, size_t n, int mode, fc5_result* res) try {
   fc5_band* b = nullptr;
   int rc = fc5_band_create(ctx, units, n, &b);
   if (rc != FC5_OK) return rc;
   rc = fc5_band_run(ctx, b, mode, res);
   fc5_band_destroy(ctx, b);
   return rc;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_last_hits(fc5_ctx* ctx, fc5_hit* out, size_t cap, size_t* n) {
+extern "C" int fc5_last_hits(fc5_ctx* ctx, fc5_hit* out, size_t cap, size_t* n) try {
   if (!ctx || !n || (!out && cap)) return FC5_E_ARG;
   *n = ctx->hits.size();
   std::copy_n(ctx->hits.begin(), std::min(cap, ctx->hits.size()), out);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_last_solutions(fc5_ctx* ctx, fc5_solution* out, size_t cap, size_t* n) {
+extern "C" int fc5_last_solutions(fc5_ctx* ctx, fc5_solution* out, size_t cap, size_t* n) try {
   if (!ctx || !n || (!out && cap)) return FC5_E_ARG;
   *n = ctx->sols.size();
   std::copy_n(ctx->sols.begin(), std::min(cap, ctx->sols.size()), out);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_units_calls(const fc5_unit* units, size_t n, uint64_t* calls) {
+extern "C" int fc5_units_calls(const fc5_unit* units, size_t n, uint64_t* calls) try {
   if ((!units && n) || !calls) return FC5_E_ARG;
   u64 c = 0;
   for (size_t i = 0; i < n; ++i) {
@@ -1163,10 +1205,12 @@ extern "C" int fc5_units_calls(const fc5_unit* units, size_t n, uint64_t* calls)
   }
   *calls = c;
   return FC5_OK;
+} catch (...) {
+  return abi_exception(nullptr);
 }
 
 extern "C" int fc5_emit_units(uint64_t lo, uint64_t hi, unsigned cls_mask, double lo_frac, double hi_frac,
-                              uint32_t* out_cls, uint64_t* out_B, uint64_t* out_u, size_t cap, size_t* n) {
+                              uint32_t* out_cls, uint64_t* out_B, uint64_t* out_u, size_t cap, size_t* n) try {
   if (!n || lo == 0 || hi < lo || (cap && (!out_cls || !out_B || !out_u))) return FC5_E_ARG;
   size_t k = 0;
   for (u64 B = lo; B <= hi; ++B) {
@@ -1185,9 +1229,11 @@ extern "C" int fc5_emit_units(uint64_t lo, uint64_t hi, unsigned cls_mask, doubl
   }
   *n = k;
   return FC5_OK;
+} catch (...) {
+  return abi_exception(nullptr);
 }
 
-extern "C" int fc5_build_pairs_file(int N, int r, const char* path, fc5_store_header* hdr_out) {
+extern "C" int fc5_build_pairs_file(int N, int r, const char* path, fc5_store_header* hdr_out) try {
   if (N < 1 || r < 1 || r > 64 || !path) return FC5_E_ARG;
   XorTable t;
   std::string msg;
@@ -1195,12 +1241,14 @@ extern "C" int fc5_build_pairs_file(int N, int r, const char* path, fc5_store_he
   if (!save_bin(path, t, msg)) return FC5_E_IO;
   if (hdr_out) std::memcpy(hdr_out, &t.hdr, sizeof *hdr_out);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(nullptr);
 }
 
 // ---------------------------------------------------------------------------
 // measurement
 // ---------------------------------------------------------------------------
-extern "C" int fc5_measure_gathers(fc5_ctx* ctx, uint64_t n, double* probe_ms, double* raw_ms) {
+extern "C" int fc5_measure_gathers(fc5_ctx* ctx, uint64_t n, double* probe_ms, double* raw_ms) try {
   if (!ctx || !probe_ms || !raw_ms || n == 0) return FC5_E_ARG;
   NOT_SERVING(ctx);
   const u32 policy = ctx->main.dev.policy;
@@ -1228,9 +1276,11 @@ extern "C" int fc5_measure_gathers(fc5_ctx* ctx, uint64_t n, double* probe_ms, d
   *probe_ms = a;
   *raw_ms = b;
   return rc;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_measure_copy(fc5_ctx* ctx, uint64_t bytes, double* ms) {
+extern "C" int fc5_measure_copy(fc5_ctx* ctx, uint64_t bytes, double* ms) try {
   if (!ctx || !ms || bytes < 8) return FC5_E_ARG;
   NOT_SERVING(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1256,4 +1306,6 @@ extern "C" int fc5_measure_copy(fc5_ctx* ctx, uint64_t bytes, double* ms) {
   (void)hipFree(dst);
   *ms = t;
   return rc;
+} catch (...) {
+  return abi_exception(ctx);
 }

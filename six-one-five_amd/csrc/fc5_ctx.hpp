@@ -8,6 +8,7 @@
 
 #include <functional>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -159,6 +160,28 @@ struct fc5_ctx {
   // resident serving loop (fc5_serve.cpp); while set, only fc5_serve_* may run
   struct fc5_server* server = nullptr;
 };
+
+// No C++ exception may cross the C ABI: every entry point catches what its
+// body throws (a failed host allocation, above all) and turns it into an
+// error code, with the text in the context when there is one.
+inline int abi_exception(fc5_ctx* ctx) noexcept {
+  int rc = FC5_E_HIP;
+  const char* what = "internal error";
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    rc = FC5_E_NOMEM;
+    what = "host allocation failed";
+  } catch (...) {
+  }
+  if (ctx) {
+    try {
+      ctx->err = what;
+    } catch (...) {
+    }
+  }
+  return rc;
+}
 
 #define NOT_SERVING(ctx)                                                              \
   do {                                                                                \

@@ -34,7 +34,7 @@ void fc5::free_exact(fc5_ctx* ctx) {
   ctx->exact_shift = 0;
 }
 
-extern "C" int fc5_exact_build(fc5_ctx* ctx, int N, const uint64_t* extra, size_t n_extra) {
+extern "C" int fc5_exact_build(fc5_ctx* ctx, int N, const uint64_t* extra, size_t n_extra) try {
   if (!ctx) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (N < 1 || N > kExactMaxN) FAIL(ctx, FC5_E_ARG, "exact table N must be 1..262143");
@@ -85,15 +85,19 @@ extern "C" int fc5_exact_build(fc5_ctx* ctx, int N, const uint64_t* extra, size_
   ctx->exact_N = N;
   ctx->exact_shift = s;
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_exact_clear(fc5_ctx* ctx) {
+extern "C" int fc5_exact_clear(fc5_ctx* ctx) try {
   if (!ctx) return FC5_E_ARG;
   NOT_SERVING(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   free_exact(ctx);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 extern "C" int fc5_exact_info(const fc5_ctx* ctx, uint64_t* n_entries, int* N, uint32_t* shift) {
@@ -104,7 +108,7 @@ extern "C" int fc5_exact_info(const fc5_ctx* ctx, uint64_t* n_entries, int* N, u
   return FC5_OK;
 }
 
-extern "C" int fc5_exact_read(fc5_ctx* ctx, uint64_t first, size_t n, uint64_t* out) {
+extern "C" int fc5_exact_read(fc5_ctx* ctx, uint64_t first, size_t n, uint64_t* out) try {
   if (!ctx || (n && !out)) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (first > ctx->n_exact || n > ctx->n_exact - first) FAIL(ctx, FC5_E_ARG, "entries out of range");
@@ -112,6 +116,8 @@ extern "C" int fc5_exact_read(fc5_ctx* ctx, uint64_t first, size_t n, uint64_t* 
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipMemcpy(out, ctx->d_exact + first, n * sizeof(u128), hipMemcpyDeviceToHost));
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 int fc5::exact_table_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
@@ -152,7 +158,7 @@ int fc5::exact_table_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
 }
 
 extern "C" int fc5_exact_lookup(fc5_ctx* ctx, const uint64_t* targets, const uint32_t* hi, size_t n, uint32_t* out,
-                                size_t cap, size_t* n_out) {
+                                size_t cap, size_t* n_out) try {
   if (!ctx || !n_out || (n && (!targets || !hi)) || (cap && !out)) return FC5_E_ARG;
   NOT_SERVING(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -177,4 +183,6 @@ extern "C" int fc5_exact_lookup(fc5_ctx* ctx, const uint64_t* targets, const uin
   }
   *n_out = k;
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }

@@ -472,7 +472,7 @@ void fc5::serve_shutdown(fc5_ctx* ctx) {
   ctx->server = nullptr;
 }
 
-extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) {
+extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) try {
   if (!ctx) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (!ctx->main.present || !ctx->main.hdr.N) FAIL(ctx, FC5_E_STATE, "serving needs a pair table with its N");
@@ -535,10 +535,12 @@ extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) {
   ctx->server = S.release();
   ctx->server->th = std::thread(serve_loop, ctx->server);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 extern "C" int fc5_serve_submit(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units, size_t n,
-                                uint64_t* band_id) {
+                                uint64_t* band_id) try {
   if (!ctx || !band_id || (!units && n)) return FC5_E_ARG;
   fc5_server* S = ctx->server;
   if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
@@ -598,10 +600,12 @@ extern "C" int fc5_serve_submit(fc5_ctx* ctx, const uint32_t* cls, const fc5_uni
   S->submitted++;
   S->cv_work.notify_all();
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 extern "C" int fc5_serve_submit_range(fc5_ctx* ctx, uint64_t lo, uint64_t hi, unsigned cls_mask, double lo_frac,
-                                      double hi_frac, uint64_t* band_id) {
+                                      double hi_frac, uint64_t* band_id) try {
   if (!ctx || !band_id) return FC5_E_ARG;
   fc5_server* S = ctx->server;
   if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
@@ -613,6 +617,8 @@ extern "C" int fc5_serve_submit_range(fc5_ctx* ctx, uint64_t lo, uint64_t hi, un
     if (rc != FC5_OK) return rc;
   }
   return fc5_serve_submit(ctx, cls.data(), units.data(), units.size(), band_id);
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 static int band_result(fc5_ctx* ctx, SBand* B, fc5_result* res) {
@@ -623,7 +629,7 @@ static int band_result(fc5_ctx* ctx, SBand* B, fc5_result* res) {
   return B->rc;
 }
 
-extern "C" int fc5_serve_poll(fc5_ctx* ctx, uint64_t band_id, fc5_result* res) {
+extern "C" int fc5_serve_poll(fc5_ctx* ctx, uint64_t band_id, fc5_result* res) try {
   if (!ctx) return FC5_E_ARG;
   fc5_server* S = ctx->server;
   if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
@@ -631,9 +637,11 @@ extern "C" int fc5_serve_poll(fc5_ctx* ctx, uint64_t band_id, fc5_result* res) {
   SBand* B = find_band(S, band_id);
   if (!B) FAIL(ctx, FC5_E_STATE, "unknown band");
   return band_result(ctx, B, res);
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_serve_wait(fc5_ctx* ctx, uint64_t band_id, double timeout_s, fc5_result* res) {
+extern "C" int fc5_serve_wait(fc5_ctx* ctx, uint64_t band_id, double timeout_s, fc5_result* res) try {
   if (!ctx) return FC5_E_ARG;
   fc5_server* S = ctx->server;
   if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
@@ -644,9 +652,11 @@ extern "C" int fc5_serve_wait(fc5_ctx* ctx, uint64_t band_id, double timeout_s, 
   if (timeout_s < 0) S->cv_done.wait(lk, ready);
   else S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready);
   return band_result(ctx, B, res);
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_serve_class_result(fc5_ctx* ctx, uint64_t band_id, int cls, fc5_result* out) {
+extern "C" int fc5_serve_class_result(fc5_ctx* ctx, uint64_t band_id, int cls, fc5_result* out) try {
   if (!ctx || !out || cls < 2 || cls > 5) return FC5_E_ARG;
   fc5_server* S = ctx->server;
   if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
@@ -656,9 +666,11 @@ extern "C" int fc5_serve_class_result(fc5_ctx* ctx, uint64_t band_id, int cls, f
   if (B->state != kDone) return FC5_E_PENDING;
   *out = B->cls_res[cls];
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_serve_hits(fc5_ctx* ctx, uint64_t band_id, fc5_hit* out, size_t cap, size_t* n) {
+extern "C" int fc5_serve_hits(fc5_ctx* ctx, uint64_t band_id, fc5_hit* out, size_t cap, size_t* n) try {
   if (!ctx || !n) return FC5_E_ARG;
   fc5_server* S = ctx->server;
   if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
@@ -669,9 +681,11 @@ extern "C" int fc5_serve_hits(fc5_ctx* ctx, uint64_t band_id, fc5_hit* out, size
   *n = B->hits.size();
   if (out) std::copy_n(B->hits.begin(), std::min(cap, B->hits.size()), out);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_serve_solutions(fc5_ctx* ctx, uint64_t band_id, fc5_solution* out, size_t cap, size_t* n) {
+extern "C" int fc5_serve_solutions(fc5_ctx* ctx, uint64_t band_id, fc5_solution* out, size_t cap, size_t* n) try {
   if (!ctx || !n) return FC5_E_ARG;
   fc5_server* S = ctx->server;
   if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
@@ -682,9 +696,11 @@ extern "C" int fc5_serve_solutions(fc5_ctx* ctx, uint64_t band_id, fc5_solution*
   *n = B->sols.size();
   if (out) std::copy_n(B->sols.begin(), std::min(cap, B->sols.size()), out);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_serve_timing_of(fc5_ctx* ctx, uint64_t band_id, fc5_serve_timing* out) {
+extern "C" int fc5_serve_timing_of(fc5_ctx* ctx, uint64_t band_id, fc5_serve_timing* out) try {
   if (!ctx || !out) return FC5_E_ARG;
   fc5_server* S = ctx->server;
   if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
@@ -693,9 +709,11 @@ extern "C" int fc5_serve_timing_of(fc5_ctx* ctx, uint64_t band_id, fc5_serve_tim
   if (!B) FAIL(ctx, FC5_E_STATE, "unknown band");
   *out = fc5_serve_timing{B->t_submit, B->t_admit, B->t_done, B->rounds};
   return B->state == kDone ? FC5_OK : FC5_E_PENDING;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_serve_release(fc5_ctx* ctx, uint64_t band_id) {
+extern "C" int fc5_serve_release(fc5_ctx* ctx, uint64_t band_id) try {
   if (!ctx) return FC5_E_ARG;
   fc5_server* S = ctx->server;
   if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
@@ -705,9 +723,11 @@ extern "C" int fc5_serve_release(fc5_ctx* ctx, uint64_t band_id) {
   if (B->state != kDone) FAIL(ctx, FC5_E_PENDING, "band not finished");
   S->bands.erase(band_id);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_serve_stats_get(fc5_ctx* ctx, fc5_serve_stats* out) {
+extern "C" int fc5_serve_stats_get(fc5_ctx* ctx, fc5_serve_stats* out) try {
   if (!ctx || !out) return FC5_E_ARG;
   fc5_server* S = ctx->server;
   if (!S) FAIL(ctx, FC5_E_STATE, "no server running");
@@ -717,12 +737,16 @@ extern "C" int fc5_serve_stats_get(fc5_ctx* ctx, fc5_serve_stats* out) {
   *out = fc5_serve_stats{S->submitted, S->admitted, S->done, (u64)S->pending.size(), active, S->rounds,
                          S->a5.used, S->a3.used, S->device_bytes, S->busy_ms};
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_serve_stop(fc5_ctx* ctx) {
+extern "C" int fc5_serve_stop(fc5_ctx* ctx) try {
   if (!ctx) return FC5_E_ARG;
   if (!ctx->server) FAIL(ctx, FC5_E_STATE, "no server running");
   const int rc = ctx->server->fail_rc;
   serve_shutdown(ctx);
   return rc;
+} catch (...) {
+  return abi_exception(ctx);
 }

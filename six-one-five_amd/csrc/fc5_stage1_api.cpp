@@ -155,7 +155,7 @@ int fc5::stage1_emit(fc5_ctx* ctx, hipStream_t s, u64 lo, u64 hi, unsigned mask,
 
 extern "C" int fc5_emit_units_device(fc5_ctx* ctx, uint64_t lo, uint64_t hi, unsigned cls_mask, double lo_frac,
                                      double hi_frac, uint32_t* out_cls, uint64_t* out_B, uint64_t* out_u, size_t cap,
-                                     size_t* n) {
+                                     size_t* n) try {
   if (!ctx || !n || (cap && (!out_cls || !out_B || !out_u))) return FC5_E_ARG;
   NOT_SERVING(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -170,10 +170,12 @@ extern "C" int fc5_emit_units_device(fc5_ctx* ctx, uint64_t lo, uint64_t hi, uns
     out_u[i] = units[i].u;
   }
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 extern "C" int fc5_band_create_range(fc5_ctx* ctx, uint64_t lo, uint64_t hi, unsigned cls_mask, double lo_frac,
-                                     double hi_frac, fc5_band** out) {
+                                     double hi_frac, fc5_band** out) try {
   if (!ctx || !out) return FC5_E_ARG;
   NOT_SERVING(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -182,4 +184,6 @@ extern "C" int fc5_band_create_range(fc5_ctx* ctx, uint64_t lo, uint64_t hi, uns
   const int rc = stage1_emit(ctx, ctx->stream, lo, hi, cls_mask, lo_frac, hi_frac, cls, units);
   if (rc != FC5_OK) return rc;
   return fc5_band_create_mixed(ctx, cls.data(), units.data(), units.size(), out);
+} catch (...) {
+  return abi_exception(ctx);
 }

@@ -298,16 +298,18 @@ void fc5::free_twin(fc5_ctx* ctx) {
   ctx->d_twin_cap = 0;
 }
 
-extern "C" int fc5_twin_cands(uint64_t B, double u_lo, double u_hi, fc5_twin_cand* out, size_t cap, size_t* n) {
+extern "C" int fc5_twin_cands(uint64_t B, double u_lo, double u_hi, fc5_twin_cand* out, size_t cap, size_t* n) try {
   if (!n || (!out && cap)) return FC5_E_ARG;
   std::vector<TwinCandRaw> v;
   twin_cands(B, u_lo, u_hi, v);
   *n = v.size();
   for (size_t i = 0; i < v.size() && i < cap; ++i) out[i] = abi_of(v[i]);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(nullptr);
 }
 
-extern "C" int fc5_twin_run(fc5_ctx* ctx, const fc5_twin_cand* cands, size_t n, fc5_twin_result* res) {
+extern "C" int fc5_twin_run(fc5_ctx* ctx, const fc5_twin_cand* cands, size_t n, fc5_twin_result* res) try {
   if (!ctx || !res || (!cands && n)) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no pair table loaded");
@@ -333,10 +335,12 @@ extern "C" int fc5_twin_run(fc5_ctx* ctx, const fc5_twin_cand* cands, size_t n, 
   const int rc = twin_verify(ctx, all);
   if (rc != FC5_OK) return rc;
   return finish(ctx, res, t0);
+} catch (...) {
+  return abi_exception(ctx);
 }
 
 extern "C" int fc5_twin_run_range(fc5_ctx* ctx, uint64_t B_lo, uint64_t B_hi, double u_lo, double u_hi,
-                                  fc5_twin_result* res) {
+                                  fc5_twin_result* res) try {
   if (!ctx || !res || B_lo > B_hi) return FC5_E_ARG;
   if (B_hi > 2353973) FAIL(ctx, FC5_E_ARG, "B_max capped at 2353973 (B^6 < 2^127)");
   std::vector<fc5_twin_cand> cands;
@@ -348,18 +352,24 @@ extern "C" int fc5_twin_run_range(fc5_ctx* ctx, uint64_t B_lo, uint64_t B_hi, do
     for (const TwinCandRaw& r : v) cands.push_back(abi_of(r));
   }
   return fc5_twin_run(ctx, cands.data(), cands.size(), res);
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_twin_last_hits(fc5_ctx* ctx, fc5_twin_hit* out, size_t cap, size_t* n) {
+extern "C" int fc5_twin_last_hits(fc5_ctx* ctx, fc5_twin_hit* out, size_t cap, size_t* n) try {
   if (!ctx || !n) return FC5_E_ARG;
   *n = ctx->twin_hits.size();
   if (out) std::copy_n(ctx->twin_hits.begin(), std::min(cap, ctx->twin_hits.size()), out);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
 
-extern "C" int fc5_twin_last_solutions(fc5_ctx* ctx, fc5_twin_solution* out, size_t cap, size_t* n) {
+extern "C" int fc5_twin_last_solutions(fc5_ctx* ctx, fc5_twin_solution* out, size_t cap, size_t* n) try {
   if (!ctx || !n) return FC5_E_ARG;
   *n = ctx->twin_sols.size();
   if (out) std::copy_n(ctx->twin_sols.begin(), std::min(cap, ctx->twin_sols.size()), out);
   return FC5_OK;
+} catch (...) {
+  return abi_exception(ctx);
 }
