@@ -190,7 +190,7 @@ int fc5_units_calls(const fc5_unit *units, size_t n, uint64_t *calls);
 /* exact check of filter survivors on the device (replaces PairRecover + the
  * exact compare of the host verify, fourcore_xor_store.hpp:23-51,
  * fourcore_cls5_gpu_v4.cu:517-522): for each 128-bit target T_i (targets[2i] low,
- * targets[2i+1] high word) every pair a <= b <= hi[i] with a^6 + b^6 = T_i.  out
+ * targets[2i+1] high word, T_i < 2^127) every pair a <= b <= hi[i] with a^6 + b^6 = T_i.  out
  * gets (i, a, b) triples (up to cap, sorted), *n_out the total.  Every run's
  * hit verification goes through this search. */
 int fc5_pair_search(fc5_ctx *ctx, const uint64_t *targets, const uint32_t *hi, size_t n, uint32_t *out,

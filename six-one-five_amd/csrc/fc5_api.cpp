@@ -581,7 +581,11 @@ extern "C" int fc5_pair_search(fc5_ctx* ctx, const uint64_t* targets, const uint
   NOT_SERVING(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   std::vector<PairTarget> t(n);
-  for (size_t i = 0; i < n; ++i) t[i] = PairTarget{((u128)targets[2 * i + 1] << 64) | targets[2 * i], hi[i], true};
+  for (size_t i = 0; i < n; ++i) {
+    // the device's sixth roots stay exact below 2^127 (every hit's target is below 2^118)
+    if (targets[2 * i + 1] >> 63) FAIL(ctx, FC5_E_ARG, "pair search targets must be below 2^127");
+    t[i] = PairTarget{((u128)targets[2 * i + 1] << 64) | targets[2 * i], hi[i], true};
+  }
   std::vector<std::vector<std::pair<u32, u32>>> pairs;
   const int rc = device_pairs(ctx, t, pairs);
   if (rc != FC5_OK) return rc;

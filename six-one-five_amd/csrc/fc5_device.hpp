@@ -67,7 +67,9 @@ __device__ __forceinline__ u128 p6_u128(u32 v) {
   return (u128)v3 * v3;
 }
 
-__device__ __forceinline__ u32 iroot6_u128(u128 X) {  // largest a with a^6 <= X (X < 2^126)
+// largest a with a^6 <= X, for X < 2^127: the root is then below 2.36e6, so
+// (a+1)^3 stays below 2^64 in the fix-up loops
+__device__ __forceinline__ u32 iroot6_u128(u128 X) {
   const double x = (double)(u64)(X >> 64) * 18446744073709551616.0 + (double)(u64)X;
   u32 a = (u32)cbrt(sqrt(x));
   while (p6_u128(a + 1) <= X) ++a;

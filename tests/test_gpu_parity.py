@@ -458,6 +458,9 @@ def test_device_pair_search_exact(ctx):
     for i, a, b in got:
         assert a <= b and a ** 6 + b ** 6 == big[i]
     assert (1, 2 ** 21 - 1, 2 ** 21 - 1) in got and (2, 77, 2 ** 21 - 5) in got
+    with pytest.raises(fc5.Fc5Error) as e:  # outside the exact range of the device sixth root
+        ctx.pair_search([2 ** 127], [2 ** 21])
+    assert e.value.code == fc5.E_ARG
 
 
 def brute_pairs(T, hi):
