@@ -159,6 +159,7 @@ static int upload_table(fc5_ctx* ctx, const StoreHeader& h, const u8* packed, si
   HIPCHK(ctx, hipSetDevice(ctx->device));
   free_table(dst);
   dst.hdr = h;
+  if (h.m_cells > (1ull << 40)) FAIL(ctx, FC5_E_FORMAT, "cell count out of range");  // 8.8 TB of cells
   const size_t want = packed_bytes(h.m_cells, h.r);
   if (len < want) FAIL(ctx, FC5_E_FORMAT, "packed length shorter than the header implies");
   if (h.n_keys == 0 || h.m_cells < 3) {  // empty filter: every query answers "no"

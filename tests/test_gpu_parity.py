@@ -184,6 +184,20 @@ def test_table_load_errors(ctx, tmp_path):
     with pytest.raises(fc5.Fc5Error) as e:
         ctx.table_load(str(tmp_path / "missing.bin"), 300)
     assert e.value.code == fc5.E_IO
+    # damaged headers are refused before any allocation or copy
+    raw = open(p, "rb").read()
+    damaged = str(tmp_path / "damaged.bin")
+    open(damaged, "wb").write(raw[:32] + (1 << 61).to_bytes(8, "little") + raw[40:])
+    with pytest.raises(fc5.Fc5Error) as e:
+        ctx.table_load(damaged, 300)
+    assert e.value.code == fc5.E_IO
+    hdr = fc5.StoreHeader.from_buffer_copy(raw[:64])
+    for m in (1 << 61, 1 << 41):
+        hdr.m_cells = m
+        with pytest.raises(fc5.Fc5Error) as e:
+            ctx.table_upload(hdr, raw[64:])
+        assert e.value.code == fc5.E_FORMAT, m
+    ctx.table_load(p, 300)  # the context still works
 
 
 def test_band_1000001_1020000_full_calls_and_64_plants(ctx):
