@@ -618,22 +618,28 @@ static int upload_segments(fc5_ctx* ctx, fc5_band* b) {
   return FC5_OK;
 }
 
-void fc5::number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3) {
-  n5 = 0;
+bool fc5::number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3) {
+  u64 t5 = 0, t3 = 0;
   for (DevUnit5& D : dev) {
-    D.item_base = n5;
-    n5 += (D.rows + (u32)E - 1) / (u32)E;
+    D.item_base = (u32)t5;
+    t5 += (D.rows + (u32)E - 1) / (u32)E;
+    if (t5 > kMaxItems) return false;
   }
-  n3 = 0;
   for (DevUnit3& D : dev3) {
-    D.item_base = n3;
-    n3 += (D.rows + (u32)E - 1) / (u32)E;
+    D.item_base = (u32)t3;
+    t3 += (D.rows + (u32)E - 1) / (u32)E;
+    if (t3 > kMaxItems) return false;
   }
+  if (t5 + t3 > kMaxItems) return false;
+  n5 = (u32)t5;
+  n3 = (u32)t3;
+  return true;
 }
 
 static int renumber(fc5_ctx* ctx, fc5_band* b) {
   if (b->rows == ctx->rows_per_item) return FC5_OK;
-  number_items(b->dev_h, b->dev3_h, ctx->rows_per_item, b->n_items, b->n_items3);
+  if (!number_items(b->dev_h, b->dev3_h, ctx->rows_per_item, b->n_items, b->n_items3))
+    FAIL(ctx, FC5_E_ARG, kTooManyItems);
   if (!b->dev_h.empty())
     HIPCHK(ctx, hipMemcpy(b->d_units, b->dev_h.data(), b->dev_h.size() * sizeof(DevUnit5), hipMemcpyHostToDevice));
   if (!b->dev3_h.empty())
@@ -714,7 +720,7 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
   b->n_dev = (u32)dev.size();
   b->n_plain = (u32)plain.size();
   b->n_dev3 = (u32)dev3.size();
-  number_items(dev, dev3, ctx->rows_per_item, b->n_items, b->n_items3);
+  if (!number_items(dev, dev3, ctx->rows_per_item, b->n_items, b->n_items3)) FAIL(ctx, FC5_E_ARG, kTooManyItems);
   b->rows = ctx->rows_per_item;
   int rc = FC5_OK;
   auto upload = [&](void** dst, const void* src, size_t bytes) {

@@ -213,8 +213,13 @@ int ensure_ndg(fc5_ctx* ctx, u32 dmax);
 int ensure_hits(fc5_ctx* ctx);
 DevUnit5 dev_unit5(const Cls5Unit& U, u32 uid, u32 band);
 DevUnit3 dev_unit3(const Cls234Unit& U, u32 uid, u32 band);
-// work items: ceil(rows / E) per unit, numbered per arena
-void number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3);
+// work items: ceil(rows / E) per unit, numbered per arena.  A launch takes
+// items from a 32-bit counter that every wave bumps once more before it
+// stops, so a band holds at most kMaxItems; false when it holds more.
+constexpr u64 kMaxItems = (1ull << 32) - (1ull << 20);
+constexpr const char* kTooManyItems =
+    "band too large for one launch (more than 2^32 - 2^20 work items at this rows_per_item): split it";
+bool number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3);
 // serving loop (fc5_serve.cpp): stops and frees the server of a context
 void serve_shutdown(fc5_ctx* ctx);
 // Stage-1 units of classes in mask (bits 2..5) for B in [lo, hi] on the device,

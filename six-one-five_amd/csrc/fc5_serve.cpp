@@ -322,6 +322,9 @@ int run_round(fc5_server* S, std::vector<SBand*>& active) {
       left -= n5 * B->cost5;
     }
     if (r3 && (left > 0 || !n5)) n3 = (u32)std::min<double>(r3, std::max(1.0, left / B->cost3));
+    // one launch's items come from a 32-bit counter (kMaxItems per launch)
+    n5 = (u32)std::min<u64>(n5, kMaxItems - total);
+    n3 = (u32)std::min<u64>(n3, kMaxItems - total - n5);
     if (n5) {
       S->h_segs[n_segs++] = Segment{total, B->next5, (u32)B->off5, (u32)B->dev5.size(), 0, 0};
       total += n5;
@@ -581,7 +584,7 @@ extern "C" int fc5_serve_submit(fc5_ctx* ctx, const uint32_t* cls, const fc5_uni
   }
   if (B->dev5.size() > S->a5.cap || B->dev3.size() > S->a3.cap)
     FAIL(ctx, FC5_E_NOMEM, "band larger than the resident unit arenas");
-  number_items(B->dev5, B->dev3, S->rows, B->items5, B->items3);
+  if (!number_items(B->dev5, B->dev3, S->rows, B->items5, B->items3)) FAIL(ctx, FC5_E_ARG, kTooManyItems);
   u64 calls5 = 0, rows3 = 0, lim3 = 0;
   for (const Cls5Unit& U : B->units5) calls5 += cls5_calls(U);
   for (const DevUnit3& D : B->dev3) {

@@ -634,3 +634,22 @@ def test_mixed_ragged_lines_match_oracle(ctx, mode):
     assert out.result["units_skipped"] >= 3
     assert out.result["units_irregular"] > 0
     assert sum(oc[k]["fail_t"] for k in (2, 3, 4)) > 0 and len(oh) > 0
+
+
+def test_band_item_count_limit(ctx):
+    """a band whose work items would overflow the launch's 32-bit item counter is
+    refused when it is created, not run with wrapped item numbers"""
+    say("work-item limit")
+    _, Bs, us = fc5.emit_units(29000003, 29000003)  # a regular unit with 17052 e-rows
+    n = 1100000
+    units = fc5.units_array(np.repeat(Bs[:1], n), np.repeat(us[:1], n))
+    ctx.table_build(300, 48)
+    ctx.set_option("rows_per_item", 4)  # 4263 items per unit: 4.7e9 in all
+    try:
+        with pytest.raises(fc5.Fc5Error) as e:
+            ctx.band(units)
+        assert e.value.code == fc5.E_ARG and "work items" in str(e.value)
+        ctx.set_option("rows_per_item", 8)  # 2132 items per unit: 2.3e9, accepted (not run)
+        ctx.band(units).close()
+    finally:
+        ctx.set_option("rows_per_item", 8)
