@@ -115,7 +115,7 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
     items += D.rows;
     max_lim = std::max(max_lim, C.lim);
   }
-  if (items >= 0xffffffffull) FAIL(ctx, FC5_E_ARG, "too many twin work items in one launch");
+  if (items > kMaxItems) FAIL(ctx, FC5_E_ARG, "too many twin work items in one launch (more than 2^32 - 2^20)");
   int rc = ensure_twin(ctx, dev.size());
   if (rc != FC5_OK) return rc;
   if ((rc = ensure_c6g(ctx, max_lim + 64)) != FC5_OK) return rc;
@@ -324,13 +324,22 @@ extern "C" int fc5_twin_run(fc5_ctx* ctx, const fc5_twin_cand* cands, size_t n, 
     if (!pack_twin(raw_of(cands[i]), all[i], why))
       FAIL(ctx, FC5_E_ARG, "candidate " + std::to_string(i) + ": " + why);
   }
-  for (size_t s = 0; s < n; s += kTwinBatch) {
-    const size_t e = std::min(n, s + kTwinBatch);
+  // launches of at most kTwinBatch candidates and kMaxItems work items each
+  for (size_t s = 0; s < n;) {
+    size_t e = s;
+    u64 items = 0;
+    while (e < n && e - s < kTwinBatch) {
+      const u64 rows = (u64)all[e].nres * (all[e].fmax1 / 64 + 1);
+      if (e > s && items + rows > kMaxItems) break;
+      items += rows;
+      ++e;
+    }
     std::vector<TwinCand> part(all.begin() + (long)s, all.begin() + (long)e);
     std::vector<u32> uid(e - s);
     for (size_t i = s; i < e; ++i) uid[i - s] = (u32)i;
     const int rc = twin_launch(ctx, part, uid, res);
     if (rc != FC5_OK) return rc;
+    s = e;
   }
   const int rc = twin_verify(ctx, all);
   if (rc != FC5_OK) return rc;
