@@ -38,6 +38,16 @@
  *   host verify      fourcore_cls5_gpu_v4.cu:513-542, fourcore_xor_store.hpp:23-51
  *       pinned by: synthetic exact identities in Python (a fabricated unit
  *       whose residual is a real pair sum) -- no (6,1,5) solution is known.
+ *   (6,1,7) twin: candidates, tri sets, k_find4_cls5 loops, verify
+ *       solve_617_v1.cu:334-346,563-657,807-906,1080-1110
+ *       pinned by: candidate lists and triple-sum sets from the reference's
+ *       own generator compiled into oracle/_ref/ref_617 (golden SHA-256 and
+ *       lists); every class-5 row of the reference's known-solution list
+ *       found by whole-B runs; a Python big-integer restatement of the loops
+ *       (all counters and the maybe-hit list, both c4 bounds, gates on/off).
+ * The exact sorted pair table (oracle.py: exact_table, quad_sum.hpp:8-45) is
+ * pinned by the reference's own QuadSumIndex compiled into oracle/_ref/ref_quad
+ * (entry lines and SHA-256 for small N, equal_sum_range of sampled targets).
  */
 #include "fc5_oracle.h"
 
