@@ -227,7 +227,12 @@ int fc5_exact_lookup(fc5_ctx *ctx, const uint64_t *targets, const uint32_t *hi, 
  * grid term 21d (the odd term).  Options: "use_gate" (shared; --no-gate),
  * "twin_tri_gate" (1; 0 = --no-tri-gate), "twin_c4_div" (3 = the reference
  * kernel's c4 lower bound 3 c4^6 >= T, solve_617_v1.cu:587; 4 = the exact
- * bound 4 c4^6 >= T for the largest of four cores), "hit_cap", "verify". */
+ * bound 4 c4^6 >= T for the largest of four cores), "twin_ref_tiles" (0 =
+ * every c4 searched once; 2048 = the reference launch's c4 tiles, which start
+ * at lo4 + 2048 z for z < zmax <= 32 and have no end, solve_617_v1.cu:607-608,
+ * 961-979, so a c4 at offset k from lo4 is searched min(32, k/2048 + 1) times
+ * and its counts and hits repeat as in the reference; other widths for tests),
+ * "hit_cap", "verify". */
 
 /* candidate (GpuCand of class 5, solve_617_v1.cu:354-363): base = B^6 - u^6 - w^6
  * (< 2^127), grid d = res[i] + mod1 * k in [1, fmax1], cores <= lim.  mod1 must be

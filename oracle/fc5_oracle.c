@@ -1195,7 +1195,7 @@ static u32 min_c_bound(u128 R, u64 k) {
 
 int or_twin_run(const or_twin_cand_t *cands, int n, const or_xor_t *f, int use_gate, int use_tri,
                 int c4_div, or_twin_counts_t *counts, or_twin_hit_t *hits, u64 cap, u64 *n_hits,
-                int threads) {
+                int threads, int tile) {
   or_gate_t g;
   or_build_gate(&g);
   u64 t504[8], t247[4];
@@ -1208,6 +1208,18 @@ int or_twin_run(const or_twin_cand_t *cands, int n, const or_xor_t *f, int use_g
   u64 jobs = 0, c4t = 0, c4s = 0, calls = 0, gated = 0, probes = 0, maybe = 0, nh = 0;
   if (threads < 1) threads = 1;
   if (c4_div != 4) c4_div = 3;
+  /* tile > 0: the reference launch's c4 tiles, zmax = ceil(max lim / tile) blocks
+   * capped at 32 (solve_617_v1.cu:961-979); block z starts at lo4 + z * tile and
+   * runs to hi4 with no tile end (solve_617_v1.cu:607-608) */
+  u32 zmax = 1;
+  if (tile > 0) {
+    u32 max_lim = 1;
+    for (int ci = 0; ci < n; ++ci)
+      if (cands[ci].lim > max_lim) max_lim = cands[ci].lim;
+    zmax = (max_lim + (u32)tile - 1) / (u32)tile;
+    if (zmax < 1) zmax = 1;
+    if (zmax > 32) zmax = 32;
+  }
   /* one work row per (cand, t); rows of a cand are its blockIdx.y values */
   u64 *row0 = (u64 *)malloc(sizeof(u64) * (size_t)(n + 1));
   row0[0] = 0;
@@ -1240,7 +1252,9 @@ int or_twin_run(const or_twin_cand_t *cands, int n, const or_xor_t *f, int use_g
       ++jobs;
       const u32 Q504 = (u32)((u64)(R % (u128)m504) / (u64)M42 % 504ULL);
       const u32 Q247 = (u32)((u64)(R % (u128)247) * inv42_247 % 247ULL);
-      for (u32 c4 = lo4; c4 <= hi4; ++c4) {
+      for (u32 z = 0; z < zmax; ++z)
+      for (u64 c4w = (u64)lo4 + (u64)z * (u64)(tile > 0 ? tile : 0); c4w <= hi4; ++c4w) {
+        const u32 c4 = (u32)c4w;
         if (use_tri) {
           ++c4t;
           u32 tr504 = Q504 + 504u - g.p6504[c4 % 504];

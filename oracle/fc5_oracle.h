@@ -176,12 +176,15 @@ typedef struct {
   uint64_t maybe;           /* maybe-hits of the table */
 } or_twin_counts_t;
 typedef struct { uint32_t cand, d, c4, c3; uint64_t fp; } or_twin_hit_t;
-/* k_find4_cls5 semantics (solve_617_v1.cu:563-657) over every (cand, d, c4)
- * once.  c4_div: 3 = the reference kernel's c4 lower bound (3 c4^6 >= T),
- * 4 = the exact bound for the largest of four cores (4 c4^6 >= T). */
+/* k_find4_cls5 semantics (solve_617_v1.cu:563-657).  c4_div: 3 = the reference
+ * kernel's c4 lower bound (3 c4^6 >= T), 4 = the exact bound for the largest of
+ * four cores (4 c4^6 >= T).  tile = 0 visits every (cand, d, c4) once; tile > 0
+ * restates the reference launch's c4 tiles of that width (2048 in the
+ * reference): block z starts at lo4 + z * tile with no tile end, so a c4 at
+ * offset k from lo4 is visited min(zmax, k / tile + 1) times. */
 int or_twin_run(const or_twin_cand_t *cands, int n, const or_xor_t *f, int use_gate, int use_tri,
                 int c4_div, or_twin_counts_t *counts, or_twin_hit_t *hits, uint64_t cap,
-                uint64_t *n_hits, int threads);
+                uint64_t *n_hits, int threads, int tile);
 /* expand_xor_hits + verify_hit cls5 (solve_617_v1.cu:1019-1110): writes up to cap
  * solutions as 7 sorted terms + gcd; returns the count (before deduplication) */
 int or_twin_verify(const or_twin_cand_t *C, uint32_t d, uint32_t c4, uint32_t c3, uint64_t fp,

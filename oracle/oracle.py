@@ -297,7 +297,7 @@ class TwinCounts(C.Structure):
 _lib.or_twin_cands.argtypes = [C.c_int64, C.c_double, C.c_double, _P, C.c_int]
 _lib.or_build_tri.argtypes = [_P, _P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
 _lib.or_twin_run.argtypes = [_P, C.c_int, C.POINTER(Xor), C.c_int, C.c_int, C.c_int, C.POINTER(TwinCounts), _P,
-                             C.c_uint64, C.POINTER(C.c_uint64), C.c_int]
+                             C.c_uint64, C.POINTER(C.c_uint64), C.c_int, C.c_int]
 _lib.or_twin_verify.argtypes = [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, _P, _P, C.c_int]
 
 
@@ -327,15 +327,16 @@ def tri_sets():
     return t504, t247, a.value, b.value
 
 
-def twin_run(cands, table, use_gate=True, use_tri=True, c4_div=3, threads=8, cap=1 << 20):
-    """k_find4_cls5 semantics: counters dict and sorted hits"""
+def twin_run(cands, table, use_gate=True, use_tri=True, c4_div=3, threads=8, cap=1 << 20, tile=0):
+    """k_find4_cls5 semantics: counters dict and sorted hits (tile > 0: the
+    reference launch's unbounded c4 tiles of that width)"""
     cands = np.ascontiguousarray(cands, dtype=TWIN_CAND_DTYPE)
     c = TwinCounts()
     hits = np.zeros(cap, TWIN_HIT_DTYPE)
     nh = C.c_uint64()
     _lib.or_twin_run(cands.ctypes.data_as(_P), len(cands), C.byref(table.x) if table is not None else None,
                      1 if use_gate else 0, 1 if use_tri else 0, c4_div, C.byref(c), hits.ctypes.data_as(_P),
-                     cap, C.byref(nh), threads)
+                     cap, C.byref(nh), threads, tile)
     hits = np.sort(hits[:min(nh.value, cap)], order=["cand", "d", "c4", "c3"])
     return {f: getattr(c, f) for f, _ in TwinCounts._fields_}, hits, nh.value
 

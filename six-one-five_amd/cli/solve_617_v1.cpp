@@ -7,7 +7,9 @@
 //                [--device K] [--r R] [--hit-cap N] [--load-table F] [--save-table F]
 //                [--no-gate] [--no-tri-gate] [--quiet] [--check-known [--known-file F]]
 // Additions: --exact-c4 (the exact c4 bound 4 c4^6 >= T instead of the reference
-// kernel's 3 c4^6 >= T), --gpu-build (build the table on the device instead of
+// kernel's 3 c4^6 >= T), --ref-tiles (count and report as the reference launch
+// does, whose unbounded 2048-wide c4 tiles search part of a window more than
+// once past B ~ 515k), --gpu-build (build the table on the device instead of
 // the host's byte-identical builder), --counters (one machine-readable line).
 // Branch B, classes 1-4 of the (6,1,7) system, --xcheck and --bench are not
 // part of this build: asking for them is an error (exit 1).
@@ -33,7 +35,7 @@ static void usage(const char* a0) {
           "  options: --branch-a-only --chunk K --device K --r R --hit-cap N\n"
           "           --save-table F --load-table F --quiet --no-gate --no-tri-gate\n"
           "           --check-known [--known-file F]  diff vs database after run\n"
-          "           --exact-c4 --gpu-build --counters\n",
+          "           --exact-c4 --ref-tiles --gpu-build --counters\n",
           a0);
 }
 
@@ -50,7 +52,7 @@ int main(int argc, char** argv) {
   long long hit_cap = 1 << 20;
   std::string save_table, load_table, known_file = "617-solutions-clean.txt";
   bool quiet = false, no_gate = false, no_tri = false, check_known = false, exact_c4 = false;
-  bool gpu_build = false, counters = false, branch_b = true;
+  bool gpu_build = false, counters = false, branch_b = true, ref_tiles = false;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     if (a == "--quiet") quiet = true;
@@ -60,6 +62,7 @@ int main(int argc, char** argv) {
     else if (a == "--check-known") check_known = true;
     else if (a == "--branch-a-only") branch_b = false;
     else if (a == "--exact-c4") exact_c4 = true;
+    else if (a == "--ref-tiles") ref_tiles = true;
     else if (a == "--gpu-build") gpu_build = true;
     else if (a == "--counters") counters = true;
     else if (a == "--known-file" && i + 1 < argc) known_file = argv[++i];
@@ -155,6 +158,7 @@ int main(int argc, char** argv) {
   fc5_set_option(ctx, "use_gate", no_gate ? 0 : 1);
   fc5_set_option(ctx, "twin_tri_gate", no_tri ? 0 : 1);
   fc5_set_option(ctx, "twin_c4_div", exact_c4 ? 4 : 3);
+  fc5_set_option(ctx, "twin_ref_tiles", ref_tiles ? 2048 : 0);
   bool loaded = false;
   if (!load_table.empty()) {
     const int rc = fc5_table_load(ctx, load_table.c_str(), N);

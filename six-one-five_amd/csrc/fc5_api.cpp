@@ -137,6 +137,9 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) try {
     ctx->twin_tri = v ? 1 : 0;
   } else if (k == "twin_plain") {
     ctx->twin_plain = v ? 1 : 0;
+  } else if (k == "twin_ref_tiles") {
+    if (v < 0 || v > 65535) FAIL(ctx, FC5_E_ARG, "twin_ref_tiles must be 0 (off) or a tile width 1..65535");
+    ctx->twin_tile = (int)v;
   } else if (k == "twin_c4_div") {
     if (v != 3 && v != 4) FAIL(ctx, FC5_E_ARG, "twin_c4_div must be 3 or 4");
     ctx->twin_c4_div = (int)v;

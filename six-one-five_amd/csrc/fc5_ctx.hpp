@@ -144,6 +144,7 @@ struct fc5_ctx {
   std::vector<fc5_solution> sols;
   // (6,1,7) class-5 twin
   int twin_tri = 1, twin_c4_div = 3, twin_plain = 0;
+  int twin_tile = 0;  // 0: every c4 once; > 0: the reference launch's unbounded c4 tiles
   u32* d_tri = nullptr;               // triple-sum sets mod 504 (words 0..15) and 247 (16..23)
   TwinCounters* d_twin_ctr = nullptr;
   TwinCounters* h_twin_ctr = nullptr; // pinned read-back

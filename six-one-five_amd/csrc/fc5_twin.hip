@@ -13,7 +13,11 @@
 // after another, the 64 lanes sweep c3 through the pair-sum gate (LDS bitmap) and compact the
 // survivors into a per-wave LDS ring that is probed 64 keys at a time
 // (key = T - c4^6 - c3^6 mod 2^64, 3 gathers each).  Every c4 of a window is
-// visited once.
+// visited once -- or, with a tile width, as often as the reference launch
+// visits it: its c4 tiles start at lo4 + z * 2048 (z < zmax <= 32) and have
+// no end (solve_617_v1.cu:607-608,961-979), so a c4 at offset k from lo4 is
+// searched min(32, k / 2048 + 1) times; the counters and hits are multiplied
+// accordingly.
 #include "fc5_device.hpp"
 #include "fc5_twin.hpp"
 
@@ -29,6 +33,9 @@ struct TwRing {
   u32 head, tail;
 };
 
+// visits of a c4 at offset k from the window start under the reference tiles
+__device__ __forceinline__ u32 tile_visits(u32 k, u32 tile) { return tile ? min(32u, k / tile + 1u) : 1u; }
+
 __device__ __forceinline__ void tw_drain(const TwinArgs& a, const TwRing& q, u32 n, u32 lane) {
   if (lane >= n) return;
   const u32 slot = (q.head + lane) & (kTwRing - 1);
@@ -36,21 +43,27 @@ __device__ __forceinline__ void tw_drain(const TwinArgs& a, const TwRing& q, u32
   const u32 c3 = x & 0xffffu, c4 = x >> 16;
   const DevTwinCand* C = a.cands + ci;
   const u64 d3 = (u64)d * d * d;
-  const u64 dq = (u64)(((u128)d3 * d3) >> 6);
+  const u128 dq = ((u128)d3 * d3) >> 6;
   const u64 a2 = (u64)c4 * c4, b2 = (u64)c3 * c3;
-  const u64 fp = C->Q_lo - dq - a2 * a2 * a2 - b2 * b2 * b2;
+  const u64 fp = C->Q_lo - (u64)dq - a2 * a2 * a2 - b2 * b2 * b2;
   if (d_query(a.main, fp)) {
-    atomicAdd((unsigned long long*)&a.ctr->maybe, 1ULL);
-    const u64 s = atomicAdd((unsigned long long*)a.nhits, 1ULL);
-    if (s < a.hit_cap) {
-      TwinHit h;
-      h.cand = C->uid;
-      h.d = d;
-      h.c4 = c4;
-      h.c3 = c3;
-      h.fp = fp;
-      a.hits[s] = h;
+    u32 visits = 1;
+    if (a.tile) {  // rare: recompute the window start of this (candidate, d)
+      const u128 T = (((u128)C->Q_hi << 64) | C->Q_lo) - dq;
+      visits = tile_visits(c4 - min_core(T, a.c4_div), a.tile);
     }
+    atomicAdd((unsigned long long*)&a.ctr->maybe, (unsigned long long)visits);
+    const u64 s = atomicAdd((unsigned long long*)a.nhits, (unsigned long long)visits);
+    for (u32 v = 0; v < visits; ++v)
+      if (s + v < a.hit_cap) {
+        TwinHit h;
+        h.cand = C->uid;
+        h.d = d;
+        h.c4 = c4;
+        h.c3 = c3;
+        h.fp = fp;
+        a.hits[s + v] = h;
+      }
   }
 }
 
@@ -106,11 +119,17 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
     u32 t4 = xg + n4;
     t4 = min(t4, t4 - kGateMod);  // (T - c4^6) mod 124488
     bool live = in;
+    const u32 visits = tile_visits(c4 - lo4, a.tile);  // 1 unless the reference tiles are asked for
     if (a.use_tri) {
       const u32 y = t4 % 504u, z = t4 % 247u;
       live = in && ((s_tri[y >> 5] >> (y & 31u)) & 1u) && ((s_tri[16 + (z >> 5)] >> (z & 31u)) & 1u);
-      n_c4 += (u64)__popcll(__ballot(in));
-      n_skip += (u64)__popcll(__ballot(in && !live));
+      if (a.tile) {
+        n_c4 += wave_sum(in ? visits : 0u);
+        n_skip += wave_sum(in && !live ? visits : 0u);
+      } else {
+        n_c4 += (u64)__popcll(__ballot(in));
+        n_skip += (u64)__popcll(__ballot(in && !live));
+      }
     }
     u32 l3 = 1, h3 = 0, x4 = 0;
     if (live) {
@@ -132,15 +151,15 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
     while (m) {
       const int i = __builtin_ctzll(m);
       m &= m - 1;
-      const u32 l = rdl(l3, i), h = rdl(h3, i), xi = rdl(x4, i);
+      const u32 l = rdl(l3, i), h = rdl(h3, i), xi = rdl(x4, i), vi = rdl(visits, i);
       const u32 tag = (c4b + (u32)i) << 16;
       for (u32 cb = l; cb <= h; cb += 64u) {
         const u32 c = cb + lane;
         const bool inn = c <= h;
         const bool pass = inn && (!a.use_gate || gate_pass(s_gate, xi, a.c6g[c]));
         const u64 pm = __ballot(pass), im = __ballot(inn);
-        n_calls += (u64)__popcll(pm);
-        n_gated += (u64)__popcll(im & ~pm);
+        n_calls += (u64)__popcll(pm) * vi;
+        n_gated += (u64)__popcll(im & ~pm) * vi;
         tw_push(a, q, pass, ci, d, c | tag, lane);
         if (h - cb < 64u) break;  // cb + 64 could wrap past 2^32
       }
@@ -269,8 +288,9 @@ __global__ __launch_bounds__(256) void k_twin_plain(PlainTwinArgs a) {
   __syncthreads();
   const u128 R = ((u128)s_rhi << 64) | s_rlo;
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-  const u32 t0 = s_lo4 + blockIdx.z * 2048u;
-  const u32 t1 = min(s_hi4, s_lo4 + blockIdx.z * 2048u + 2047u);
+  // tiles of 2048 with an end (every c4 once), or the reference's unbounded tiles
+  const u32 t0 = s_lo4 + blockIdx.z * (a.tile ? a.tile : 2048u);
+  const u32 t1 = a.tile ? s_hi4 : min(s_hi4, t0 + 2047u);
   u64 n_c4 = 0, n_skip = 0, n_calls = 0, n_gated = 0;
   for (u32 c4 = t0 + wave; c4 <= t1; c4 += waves) {
     u32 t4504 = s_q504 + 504u - s_p504[c4 % 504u];

@@ -41,7 +41,8 @@ struct TwinArgs {
   const u32* tri;   // triple-sum sets: 504-bit map in words 0..15, 247-bit map in 16..23
   const u32* c6g;   // (124488 - c^6 mod 124488) for c = 0..max lim
   u32 use_gate, use_tri;
-  u32 c4_div, pad;  // 3: the reference kernel's c4 bound; 4: exact bound
+  u32 c4_div;       // 3: the reference kernel's c4 bound; 4: exact bound
+  u32 tile;         // 0: every c4 once; > 0: the reference launch's unbounded c4 tiles of this width
   XorDev main;
   TwinHit* hits;
   u64 hit_cap;
@@ -61,6 +62,7 @@ static_assert(sizeof(PlainTwinCand) == 64, "PlainTwinCand is 64 bytes");
 struct PlainTwinArgs {
   const PlainTwinCand* cands;
   u32 n_cands, use_gate, use_tri, c4_div;
+  u32 tile, pad;    // as TwinArgs::tile (0: bounded tiles of 2048, each c4 once)
   GateWords gate;   // pair-sum sets mod 504 and 247
   const u32* tri;   // as TwinArgs::tri
   const u16* p6;    // x^6 mod 504 (504 entries) then mod 247 (247 entries)
@@ -74,7 +76,7 @@ struct PlainTwinArgs {
 
 int launch_twin_batched(const TwinArgs& a, int grid, hipStream_t s);
 int twin_blocks_per_cu();
-// the reference launch shape: grid (candidate, d-row, c4 tile of 2048)
+// the reference launch shape: grid (candidate, d-row, c4 tile of 2048 or a.tile)
 int launch_twin_plain(const PlainTwinArgs& a, u32 ymax, u32 zmax, hipStream_t s);
 
 }  // namespace fc5

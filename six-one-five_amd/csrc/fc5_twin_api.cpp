@@ -132,6 +132,7 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
   a.use_gate = (u32)ctx->use_gate;
   a.use_tri = (u32)ctx->twin_tri;
   a.c4_div = (u32)ctx->twin_c4_div;
+  a.tile = (u32)ctx->twin_tile;
   a.main = ctx->main.dev;
   a.hits = reinterpret_cast<TwinHit*>(ctx->d_hits);
   a.hit_cap = ctx->hit_cap;
@@ -184,7 +185,11 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
     p.hit_cap = a.hit_cap;
     p.nhits = a.nhits;
     p.ctr = a.ctr;
-    const u32 zmax = lmax / 2048 + 1;
+    // bounded 2048-wide tiles, or the reference launch's zmax = ceil(max lim / tile)
+    // unbounded tiles, capped at 32 (solve_617_v1.cu:961-979)
+    const u32 zmax = ctx->twin_tile ? std::min<u32>(32, std::max<u32>(1, (lmax + ctx->twin_tile - 1) / ctx->twin_tile))
+                                    : lmax / 2048 + 1;
+    p.tile = (u32)ctx->twin_tile;
     if (hipMemcpy(d_plain, plain.data(), plain.size() * sizeof(PlainTwinCand), hipMemcpyHostToDevice) != hipSuccess ||
         launch_twin_plain(p, ymax, zmax, ctx->stream) != 0) {
       (void)hipFree(d_plain);

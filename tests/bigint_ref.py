@@ -160,9 +160,11 @@ T504 = triple_sum_set(504)
 T247 = triple_sum_set(247)
 
 
-def twin_counts(cand, contains=None, c4_div=3, use_gate=True, use_tri=True):
-    """k_find4_cls5 from its definition: every grid d, every c4 of the window
-    (each once), every c3 of the inner window; Python integers throughout."""
+def twin_counts(cand, contains=None, c4_div=3, use_gate=True, use_tri=True, tile=0, zmax=1):
+    """k_find4_cls5 from its definition: every grid d, every c4 of the window,
+    every c3 of the inner window; Python integers throughout.  tile = 0 visits
+    each c4 once; tile > 0 runs the reference launch's zmax c4 tiles, tile z
+    from lo4 + z * tile to hi4 (no tile end)."""
     base = cand["base"]
     lim, fmax1 = cand["lim"], cand["fmax1"]
     c = dict(jobs=0, c4_total=0, c4_tri_skipped=0, calls=0, gated=0, probes=0, maybe=0)
@@ -183,7 +185,8 @@ def twin_counts(cand, contains=None, c4_div=3, use_gate=True, use_tri=True):
         if lo4 > hi4:
             continue
         c["jobs"] += 1
-        for c4 in range(lo4, hi4 + 1):
+        starts = [lo4 + z * tile for z in range(zmax)] if tile else [lo4]
+        for c4 in (c4 for s in starts for c4 in range(s, hi4 + 1)):
             T4 = T - c4 ** 6
             if use_tri:
                 c["c4_total"] += 1

@@ -205,3 +205,48 @@ def test_cli_617_finds_and_checks_known(tmp_path):
     assert {int(l.split("B=")[1].split()[0]) for l in sols} == want
     assert "[check-known]" in r.stderr and "PASS" in r.stderr
     assert any(l.startswith("COUNTERS cands=") for l in r.stdout.splitlines())
+
+
+@pytest.mark.parametrize("tile", [16, 60])
+@pytest.mark.parametrize("plain", [0, 1])
+def test_twin_reference_tiles_small_B(ctx, tile, plain):
+    """the reference launch's unbounded c4 tiles (twin_ref_tiles) at small tile
+    widths, so that repeated visits (and the 32-tile cap, at 16) occur at small
+    B: counters and the maybe-hit list, duplicates included, equal the oracle's
+    restatement of the reference loops, in both kernel forms"""
+    say(f"twin reference tiles={tile} plain={plain}")
+    t = device_table(ctx, 1500, 16)
+    cands = np.concatenate([O.twin_cands(54347)[::3], O.twin_cands(59819)[::5]])
+    ctx.set_option("twin_ref_tiles", tile)
+    ctx.set_option("twin_plain", plain)
+    try:
+        res, hits, _ = ctx.twin_run(cands.view(fc5.TWIN_CAND_DTYPE))
+    finally:
+        ctx.set_option("twin_ref_tiles", 0)
+        ctx.set_option("twin_plain", 0)
+    oc, oh, _ = O.twin_run(cands, t, threads=THREADS, tile=tile)
+    compare(res, hits, oc, oh)
+    once, _, _ = O.twin_run(cands, t, threads=THREADS)
+    assert oc["calls"] > once["calls"] and oc["maybe"] > once["maybe"] > 0
+
+
+def test_twin_reference_tiles_2048_past_515k(ctx):
+    """at B = 600011 the c4 windows of the first grid rows exceed 2048, so the
+    reference's 2048-wide unbounded tiles search part of them twice: with
+    twin_ref_tiles = 2048 the counters and hits repeat exactly as the oracle's
+    restatement of the reference loops says; without, every c4 once"""
+    say("twin reference tiles 2048 at B=600011")
+    B = 600011
+    t = device_table(ctx, B // 42, 16)
+    cands = O.twin_cands(B)[[97, 194, 679, 776]]  # candidates whose c3 sweeps are not all gated away
+    ctx.set_option("twin_ref_tiles", 2048)
+    try:
+        res, hits, _ = ctx.twin_run(cands.view(fc5.TWIN_CAND_DTYPE))
+    finally:
+        ctx.set_option("twin_ref_tiles", 0)
+    oc, oh, _ = O.twin_run(cands, t, threads=THREADS, tile=2048)
+    compare(res, hits, oc, oh)
+    res0, hits0, _ = ctx.twin_run(cands.view(fc5.TWIN_CAND_DTYPE))
+    once, oh0, _ = O.twin_run(cands, t, threads=THREADS)
+    compare(res0, hits0, once, oh0)
+    assert oc["c4_total"] > once["c4_total"] and oc["calls"] > once["calls"]

@@ -103,6 +103,33 @@ def test_twin_oracle_matches_bigint_restatement(c4_div, use_gate, use_tri):
         assert [(int(h["d"]), int(h["c4"]), int(h["c3"]), int(h["fp"])) for h in hits] == sorted(whits)
 
 
+@pytest.mark.parametrize("tile", [16, 60, 2048])
+def test_twin_oracle_reference_tiles_match_bigint_restatement(tile):
+    """the reference launch's unbounded c4 tiles (solve_617_v1.cu:607-608,961-979):
+    block z runs from lo4 + z * tile to hi4, so c4 values far into a window are
+    counted (and hit) once per block that reaches them; small tiles make the
+    duplicates appear at small B, tile 16 also reaches the 32-block cap"""
+    N = 700
+    tab = O.Table.build_pairs(N, 16)
+    k = KNOWN[0]
+    rnd = random.Random(618)
+    c = O.twin_cands(63631)
+    x = c[rnd.randrange(len(c))]
+    cases = [small_cand(k["B"], k["u"], k["w"], fmax1=k["d"], res=[k["d"] % 64]),
+             small_cand(63631, int(x["u"]), int(x["w"]), fmax1=100)]
+    for cand, spec in cases:
+        zmax = min(32, max(1, -(-max(spec["lim"], 1) // tile)))
+        got, hits, nh = O.twin_run(cand, tab, tile=tile)
+        want, whits = br.twin_counts(spec, contains=tab.contains, tile=tile, zmax=zmax)
+        assert got == want
+        assert [(int(h["d"]), int(h["c4"]), int(h["c3"]), int(h["fp"])) for h in hits] == sorted(whits)
+        once, _, _ = O.twin_run(cand, tab)
+        if tile == 2048:  # windows below 2048 at these B: the same as visiting each c4 once
+            assert got == once
+        else:
+            assert got["c4_total"] > once["c4_total"] and got["calls"] > once["calls"]
+
+
 def test_twin_oracle_finds_known_solutions_whole_B():
     """every candidate of B, table N = B/42 (the reference's N = B_max/42)"""
     for k in [k for k in KNOWN if k["B"] < 70000]:
