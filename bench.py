@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-every", type=int, default=400, help="CPU oracle on every k-th unit")
     ap.add_argument("--cpu-offset", type=int, default=0, help="first unit of the CPU oracle sample")
+    ap.add_argument("--cpu-ref-span", type=int, default=100,
+                    help="integers of the band (from its start) the reference CPU solver searches "
+                         "(bands below 2.2M; whole band if smaller)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-measure", action="store_true", help="skip the gather/copy microbenchmarks")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape")
@@ -278,9 +281,13 @@ def main():
     if os.path.exists(tpath) and not args.band:
         traffic = json.load(open(tpath)).get("traffic_bytes_per_launch")
 
-    cpu = None
+    cpu = cpu_port = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(ctx, units, ucls, args.cpu_every, cfg, r, pkeys, args.cpu_offset)
+        if not mixed:  # below 2.2M the reference's own CPU solver is the baseline, the port beside it
+            ref = cpu_reference(cfg, cpu["cores"], args.cpu_ref_span, calls_closed)
+            if ref is not None:
+                cpu, cpu_port = ref, cpu
 
     line = {
         "metric": METRIC, "value": value, "unit": "(d,e) grid points/s", "n_gpus": 1,
@@ -316,6 +323,7 @@ def main():
         "graph": not args.no_graph,
         "plain_form": plain,
         "cpu_baseline": cpu,
+        "cpu_port": cpu_port,
         "table_build_s": t_build,
         "exact_table": exact,
     }
@@ -655,6 +663,45 @@ def cpu_baseline(ctx, units, ucls, every, cfg, r, pkeys, offset=0):
                       + f" ({len(sub)} units, {c['calls']} grid points, "
                       f"{dt:.1f} s), same table bytes; counters and the {len(oh)}-entry maybe-hit list "
                       f"identical to the GPU on that sample"}
+
+
+def cpu_reference(cfg, threads, span, band_points):
+    """The reference's own CPU search of the class-5 (d,e) grid: solve_516_v2
+    (every e, every d, an exact lookup in its sorted pair index,
+    solve_516_v2.cpp:400-416), compiled from the reference's sources into
+    oracle/_ref and timed on this host over the whole band when it holds at
+    most 2e9 grid points, else over its first `span` integers (its own
+    pair-index build excluded, as it reports it); the rate is the closed-form
+    grid-point count of the same units over that time.  None when the binary
+    is absent or the band is past its B <= 2.2M limit."""
+    import re
+    import subprocess
+
+    exe = os.path.join(REPO, "oracle", "_ref", "solve_516_v2")
+    if not os.path.exists(exe) or cfg["hi"] > 2200000:
+        return None
+    lo = cfg["lo"]
+    hi = cfg["hi"] if band_points <= 2e9 else min(cfg["hi"], lo + span - 1)  # small bands in full
+    _, Bs, us = fc5.emit_units(lo, hi)
+    points = fc5.units_calls(fc5.units_array(Bs, us))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    t0 = time.perf_counter()
+    r = subprocess.run([exe, str(lo), str(hi), "5"], capture_output=True, text=True, env=env, timeout=600)
+    wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise SystemExit(f"reference solver failed: {r.stderr[-400:]}")
+    build_ms = int(re.search(r"index built in (\d+) ms", r.stderr).group(1))
+    units = int(re.search(r"unit candidates tested: (\d+)", r.stderr).group(1))
+    sols = int(re.search(r"total solutions: (\d+)", r.stderr).group(1))
+    assert units == len(Bs) and sols == 0, (units, len(Bs), sols)
+    dt = wall - build_ms / 1000.0
+    log(f"[bench] reference CPU solver {points / dt:.4e} points/s on B in [{lo},{hi}] ({units} units), "
+        f"{threads} threads, {dt:.1f}s (+{build_ms / 1000:.1f}s pair index)")
+    return {"value": points / dt, "unit": "(d,e) grid points/s", "cores": threads, "kind": "reference",
+            "host": host_cpu(),
+            "sample": f"B in [{lo},{hi}] ({units} class-5 units, {points} grid points): the reference's "
+                      f"solve_516_v2 class-5 search built from its sources, {dt:.1f} s after its "
+                      f"{build_ms / 1000:.1f} s pair-index build; 0 solutions as expected"}
 
 
 if __name__ == "__main__":
