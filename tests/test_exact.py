@@ -59,3 +59,18 @@ def test_oracle_matches_reference_binary_n500():
     out = subprocess.run([os.path.join(REF_DIR, "ref_quad"), "table", "500"], capture_output=True, text=True,
                          check=True).stdout.split("\n")[:-1]
     assert lines_of(500)[0] == out
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_DIR, "solve_516_v2")), reason="oracle/_ref not built")
+def test_reference_cpu_solver_agrees_on_units():
+    """the reference's own CPU solver (the CPU baseline below 2.2M) passes its
+    self-test and searches the same class-5 units as Stage 1 here, finding no
+    solution, on B in [100001, 102000]"""
+    import fc5
+
+    exe = os.path.join(REF_DIR, "solve_516_v2")
+    assert subprocess.run([exe, "--selftest"], capture_output=True, text=True).returncode == 0
+    r = subprocess.run([exe, "100001", "102000", "5"], capture_output=True, text=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="4"), check=True)
+    _, Bs, _ = fc5.emit_units(100001, 102000)
+    assert f"unit candidates tested: {len(Bs)}" in r.stderr and "total solutions: 0" in r.stderr
