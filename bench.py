@@ -293,7 +293,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "(d,e) grid points/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
-        "data": "synthetic (deterministic Stage-1 enumeration of the band; seeded planted keys)",
+        "data": "synthetic (deterministic Stage-1 enumeration of the band"
+                + ("; seeded planted keys)" if plants else ")"),
         "config": {"workload": ("band" if args.band else f"config {args.config}") + ": batched persistent cls5 peel, eligible B in "
                                f"[{cfg['lo']},{cfg['hi']}] ({n_B} B, {len(units)} units, "
                                f"{r['calls']} grid points), xor pair table N={hdr.N} r={hdr.r} "
