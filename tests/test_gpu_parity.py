@@ -513,6 +513,40 @@ def test_cli_cls5_units_bbj_buc_and_exact_table(tmp_path):
     assert outs[0] == outs[1] == outs[2] and outs[0]
 
 
+def test_cli_xor_build_save_on_device(ctx, tmp_path):
+    """xor_build_save --gpu writes a v2 .bin with the host builder's geometry and
+    every key; the search over it (cls5 CLI, --load-table) matches the oracle
+    reading the same file, FP maybe-hits included"""
+    import re
+    import subprocess
+    from conftest import BIN
+    say("xor_build_save --gpu")
+    p = str(tmp_path / "dev.bin")
+    r = subprocess.run([os.path.join(BIN, "xor_build_save"), "--gpu", "--N", "2428", "--r", "16", "--out", p],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-1000:]
+    assert "(device)" in r.stderr and "[xor_build_save] wrote" in r.stdout
+    t = O.Table.load(p, 2428)
+    host = O.Table.build_pairs(2428, 16)
+    hd = t.header()
+    assert (hd["n_keys"], hd["m_cells"], hd["r"], hd["N"]) == (host.x.hdr.n_keys, host.x.hdr.m_cells, 16, 2428)
+    ctx.table_load(p, 2428)
+    assert ctx.table_query(_pair_keys(2428)).all()
+    u = str(tmp_path / "u.buc")
+    subprocess.run([os.path.join(BIN, "fourcore_hunt_v3"), "--lo", "100001", "--hi", "102000", "--classes", "5",
+                    "--emit-units", u], check=True, capture_output=True)
+    r = subprocess.run([os.path.join(BIN, "fourcore_cls5_gpu_v4"), "--units", u, "--N", "2428", "--r", "16",
+                        "--load-table", p, "--quiet", "--counters"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-1000:]
+    cls, Bs, us = fc5.emit_units(100001, 102000, classes=(5,))
+    oc, oh = _oracle_run(Bs, us, t)
+    line = next(l for l in r.stdout.splitlines() if l.startswith("COUNTERS"))
+    got = dict((k, int(v)) for k, v in re.findall(r"(\w+)=(\d+)\b", line))
+    for k in COUNTERS:
+        assert got[k] == oc[k], (k, got[k], oc[k])
+    assert oc["maybe"] > 0
+
+
 @pytest.mark.parametrize("lo,hi,classes,band", [
     (100001, 102000, (2, 3, 4, 5), (0.0, 1.0)),
     (1, 3000, (2, 3, 4, 5), (0.0, 1.0)),
