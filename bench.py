@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--plain-every", type=int, default=64, help="plain form on every k-th unit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-every", type=int, default=400, help="CPU oracle on every k-th unit")
+    ap.add_argument("--cpu-every", type=int, default=160, help="CPU oracle on every k-th unit")
     ap.add_argument("--cpu-offset", type=int, default=0, help="first unit of the CPU oracle sample")
     ap.add_argument("--cpu-ref-span", type=int, default=100,
                     help="integers of the band (from its start) the reference CPU solver searches "

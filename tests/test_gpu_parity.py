@@ -622,6 +622,34 @@ def test_ragged_and_irregular_units_match_oracle(ctx, mode):
     assert len(oh) > 0
 
 
+def test_units_either_side_of_the_2_pow_21_e_bound_match_oracle(ctx):
+    """the top of the batched 128-bit T-space path: e's free range reaches 2^21
+    at B = 14 * 2^21 = 29360128, so these Stage-1 units at B = 29360119 and
+    29360123 (fmax1 = 2^21 - 1) run in the batched kernel and those at
+    B = 29360141 and 29360143 on the per-point 192-bit path.  Units chosen so
+    that the pair-sum gate passes some points (at B = 29360125 the residual is
+    3, 4 or 5 mod 7 for every point, and sixth-power pair sums are 0, 1 or 2
+    mod 7).  About 1.5e9 grid points per unit: the oracle needs ~90 s"""
+    say("units either side of B = 14 * 2^21 (oracle ~90 s)")
+    pick = [(29360119, 106578), (29360119, 14717196), (29360123, 11148),
+            (29360141, 65340), (29360143, 165162), (29360143, 14658612)]
+    Bs = np.array([b for b, _ in pick], np.uint64)
+    us = np.array([u for _, u in pick], np.uint64)
+    _, eB, eu = fc5.emit_units(29360119, 29360143)
+    assert set(pick) <= set(zip(eB.tolist(), eu.tolist()))
+    t = _load(ctx, 2428, 16)
+    b = ctx.band(fc5.units_array(Bs, us))
+    outb = b.run(fc5.MODE_BATCHED)
+    outp = b.run(fc5.MODE_PLAIN)
+    b.close()
+    assert outb.result["units_irregular"] == 3
+    oc, oh = _oracle_run(Bs, us, t)
+    say("oracle", oc)
+    _compare(outb, oc, oh)
+    _compare(outp, oc, oh)
+    assert oc["probes"] > 0 and len(oh) > 0
+
+
 @pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN], ids=["batched", "plain"])
 def test_empty_band(ctx, mode):
     say(f"empty band, mode={mode}")
