@@ -143,3 +143,37 @@ def test_serving_refusals(ctx):
         assert ctx.serve_result(empty).result["units"] == 0
     finally:
         ctx.serve_stop()
+
+
+def test_serving_hit_overflow_is_flagged(ctx):
+    """a round whose hits exceed hit_cap: the band finishes with FC5_E_OVERFLOW,
+    hits_dropped = 1 (the round's buffer is shared, so the count is not split
+    among bands), the kept hits are a subset of the band's hits alone, and the
+    counters are unaffected"""
+    say("serving: hit overflow in a round")
+    cls, Bs, us = fc5.emit_units(1000001, 1002000, classes=(4, 5))
+    units = fc5.units_array(Bs, us)
+    alone = ctx.run(units, fc5.MODE_BATCHED, cls=cls)
+    n = alone.result["n_hits"]
+    assert n >= 2
+    ctx.set_option("hit_cap", n - 1)
+    try:
+        ctx.serve_start(round_points=1e15)  # the whole band in one round
+        try:
+            bid = ctx.serve_submit(units, cls)
+            assert ctx.serve_wait(bid, 120.0)
+            with pytest.raises(fc5.Fc5Error) as e:
+                ctx.serve_result(bid)
+            assert e.value.code == fc5.E_OVERFLOW
+            out = ctx.serve_result(bid, allow_overflow=True)
+            assert out.rc == fc5.E_OVERFLOW
+            assert out.result["hits_dropped"] == 1
+            assert out.result["n_hits"] == len(out.hits) == n - 1
+            key = lambda h: (int(h["unit"]), int(h["e"]), int(h["d"]), int(h["flags"]))
+            assert {key(h) for h in out.hits} <= {key(h) for h in alone.hits}
+            for f in ("calls", "neg", "zero", "gated", "probes", "maybe", "jobs", "fail_t"):
+                assert out.result[f] == alone.result[f], f
+        finally:
+            ctx.serve_stop()
+    finally:
+        ctx.set_option("hit_cap", 1 << 20)

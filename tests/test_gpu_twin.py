@@ -176,6 +176,9 @@ def test_twin_errors(ctx):
     with pytest.raises(fc5.Fc5Error) as e:
         ctx.twin_run(bad)
     assert e.value.code == fc5.E_ARG
+    with pytest.raises(fc5.Fc5Error) as e:  # B^6 < 2^127 cap, solve_617_v1.cu:146,1775
+        ctx.twin_run_range(2353973, 2353974)
+    assert e.value.code == fc5.E_ARG
     ctx.set_option("hit_cap", 4)  # r = 8: many maybe-hits
     try:
         with pytest.raises(fc5.Fc5Error) as e:
