@@ -147,6 +147,12 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
     FAIL(ctx, FC5_E_HIP, "counter reset");
   std::vector<PlainTwinCand> plain;
   PlainTwinCand* d_plain = nullptr;
+  struct FreeOnExit {  // hipFree waits for the launch that reads the buffer
+    PlainTwinCand*& p;
+    ~FreeOnExit() {
+      if (p) (void)hipFree(p);
+    }
+  } free_plain{d_plain};
   if (items && ctx->twin_plain) {
     // the reference launch shape over the same candidates
     u32 ymax = 1, lmax = 0;
@@ -191,10 +197,8 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
                                     : lmax / 2048 + 1;
     p.tile = (u32)ctx->twin_tile;
     if (hipMemcpy(d_plain, plain.data(), plain.size() * sizeof(PlainTwinCand), hipMemcpyHostToDevice) != hipSuccess ||
-        launch_twin_plain(p, ymax, zmax, ctx->stream) != 0) {
-      (void)hipFree(d_plain);
+        launch_twin_plain(p, ymax, zmax, ctx->stream) != 0)
       FAIL(ctx, FC5_E_HIP, "plain twin launch");
-    }
   } else if (items) {
     const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : twin_blocks_per_cu();
     const u64 want = (items + 3) / 4;  // 4 waves per workgroup
@@ -205,9 +209,7 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
                              ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->h_nhits, ctx->d_nhits, 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-  const hipError_t sync = hipStreamSynchronize(ctx->stream);
-  if (d_plain) (void)hipFree(d_plain);
-  HIPCHK(ctx, sync);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   float ms = 0;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   const TwinCounters c = *ctx->h_twin_ctr;
