@@ -37,7 +37,8 @@ CONFIGS = {
             exact=True),
     3: dict(lo=2000001, hi=2010000, N=47857, plants=0, plant_mode=None, seed=0x615 << 8 | 3,
             classes=(2, 3, 4, 5)),
-    4: dict(lo=3200001, hi=3210000, N=76428, plants=256, plant_mode="side", seed=0x615 << 8 | 4, classes=(5,)),
+    4: dict(lo=3200001, hi=3210000, N=76428, plants=256, plant_mode="side", seed=0x615 << 8 | 4, classes=(5,),
+            exact=True),
 }
 COUNTER_KEYS = ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe", "jobs", "fail_t")
 
@@ -88,7 +89,7 @@ def main():
     ap.add_argument("--r", type=int, default=48)
     ap.add_argument("--exact-table", choices=["auto", "on", "off"], default="auto",
                     help="exact sorted pair table beside the xor filter, planted targets in both, hits "
-                         "checked by lookup (auto: on for config 2)")
+                         "checked by lookup (auto: on for configs 2 and 4)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--plain-every", type=int, default=64, help="plain form on every k-th unit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
