@@ -55,6 +55,12 @@ def test_units_calls_closed_form():
     _, Bs, us = fc5.emit_units(3200001, 3200100)
     closed = sum(len(es) * len(ds) for es, ds in (br.cls5_grid(int(b), int(u)) for b, u in zip(Bs, us)))
     assert fc5.units_calls(fc5.units_array(Bs, us)) == closed
+    # either side of B = 14 * 2^21, where e's free range reaches 2^21 (batched path
+    # below, per-point 192-bit path above): 9,049,577,666 points, the oracle's count
+    pick = [(29360119, 106578), (29360119, 14717196), (29360123, 11148),
+            (29360141, 65340), (29360143, 165162), (29360143, 14658612)]
+    closed = sum(len(es) * len(ds) for es, ds in (br.cls5_grid(b, u) for b, u in pick))
+    assert fc5.units_calls(fc5.units_array([b for b, _ in pick], [u for _, u in pick])) == closed == 9049577666
 
 
 @pytest.mark.parametrize("N,r", [(300, 48), (2000, 16), (2428, 48)])
