@@ -2,7 +2,7 @@
  * fc5_oracle.c -- CPU restatement of the reference class-5 free peel.
  *
  * TEST INFRASTRUCTURE ONLY (tests/, __graft_entry__.smoke(), bench.py's
- * cpu_baseline leg).  It is the checker, never the thing measured or shipped.
+ * cpu_baseline leg).  It checks results; it is never the thing measured or shipped.
  *
  * Function -> reference lines it restates -> what pins it (tests/):
  *   class contracts  fourcore_classes_v3.hpp:37-232

@@ -1,7 +1,7 @@
 """ctypes binding of the oracle (oracle/libfc5_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker.  The oracle restates the
+bench.py's cpu_baseline leg, to check the product's results.  It restates the
 reference algorithm on the CPU (see fc5_oracle.c for the file:line map).
 """
 from __future__ import annotations

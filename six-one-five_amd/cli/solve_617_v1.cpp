@@ -1,7 +1,7 @@
 // solve_617_v1 -- CLI front end over libfc5 for the (6,1,7) Branch-A class-5 twin.
 //
 // Same positional arguments, flags, exit codes and stdout/stderr line shapes as
-// the reference program (solve_617_v1.cu:1686-2045) for the part this build
+// the reference program (solve_617_v1.cu:1686-2045) for the part this program
 // provides, Branch A class 5:
 //   solve_617_v1 <B_min> <B_max> 5 [u_lo] [u_hi] --branch-a-only [--chunk K]
 //                [--device K] [--r R] [--hit-cap N] [--load-table F] [--save-table F]
@@ -12,7 +12,7 @@
 // once past B ~ 515k), --gpu-build (build the table on the device instead of
 // the host's byte-identical builder), --counters (one machine-readable line).
 // Branch B, classes 1-4 of the (6,1,7) system, --xcheck and --bench are not
-// part of this build: asking for them is an error (exit 1).
+// provided here: asking for them is an error (exit 1).
 #include <algorithm>
 #include <array>
 #include <cstdio>
@@ -30,7 +30,7 @@ static constexpr long long kBHardMax = 2353973;     // solve_617_v1.cu:146 (B^6 
 static void usage(const char* a0) {
   fprintf(stderr,
           "usage: %s <B_min> <B_max> [classes] [u_lo] [u_hi] [options]\n"
-          "  (6,1,7): a1^6+..+a7^6 = B^6, primitive; this build runs Branch A class 5.\n"
+          "  (6,1,7): a1^6+..+a7^6 = B^6, primitive; this program runs Branch A class 5.\n"
           "  classes: \"5\"; u band as fractions of B\n"
           "  options: --branch-a-only --chunk K --device K --r R --hit-cap N\n"
           "           --save-table F --load-table F --quiet --no-gate --no-tri-gate\n"
@@ -74,7 +74,7 @@ int main(int argc, char** argv) {
     else if (a == "--load-table" && i + 1 < argc) load_table = argv[++i];
     else if (a == "--branch-b-only" || a == "--xcheck" || a == "--selftest" || a == "--bench" ||
              a == "--branch-b-lim") {
-      fprintf(stderr, "%s is not part of this build (Branch A class 5 only)\n", a.c_str());
+      fprintf(stderr, "%s is not provided by this program (Branch A class 5 only)\n", a.c_str());
       return 1;
     } else if (!a.empty() && a[0] == '-') {
       fprintf(stderr, "unknown option %s\n", a.c_str());
@@ -89,11 +89,11 @@ int main(int argc, char** argv) {
   }
   const long long B_min = atoll(pos[0].c_str()), B_max = atoll(pos[1].c_str());
   if (pos.size() >= 3 && pos[2] != "5") {
-    fprintf(stderr, "classes \"%s\": this build runs class 5 only\n", pos[2].c_str());
+    fprintf(stderr, "classes \"%s\": this program runs class 5 only\n", pos[2].c_str());
     return 1;
   }
   if (branch_b) {
-    fprintf(stderr, "Branch B (7 | B) is not part of this build: pass --branch-a-only\n");
+    fprintf(stderr, "Branch B (7 | B) is not provided by this program: pass --branch-a-only\n");
     return 1;
   }
   const double u_lo = pos.size() >= 4 ? atof(pos[3].c_str()) : 0.0;
