@@ -89,7 +89,7 @@ def main():
     ap.add_argument("--r", type=int, default=48)
     ap.add_argument("--exact-table", choices=["auto", "on", "off"], default="auto",
                     help="exact sorted pair table beside the xor filter, planted targets in both, hits "
-                         "checked by lookup (auto: on for configs 2 and 4)")
+                         "checked by lookup (auto: on for --config 2 and 4)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--plain-every", type=int, default=64, help="plain form on every k-th unit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -128,7 +128,7 @@ def main():
     units = fc5.units_array(Bs, us)
     n_B = eligible_count(cfg["lo"], cfg["hi"])
     calls_closed = fc5.units_calls(units[ucls == 5]) if mixed else fc5.units_calls(units)
-    log(f"[bench] {'band' if args.band else f'config {args.config}'}: B=[{cfg['lo']},{cfg['hi']}] eligible_B={n_B} units={len(units)} "
+    log(f"[bench] band: B=[{cfg['lo']},{cfg['hi']}] eligible_B={n_B} units={len(units)} "
         f"calls={calls_closed:.4e}")
 
     ctx = fc5.Context(0)
@@ -296,7 +296,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (deterministic Stage-1 enumeration of the band"
                 + ("; seeded planted keys)" if plants else ")"),
-        "config": {"workload": ("band" if args.band else f"config {args.config}") + ": batched persistent cls5 peel, eligible B in "
+        "config": {"workload": "batched persistent cls5 peel, eligible B in "
                                f"[{cfg['lo']},{cfg['hi']}] ({n_B} B, {len(units)} units, "
                                f"{r['calls']} grid points), xor pair table N={hdr.N} r={hdr.r} "
                                f"({fc5_packed_gb(hdr):.2f} GB packed, device-built), "
@@ -463,7 +463,7 @@ def bench_twin(args):
 
 
 def serve_requests(n, seed):
-    """Config 5: n band requests, widths log-uniform on [500, 20000] integers,
+    """Serving workload (--config 5): n band requests, widths log-uniform on [500, 20000] integers,
     starts uniform on [1.0M, 3.5M], classes 4 and 5 (seeded, deterministic)."""
     import random
     rnd = random.Random(seed)
@@ -577,7 +577,7 @@ def bench_serve(args):
         "steps": 1, "warmup": 0, "ms_per_step": 1000.0 * span, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (seeded request stream; deterministic Stage-1 enumeration per band)",
-        "config": {"workload": f"config 5: {len(bands)} band requests (widths log-uniform [500,20000], starts "
+        "config": {"workload": f"serving: {len(bands)} band requests (widths log-uniform [500,20000], starts "
                                f"uniform [1.0M,3.5M], classes 4+5), Poisson arrivals at {args.serve_load} of the "
                                f"measured capacity, one device table N={N} r={hdr.r} ({hdr.m_cells} cells)",
                    "requests": len(bands), "load": args.serve_load, "N": N, "r": int(hdr.r)},
