@@ -738,7 +738,6 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
   b->dev3_h = std::move(dev3);
   if (rc == FC5_OK) rc = upload_segments(ctx, b.get());
   if (rc == FC5_OK && !b->units.empty()) rc = ensure_ndg(ctx, b->max_fmax2);
-  if (rc == FC5_OK && !b->units3.empty()) rc = ensure_c6g(ctx, b->max_lim3 + 64);
   if (rc != FC5_OK) {
     fc5_band_destroy(ctx, b.release());
     return rc;
@@ -912,7 +911,6 @@ static BatchedArgs batched_args(fc5_ctx* ctx, fc5_band* b) {
   a.item_ctr = ctx->d_item_ctr;
   a.inv729 = inv_mod_2_64(729);
   a.units3 = b->d_units3;
-  a.c6g = ctx->d_c6g;
   return a;
 }
 

@@ -56,6 +56,22 @@ __device__ __forceinline__ bool gate_pass(const u32* s_gate, u32 xg, u32 n) {
   return (s_gate[t >> 5] >> (t & 31u)) & 1u;
 }
 
+// (x y) mod 124488 in 32-bit steps for x y < 2^51: the high word of the
+// product (below 2^19) times 2^32 mod 124488 = 6808, plus the reduced low
+// word, stays below 2^32
+__device__ __forceinline__ u32 mulmod_gate(u32 x, u32 y) {
+  const u64 p = (u64)x * y;
+  const u32 r = (u32)p % kGateMod + (u32)(p >> 32) * 6808u;
+  return r % kGateMod;
+}
+
+// 124488 - (c^6 mod 124488), in [1, 124488]: the n of gate_pass for the
+// residual T - c^6, computed in registers (c < 2^25)
+__device__ __forceinline__ u32 neg_p6_gate(u32 c) {
+  const u32 c2 = mulmod_gate(c, c), c3 = mulmod_gate(c2, c);
+  return kGateMod - mulmod_gate(c3, c3);
+}
+
 // X mod 124488 for a 128-bit X
 __device__ __forceinline__ u32 mod_gate(u128 X) {
   const u64 xh = (u64)(X >> 64), xl = (u64)X;

@@ -266,7 +266,7 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_u
     for (u32 cb = l; cb <= h; cb += 64u) {
       const u32 c = cb + lane;
       const bool in = c <= h;
-      const u32 n = in ? a.c6g[c] : 0u;
+      const u32 n = in ? neg_p6_gate(c) : 0u;  // c <= lim < 2^21
       const bool pass = in && gate_pass(s_gate, xgi, n);
       probes_item += ring_push(a, q, pass, ui | kFind3Bit, di, c, lane);
       if (h - cb < 64u) break;  // cb + 64 could wrap past 2^32

@@ -117,7 +117,6 @@ struct BatchedArgs {
   u32* item_ctr;
   u64 inv729;  // 729^-1 mod 2^64
   const DevUnit3* units3;  // class 2-4 unit arena
-  const u32* c6g;  // (124488 - c^6 mod 124488) for c = 0..lim
 };
 
 // reference-shaped find3 candidate (Cand, fourcore_find_v4.cu:102-109)

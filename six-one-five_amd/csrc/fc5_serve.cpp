@@ -353,7 +353,6 @@ int run_round(fc5_server* S, std::vector<SBand*>& active) {
   a.ctr = S->d_ctr;
   a.item_ctr = ctx->d_item_ctr;
   a.inv729 = inv_mod_2_64(729);
-  a.c6g = ctx->d_c6g;
   const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(S->rows);
   const int grid = (int)std::max<u64>(1, std::min<u64>((u64)ctx->num_cu * (u64)bpc, total));
   HIPCHK(ctx, hipMemcpyAsync(S->d_segs, S->h_segs, n_segs * sizeof(Segment), hipMemcpyHostToDevice, ctx->stream));
@@ -498,7 +497,6 @@ extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) try {
   // every band the table can serve: B <= 42 (N + 1) gives fmax2 <= 2N + 2 and lim <= N + 1
   const u32 N = ctx->main.hdr.N;
   int rc = ensure_ndg(ctx, 2 * N + 2);
-  if (rc == FC5_OK) rc = ensure_c6g(ctx, N + 64);
   if (rc == FC5_OK) rc = ensure_hits(ctx);
   if (rc != FC5_OK) return rc;
   const u64 table_bytes = ctx->main.hdr.m_cells * 8 + ctx->side.hdr.m_cells * 8 + ctx->n_exact * sizeof(u128);
