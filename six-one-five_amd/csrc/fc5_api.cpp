@@ -88,8 +88,7 @@ extern "C" void fc5_ctx_destroy(fc5_ctx* ctx) {
   free_table(ctx->main);
   free_table(ctx->side);
   for (void* p : {(void*)ctx->d_gate, (void*)ctx->d_gate_ones, (void*)ctx->d_ndg, (void*)ctx->d_hits,
-                  (void*)ctx->d_ctr, (void*)ctx->d_nhits, (void*)ctx->d_item_ctr, (void*)ctx->d_p6,
-                  (void*)ctx->d_c6g})
+                  (void*)ctx->d_ctr, (void*)ctx->d_nhits, (void*)ctx->d_item_ctr, (void*)ctx->d_p6})
     if (p) (void)hipFree(p);
   if (ctx->h_ctr) (void)hipHostFree(ctx->h_ctr);
   if (ctx->h_nhits) (void)hipHostFree(ctx->h_nhits);
@@ -436,22 +435,6 @@ int fc5::ensure_ndg(fc5_ctx* ctx, u32 dmax) {
   HIPCHK(ctx, hipMemcpy(ctx->d_ndg, h.data(), h.size() * 4, hipMemcpyHostToDevice));
   ctx->ndg_K = K;
   ctx->ndg_dmax = 64 * K - 1;
-  return FC5_OK;
-}
-
-int fc5::ensure_c6g(fc5_ctx* ctx, u32 cmax) {
-  if (ctx->d_c6g && ctx->c6g_max >= cmax) return FC5_OK;
-  std::vector<u32> h((size_t)cmax + 1);
-  for (u32 c = 0; c <= cmax; ++c) {
-    const u64 c3 = (u64)c * c * c;
-    const u128 c6 = (u128)c3 * c3;
-    h[c] = (u32)(kGateModU - (u64)(c6 % kGateModU));
-  }
-  if (ctx->d_c6g) (void)hipFree(ctx->d_c6g);
-  ctx->d_c6g = nullptr;
-  if (hipMalloc(&ctx->d_c6g, h.size() * 4) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "c table");
-  HIPCHK(ctx, hipMemcpy(ctx->d_c6g, h.data(), h.size() * 4, hipMemcpyHostToDevice));
-  ctx->c6g_max = cmax;
   return FC5_OK;
 }
 

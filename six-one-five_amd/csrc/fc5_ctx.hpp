@@ -134,8 +134,6 @@ struct fc5_ctx {
   u64 d_hits_cap = 0;
   Counters* d_ctr = nullptr;  // [cls] batched kernel, [8 + cls] plain kernels
   u16* d_p6 = nullptr;        // x^6 mod 504 (504 entries) then mod 247 (247 entries)
-  u32* d_c6g = nullptr;       // (124488 - c^6 mod 124488) for c = 0..c6g_max
-  u32 c6g_max = 0;
   fc5_result class_res[6]{};
   u64* d_nhits = nullptr;
   u32* d_item_ctr = nullptr;
@@ -209,7 +207,6 @@ inline int abi_exception(fc5_ctx* ctx) noexcept {
 
 namespace fc5 {
 // shared run-buffer helpers (fc5_api.cpp)
-int ensure_c6g(fc5_ctx* ctx, u32 cmax);
 int ensure_ndg(fc5_ctx* ctx, u32 dmax);
 int ensure_hits(fc5_ctx* ctx);
 DevUnit5 dev_unit5(const Cls5Unit& U, u32 uid, u32 band);

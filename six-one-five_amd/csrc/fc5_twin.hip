@@ -115,7 +115,7 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
   for (u32 c4b = lo4; c4b <= hi4; c4b += 64u) {
     const u32 c4 = c4b + lane;
     const bool in = c4 <= hi4;
-    const u32 n4 = in ? a.c6g[c4] : 0u;
+    const u32 n4 = in ? neg_p6_gate(c4) : 0u;  // c4 <= lim < 2^16
     u32 t4 = xg + n4;
     t4 = min(t4, t4 - kGateMod);  // (T - c4^6) mod 124488
     bool live = in;
@@ -156,7 +156,7 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
       for (u32 cb = l; cb <= h; cb += 64u) {
         const u32 c = cb + lane;
         const bool inn = c <= h;
-        const bool pass = inn && (!a.use_gate || gate_pass(s_gate, xi, a.c6g[c]));
+        const bool pass = inn && (!a.use_gate || gate_pass(s_gate, xi, neg_p6_gate(c)));
         const u64 pm = __ballot(pass), im = __ballot(inn);
         n_calls += (u64)__popcll(pm) * vi;
         n_gated += (u64)__popcll(im & ~pm) * vi;

@@ -39,7 +39,6 @@ struct TwinArgs {
   u32 n_cands, n_items;
   const u32* gate;  // 124488-bit pair-sum bitmap (g504 && g247)
   const u32* tri;   // triple-sum sets: 504-bit map in words 0..15, 247-bit map in 16..23
-  const u32* c6g;   // (124488 - c^6 mod 124488) for c = 0..max lim
   u32 use_gate, use_tri;
   u32 c4_div;       // 3: the reference kernel's c4 bound; 4: exact bound
   u32 tile;         // 0: every c4 once; > 0: the reference launch's unbounded c4 tiles of this width

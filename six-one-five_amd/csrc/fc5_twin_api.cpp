@@ -98,7 +98,6 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
                 fc5_twin_result* res) {
   std::vector<DevTwinCand> dev(cands.size());
   u64 items = 0;
-  u32 max_lim = 0;
   for (size_t i = 0; i < cands.size(); ++i) {
     const TwinCand& C = cands[i];
     DevTwinCand& D = dev[i];
@@ -113,12 +112,10 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
     D.nres = C.nres;
     for (u32 k = 0; k < C.nres; ++k) D.res[k] = (u8)C.res[k];
     items += D.rows;
-    max_lim = std::max(max_lim, C.lim);
   }
   if (items > kMaxItems) FAIL(ctx, FC5_E_ARG, "too many twin work items in one launch (more than 2^32 - 2^20)");
   int rc = ensure_twin(ctx, dev.size());
   if (rc != FC5_OK) return rc;
-  if ((rc = ensure_c6g(ctx, max_lim + 64)) != FC5_OK) return rc;
   if ((rc = ensure_hits(ctx)) != FC5_OK) return rc;
   if (!dev.empty())
     HIPCHK(ctx, hipMemcpy(ctx->d_twin, dev.data(), dev.size() * sizeof(DevTwinCand), hipMemcpyHostToDevice));
@@ -128,7 +125,6 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
   a.n_items = (u32)items;
   a.gate = ctx->d_gate;
   a.tri = ctx->d_tri;
-  a.c6g = ctx->d_c6g;
   a.use_gate = (u32)ctx->use_gate;
   a.use_tri = (u32)ctx->twin_tri;
   a.c4_div = (u32)ctx->twin_c4_div;
