@@ -38,8 +38,9 @@ __global__ __launch_bounds__(256) void k_probe_only(XorDev x, u64 n, u64 salt, u
   if (acc) atomicAdd(sink, (unsigned long long)acc);
 }
 
-__global__ __launch_bounds__(256) void k_gather_raw(const u64* cells, u64 m, ModDiv md, u64 n, u64 salt,
-                                                    u32 policy, unsigned long long* sink) {
+// md divides by the cell count, so every index is below it
+__global__ __launch_bounds__(256) void k_gather_raw(const u64* cells, ModDiv md, u64 n, u64 salt, u32 policy,
+                                                    unsigned long long* sink) {
   u64 acc = 0;
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
     acc ^= load_cell_policy(cells + mod_fast(m_mix64(i + salt), md), policy);
@@ -58,7 +59,7 @@ int launch_probe_only(const XorDev& x, u64 n, u64 salt, unsigned long long* sink
 
 int launch_gather_raw(const u64* cells, u64 m, u64 n, u64 salt, u32 policy, unsigned long long* sink, int grid,
                       hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_raw, dim3(grid), dim3(256), 0, s, cells, m, make_moddiv(m), n, salt, policy, sink);
+  hipLaunchKernelGGL(k_gather_raw, dim3(grid), dim3(256), 0, s, cells, make_moddiv(m), n, salt, policy, sink);
   return (int)hipGetLastError();
 }
 
