@@ -161,3 +161,14 @@ def test_cli_hunt_bbj_units_and_jobs(tmp_path):
     text = [tuple(map(int, l.split())) for l in open(tt).read().split("\n") if l]
     assert [(c, B, u, f1, f2, tl, th) for c, B, u, f1, f2, tl, th in jobs] == text
     assert len(text) > 1000
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asking for more exits non-zero with the reason"""
+    import subprocess
+    import sys
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(repo, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stdout + r.stderr
