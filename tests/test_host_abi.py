@@ -172,3 +172,22 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("r", [12, 24, 40, 64])
+def test_host_builder_other_widths_match_oracle_and_reference(r, tmp_path):
+    """cells of 12, 24 and 40 bits straddle byte and word boundaries, 64 bits
+    fill the word: the product's builder writes the oracle's bytes, and the
+    reference xor_build_save's where oracle/_ref is built"""
+    import subprocess
+    p = str(tmp_path / "mine.bin")
+    fc5.build_pairs_file(500, r, p)
+    q = str(tmp_path / "oracle.bin")
+    O.Table.build_pairs(500, r).save(q)
+    mine = open(p, "rb").read()
+    assert mine == open(q, "rb").read()
+    ref = os.path.join(REPO, "oracle", "_ref", "xor_build_save")
+    if os.path.exists(ref):
+        s = str(tmp_path / "ref.bin")
+        subprocess.run([ref, "--N", "500", "--r", str(r), "--out", s], check=True, capture_output=True)
+        assert mine == open(s, "rb").read()
