@@ -222,3 +222,81 @@ def test_errors(ctx):
     assert e.value.code == fc5.E_STATE
     assert ctx.exact_lookup([65, 66, 2], [10, 10, 10]) == [(0, 1, 2), (2, 1, 1)]
     ctx.exact_clear()
+
+
+def test_find3_verify_table_flags_planted_targets(ctx):
+    """classes 2-4: planted keys (T(d) - c3^6) mod 2^64 in the xor table and their
+    exact targets T(d) - c3^6 as extra values of the exact table; verifying the
+    maybe-hits by lookup flags exactly the plants and leaves every other hit as
+    the pair search has it"""
+    say("find3 with verify_table")
+    cls, Bs, us = fc5.emit_units(2000001, 2000400, classes=(2, 3, 4))
+    rng = np.random.default_rng(12)
+    pts = []
+    for i in rng.choice(len(Bs), 16, replace=False):
+        k, B, u = int(cls[i]), int(Bs[i]), int(us[i])
+        res, mod, fmax = O.free_d_cls234(B, u, k)
+        T = None
+        for _ in range(200):
+            d = int(res[rng.integers(len(res))] + mod * rng.integers(fmax // mod + 1))
+            T = O.T_peel1(B, u, {2: 14, 3: 21, 4: 7}[k], d) if 1 <= d <= fmax else None
+            if T:
+                break
+        if not T:
+            continue
+        hi = min(int(round(T ** (1 / 6))) + 2, (B - 1) // 42)
+        while hi ** 6 > T:
+            hi -= 1
+        lo = 1
+        while lo <= hi and 3 * lo ** 6 < T:
+            lo += 1
+        if lo > hi:
+            continue
+        c3 = int(rng.integers(lo, hi + 1))
+        pts.append((int(i), d, c3, T - c3 ** 6))
+    assert len(pts) >= 8
+    ctx.table_build(2428, 48, extra=np.array([v % 2 ** 64 for *_, v in pts], np.uint64))
+    units = fc5.units_array(Bs, us)
+    ctx.set_option("use_gate", 0)  # plants need not pass the residue gate
+    try:
+        ctx.set_option("verify_table", 0)
+        ref = ctx.run(units, cls=cls)
+        ctx.exact_build(2428, [v for *_, v in pts])
+        ctx.set_option("verify_table", 1)
+        out = ctx.run(units, cls=cls)
+    finally:
+        ctx.set_option("verify_table", 0)
+        ctx.set_option("use_gate", 1)
+        ctx.exact_clear()
+    flagged = {(int(h["unit"]), int(h["d"]), int(h["e"])) for h in out.hits if h["flags"] & fc5.HIT_EXTRA}
+    assert flagged == {(i, d, c) for i, d, c, _ in pts}
+    stripped = out.hits.copy()
+    stripped["flags"] &= ~np.uint32(fc5.HIT_EXTRA)
+    assert np.array_equal(stripped, ref.hits)
+    assert out.solutions == ref.solutions
+
+
+def test_twin_verify_table_finds_the_known(ctx):
+    """the 54 known (6,1,7) class-5 solutions, verified by lookups of
+    T - c4^6 - c3^6 in the exact sorted pair table instead of the pair search"""
+    import json
+    say("twin with verify_table")
+    known = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "known_617_cls5.json")))
+    N = max(k["B"] for k in known) // 42
+    ctx.table_build(N, 48, on_gpu=True)
+    rows = []
+    for k in known:
+        c = fc5.twin_cands(k["B"])
+        rows.append(c[(c["u"] == k["u"]) & (c["w"] == k["w"])])
+    cands = np.concatenate(rows)
+    res0, hits0, sols0 = ctx.twin_run(cands)
+    ctx.exact_build(N)
+    ctx.set_option("verify_table", 1)
+    try:
+        res1, hits1, sols1 = ctx.twin_run(cands)
+    finally:
+        ctx.set_option("verify_table", 0)
+        ctx.exact_clear()
+    assert sols1 == sols0
+    assert sorted(tuple(s["terms"]) for s in sols1 if s["gcd"] == 1) == sorted(tuple(k["terms"]) for k in known)
+    assert np.array_equal(hits1, hits0)
