@@ -191,3 +191,27 @@ def test_host_builder_other_widths_match_oracle_and_reference(r, tmp_path):
         s = str(tmp_path / "ref.bin")
         subprocess.run([ref, "--N", "500", "--r", str(r), "--out", s], check=True, capture_output=True)
         assert mine == open(s, "rb").read()
+
+
+def test_gate_residue_reduction_in_device_header():
+    """the device computes c^6 mod 124488 in 32-bit steps (mulmod_gate in
+    fc5_device.hpp): its constant must be 2^32 mod 124488, and the reduction,
+    emulated with 32-bit wrap-around, equals direct modular powers at the edges
+    of its domain (c < 2^25) and on a random sample"""
+    import random
+    src = open(os.path.join(REPO, "six-one-five_amd", "csrc", "fc5_device.hpp")).read()
+    m = re.search(r"\(u32\)\(p >> 32\) \* (\d+)u", src)
+    assert m and int(m.group(1)) == 2 ** 32 % 124488
+    k = int(m.group(1))
+
+    def mm(x, y):
+        p = x * y
+        assert p < 2 ** 51
+        r = ((p & 0xFFFFFFFF) % 124488 + (p >> 32) * k) & 0xFFFFFFFF
+        return r % 124488
+
+    rnd = random.Random(6)
+    cs = list(range(0, 2048)) + [2 ** 25 - 1 - i for i in range(2048)] + [rnd.randrange(2 ** 25) for _ in range(20000)]
+    for c in cs:
+        c2 = mm(c, c)
+        assert mm(mm(c2, c), mm(c2, c)) == pow(c, 6, 124488), c
