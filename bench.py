@@ -38,7 +38,7 @@ CONFIGS = {
     3: dict(lo=2000001, hi=2010000, N=47857, plants=0, plant_mode=None, seed=0x615 << 8 | 3,
             classes=(2, 3, 4, 5)),
     4: dict(lo=3200001, hi=3210000, N=76428, plants=256, plant_mode="side", seed=0x615 << 8 | 4, classes=(5,),
-            exact=True),
+            exact=True, from_file=True),
 }
 COUNTER_KEYS = ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe", "jobs", "fail_t")
 
@@ -95,9 +95,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-every", type=int, default=160, help="CPU oracle on every k-th unit")
     ap.add_argument("--cpu-offset", type=int, default=0, help="first unit of the CPU oracle sample")
-    ap.add_argument("--cpu-ref-span", type=int, default=100,
-                    help="integers of the band (from its start) the reference CPU solver searches "
-                         "(bands below 2.2M; whole band if smaller)")
+    ap.add_argument("--table-dir", default=None,
+                    help="4: directory for the table file written and loaded back before timing (default: TMPDIR)")
+    ap.add_argument("--no-table-file", action="store_true",
+                    help="4: search the device-built table directly instead of one loaded from a file")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-measure", action="store_true", help="skip the gather/copy microbenchmarks")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape")
@@ -145,6 +146,10 @@ def main():
     t_build = time.time() - t0
     log(f"[bench] device table N={hdr.N} keys={hdr.n_keys} cells={hdr.m_cells} "
         f"packed={fc5_packed_gb(hdr):.2f} GB built in {t_build:.1f}s")
+    table_file = None
+    if cfg.get("from_file") and not args.band and not args.no_table_file:
+        table_file = table_via_file(ctx, hdr, args.table_dir)
+        hdr = ctx.table_info()
     fpr = ctx.fpr_smoke()
     if fpr > 16:
         raise SystemExit(f"FPR smoke {fpr} > 16: refuse")
@@ -248,47 +253,49 @@ def main():
         t1 = time.perf_counter()
         ob2 = b2.run(fc5.MODE_BATCHED)
         tb = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        oc = b2.run(fc5.MODE_CHUNKED)
+        tc = time.perf_counter() - t1
         b2.close()
         for cl in (2, 3, 4, 5):
             for k in COUNTER_KEYS:
-                assert op.by_class[cl][k] == ob.by_class[cl][k] == ob2.by_class[cl][k], (cl, k)
+                assert op.by_class[cl][k] == ob.by_class[cl][k] == ob2.by_class[cl][k] == oc.by_class[cl][k], (cl, k)
         hb = np.sort(ob.hits, order=["unit", "e", "d", "flags"])
         hp = np.sort(op.hits, order=["unit", "e", "d", "flags"])
-        assert np.array_equal(hb, hp), "plain and batched hit lists differ"
-        pv = op.result["calls"] / tp
+        hc = np.sort(oc.hits, order=["unit", "e", "d", "flags"])
+        assert np.array_equal(hb, hp) and np.array_equal(hb, hc), "plain, chunked and batched hit lists differ"
+        pv, bv, cv = op.result["calls"] / tp, ob2.result["calls"] / tb, oc.result["calls"] / tc
         plain = {"subset": f"every {args.plain_every}th unit ({len(sub)} units)", "points_per_s": pv,
-                 "batched_points_per_s_same_subset": ob2.result["calls"] / tb,
+                 "batched_points_per_s_same_subset": bv,
+                 "chunked_points_per_s_same_subset": cv,
                  "eligible_B_per_hour": n_B * 3600.0 * pv / r["calls"],
-                 "speedup_batched_vs_plain": value / pv, "counters_identical": True}
-        log(f"[bench] plain form {pv:.4e} points/s on the subset; batched/plain = {value / pv:.1f}x")
+                 "speedup_batched_vs_plain": value / pv,
+                 "speedup_batched_vs_plain_same_subset": bv / pv,
+                 "speedup_batched_vs_chunked_same_subset": bv / cv,
+                 "speedup_chunked_vs_plain_same_subset": cv / pv,
+                 "chunked": "the T-space kernel launched once per 16 units with a synchronize after each "
+                            "(the reference's launch pattern): batched/chunked isolates the persistent "
+                            "single launch, chunked/plain the T-space arithmetic",
+                 "counters_identical": True}
+        log(f"[bench] plain form {pv:.4e} points/s on the subset; batched/plain = {value / pv:.1f}x; "
+            f"same subset: batched {bv:.4e}, chunked {cv:.4e} (batched/chunked {bv / cv:.2f}x, "
+            f"chunked/plain {cv / pv:.1f}x)")
 
     ceiling = None
     if not args.no_measure:
-        n = 1 << 31
-        pms, rms = ctx.measure_gathers(n)
-        cms = ctx.measure_copy(4 << 30)
-        probe_rate = n / (pms / 1000.0)
-        kernel_probe_rate = r["probes"] / kern_s
-        ceiling = {"probe_only_probes_per_s": probe_rate, "raw_random_8B_gathers_per_s": 3 * n / (rms / 1000.0),
-                   "probe_only_transaction_GBps_at_64B": probe_rate * 3 * 64 / 1e9,
-                   "copy_GBps": 2 * (4 << 30) / (cms / 1000.0) / 1e9,
-                   "kernel_probe_rate_frac_of_probe_only": kernel_probe_rate / probe_rate}
-        log(f"[bench] gather ceiling: {probe_rate:.4e} probes/s probe-only, "
-            f"{3 * n / (rms / 1000.0):.4e} raw gathers/s; kernel at {kernel_probe_rate / probe_rate:.1%}; "
-            f"copy {ceiling['copy_GBps']:.0f} GB/s")
+        ceiling = measure_ceilings(ctx, hdr, r["probes"] / kern_s)
 
-    traffic = None
-    tpath = os.path.join(REPO, "profiles", f"traffic_cls5_batched_config{args.config}.json")
-    if os.path.exists(tpath) and not args.band:
-        traffic = json.load(open(tpath)).get("traffic_bytes_per_launch")
+    traffic, valu = None, None
+    if not args.band:
+        tprof = load_profile(f"traffic_cls5_batched_B{cfg['lo']}-{cfg['hi']}.json")
+        traffic = tprof.get("traffic_bytes_per_launch") if tprof else None
+        valu = valu_roofline(load_profile(f"valu_cls5_batched_B{cfg['lo']}-{cfg['hi']}.json"), ceiling)
 
-    cpu = cpu_port = None
+    cpu = cpu_ref = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(ctx, units, ucls, args.cpu_every, cfg, r, pkeys, args.cpu_offset)
-        if not mixed:  # below 2.2M the reference's own CPU solver is the baseline, the port beside it
-            ref = cpu_reference(cfg, cpu["cores"], args.cpu_ref_span, calls_closed)
-            if ref is not None:
-                cpu, cpu_port = ref, cpu
+        if not args.band:
+            cpu_ref = cpu_reference(args.config)
 
     line = {
         "metric": METRIC, "value": value, "unit": "(d,e) grid points/s", "n_gpus": 1,
@@ -299,7 +306,9 @@ def main():
         "config": {"workload": "batched persistent cls5 peel, eligible B in "
                                f"[{cfg['lo']},{cfg['hi']}] ({n_B} B, {len(units)} units, "
                                f"{r['calls']} grid points), xor pair table N={hdr.N} r={hdr.r} "
-                               f"({fc5_packed_gb(hdr):.2f} GB packed, device-built), "
+                               f"({fc5_packed_gb(hdr):.2f} GB packed, "
+                               + ("device-built, written to a .bin file and loaded from it" if table_file and
+                                  table_file.get("loaded") else "device-built") + "), "
                                f"{len(plants)} planted keys ({cfg['plant_mode'] or 'none'} table)",
                    "B_lo": cfg["lo"], "B_hi": cfg["hi"], "N": int(hdr.N), "r": int(hdr.r),
                    "units": int(len(units)), "parallelism": "single GPU"},
@@ -319,14 +328,16 @@ def main():
                      "transaction_GBps_at_64B": r["probes"] * 3 * 64 / kern_s / 1e9,
                      # measured HBM traffic (128-byte lines per random gather) over the same launch time
                      "traffic_GBps": traffic / kern_s / 1e9 if traffic else None,
-                     "traffic_frac": traffic / kern_s / 1e9 / HBM_PEAK_GBS if traffic else None},
+                     "traffic_frac": traffic / kern_s / 1e9 / HBM_PEAK_GBS if traffic else None,
+                     "valu": valu},
         "gather_ceiling": ceiling,
         "autotune": tune,
         "graph": not args.no_graph,
         "plain_form": plain,
         "cpu_baseline": cpu,
-        "cpu_port": cpu_port,
+        "cpu_reference": cpu_ref,
         "table_build_s": t_build,
+        "table_file": table_file,
         "exact_table": exact,
     }
     print(json.dumps(line), flush=True)
@@ -592,6 +603,102 @@ def bench_serve(args):
     ctx.close()
 
 
+def table_via_file(ctx, hdr, where):
+    """The --load-table path (fourcore_cls5_gpu_v4.cu:568-594, fourcore_xor_store.hpp:147-181):
+    write the resident table as a StoreHeader-v2 .bin file, then load it back with
+    the exact-N check and the FPR-smoke refusal, which re-uploads and widens the
+    cells.  The file is removed afterwards.  Skipped (said loudly, and the
+    workload says so) when the directory lacks room for it."""
+    import shutil
+    import tempfile
+    d = where or tempfile.gettempdir()
+    path = os.path.join(d, f"fc5_xor_N{hdr.N}_r{hdr.r}_{os.getpid()}.bin")
+    need = (hdr.m_cells * hdr.r + 7) // 8 + 8 + 64
+    free = shutil.disk_usage(d).free
+    if free < need + (2 << 30):
+        log(f"[bench] NOT loading the table from a file: {d} has {free / 1e9:.1f} GB free, the file needs "
+            f"{need / 1e9:.1f} GB; the device-built table is searched")
+        return {"loaded": False, "reason": f"{d}: {free / 1e9:.1f} GB free < {need / 1e9:.1f} GB"}
+    before = {f: getattr(hdr, f) for f, _ in fc5.StoreHeader._fields_}
+    try:
+        t0 = time.time()
+        ctx.table_save(path)
+        t_save = time.time() - t0
+        t0 = time.time()
+        ctx.table_load(path, hdr.N)
+        t_load = time.time() - t0
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+    after = ctx.table_info()
+    assert {f: getattr(after, f) for f, _ in fc5.StoreHeader._fields_} == before, "loaded header differs"
+    log(f"[bench] table written to {path} ({need / 1e9:.2f} GB) in {t_save:.1f}s and loaded back in "
+        f"{t_load:.1f}s (read, exact-N check N={hdr.N}, FPR smoke, upload, widen)")
+    return {"loaded": True, "bytes": need, "save_s": t_save, "load_s": t_load, "dir": d,
+            "load": "fc5_table_load: read, exact-N check, FPR smoke on the host bytes, H2D copy, cell widening"}
+
+
+def load_profile(name):
+    """a committed rocprofv3 counter summary (JSON) under profiles/, or None (said loudly)"""
+    p = os.path.join(REPO, "profiles", name)
+    if not os.path.exists(p):
+        log(f"[bench] no counter summary profiles/{name}: its roofline fields stay null")
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def measure_ceilings(ctx, hdr, kernel_probe_rate):
+    """Roofline microbenchmarks on the resident main table: the probe-only and
+    raw random-gather rates, random gathers confined to growing regions of the
+    table (L2 / memory-side cache / HBM), a read-only stream over a 24 GiB
+    buffer, a copy (calibrates the profiler's byte counters) and the integer
+    VALU issue rates."""
+    n = 1 << 31
+    pms, rms = ctx.measure_gathers(n)
+    cms = ctx.measure_copy(4 << 30)
+    read_bytes = 24 << 30
+    sms = ctx.measure_read(read_bytes)
+    probe_rate = n / (pms / 1000.0)
+    regions = []
+    for mib in (4, 16, 64, 128, 192, 256, 384, 512, 1024, 4096):
+        cells = (mib << 20) // 8
+        if cells > hdr.m_cells:
+            break
+        g = 1 << 29
+        regions.append({"MiB": mib, "gathers_per_s": g / (ctx.measure_gathers_region(cells, g) / 1000.0)})
+    regions.append({"MiB": hdr.m_cells * 8 / 2 ** 20, "gathers_per_s": 3 * n / (rms / 1000.0)})
+    valu = {name: ctx.measure_valu(k) for k, name in enumerate(("int32_xad", "mad_u64_u32", "add_u128_carry",
+                                                                  "mix_7pct_int64"))}
+    ceiling = {"probe_only_probes_per_s": probe_rate, "raw_random_8B_gathers_per_s": 3 * n / (rms / 1000.0),
+               "probe_only_transaction_GBps_at_64B": probe_rate * 3 * 64 / 1e9,
+               "copy_GBps": 2 * (4 << 30) / (cms / 1000.0) / 1e9,
+               "read_stream_GBps": read_bytes / (sms / 1000.0) / 1e9,
+               "random_gathers_by_region": regions,
+               "valu_wave_instr_per_s": valu,
+               "kernel_probe_rate_frac_of_probe_only": kernel_probe_rate / probe_rate}
+    log(f"[bench] gather ceiling: {probe_rate:.4e} probes/s probe-only, "
+        f"{3 * n / (rms / 1000.0):.4e} raw gathers/s; kernel at {kernel_probe_rate / probe_rate:.1%}; "
+        f"copy {ceiling['copy_GBps']:.0f} GB/s, read stream {ceiling['read_stream_GBps']:.0f} GB/s")
+    log("[bench] random gathers by region: " + ", ".join(f"{x['MiB']:.0f} MiB {x['gathers_per_s']:.3e}/s"
+                                                        for x in regions))
+    log("[bench] VALU wave-instructions/s: " + ", ".join(f"{k} {v:.3e}" for k, v in valu.items()))
+    return ceiling
+
+
+def valu_roofline(prof, ceiling):
+    """VALU side of the roofline: the kernel's VALU instruction rate (rocprofv3
+    SQ_INSTS_VALU over the launch time of the same run) against the issue rate of
+    the microbenchmark with the kernel's INT64 share, and VALUBusy"""
+    if not prof:
+        return None
+    rate = prof["SQ_INSTS_VALU"] / (prof["kernel_ms"] / 1000.0)
+    ceil = ceiling["valu_wave_instr_per_s"]["mix_7pct_int64"] if ceiling else None
+    return {"busy": prof["VALUBusy"] / 100.0, "achieved": rate, "ceiling": ceil, "unit": "wave instructions/s",
+            "frac": rate / ceil if ceil else None, "int64_share": prof["SQ_INSTS_VALU_INT64"] / prof["SQ_INSTS_VALU"],
+            "source": prof.get("commands")}
+
+
 def twin_traffic(lo, hi):
     """HBM bytes per launch from the committed rocprofv3 counter summary of the
     same workload, or None"""
@@ -667,43 +774,24 @@ def cpu_baseline(ctx, units, ucls, every, cfg, r, pkeys, offset=0):
                       f"identical to the GPU on that sample"}
 
 
-def cpu_reference(cfg, threads, span, band_points):
-    """The reference's own CPU search of the class-5 (d,e) grid: solve_516_v2
-    (every e, every d, an exact lookup in its sorted pair index,
-    solve_516_v2.cpp:400-416), compiled from the reference's sources into
-    oracle/_ref and timed on this host over the whole band when it holds at
-    most 2e9 grid points, else over its first `span` integers (its own
-    pair-index build excluded, as it reports it); the rate is the closed-form
-    grid-point count of the same units over that time.  None when the binary
-    is absent or the band is past its B <= 2.2M limit."""
-    import re
-    import subprocess
-
-    exe = os.path.join(REPO, "oracle", "_ref", "solve_516_v2")
-    if not os.path.exists(exe) or cfg["hi"] > 2200000:
+def cpu_reference(config):
+    """The reference's own CPU class-5 search (solve_516_v2, every (e, d) with an
+    exact pair-index lookup, solve_516_v2.cpp:400-416) as timed in the build
+    container by oracle/time_ref_cpu.py -- it is never run on this host.  Its
+    figures for configs 1 and 2 (the solver stops at B <= 2.2M), or None."""
+    p = os.path.join(REPO, "oracle", "ref_cpu_timing.json")
+    if not os.path.exists(p):
         return None
-    lo = cfg["lo"]
-    hi = cfg["hi"] if band_points <= 2e9 else min(cfg["hi"], lo + span - 1)  # small bands in full
-    _, Bs, us = fc5.emit_units(lo, hi)
-    points = fc5.units_calls(fc5.units_array(Bs, us))
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    t0 = time.perf_counter()
-    r = subprocess.run([exe, str(lo), str(hi), "5"], capture_output=True, text=True, env=env, timeout=600)
-    wall = time.perf_counter() - t0
-    if r.returncode != 0:
-        raise SystemExit(f"reference solver failed: {r.stderr[-400:]}")
-    build_ms = int(re.search(r"index built in (\d+) ms", r.stderr).group(1))
-    units = int(re.search(r"unit candidates tested: (\d+)", r.stderr).group(1))
-    sols = int(re.search(r"total solutions: (\d+)", r.stderr).group(1))
-    assert units == len(Bs) and sols == 0, (units, len(Bs), sols)
-    dt = wall - build_ms / 1000.0
-    log(f"[bench] reference CPU solver {points / dt:.4e} points/s on B in [{lo},{hi}] ({units} units), "
-        f"{threads} threads, {dt:.1f}s (+{build_ms / 1000:.1f}s pair index)")
-    return {"value": points / dt, "unit": "(d,e) grid points/s", "cores": threads, "kind": "reference",
-            "host": host_cpu(),
-            "sample": f"B in [{lo},{hi}] ({units} class-5 units, {points} grid points): the reference's "
-                      f"solve_516_v2 class-5 search built from its sources, {dt:.1f} s after its "
-                      f"{build_ms / 1000:.1f} s pair-index build; 0 solutions as expected"}
+    with open(p) as f:
+        t = json.load(f)
+    c = t["configs"].get(str(config))
+    if c is None:
+        return None
+    return {"value": c["points_per_s"], "unit": "(d,e) grid points/s", "cores": t["threads"], "kind": "reference",
+            "host": t["host"], "where": "build container (oracle/time_ref_cpu.py), not this host",
+            "sample": f"B in [{c['B_lo']},{c['B_hi']}] ({c['units']} class-5 units, {c['grid_points']} grid points): "
+                      f"the reference's solve_516_v2 class-5 search built from its sources, {c['search_s']:.1f} s "
+                      f"after its {c['index_build_s']:.1f} s pair-index build; {c['solutions']} solutions"}
 
 
 if __name__ == "__main__":

@@ -45,7 +45,9 @@ enum {
 /* search modes of fc5_band_run */
 enum {
   FC5_MODE_PLAIN = 0,   /* reference shape: 16-unit chunks, one synchronous launch each */
-  FC5_MODE_BATCHED = 1  /* one persistent launch fed by a device work queue */
+  FC5_MODE_BATCHED = 1, /* one persistent launch fed by a device work queue */
+  FC5_MODE_CHUNKED = 2  /* the batched kernel in the reference's launch pattern: one
+                           synchronous launch per 16 units (fourcore_cls5_gpu_v4.cu:468-501) */
 };
 
 /* StoreHeader v2 (spike/ribbon/store_header.hpp:23-37): 64 bytes, little-endian */
@@ -348,6 +350,15 @@ int fc5_serve_stop(fc5_ctx *ctx);
 int fc5_measure_gathers(fc5_ctx *ctx, uint64_t n, double *probe_ms, double *raw_ms);
 /* a streaming device copy of `bytes` bytes (reads and writes that many) */
 int fc5_measure_copy(fc5_ctx *ctx, uint64_t bytes, double *ms);
+/* n raw random 8-byte loads confined to the first region_cells cells of the
+ * main table (the L2 / memory-side-cache / HBM curve of random gathers) */
+int fc5_measure_gathers_region(fc5_ctx *ctx, uint64_t region_cells, uint64_t n, double *ms);
+/* a read-only streaming pass over a fresh device buffer of `bytes` bytes */
+int fc5_measure_read(fc5_ctx *ctx, uint64_t bytes, double *ms);
+/* integer VALU issue rate in wavefront instructions per second at full
+ * occupancy: kind 0 = 32-bit (v_xad_u32), 1 = v_mad_u64_u32, 2 = 128-bit
+ * add-with-carry chains, 3 = one v_mad_u64_u32 per 13 32-bit operations */
+int fc5_measure_valu(fc5_ctx *ctx, int kind, double *instr_per_s);
 
 /* ---- host helpers shared by the CLIs (no device use) ---- */
 /* Stage-1 emission over [lo,hi] for classes in cls_mask (bit c = class c),

@@ -942,10 +942,54 @@ static int ensure_graph(fc5_ctx* ctx, fc5_band* b, const BatchedArgs& a, int gri
   return FC5_OK;
 }
 
+// The batched kernel in the reference's launch pattern: one launch per 16
+// units of one arena (fourcore_cls5_gpu_v4.cu:468-501), each followed by a
+// synchronize.  Same kernel and counters as the persistent launch; it
+// isolates what batching into one launch buys from what the T-space kernel does.
+static int run_chunked(fc5_ctx* ctx, fc5_band* b, BatchedArgs a, int bpc) {
+  constexpr u32 kChunk = 16;
+  std::vector<Segment> segs;
+  std::vector<u32> items;
+  for (u32 u0 = 0; u0 < b->n_dev; u0 += kChunk) {
+    const u32 n = std::min(kChunk, b->n_dev - u0);
+    const u32 i0 = b->dev_h[u0].item_base, i1 = u0 + n < b->n_dev ? b->dev_h[u0 + n].item_base : b->n_items;
+    segs.push_back(Segment{0, i0, u0, n, 0, 0});
+    items.push_back(i1 - i0);
+  }
+  for (u32 u0 = 0; u0 < b->n_dev3; u0 += kChunk) {
+    const u32 n = std::min(kChunk, b->n_dev3 - u0);
+    const u32 i0 = b->dev3_h[u0].item_base, i1 = u0 + n < b->n_dev3 ? b->dev3_h[u0 + n].item_base : b->n_items3;
+    segs.push_back(Segment{0, i0, u0, n, 1, 0});
+    items.push_back(i1 - i0);
+  }
+  if (segs.empty()) return FC5_OK;
+  Segment* d_segs = nullptr;
+  if (hipMalloc(&d_segs, segs.size() * sizeof(Segment)) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "chunk segments");
+  int rc = FC5_OK;
+  if (hipMemcpy(d_segs, segs.data(), segs.size() * sizeof(Segment), hipMemcpyHostToDevice) != hipSuccess)
+    rc = FC5_E_HIP;
+  for (size_t i = 0; i < segs.size() && rc == FC5_OK; ++i) {
+    if (!items[i]) continue;
+    a.segs = d_segs + i;
+    a.n_segs = 1;
+    a.total = items[i];
+    const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)items[i]));
+    if (launch_fill_u64((u64*)ctx->d_item_ctr, 1, 0, ctx->stream) != 0 ||
+        launch_cls5_batched(a, grid, ctx->rows_per_item, ctx->stream) != 0 ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      rc = FC5_E_HIP;
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_segs);
+  if (rc != FC5_OK) ctx->err = "chunked launch";
+  return rc;
+}
+
 extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res) try {
   if (!ctx || !b || !res) return FC5_E_ARG;
   NOT_SERVING(ctx);
-  if (mode != FC5_MODE_PLAIN && mode != FC5_MODE_BATCHED) FAIL(ctx, FC5_E_ARG, "unknown mode");
+  if (mode != FC5_MODE_PLAIN && mode != FC5_MODE_BATCHED && mode != FC5_MODE_CHUNKED)
+    FAIL(ctx, FC5_E_ARG, "unknown mode");
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no pair table loaded");
   const auto t0 = std::chrono::steady_clock::now();
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -958,12 +1002,13 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   u64 fail_host[6] = {0}, jobs_host[6] = {0};
   rc = renumber(ctx, b);
   if (rc != FC5_OK) return rc;
-  const bool batched = mode == FC5_MODE_BATCHED;
+  const bool batched = mode != FC5_MODE_PLAIN;  // the T-space kernel, in one launch or in chunks
   const bool any_items = b->n_items + b->n_items3 > 0;
   BatchedArgs a = batched_args(ctx, b);
   const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(ctx->rows_per_item);
   const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)(b->n_items + b->n_items3)));
-  const bool graph = batched && any_items && ctx->use_graph && b->irregular.empty() && b->irregular3.empty();
+  const bool graph = mode == FC5_MODE_BATCHED && any_items && ctx->use_graph && b->irregular.empty() &&
+                     b->irregular3.empty();
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (graph) {
     rc = ensure_graph(ctx, b, a, grid);
@@ -973,8 +1018,12 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
     rc = reset_counters(ctx);
     if (rc != FC5_OK) return rc;
     if (batched) {
-      if (any_items && launch_cls5_batched(a, grid, ctx->rows_per_item, ctx->stream) != 0)
+      if (mode == FC5_MODE_CHUNKED) {
+        rc = run_chunked(ctx, b, a, bpc);
+        if (rc != FC5_OK) return rc;
+      } else if (any_items && launch_cls5_batched(a, grid, ctx->rows_per_item, ctx->stream) != 0) {
         FAIL(ctx, FC5_E_HIP, "batched launch");
+      }
       if (!b->irregular.empty()) {
         rc = run_plain_units(ctx, b, b->irregular, false);
         if (rc != FC5_OK) return rc;
@@ -1270,6 +1319,91 @@ extern "C" int fc5_measure_gathers(fc5_ctx* ctx, uint64_t n, double* probe_ms, d
   (void)hipFree(sink);
   *probe_ms = a;
   *raw_ms = b;
+  return rc;
+} catch (...) {
+  return abi_exception(ctx);
+}
+
+extern "C" int fc5_measure_gathers_region(fc5_ctx* ctx, uint64_t region_cells, uint64_t n, double* ms) try {
+  if (!ctx || !ms || n == 0 || region_cells == 0) return FC5_E_ARG;
+  NOT_SERVING(ctx);
+  if (!ctx->main.present || !ctx->main.cells) FAIL(ctx, FC5_E_STATE, "no device table");
+  if (region_cells > ctx->main.hdr.m_cells) FAIL(ctx, FC5_E_ARG, "region larger than the table");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  unsigned long long* sink = nullptr;
+  if (hipMalloc(&sink, 8) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "sink");
+  const int grid = ctx->num_cu * 8;
+  const u32 policy = ctx->main.dev.policy;
+  float t = 0;
+  int rc = FC5_OK;
+  if (launch_gather_raw(ctx->main.cells, region_cells, n / 4, 3, policy, sink, grid, ctx->stream) != 0 ||  // warm-up
+      hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess ||
+      launch_gather_raw(ctx->main.cells, region_cells, n, 13, policy, sink, grid, ctx->stream) != 0 ||
+      hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess ||
+      hipEventElapsedTime(&t, ctx->ev0, ctx->ev1) != hipSuccess) {
+    ctx->err = "gather measurement failed";
+    rc = FC5_E_HIP;
+  }
+  (void)hipFree(sink);
+  *ms = t;
+  return rc;
+} catch (...) {
+  return abi_exception(ctx);
+}
+
+extern "C" int fc5_measure_read(fc5_ctx* ctx, uint64_t bytes, double* ms) try {
+  if (!ctx || !ms || bytes < 16) return FC5_E_ARG;
+  NOT_SERVING(ctx);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const u64 n = bytes / 16 * 2;  // u64 words of a whole number of 16-byte words
+  u64* src = nullptr;
+  unsigned long long* sink = nullptr;
+  if (hipMalloc(&src, n * 8) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "read buffer");
+  if (hipMalloc(&sink, 8) != hipSuccess) {
+    (void)hipFree(src);
+    FAIL(ctx, FC5_E_NOMEM, "sink");
+  }
+  const int grid = ctx->num_cu * 8;
+  float t = 0;
+  int rc = FC5_OK;
+  if (launch_fill_u64(src, n, 0x0123456789abcdefULL, ctx->stream) != 0 ||
+      launch_read_stream(src, n * 8, sink, grid, ctx->stream) != 0 ||  // warm-up
+      hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess ||
+      launch_read_stream(src, n * 8, sink, grid, ctx->stream) != 0 ||
+      hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess ||
+      hipEventElapsedTime(&t, ctx->ev0, ctx->ev1) != hipSuccess) {
+    ctx->err = "read measurement failed";
+    rc = FC5_E_HIP;
+  }
+  (void)hipFree(src);
+  (void)hipFree(sink);
+  *ms = t;
+  return rc;
+} catch (...) {
+  return abi_exception(ctx);
+}
+
+extern "C" int fc5_measure_valu(fc5_ctx* ctx, int kind, double* instr_per_s) try {
+  if (!ctx || !instr_per_s || kind < 0 || kind > 3) return FC5_E_ARG;
+  NOT_SERVING(ctx);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  unsigned long long* sink = nullptr;
+  if (hipMalloc(&sink, 8) != hipSuccess) FAIL(ctx, FC5_E_NOMEM, "sink");
+  const int grid = ctx->num_cu * 8;  // 32 waves per CU: the occupancy limit of 256-thread groups
+  const u32 iters = (u32)(4500000u / kValuInstrs[kind]);
+  float t = 0;
+  int rc = FC5_OK;
+  if (launch_valu(kind, iters / 8, 1, sink, grid, ctx->stream) != 0 ||  // warm-up
+      hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess ||
+      launch_valu(kind, iters, 0x2545F491u, sink, grid, ctx->stream) != 0 ||
+      hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess ||
+      hipEventElapsedTime(&t, ctx->ev0, ctx->ev1) != hipSuccess) {
+    ctx->err = "VALU measurement failed";
+    rc = FC5_E_HIP;
+  }
+  (void)hipFree(sink);
+  const double waves = (double)grid * 256 / 64;
+  *instr_per_s = t > 0 ? waves * iters * kValuInstrs[kind] / (t / 1000.0) : 0.0;
   return rc;
 } catch (...) {
   return abi_exception(ctx);

@@ -35,26 +35,12 @@
 #include <vector>
 
 #include "fc5_build.hpp"
+#include "fc5_mix.hpp"
 #include "fc5_moddiv.hpp"
 
 namespace fc5 {
 
 typedef unsigned long long ull;
-
-__device__ __forceinline__ u64 b_mix64(u64 x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdULL;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ULL;
-  x ^= x >> 33;
-  return x;
-}
-
-__device__ __forceinline__ void b_cells(u64 h, u64 block, const ModDiv& md, u64 c[3]) {
-  c[0] = mod_fast(h, md);
-  c[1] = mod_fast(b_mix64(h ^ 0x9E3779B97F4A7C15ULL), md) + block;
-  c[2] = mod_fast(b_mix64(h ^ 0xBF58476D1CE4E5B9ULL), md) + 2 * block;
-}
 
 __device__ __forceinline__ u64 shfl64(u64 v, int lane) {
   const u32 lo = (u32)__shfl((int)(u32)v, lane, 64), hi = (u32)__shfl((int)(u32)(v >> 32), lane, 64);
@@ -86,7 +72,7 @@ __device__ __forceinline__ void set_insert(u64* slots, u64 nslots, u32* zero_fla
     atomicOr(zero_flag, 1u);
     return;
   }
-  u64 pos = __umul64hi(b_mix64(key ^ 0x2545F4914F6CDD1DULL), nslots);
+  u64 pos = __umul64hi(mix64(key ^ 0x2545F4914F6CDD1DULL), nslots);
   for (;;) {
     const u64 prev = atomicCAS((ull*)&slots[pos], 0ULL, (ull)key);
     if (prev == 0 || prev == key) return;
@@ -126,9 +112,9 @@ __global__ void k_count_slots(const u64* slots, u64 n, ull* count) {
 
 __device__ __forceinline__ void add_key(u64 key, u64 seed, u64 block, const ModDiv& md, u64* setxor, u32* counts,
                                         u32* overflow) {
-  const u64 h = b_mix64(key + seed);
+  const u64 h = mix64(key + seed);
   u64 c[3];
-  b_cells(h, block, md, c);
+  xor_cells(h, block, md, c);
   for (int t = 0; t < 3; ++t) {
     atomicXor((ull*)&setxor[c[t]], (ull)h);
     if (cnt_add(counts, c[t], 1) >= 0xffffu) atomicOr(overflow, 1u);
@@ -169,7 +155,7 @@ __global__ void k_claim(const u64* front, u64 nf, const u32* counts, const u64* 
       i = front[k];
       if (cnt_get(counts, i) == 1) {
         u64 c[3];
-        b_cells(setxor[i], block, md, c);
+        xor_cells(setxor[i], block, md, c);
         const u64 first = cnt_get(counts, c[0]) == 1 ? c[0] : (cnt_get(counts, c[1]) == 1 ? c[1] : c[2]);
         take = first == i;
       }
@@ -190,7 +176,7 @@ __global__ void k_apply(const u64* peel, u64 np, u32* counts, u64* setxor, u64 b
     if (k < np) {
       const u64 i = peel[k];
       const u64 h = setxor[i];
-      b_cells(h, block, md, c);
+      xor_cells(h, block, md, c);
       for (int q = 0; q < 3; ++q) {
         if (c[q] == i) continue;
         atomicXor((ull*)&setxor[c[q]], (ull)h);
@@ -210,7 +196,7 @@ __global__ void k_assign(const u64* peel, u64 np, u64* cells, u64 block, ModDiv 
     const u64 i = peel[k];
     const u64 h = cells[i];
     u64 c[3];
-    b_cells(h, block, md, c);
+    xor_cells(h, block, md, c);
     u64 v = h & mask;
     for (int q = 0; q < 3; ++q)
       if (c[q] != i) v ^= cells[c[q]];

@@ -1,10 +1,11 @@
 // fc5_device.hpp -- device helpers shared by the search kernels (gfx950).
 //
-// 64-bit mixer and xor-filter query on expanded cells, wavefront reductions
+// xor-filter query on expanded cells, wavefront reductions
 // and lane reads, the LDS residue gate test and an exact sixth root.
 #pragma once
 
 #include "fc5_kernels.hpp"
+#include "fc5_mix.hpp"
 
 namespace fc5 {
 
@@ -13,26 +14,14 @@ typedef unsigned __int128 u128;
 constexpr u32 kGateMod = 124488;  // 504 * 247
 constexpr u32 kGateWords = (kGateMod + 31) / 32;
 
-__device__ __forceinline__ u64 d_mix64(u64 x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdULL;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ULL;
-  x ^= x >> 33;
-  return x;
-}
-
 // xor filter query on expanded cells (xor_filter.hpp:53-64,472-483)
 __device__ __forceinline__ bool d_query(const XorDev& x, u64 key) {
   if (!x.present) return false;
-  const u64 h = d_mix64(key + x.seed);
-  const u64 h1 = d_mix64(h ^ 0x9E3779B97F4A7C15ULL);
-  const u64 h2 = d_mix64(h ^ 0xBF58476D1CE4E5B9ULL);
-  const u64 i0 = mod_fast(h, x.md);
-  const u64 i1 = mod_fast(h1, x.md) + x.block;
-  const u64 i2 = mod_fast(h2, x.md) + 2 * x.block;
-  const u64 got = load_cell_policy(x.cells + i0, x.policy) ^ load_cell_policy(x.cells + i1, x.policy) ^
-                  load_cell_policy(x.cells + i2, x.policy);
+  const u64 h = mix64(key + x.seed);
+  u64 c[3];
+  xor_cells(h, x.block, x.md, c);
+  const u64 got = load_cell_policy(x.cells + c[0], x.policy) ^ load_cell_policy(x.cells + c[1], x.policy) ^
+                  load_cell_policy(x.cells + c[2], x.policy);
   return got == (h & x.mask);
 }
 

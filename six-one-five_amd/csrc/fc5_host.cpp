@@ -456,7 +456,10 @@ bool save_bin(const std::string& path, const XorTable& t, std::string& msg) {
     msg = "short write " + path;
     return false;
   }
-  std::fclose(f);
+  if (std::fclose(f) != 0) {  // buffered bytes that could not be written (a full disk)
+    msg = "short write " + path;
+    return false;
+  }
   return true;
 }
 
@@ -514,8 +517,8 @@ void store_cell(u8* p, u64 cell, u32 r, u64 v) {
 
 void cell_indices(u64 h, u64 block, u64 out[3]) {
   out[0] = h % block;
-  out[1] = mix64(h ^ 0x9E3779B97F4A7C15ULL) % block + block;
-  out[2] = mix64(h ^ 0xBF58476D1CE4E5B9ULL) % block + 2 * block;
+  out[1] = mix64(h ^ kCellSalt1) % block + block;
+  out[2] = mix64(h ^ kCellSalt2) % block + 2 * block;
 }
 
 bool xor_query(const XorTable& t, u64 key) {

@@ -23,6 +23,8 @@
 #include <string>
 #include <vector>
 
+#include "fc5_mix.hpp"
+
 namespace fc5 {
 
 using u8 = std::uint8_t;
@@ -142,14 +144,6 @@ bool save_bin(const std::string& path, const XorTable& t, std::string& msg);
 // ---------------------------------------------------------------------------
 // xor filter
 // ---------------------------------------------------------------------------
-inline u64 mix64(u64 x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdULL;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ULL;
-  x ^= x >> 33;
-  return x;
-}
 inline u64 fp_mask(u32 r) { return r >= 64 ? ~0ULL : ((1ULL << r) - 1ULL); }
 size_t packed_bytes(u64 m_cells, u32 r);
 size_t capacity_for(size_t n);

@@ -192,6 +192,10 @@ int launch_probe_only(const XorDev& x, u64 n, u64 salt, unsigned long long* sink
 int launch_gather_raw(const u64* cells, u64 m, u64 n, u64 salt, u32 policy, unsigned long long* sink, int grid,
                       hipStream_t s);
 int launch_copy_u64(const u64* src, u64* dst, u64 n, int grid, hipStream_t s);
+int launch_read_stream(const void* src, u64 bytes, unsigned long long* sink, int grid, hipStream_t s);
+// integer VALU issue microbenchmark; kind 0..3, VALU instructions per loop step per wave: kValuInstrs[kind]
+constexpr u32 kValuInstrs[4] = {8, 8, 32, 112};
+int launch_valu(int kind, u32 iters, u32 seed, unsigned long long* sink, int grid, hipStream_t s);
 int launch_stage1(const Stage1Args& a, int pass, hipStream_t s);
 // exclusive prefix sum of n u64 (temp == nullptr: query temp_bytes)
 int stage1_scan(const u64* counts_in, u64* offsets_out, u64 n, void* temp, size_t& temp_bytes, hipStream_t s);

@@ -26,7 +26,7 @@ _lib = C.CDLL(LIB_PATH)
 
 OK, E_ARG, E_IO, E_FORMAT, E_NMISMATCH, E_FPR, E_HIP, E_NOMEM, E_STATE, E_OVERFLOW, E_BUILD, E_PENDING = (
     0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10, -11)
-MODE_PLAIN, MODE_BATCHED = 0, 1
+MODE_PLAIN, MODE_BATCHED, MODE_CHUNKED = 0, 1, 2
 
 
 class StoreHeader(C.Structure):
@@ -128,6 +128,9 @@ _sig = {
     "fc5_units_calls": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_uint64)]),
     "fc5_measure_gathers": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "fc5_measure_copy": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double)]),
+    "fc5_measure_gathers_region": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
+    "fc5_measure_read": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double)]),
+    "fc5_measure_valu": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
     "fc5_emit_units": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double,
                                  _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_build_pairs_file": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.POINTER(StoreHeader)]),
@@ -368,6 +371,24 @@ class Context:
     def measure_copy(self, nbytes):
         t = C.c_double()
         self._check(_lib.fc5_measure_copy(self.h, nbytes, C.byref(t)))
+        return t.value
+
+    def measure_gathers_region(self, region_cells, n):
+        """ms for n raw random cell loads within the first region_cells cells of the main table"""
+        t = C.c_double()
+        self._check(_lib.fc5_measure_gathers_region(self.h, region_cells, n, C.byref(t)))
+        return t.value
+
+    def measure_read(self, nbytes):
+        """ms for one read-only streaming pass over a fresh device buffer of nbytes"""
+        t = C.c_double()
+        self._check(_lib.fc5_measure_read(self.h, nbytes, C.byref(t)))
+        return t.value
+
+    def measure_valu(self, kind):
+        """integer VALU issue rate (wavefront instructions/s) of microbenchmark kind 0..3"""
+        t = C.c_double()
+        self._check(_lib.fc5_measure_valu(self.h, kind, C.byref(t)))
         return t.value
 
     def band(self, units, cls=None):

@@ -63,7 +63,8 @@ def _compare(out, oc, ohits):
         assert np.array_equal(got[f], ohits[f]), f
 
 
-@pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN], ids=["batched", "plain"])
+@pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN, fc5.MODE_CHUNKED],
+                         ids=["batched", "plain", "chunked"])
 @pytest.mark.parametrize("r", [48, 16])
 def test_band_100001_102000_full_matches_oracle(ctx, mode, r):
     say(f"B in [100001,102000] full r={r} mode={mode}")
@@ -338,7 +339,8 @@ def _compare_mixed(out, oc, oh):
         assert np.array_equal(got[f], oh[f]), f
 
 
-@pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN], ids=["batched", "plain"])
+@pytest.mark.parametrize("mode", [fc5.MODE_BATCHED, fc5.MODE_PLAIN, fc5.MODE_CHUNKED],
+                         ids=["batched", "plain", "chunked"])
 def test_mixed_classes_band_100001_102000_match_oracle(ctx, mode):
     """classes 2-5 of B in [100001,102000] in one band: cls2-4 run the find3
     leaf in the same persistent kernel as cls5 (batched) or the reference

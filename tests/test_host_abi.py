@@ -215,3 +215,28 @@ def test_gate_residue_reduction_in_device_header():
     for c in cs:
         c2 = mm(c, c)
         assert mm(mm(c2, c), mm(c2, c)) == pow(c, 6, 124488), c
+
+
+def test_bench_default_config_finds_its_counter_summaries():
+    """the default bench line (config 4) fills roofline.traffic and roofline.valu
+    from the committed rocprofv3 summaries named for its band"""
+    import bench
+    cfg = bench.CONFIGS[4]
+    band = f"B{cfg['lo']}-{cfg['hi']}"
+    t = bench.load_profile(f"traffic_cls5_batched_{band}.json")
+    v = bench.load_profile(f"valu_cls5_batched_{band}.json")
+    assert t and t["traffic_bytes_per_launch"] > t["algorithmic_bytes_per_launch"] > 0
+    assert 1.9 < t["calibration"]["fetch_factor"] < 2.1
+    assert v and 0 < v["VALUBusy"] < 100 and v["SQ_INSTS_VALU"] > v["SQ_INSTS_VALU_INT64"] > 0
+    roof = bench.valu_roofline(v, {"valu_wave_instr_per_s": {"mix_7pct_int64": 6e11}})
+    assert 0 < roof["frac"] < 1 and 0 < roof["int64_share"] < 0.2
+
+
+def test_bench_runs_no_reference_binary():
+    """bench.py never executes a program built from the reference's sources (the
+    reference CPU figure is read from the container timing), so nothing built from
+    the reference runs on a GPU host"""
+    src = open(os.path.join(REPO, "bench.py")).read()
+    assert "subprocess" not in src and "solve_516_v2\"" not in src and "_ref\"" not in src
+    ref = json.load(open(os.path.join(REPO, "oracle", "ref_cpu_timing.json")))
+    assert set(ref["configs"]) == {"1", "2"} and ref["configs"]["1"]["grid_points"] == 10973276
