@@ -582,6 +582,21 @@ def bench_serve(args):
     n_B = sum(eligible_count(lo, hi) for lo, hi in reqs)
     log(f"[bench] served {len(bands)} bands in {span:.1f}s: p50 {np.percentile(lat, 50):.2f}s "
         f"p99 {np.percentile(lat, 99):.2f}s, {points / span:.4e} points/s, device busy {busy_ms / 1000 / span:.1%}")
+    # roofline of the serving rounds: algorithmic cell bytes of every probe over the summed round time
+    cell_bytes = math.ceil(args.r / 8)
+    alg = probes * 3 * cell_bytes
+    roof = {"bound": "hbm", "achieved": alg / (busy_ms / 1000.0) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": alg / (busy_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, "traffic": None, "kernel": "k_cls5_batched",
+            "per_unit": f"{3 * cell_bytes} B per filter probe (3 cells x {cell_bytes} B), over the summed "
+                        "event time of the serving rounds"}
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle on a unit sample of the bands checked alone, over the same table bytes
+        cu = np.concatenate([bands[i][3] for i in check])
+        cc = np.concatenate([bands[i][2] for i in check]).astype(np.uint32)
+        every = max(1, len(cu) // 600)
+        cpu = cpu_baseline(ctx, cu, cc, every, {"plant_mode": None}, None, np.zeros(0, np.uint64))
+        cpu["sample"] = f"units of the {len(check)} bands checked alone: " + cpu["sample"]
     line = {
         "metric": "serving: band latency p50/p99 + throughput, Poisson arrivals, 1 GPU, bit-exact",
         "value": points / span, "unit": "(d,e) grid points + c3 values / s", "n_gpus": 1,
@@ -598,6 +613,7 @@ def bench_serve(args):
         "eligible_B_per_hour": n_B * 3600.0 / span, "filter_probes_per_s": probes / span,
         "device_busy_frac": busy_ms / 1000.0 / span, "rounds": st1["rounds"] - st0["rounds"],
         "bands_checked_alone": len(alone), "n_hits": n_hits,
+        "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

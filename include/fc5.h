@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FC5_ABI_VERSION 1
+#define FC5_ABI_VERSION 2
 
 enum {
   FC5_OK = 0,
@@ -308,6 +308,8 @@ typedef struct {
 typedef struct {
   double t_submit, t_admit, t_done;  /* seconds on the server's steady clock */
   uint64_t rounds;                   /* scheduling rounds the band took part in */
+  double t_release;                  /* its last round ended: counter slot and arena ranges
+                                        given back (the exact check of its hits follows) */
 } fc5_serve_timing;
 typedef struct {
   uint64_t submitted, admitted, done, pending, active, rounds;
@@ -321,11 +323,11 @@ int fc5_serve_start(fc5_ctx *ctx, const fc5_serve_opts *opts); /* opts may be NU
  * classes and lines the packing refuses are skipped and counted */
 int fc5_serve_submit(fc5_ctx *ctx, const uint32_t *cls, const fc5_unit *units, size_t n, uint64_t *band_id);
 /* FC5_OK and the band's result when done, FC5_E_PENDING otherwise; a finished
- * band whose hit buffer overflowed returns FC5_E_OVERFLOW with the result.
- * The bands of a round share its hit buffer (hit_cap records), so the number
- * of hits a round dropped cannot be split among them: for a served band
- * n_hits counts the hits kept and hits_dropped is 1 when a round it took part
- * in overflowed (every band of that round is flagged), 0 otherwise */
+ * band some of whose hits did not fit a round's hit buffer (hit_cap records,
+ * shared by the bands of the round) returns FC5_E_OVERFLOW with the result:
+ * n_hits counts the hits kept, hits_dropped the band's own hits that were
+ * dropped (the kernel counts every band's hit records, so bands of the same
+ * round whose hits were all kept are not flagged) */
 int fc5_serve_poll(fc5_ctx *ctx, uint64_t band_id, fc5_result *res);
 /* blocks until done or timeout_s elapses (timeout_s < 0: no limit) */
 int fc5_serve_wait(fc5_ctx *ctx, uint64_t band_id, double timeout_s, fc5_result *res);

@@ -497,7 +497,8 @@ DevUnit3 fc5::dev_unit3(const Cls234Unit& U, u32 uid, u32 band) {
 }
 
 int fc5::run_pair_jobs(fc5_ctx* ctx, const std::vector<PairJob>& jobs, const PairLaunch& launch,
-                       std::vector<PairHit>& out) {
+                       std::vector<PairHit>& out, hipStream_t st) {
+  if (!st) st = ctx->stream;
   out.clear();
   if (jobs.empty()) return FC5_OK;
   if (jobs.size() > 0xffffffffull) FAIL(ctx, FC5_E_ARG, "too many hits to verify");
@@ -520,10 +521,10 @@ int fc5::run_pair_jobs(fc5_ctx* ctx, const std::vector<PairJob>& jobs, const Pai
     }
     u64 count = 0;
     if (hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice) != hipSuccess ||
-        launch_fill_u64(d_count, 1, 0, ctx->stream) != 0 ||
-        launch(d_jobs, (u32)jobs.size(), d_out, cap, reinterpret_cast<u32*>(d_count), ctx->stream) != 0 ||
-        hipMemcpyAsync(&count, d_count, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        launch_fill_u64(d_count, 1, 0, st) != 0 ||
+        launch(d_jobs, (u32)jobs.size(), d_out, cap, reinterpret_cast<u32*>(d_count), st) != 0 ||
+        hipMemcpyAsync(&count, d_count, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
       release();
       FAIL(ctx, FC5_E_HIP, "pair search");
     }
@@ -543,8 +544,8 @@ int fc5::run_pair_jobs(fc5_ctx* ctx, const std::vector<PairJob>& jobs, const Pai
 }
 
 int fc5::device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
-                      std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra) {
-  if (ctx->verify_table) return exact_table_pairs(ctx, targets, pairs, extra);
+                      std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra, hipStream_t st) {
+  if (ctx->verify_table) return exact_table_pairs(ctx, targets, pairs, extra, st);
   if (extra) extra->assign(targets.size(), 0);
   pairs.assign(targets.size(), {});
   std::vector<PairJob> jobs;
@@ -555,7 +556,7 @@ int fc5::device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
       which.push_back(i);
     }
   std::vector<PairHit> out;
-  const int rc = run_pair_jobs(ctx, jobs, launch_pair_search, out);
+  const int rc = run_pair_jobs(ctx, jobs, launch_pair_search, out, st);
   if (rc != FC5_OK) return rc;
   for (const PairHit& h : out) pairs[which[h.job]].emplace_back(h.a, h.b);
   for (auto& v : pairs) std::sort(v.begin(), v.end());
@@ -1363,7 +1364,7 @@ extern "C" int fc5_measure_read(fc5_ctx* ctx, uint64_t bytes, double* ms) try {
     (void)hipFree(src);
     FAIL(ctx, FC5_E_NOMEM, "sink");
   }
-  const int grid = ctx->num_cu * 8;
+  const int grid = ctx->num_cu * 32;
   float t = 0;
   int rc = FC5_OK;
   if (launch_fill_u64(src, n, 0x0123456789abcdefULL, ctx->stream) != 0 ||

@@ -229,15 +229,19 @@ int stage1_emit(fc5_ctx* ctx, hipStream_t s, u64 lo, u64 hi, unsigned mask, doub
 // search (k_pair_search), or from the exact sorted table when the option
 // verify_table is set, which also marks in extra (if given) the targets equal
 // to one of its extra values
+// (on stream st; nullptr: the context's stream)
 int device_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
-                 std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra = nullptr);
+                 std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra = nullptr,
+                 hipStream_t st = nullptr);
 // pair jobs on the device through launch (the pair search or the exact-table
 // lookup), into an output buffer that grows once when the first pass overflows
 using PairLaunch = std::function<int(const PairJob*, u32, PairHit*, u32, u32*, hipStream_t)>;
-int run_pair_jobs(fc5_ctx* ctx, const std::vector<PairJob>& jobs, const PairLaunch& launch, std::vector<PairHit>& out);
+int run_pair_jobs(fc5_ctx* ctx, const std::vector<PairJob>& jobs, const PairLaunch& launch, std::vector<PairHit>& out,
+                  hipStream_t st = nullptr);
 // the exact-table half of device_pairs (fc5_exact_api.cpp)
 int exact_table_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
-                      std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra);
+                      std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra,
+                      hipStream_t st = nullptr);
 void free_exact(fc5_ctx* ctx);
 // twin buffers (fc5_twin_api.cpp)
 void free_twin(fc5_ctx* ctx);

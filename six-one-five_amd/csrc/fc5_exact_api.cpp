@@ -121,7 +121,8 @@ extern "C" int fc5_exact_read(fc5_ctx* ctx, uint64_t first, size_t n, uint64_t* 
 }
 
 int fc5::exact_table_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
-                           std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra) {
+                           std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra,
+                           hipStream_t st) {
   pairs.assign(targets.size(), {});
   if (extra) extra->assign(targets.size(), 0);
   if (!ctx->d_exact) FAIL(ctx, FC5_E_STATE, "verify_table is set but no exact table is built");
@@ -144,7 +145,7 @@ int fc5::exact_table_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
       [keys, n, s](const PairJob* j, u32 nj, PairHit* o, u32 cap, u32* count, hipStream_t st) {
         return launch_exact_lookup(keys, n, s, j, nj, o, cap, count, st);
       },
-      out);
+      out, st);
   if (rc != FC5_OK) return rc;
   for (const PairHit& h : out) {
     if (h.pad) {

@@ -861,17 +861,26 @@ static PairTarget cls5_target(int N, u64 B, u64 u, u32 lim, u32 d, u32 e, u64 fp
   return t;
 }
 
-static int cls5_finish(u64 B, u64 u, u32 d, u32 e, u32 a, u32 b, std::vector<Solution>& out) {
-  if (!verify_cls5(B, u, d, e, a, b)) return 0;
-  i64 t[5] = {42LL * a, 42LL * b, 21LL * (i64)d, 14LL * (i64)e, (i64)u};
+bool solution_record(u64 B, u64 u, u32 cls, u32 d, u32 e, u32 a, u32 b, u32 c, const i64 terms[5], Solution& out) {
+  i64 t[5];
+  std::copy(terms, terms + 5, t);
   std::sort(t, t + 5);
-  bool bad = false;
-  for (int i = 0; i < 4; ++i) bad |= t[i] == t[i + 1];
-  if (bad || t[4] >= (i64)B) return 0;
+  for (int i = 0; i < 4; ++i)
+    if (t[i] == t[i + 1]) return false;
+  if (t[4] >= (i64)B || t[0] < 1) return false;
   i64 g = t[0];
   for (int i = 1; i < 5; ++i) g = gcd_i64(g, t[i]);
   g = gcd_i64(g, (i64)B);
-  out.push_back(Solution{B, u, 5, d, e, a, b, 0, {t[0], t[1], t[2], t[3], t[4]}, g});
+  out = Solution{B, u, cls, d, e, a, b, c, {t[0], t[1], t[2], t[3], t[4]}, g};
+  return true;
+}
+
+static int cls5_finish(u64 B, u64 u, u32 d, u32 e, u32 a, u32 b, std::vector<Solution>& out) {
+  if (!verify_cls5(B, u, d, e, a, b)) return 0;
+  const i64 t[5] = {42LL * a, 42LL * b, 21LL * (i64)d, 14LL * (i64)e, (i64)u};
+  Solution s;
+  if (!solution_record(B, u, 5, d, e, a, b, 0, t, s)) return 0;
+  out.push_back(s);
   return 1;
 }
 
@@ -907,15 +916,10 @@ static PairTarget find3_target(int N, int cls, u64 B, u64 u, u32 d, u32 c, u64 f
 static int find3_finish(int cls, u64 B, u64 u, u32 d, u32 c, u32 a, u32 b, std::vector<Solution>& out) {
   const u64 f = (u64)free_factor(cls);
   if (!verify_cls234(B, u, f, d, a, b, c)) return 0;
-  i64 t[5] = {42LL * a, 42LL * b, 42LL * c, (i64)f * (i64)d, (i64)u};
-  std::sort(t, t + 5);
-  bool bad = false;
-  for (int i = 0; i < 4; ++i) bad |= t[i] == t[i + 1];
-  if (bad || t[4] >= (i64)B || t[0] < 1) return 0;
-  i64 g = t[0];
-  for (int i = 1; i < 5; ++i) g = std::gcd(g, t[i]);
-  g = std::gcd(g, (i64)B);
-  out.push_back(Solution{B, u, (u32)cls, d, 0, a, b, c, {t[0], t[1], t[2], t[3], t[4]}, g});
+  const i64 t[5] = {42LL * a, 42LL * b, 42LL * c, (i64)f * (i64)d, (i64)u};
+  Solution s;
+  if (!solution_record(B, u, (u32)cls, d, 0, a, b, c, t, s)) return 0;
+  out.push_back(s);
   return 1;
 }
 

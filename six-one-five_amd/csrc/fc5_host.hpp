@@ -222,6 +222,12 @@ struct Solution {
   i64 terms[5];
   i64 gcd;
 };
+// The record of an identity that has been checked exactly (the SOLUTION line,
+// fourcore_cls5_gpu_v4.cu:524-536, fourcore_find_v4.cu:427-457): the five terms
+// sorted ascending; refused (false) when two terms are equal, the smallest is
+// not positive or the largest is not below B; gcd = gcd of the terms and B
+// (gcd 1: "primitive").
+bool solution_record(u64 B, u64 u, u32 cls, u32 d, u32 e, u32 a, u32 b, u32 c, const i64 terms[5], Solution& out);
 // Exact pairs a <= b <= hi with a^6 + b^6 == T, by two pointers over x^6.
 // The reference recovers every pair with a^6+b^6 = fp (mod 2^64) and then keeps
 // the exact ones (fourcore_cls5_gpu_v4.cu:517-522); fp = T mod 2^64 for every

@@ -63,6 +63,7 @@ __device__ __forceinline__ void drain(const BatchedArgs& a, const u32* qu, const
       Counters* c = a.ctr + band;
       if (flags & 1u) atomicAdd((unsigned long long*)&c->maybe, 1ULL);
       if (flags & 2u) atomicAdd((unsigned long long*)&c->side_maybe, 1ULL);
+      atomicAdd((unsigned long long*)&c->recs, 1ULL);
       const u64 s = atomicAdd((unsigned long long*)a.nhits, 1ULL);
       if (s < a.hit_cap) {
         HitRec h;
@@ -448,6 +449,7 @@ __global__ __launch_bounds__(256) void k_find3_plain(PlainF3Args a) {
       Counters* c = a.ctr + C.band;
       if (flags & 1u) atomicAdd((unsigned long long*)&c->maybe, 1ULL);
       if (flags & 2u) atomicAdd((unsigned long long*)&c->side_maybe, 1ULL);
+      atomicAdd((unsigned long long*)&c->recs, 1ULL);
       const u64 s = atomicAdd((unsigned long long*)a.nhits, 1ULL);
       if (s < a.hit_cap) {
         HitRec h;

@@ -83,6 +83,7 @@ struct PlainUnit {
 
 struct Counters {
   u64 calls, neg, zero, gated, probes, maybe, side_maybe, pos, jobs, failT;
+  u64 recs;  // hit records produced (written or dropped at hit_cap), batched kernel
 };
 
 struct HitRec {

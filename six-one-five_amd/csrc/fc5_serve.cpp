@@ -6,12 +6,18 @@
 // resident unit arenas (first-fit ranges of two device arrays sized from the
 // memory budget).  Every scheduling round builds a segment list that gives
 // each admitted band an equal share of the round's grid points (its next work
-// items), launches the persistent kernel once over all of them, reads back the
-// per-slot counters and the round's hits, and finalizes the bands whose last
-// items ran.  Bands that arrive while others run join at the next round and
-// share the device with them (continuous batching).  Counters are sums over
-// work items and hits are sorted, so a band's result does not depend on what
-// it shared rounds with.
+// items) and launches the persistent kernel once over all of them; the
+// per-slot counters and the round's hit count are copied back behind it on the
+// same stream.  Two rounds are in flight: round k+1 is planned and enqueued
+// before round k's results are taken apart (hits read back on a copy stream
+// while round k+1 runs, demultiplexed by counter slot, work estimates
+// updated), so the device does not wait for the host between rounds.  A band
+// whose last items ran gives back its slot and arena ranges at once and goes
+// to a finalizer thread, which verifies its hits (exact check on a stream of
+// its own) and publishes its result.  Bands that arrive while others run join
+// at the next planned round and share the device with them (continuous
+// batching).  Counters are sums over work items and hits are sorted, so a
+// band's result does not depend on what it shared rounds with.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -79,7 +85,7 @@ struct Arena {
   }
 };
 
-enum BandState { kPending, kActive, kDone };
+enum BandState { kPending, kActive, kFinalizing, kDone };
 
 struct SBand {
   u64 id = 0;
@@ -92,19 +98,38 @@ struct SBand {
   std::vector<Cls234Unit> units3;
   std::vector<DevUnit5> dev5;
   std::vector<DevUnit3> dev3;
-  u32 items5 = 0, items3 = 0, next5 = 0, next3 = 0;
+  u32 items5 = 0, items3 = 0;
+  u32 next5 = 0, next3 = 0;  // items given to planned rounds
+  u32 done5 = 0, done3 = 0;  // items of rounds that have completed
   double cost5 = 1, cost3 = 1;  // estimated grid points (c3 values) per work item
   u64 off5 = 0, off3 = 0;
   int slot = -1;
-  bool overflow = false;
+  u64 recs_seen = 0;  // hit records the kernel produced for this band so far
+  u64 dropped = 0;    // of them, records beyond a round's hit_cap
+  Counters ctr[8]{};  // the band's counter slots after its last round
   std::vector<fc5_hit> hits;
   std::vector<fc5_solution> sols;
   fc5_result res{};
   fc5_result cls_res[6]{};
-  double t_submit = 0, t_admit = 0, t_done = 0;
+  double t_submit = 0, t_admit = 0, t_done = 0, t_release = 0;
   u64 rounds = 0;
   double kernel_ms = 0;
-  bool finished_items() const { return next5 >= items5 && next3 >= items3; }
+  bool unplanned() const { return next5 < items5 || next3 < items3; }
+  bool finished_items() const { return done5 >= items5 && done3 >= items3; }
+};
+
+// one scheduling round's buffers; two alternate so that one round can be
+// planned and enqueued while the other still runs
+struct Round {
+  Segment* d_segs = nullptr;
+  Segment* h_segs = nullptr;  // pinned
+  HitRec* d_hits = nullptr;
+  u64* d_nhits = nullptr;
+  HitRec* h_hits = nullptr;   // pinned
+  u64* h_nhits = nullptr;     // pinned
+  Counters* h_ctr = nullptr;  // pinned snapshot of every counter slot after the round
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
+  std::vector<std::pair<SBand*, std::pair<u32, u32>>> took;  // band, items of each kind
 };
 
 }  // namespace
@@ -125,15 +150,20 @@ struct fc5_server {
   Arena a5, a3;
   DevUnit5* d5 = nullptr;
   DevUnit3* d3 = nullptr;
-  Segment* d_segs = nullptr;
-  Segment* h_segs = nullptr;  // pinned
   Counters* d_ctr = nullptr;
-  Counters* h_ctr = nullptr;  // pinned
-  HitRec* h_hits = nullptr;   // pinned
-  u64* h_nhits = nullptr;     // pinned
+  HitRec* d_hits2 = nullptr;  // second round's hit buffer (the first is the context's)
+  u64* d_nhits2 = nullptr;
+  Round rr[2];
+  u64 planned = 0, processed = 0;  // rounds enqueued / taken apart; round k uses rr[k % 2]
   u32 max_fmax2 = 0, max_lim = 0;
   hipStream_t side = nullptr;  // Stage-1 emission of submitted ranges
   std::mutex side_mu;
+  hipStream_t copy = nullptr;     // hit read-back beside the next round
+  hipStream_t vstream = nullptr;  // exact checks of finished bands (finalizer thread)
+  std::thread fin_th;
+  std::deque<SBand*> fin_q;
+  std::condition_variable cv_fin;
+  bool fin_stop = false;
   u64 submitted = 0, admitted = 0, done = 0, rounds = 0, device_bytes = 0;
   double busy_ms = 0;
   Clock::time_point t0 = Clock::now();
@@ -143,23 +173,31 @@ struct fc5_server {
 namespace {
 
 void free_server_buffers(fc5_server* S) {
-  for (void* p : {(void*)S->d5, (void*)S->d3, (void*)S->d_segs, (void*)S->d_ctr})
+  for (void* p : {(void*)S->d5, (void*)S->d3, (void*)S->d_ctr, (void*)S->d_hits2, (void*)S->d_nhits2})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)S->h_segs, (void*)S->h_ctr, (void*)S->h_hits, (void*)S->h_nhits})
-    if (p) (void)hipHostFree(p);
   S->d5 = nullptr;
   S->d3 = nullptr;
-  S->d_segs = nullptr;
   S->d_ctr = nullptr;
-  S->h_segs = nullptr;
-  S->h_ctr = nullptr;
-  S->h_hits = nullptr;
-  S->h_nhits = nullptr;
+  S->d_hits2 = nullptr;
+  S->d_nhits2 = nullptr;
+  for (Round& R : S->rr) {
+    if (R.d_segs) (void)hipFree(R.d_segs);
+    for (void* p : {(void*)R.h_segs, (void*)R.h_hits, (void*)R.h_nhits, (void*)R.h_ctr})
+      if (p) (void)hipHostFree(p);
+    for (hipEvent_t e : {R.ev0, R.ev1, R.ev_done})
+      if (e) (void)hipEventDestroy(e);
+    R = Round{};
+  }
+  for (hipStream_t* st : {&S->side, &S->copy, &S->vstream})
+    if (*st) {
+      (void)hipStreamDestroy(*st);
+      *st = nullptr;
+    }
 }
 
 // the band's result from its counter slots (same reduction as fc5_band_run)
-void collect(fc5_server* S, SBand& B) {
-  const Counters* c = S->h_ctr + (size_t)B.slot * kSlotStride;
+void collect(SBand& B) {
+  const Counters* c = B.ctr;
   std::memset(&B.res, 0, sizeof B.res);
   std::memset(B.cls_res, 0, sizeof B.cls_res);
   for (int k = 2; k <= 5; ++k) {
@@ -193,7 +231,7 @@ void collect(fc5_server* S, SBand& B) {
 // under the lock afterwards, and nothing else touches an open band's results.
 void finalize(fc5_server* S, SBand& B) {
   fc5_ctx* ctx = S->ctx;
-  if (B.slot >= 0) collect(S, B);
+  collect(B);
   std::sort(B.hits.begin(), B.hits.end(), [](const fc5_hit& x, const fc5_hit& y) {
     if (x.unit != y.unit) return x.unit < y.unit;
     if (x.e != y.e) return x.e < y.e;
@@ -224,7 +262,7 @@ void finalize(fc5_server* S, SBand& B) {
     std::vector<std::vector<std::pair<u32, u32>>> pairs;
     std::vector<u8> extra;
     std::vector<Solution> sols;
-    const int prc = device_pairs(ctx, targets, pairs, &extra);
+    const int prc = device_pairs(ctx, targets, pairs, &extra, S->vstream);
     if (prc == FC5_OK) {
       verify_finish_all(jobs, pairs, false, sols);
       for (size_t i = 0; i < extra.size(); ++i)
@@ -248,14 +286,14 @@ void finalize(fc5_server* S, SBand& B) {
     }
   }
   B.res.n_hits = B.hits.size();
-  B.res.hits_dropped = B.overflow ? 1 : 0;
+  B.res.hits_dropped = B.dropped;
   B.res.solutions = B.sols.size();
   B.res.units = B.units5.size() + B.units3.size();
   B.res.units_skipped = B.n_skipped;
   B.res.kernel_ms = B.kernel_ms;
   B.t_done = S->now();
   B.res.wall_ms = 1000.0 * (B.t_done - B.t_submit);
-  if (B.overflow) B.rc = FC5_E_OVERFLOW;
+  if (B.dropped) B.rc = FC5_E_OVERFLOW;
   // the unit arrays are no longer needed
   B.dev5 = {};
   B.dev3 = {};
@@ -269,7 +307,7 @@ int admit_locked(fc5_server* S) {
     SBand& B = *S->bands.at(S->pending.front());
     if (B.items5 + B.items3 == 0) {  // nothing to run
       S->pending.pop_front();
-      B.t_admit = S->now();
+      B.t_admit = B.t_release = S->now();
       finalize(S, B);
       B.state = kDone;
       S->done++;
@@ -307,13 +345,16 @@ int admit_locked(fc5_server* S) {
   return FC5_OK;
 }
 
-// one scheduling round over the admitted bands
-int run_round(fc5_server* S, std::vector<SBand*>& active) {
+// Plan one scheduling round over the bands with unplanned items and enqueue
+// it on the context's stream into round buffers R: segment upload, hit-count
+// and work-counter reset, the persistent kernel, then the counter snapshot and
+// the hit count copied back; R.ev_done marks the end.
+int plan_round(fc5_server* S, const std::vector<SBand*>& bands, Round& R) {
   fc5_ctx* ctx = S->ctx;
-  const double share = S->opt.round_points / (double)std::max<size_t>(1, active.size());
+  const double share = S->opt.round_points / (double)std::max<size_t>(1, bands.size());
   u32 n_segs = 0, total = 0;
-  std::vector<std::pair<SBand*, std::pair<u32, u32>>> took;  // band, items of each kind
-  for (SBand* B : active) {
+  R.took.clear();
+  for (SBand* B : bands) {
     const u32 r5 = B->items5 - B->next5, r3 = B->items3 - B->next3;
     u32 n5 = 0, n3 = 0;
     double left = share;
@@ -326,19 +367,21 @@ int run_round(fc5_server* S, std::vector<SBand*>& active) {
     n5 = (u32)std::min<u64>(n5, kMaxItems - total);
     n3 = (u32)std::min<u64>(n3, kMaxItems - total - n5);
     if (n5) {
-      S->h_segs[n_segs++] = Segment{total, B->next5, (u32)B->off5, (u32)B->dev5.size(), 0, 0};
+      R.h_segs[n_segs++] = Segment{total, B->next5, (u32)B->off5, (u32)B->dev5.size(), 0, 0};
       total += n5;
     }
     if (n3) {
-      S->h_segs[n_segs++] = Segment{total, B->next3, (u32)B->off3, (u32)B->dev3.size(), 1, 0};
+      R.h_segs[n_segs++] = Segment{total, B->next3, (u32)B->off3, (u32)B->dev3.size(), 1, 0};
       total += n3;
     }
-    took.push_back({B, {n5, n3}});
+    B->next5 += n5;
+    B->next3 += n3;
+    R.took.push_back({B, {n5, n3}});
   }
   BatchedArgs a{};
   a.units = S->d5;
   a.units3 = S->d3;
-  a.segs = S->d_segs;
+  a.segs = R.d_segs;
   a.n_segs = n_segs;
   a.total = total;
   a.gate = ctx->use_gate ? ctx->d_gate : ctx->d_gate_ones;
@@ -347,105 +390,157 @@ int run_round(fc5_server* S, std::vector<SBand*>& active) {
   a.use_gate = (u32)ctx->use_gate;
   a.main = ctx->main.dev;
   a.side = ctx->side.present ? ctx->side.dev : XorDev{};
-  a.hits = ctx->d_hits;
+  a.hits = R.d_hits;
   a.hit_cap = ctx->hit_cap;
-  a.nhits = ctx->d_nhits;
+  a.nhits = R.d_nhits;
   a.ctr = S->d_ctr;
-  a.item_ctr = ctx->d_item_ctr;
+  a.item_ctr = ctx->d_item_ctr;  // rounds run in stream order, so one work counter serves both
   a.inv729 = inv_mod_2_64(729);
   const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(S->rows);
   const int grid = (int)std::max<u64>(1, std::min<u64>((u64)ctx->num_cu * (u64)bpc, total));
-  HIPCHK(ctx, hipMemcpyAsync(S->d_segs, S->h_segs, n_segs * sizeof(Segment), hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  if (launch_fill_u64(ctx->d_nhits, 1, 0, ctx->stream) != 0 ||
+  const size_t nslots = S->slots.size() * kSlotStride;
+  HIPCHK(ctx, hipMemcpyAsync(R.d_segs, R.h_segs, n_segs * sizeof(Segment), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(R.ev0, ctx->stream));
+  if (launch_fill_u64(R.d_nhits, 1, 0, ctx->stream) != 0 ||
       launch_fill_u64(reinterpret_cast<u64*>(ctx->d_item_ctr), 1, 0, ctx->stream) != 0)
     FAIL(ctx, FC5_E_HIP, "counter reset");
   if (launch_cls5_batched(a, grid, S->rows, ctx->stream) != 0) FAIL(ctx, FC5_E_HIP, "serving launch");
-  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-  const size_t nslots = S->slots.size() * kSlotStride;
-  HIPCHK(ctx, hipMemcpyAsync(S->h_ctr, S->d_ctr, nslots * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(S->h_nhits, ctx->d_nhits, 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipEventRecord(R.ev1, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(R.h_ctr, S->d_ctr, nslots * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(R.h_nhits, R.d_nhits, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(R.ev_done, ctx->stream));
+  return FC5_OK;
+}
+
+// Take apart a completed round: read its hits back (on the copy stream, while
+// the next round runs), hand them to their bands by counter slot, charge
+// each band the records that did not fit the round's hit buffer, update the
+// work estimates, and release the bands whose last items ran (their counters
+// are kept, their slot and arena ranges are freed) to the finalizer.
+int process_round(fc5_server* S, Round& R) {
+  fc5_ctx* ctx = S->ctx;
+  HIPCHK(ctx, hipEventSynchronize(R.ev_done));
   float ms = 0;
-  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  const u64 nh = *S->h_nhits, stored = std::min<u64>(nh, ctx->hit_cap);
-  if (stored)
-    HIPCHK(ctx, hipMemcpy(S->h_hits, ctx->d_hits, stored * sizeof(HitRec), hipMemcpyDeviceToHost));
-  for (u64 i = 0; i < stored; ++i) {
-    const HitRec& h = S->h_hits[i];
-    const u32 slot = (h.flags >> 8) / kSlotStride;
-    if (slot < S->slots.size() && S->slots[slot])
-      S->slots[slot]->hits.push_back(fc5_hit{h.unit, h.d, h.e, h.flags & 0xffu, h.fp});
+  HIPCHK(ctx, hipEventElapsedTime(&ms, R.ev0, R.ev1));
+  const u64 nh = *R.h_nhits, stored = std::min<u64>(nh, ctx->hit_cap);
+  if (stored) {
+    HIPCHK(ctx, hipMemcpyAsync(R.h_hits, R.d_hits, stored * sizeof(HitRec), hipMemcpyDeviceToHost, S->copy));
+    HIPCHK(ctx, hipStreamSynchronize(S->copy));
   }
+  std::vector<u64> kept(S->slots.size(), 0);
+  for (u64 i = 0; i < stored; ++i) {
+    const HitRec& h = R.h_hits[i];
+    const u32 slot = (h.flags >> 8) / kSlotStride;
+    if (slot < S->slots.size() && S->slots[slot]) {
+      S->slots[slot]->hits.push_back(fc5_hit{h.unit, h.d, h.e, h.flags & 0xffu, h.fp});
+      kept[slot]++;
+    }
+  }
+  std::vector<SBand*> fin;
+  for (auto& [B, n] : R.took) {
+    B->done5 += n.first;
+    B->done3 += n.second;
+    const Counters* c = R.h_ctr + (size_t)B->slot * kSlotStride;
+    u64 recs = 0;
+    for (int k = 2; k <= 5; ++k) recs += c[k].recs;
+    B->dropped += (recs - B->recs_seen) - kept[(size_t)B->slot];
+    B->recs_seen = recs;
+    // running estimates of the work per item from the band's own counters
+    if (B->done5) B->cost5 = std::max(1.0, (double)c[5].calls / B->done5);
+    const double c3 = (double)(c[2].calls + c[3].calls + c[4].calls);
+    if (B->done3 && c3 > 0) B->cost3 = std::max(1.0, c3 / B->done3);
+    if (B->finished_items()) {
+      std::memcpy(B->ctr, c, sizeof B->ctr);
+      fin.push_back(B);
+    }
+  }
+  std::lock_guard<std::mutex> lk(S->mu);  // what pollers and the stats read
   S->busy_ms += ms;
   S->rounds++;
-  for (auto& [B, n] : took) {
-    B->next5 += n.first;
-    B->next3 += n.second;
+  for (auto& [B, n] : R.took) {
     B->rounds++;
     B->kernel_ms += ms;
-    if (nh > stored) B->overflow = true;
-    // running estimates of the work per item from the band's own counters
-    const Counters* c = S->h_ctr + (size_t)B->slot * kSlotStride;
-    if (B->next5) B->cost5 = std::max(1.0, (double)c[5].calls / B->next5);
-    const double c3 = (double)(c[2].calls + c[3].calls + c[4].calls);
-    if (B->next3 && c3 > 0) B->cost3 = std::max(1.0, c3 / B->next3);
   }
+  R.took.clear();
+  for (SBand* B : fin) {
+    B->state = kFinalizing;
+    B->t_release = S->now();
+    S->a5.release(B->off5, B->units5.size());
+    S->a3.release(B->off3, B->units3.size());
+    S->slots[(size_t)B->slot] = nullptr;
+    S->fin_q.push_back(B);
+  }
+  if (!fin.empty()) S->cv_fin.notify_all();
   return FC5_OK;
+}
+
+void fail_open_bands_locked(fc5_server* S) {
+  for (auto& [id, B] : S->bands)
+    if (B->state == kPending || B->state == kActive) {
+      B->rc = S->fail_rc;
+      B->state = kDone;
+    }
+  S->pending.clear();
+  S->cv_done.notify_all();
 }
 
 void serve_loop(fc5_server* S) {
   fc5_ctx* ctx = S->ctx;
   (void)hipSetDevice(ctx->device);
   for (;;) {
-    std::vector<SBand*> active;
+    std::vector<SBand*> plan;
+    const bool inflight = S->planned > S->processed;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      int rc = admit_locked(S);
-      if (rc == FC5_OK) {
-        for (SBand* B : S->slots)
-          if (B) active.push_back(B);
-        if (active.empty()) {
-          if (S->stopping && S->pending.empty()) return;
-          S->cv_work.wait(lk);
-          continue;
-        }
-      } else {
-        S->fail_rc = rc;
+      if (S->fail_rc == FC5_OK) {
+        const int rc = admit_locked(S);
+        if (rc != FC5_OK) S->fail_rc = rc;
       }
       if (S->fail_rc != FC5_OK) {  // a device error ends the server: fail every open band
-        for (auto& [id, B] : S->bands)
-          if (B->state != kDone) {
-            B->rc = S->fail_rc;
-            B->state = kDone;
-          }
-        S->pending.clear();
-        S->cv_done.notify_all();
+        fail_open_bands_locked(S);
         return;
       }
+      for (SBand* B : S->slots)
+        if (B && B->unplanned()) plan.push_back(B);
+      if (plan.empty() && !inflight) {
+        bool any = false;
+        for (SBand* B : S->slots) any |= B != nullptr;
+        if (S->stopping && S->pending.empty() && !any) return;
+        S->cv_work.wait(lk);
+        continue;
+      }
     }
-    const int rc = run_round(S, active);
+    int rc = FC5_OK;
+    if (!plan.empty() && S->planned - S->processed < 2) {
+      rc = plan_round(S, plan, S->rr[S->planned % 2]);
+      if (rc == FC5_OK) S->planned++;
+    } else {
+      rc = process_round(S, S->rr[S->processed % 2]);
+      if (rc == FC5_OK) S->processed++;
+    }
     if (rc != FC5_OK) {
       std::lock_guard<std::mutex> lk(S->mu);
       S->fail_rc = rc;
-      continue;
     }
-    // finished bands are verified outside the lock (device pair search, host
-    // identity checks), so that submitters and pollers are not held up
-    std::vector<SBand*> fin;
-    for (SBand* B : active)
-      if (B->finished_items()) {
-        finalize(S, *B);
-        fin.push_back(B);
-      }
+  }
+}
+
+// finalizer thread: exact checks and results of the bands whose items have all run
+void finalize_loop(fc5_server* S) {
+  (void)hipSetDevice(S->ctx->device);
+  for (;;) {
+    SBand* B = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_fin.wait(lk, [&] { return !S->fin_q.empty() || S->fin_stop; });
+      if (S->fin_q.empty()) return;
+      B = S->fin_q.front();
+      S->fin_q.pop_front();
+    }
+    finalize(S, *B);  // outside the lock: submitters and pollers are not held up
     std::lock_guard<std::mutex> lk(S->mu);
-    for (SBand* B : fin) {
-      B->state = kDone;
-      S->a5.release(B->off5, B->units5.size());
-      S->a3.release(B->off3, B->units3.size());
-      S->slots[(size_t)B->slot] = nullptr;
-      S->done++;
-    }
+    B->state = kDone;
+    S->done++;
     S->cv_done.notify_all();
   }
 }
@@ -466,9 +561,14 @@ void fc5::serve_shutdown(fc5_ctx* ctx) {
   }
   S->cv_work.notify_all();
   if (S->th.joinable()) S->th.join();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->fin_stop = true;
+  }
+  S->cv_fin.notify_all();
+  if (S->fin_th.joinable()) S->fin_th.join();
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  if (S->side) (void)hipStreamDestroy(S->side);
   free_server_buffers(S);
   delete S;
   ctx->server = nullptr;
@@ -501,7 +601,7 @@ extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) try {
   if (rc != FC5_OK) return rc;
   const u64 table_bytes = ctx->main.hdr.m_cells * 8 + ctx->side.hdr.m_cells * 8 + ctx->n_exact * sizeof(u128);
   const u64 arena_bytes = o.arena_units5 * sizeof(DevUnit5) + o.arena_units3 * sizeof(DevUnit3);
-  const u64 run_bytes = ctx->hit_cap * sizeof(HitRec) + (u64)o.max_bands * kSlotStride * sizeof(Counters);
+  const u64 run_bytes = 2 * ctx->hit_cap * sizeof(HitRec) + (u64)o.max_bands * kSlotStride * sizeof(Counters);
   if (table_bytes + arena_bytes + run_bytes > o.mem_budget)
     FAIL(ctx, FC5_E_NOMEM, "table + unit arenas + run buffers exceed the memory budget");
   auto S = std::make_unique<fc5_server>();
@@ -515,26 +615,39 @@ extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) try {
   S->max_lim = N + 1;
   S->device_bytes = table_bytes + arena_bytes + run_bytes;
   const size_t nslots = (size_t)o.max_bands * kSlotStride;
-  if (hipMalloc(&S->d5, o.arena_units5 * sizeof(DevUnit5)) != hipSuccess ||
-      hipMalloc(&S->d3, o.arena_units3 * sizeof(DevUnit3)) != hipSuccess ||
-      hipMalloc(&S->d_segs, 2 * (size_t)o.max_bands * sizeof(Segment)) != hipSuccess ||
-      hipMalloc(&S->d_ctr, nslots * sizeof(Counters)) != hipSuccess ||
-      hipHostMalloc((void**)&S->h_segs, 2 * (size_t)o.max_bands * sizeof(Segment), hipHostMallocDefault) !=
-          hipSuccess ||
-      hipHostMalloc((void**)&S->h_ctr, nslots * sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&S->h_hits, ctx->hit_cap * sizeof(HitRec), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&S->h_nhits, 8, hipHostMallocDefault) != hipSuccess) {
+  const size_t seg_bytes = 2 * (size_t)o.max_bands * sizeof(Segment);
+  bool ok = hipMalloc(&S->d5, o.arena_units5 * sizeof(DevUnit5)) == hipSuccess &&
+            hipMalloc(&S->d3, o.arena_units3 * sizeof(DevUnit3)) == hipSuccess &&
+            hipMalloc(&S->d_ctr, nslots * sizeof(Counters)) == hipSuccess &&
+            hipMalloc(&S->d_hits2, ctx->hit_cap * sizeof(HitRec)) == hipSuccess &&
+            hipMalloc(&S->d_nhits2, 8) == hipSuccess;
+  for (int k = 0; k < 2 && ok; ++k) {
+    Round& R = S->rr[k];
+    R.d_hits = k ? S->d_hits2 : ctx->d_hits;
+    R.d_nhits = k ? S->d_nhits2 : ctx->d_nhits;
+    ok = hipMalloc(&R.d_segs, seg_bytes) == hipSuccess &&
+         hipHostMalloc((void**)&R.h_segs, seg_bytes, hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc((void**)&R.h_ctr, nslots * sizeof(Counters), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc((void**)&R.h_hits, ctx->hit_cap * sizeof(HitRec), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc((void**)&R.h_nhits, 8, hipHostMallocDefault) == hipSuccess &&
+         hipEventCreate(&R.ev0) == hipSuccess && hipEventCreate(&R.ev1) == hipSuccess &&
+         hipEventCreateWithFlags(&R.ev_done, hipEventDisableTiming) == hipSuccess;
+  }
+  if (!ok) {
     free_server_buffers(S.get());
     FAIL(ctx, FC5_E_NOMEM, "serving buffers");
   }
   if (launch_fill_u64(reinterpret_cast<u64*>(S->d_ctr), nslots * sizeof(Counters) / 8, 0, ctx->stream) != 0 ||
       hipStreamSynchronize(ctx->stream) != hipSuccess ||
-      hipStreamCreateWithFlags(&S->side, hipStreamNonBlocking) != hipSuccess) {
+      hipStreamCreateWithFlags(&S->side, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&S->vstream, hipStreamNonBlocking) != hipSuccess) {
     free_server_buffers(S.get());
     FAIL(ctx, FC5_E_HIP, "serving counter reset");
   }
   ctx->server = S.release();
   ctx->server->th = std::thread(serve_loop, ctx->server);
+  ctx->server->fin_th = std::thread(finalize_loop, ctx->server);
   return FC5_OK;
 } catch (...) {
   return abi_exception(ctx);
@@ -708,7 +821,7 @@ extern "C" int fc5_serve_timing_of(fc5_ctx* ctx, uint64_t band_id, fc5_serve_tim
   std::lock_guard<std::mutex> lk(S->mu);
   SBand* B = find_band(S, band_id);
   if (!B) FAIL(ctx, FC5_E_STATE, "unknown band");
-  *out = fc5_serve_timing{B->t_submit, B->t_admit, B->t_done, B->rounds};
+  *out = fc5_serve_timing{B->t_submit, B->t_admit, B->t_done, B->rounds, B->t_release};
   return B->state == kDone ? FC5_OK : FC5_E_PENDING;
 } catch (...) {
   return abi_exception(ctx);

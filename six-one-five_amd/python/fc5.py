@@ -85,7 +85,8 @@ class ServeOpts(C.Structure):
 
 
 class ServeTiming(C.Structure):
-    _fields_ = [("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double), ("rounds", C.c_uint64)]
+    _fields_ = [("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double), ("rounds", C.c_uint64),
+                ("t_release", C.c_double)]
 
 
 class ServeStats(C.Structure):

@@ -98,11 +98,13 @@ def test_serving_admission_queues_by_arena_space(ctx):
         t = [ctx.serve_timing(i) for i in ids]
         size5 = [int((c == 5).sum()) for c, _ in bands]
         size3 = [int((c != 5).sum()) for c, _ in bands]
-        # bands resident at the same time always fit the arenas together
+        # bands resident at the same time always fit the arenas together (a band holds its
+        # arena ranges from admission until its last round has ended)
         for j in range(6):
-            live = [k for k in range(6) if t[k]["t_admit"] <= t[j]["t_admit"] < t[k]["t_done"]]
+            live = [k for k in range(6) if t[k]["t_admit"] <= t[j]["t_admit"] < t[k]["t_release"]]
             assert sum(size5[k] for k in live) <= n5 + 16 and sum(size3[k] for k in live) <= n3 + 16
-        assert max(x["t_admit"] for x in t) > min(x["t_done"] for x in t)  # some band waited for room
+        assert all(x["t_admit"] <= x["t_release"] <= x["t_done"] for x in t)
+        assert max(x["t_admit"] for x in t) >= min(x["t_release"] for x in t)  # some band waited for room
     finally:
         ctx.serve_stop()
 
@@ -145,34 +147,44 @@ def test_serving_refusals(ctx):
         ctx.serve_stop()
 
 
-def test_serving_hit_overflow_is_flagged(ctx):
-    """a round whose hits exceed hit_cap: the band finishes with FC5_E_OVERFLOW,
-    hits_dropped = 1 (the round's buffer is shared, so the count is not split
-    among bands), the kept hits are a subset of the band's hits alone, and the
-    counters are unaffected"""
-    say("serving: hit overflow in a round")
-    cls, Bs, us = fc5.emit_units(1000001, 1002000, classes=(4, 5))
-    units = fc5.units_array(Bs, us)
-    alone = ctx.run(units, fc5.MODE_BATCHED, cls=cls)
-    n = alone.result["n_hits"]
-    assert n >= 2
-    ctx.set_option("hit_cap", n - 1)
+def test_serving_hit_overflow_is_charged_per_band(ctx):
+    """rounds whose hits exceed hit_cap: each band is charged exactly its own
+    dropped hits (n_hits + hits_dropped equals its hit count alone), returns
+    FC5_E_OVERFLOW only when some of its hits were dropped, keeps a subset of
+    its hits alone, and its counters are unaffected"""
+    say("serving: hit overflow charged per band")
+    ranges = [(1000001, 1002000), (1100001, 1101000), (1150001, 1150600)]
+    bands = []
+    for lo, hi in ranges:
+        cls, Bs, us = fc5.emit_units(lo, hi, classes=(4, 5))
+        units = fc5.units_array(Bs, us)
+        bands.append((cls, units, ctx.run(units, fc5.MODE_BATCHED, cls=cls)))
+    total = sum(a.result["n_hits"] for *_, a in bands)
+    assert min(a.result["n_hits"] for *_, a in bands) >= 2
+    cap = total // 3  # below the largest band's hit count
+    ctx.set_option("hit_cap", cap)
     try:
-        ctx.serve_start(round_points=1e15)  # the whole band in one round
+        ctx.serve_start(max_bands=4, round_points=1e15)  # a band's items in one round
         try:
-            bid = ctx.serve_submit(units, cls)
-            assert ctx.serve_wait(bid, 120.0)
-            with pytest.raises(fc5.Fc5Error) as e:
-                ctx.serve_result(bid)
-            assert e.value.code == fc5.E_OVERFLOW
-            out = ctx.serve_result(bid, allow_overflow=True)
-            assert out.rc == fc5.E_OVERFLOW
-            assert out.result["hits_dropped"] == 1
-            assert out.result["n_hits"] == len(out.hits) == n - 1
-            key = lambda h: (int(h["unit"]), int(h["e"]), int(h["d"]), int(h["flags"]))
-            assert {key(h) for h in out.hits} <= {key(h) for h in alone.hits}
-            for f in ("calls", "neg", "zero", "gated", "probes", "maybe", "jobs", "fail_t"):
-                assert out.result[f] == alone.result[f], f
+            ids = [ctx.serve_submit(u, c) for c, u, _ in bands]
+            for i in ids:
+                assert ctx.serve_wait(i, 120.0)
+            dropped = 0
+            for i, (_, _, alone) in zip(ids, bands):
+                out = ctx.serve_result(i, allow_overflow=True)
+                n = alone.result["n_hits"]
+                assert out.result["n_hits"] == len(out.hits)
+                assert out.result["n_hits"] + out.result["hits_dropped"] == n
+                assert (out.rc == fc5.E_OVERFLOW) == (out.result["hits_dropped"] > 0)
+                key = lambda h: (int(h["unit"]), int(h["e"]), int(h["d"]), int(h["flags"]))
+                assert {key(h) for h in out.hits} <= {key(h) for h in alone.hits}
+                if out.result["hits_dropped"] == 0:
+                    assert np.array_equal(out.hits, alone.hits)
+                for f in ("calls", "neg", "zero", "gated", "probes", "maybe", "jobs", "fail_t"):
+                    assert out.result[f] == alone.result[f], f
+                dropped += out.result["hits_dropped"]
+            # however the bands were grouped into rounds, a band's hits beyond the cap are dropped
+            assert dropped >= max(a.result["n_hits"] for *_, a in bands) - cap > 0
         finally:
             ctx.serve_stop()
     finally:

@@ -23,3 +23,15 @@ def test_exact_pairs_and_verify_reject(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout
     assert "pairs ok" in out.stdout
+
+
+def test_solution_record_fields(tmp_path):
+    """the SOLUTION record of a verified identity: sorted terms, refusals, gcd"""
+    exe = str(tmp_path / "solution_check")
+    csrc = os.path.join(PKG, "csrc")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-I", csrc,
+                    os.path.join(REPO, "tests", "native", "solution_check.cpp"), os.path.join(csrc, "fc5_host.cpp"),
+                    "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout
+    assert "solution ok" in out.stdout
