@@ -325,7 +325,7 @@ def main():
                      "kernel": "k_cls5_batched",
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "per_unit": f"{3 * cell_bytes} B per filter probe (3 cells x {cell_bytes} B)",
-                     "transaction_GBps_at_64B": r["probes"] * 3 * 64 / kern_s / 1e9,
+                     "line_fill_GBps_at_128B": r["probes"] * 3 * 128 / kern_s / 1e9,
                      # measured HBM traffic (128-byte lines per random gather) over the same launch time
                      "traffic_GBps": traffic / kern_s / 1e9 if traffic else None,
                      "traffic_frac": traffic / kern_s / 1e9 / HBM_PEAK_GBS if traffic else None,
@@ -516,6 +516,8 @@ def bench_serve(args):
     ctx.set_option("rows_per_item", 16)
     ctx.serve_start(max_bands=16)
 
+    arrivals = {}
+
     def run(order, times):
         """submit band i at times[i] seconds after start; returns per-band timings"""
         start = time.perf_counter()
@@ -528,6 +530,9 @@ def bench_serve(args):
                 continue
             _, i = pending.pop(0)
             ids[i] = ctx.serve_submit(bands[i][3], bands[i][2])
+            if not pending:  # the device's busy time over the arrival window
+                arrivals["span"] = time.perf_counter() - start
+                arrivals["busy_ms"] = ctx.serve_stats()["busy_ms"]
             if len(ids) % 25 == 0:
                 st = ctx.serve_stats()
                 log(f"[bench]   t={time.perf_counter() - start:.1f}s submitted {st['submitted']} done {st['done']} "
@@ -539,7 +544,9 @@ def bench_serve(args):
 
     # closed-loop capacity: a sample of bands submitted at once
     cal = list(range(min(args.serve_calibrate, len(bands))))
+    b0 = ctx.serve_stats()["busy_ms"]
     ids, span = run(cal, [0.0] * len(cal))
+    cal_busy = (ctx.serve_stats()["busy_ms"] - b0) / 1000.0 / span
     cal_points = sum(ctx.serve_result(ids[i]).result["calls"] for i in cal)
     for i in cal:
         ctx.serve_release(ids[i])
@@ -548,8 +555,8 @@ def bench_serve(args):
     cal_c5 = sum(bands[i][4] for i in cal)
     # capacity in requests/s, scaled from the calibration's class-5 share of its points
     rate = args.serve_load * cap_points * (cal_c5 / cal_points) / mean_points
-    log(f"[bench] capacity {cap_points:.4e} grid points/s on {len(cal)} bands at once; "
-        f"arrival rate {rate:.3f} requests/s (load {args.serve_load})")
+    log(f"[bench] capacity {cap_points:.4e} grid points/s on {len(cal)} bands at once (device busy "
+        f"{cal_busy:.1%} of that run); arrival rate {rate:.3f} requests/s (load {args.serve_load})")
     rnd = random.Random(seed + 1)
     times, t = [], 0.0
     for _ in bands:
@@ -580,8 +587,10 @@ def bench_serve(args):
     lat = np.array(lat)
     busy_ms = st1["busy_ms"] - st0["busy_ms"]
     n_B = sum(eligible_count(lo, hi) for lo, hi in reqs)
+    busy_arr = (arrivals["busy_ms"] - st0["busy_ms"]) / 1000.0 / arrivals["span"]
     log(f"[bench] served {len(bands)} bands in {span:.1f}s: p50 {np.percentile(lat, 50):.2f}s "
-        f"p99 {np.percentile(lat, 99):.2f}s, {points / span:.4e} points/s, device busy {busy_ms / 1000 / span:.1%}")
+        f"p99 {np.percentile(lat, 99):.2f}s, {points / span:.4e} points/s, device busy {busy_ms / 1000 / span:.1%} "
+        f"of the whole run, {busy_arr:.1%} over the {arrivals['span']:.1f}s arrival window")
     # roofline of the serving rounds: algorithmic cell bytes of every probe over the summed round time
     cell_bytes = math.ceil(args.r / 8)
     alg = probes * 3 * cell_bytes
@@ -611,7 +620,14 @@ def bench_serve(args):
                       "p99": float(np.percentile(lat, 99)), "max": float(lat.max()), "mean": float(lat.mean())},
         "arrival_rate_per_s": rate, "capacity_points_per_s": cap_points,
         "eligible_B_per_hour": n_B * 3600.0 / span, "filter_probes_per_s": probes / span,
-        "device_busy_frac": busy_ms / 1000.0 / span, "rounds": st1["rounds"] - st0["rounds"],
+        "device_busy_frac": busy_ms / 1000.0 / span,
+        "device_busy_frac_arrival_window": busy_arr, "arrival_window_s": arrivals["span"],
+        "device_busy_frac_closed_loop": cal_busy,
+        # the offered work as realized by the seeded arrivals, and the busy fraction it implies
+        # at the closed-loop rate of busy device time (points per busy second)
+        "offered_load_realized": points / (cap_points * arrivals["span"]),
+        "device_busy_frac_predicted_from_work": points / (cap_points / cal_busy) / span,
+        "rounds": st1["rounds"] - st0["rounds"],
         "bands_checked_alone": len(alone), "n_hits": n_hits,
         "roofline": roof, "cpu_baseline": cpu,
     }
@@ -687,7 +703,7 @@ def measure_ceilings(ctx, hdr, kernel_probe_rate):
     valu = {name: ctx.measure_valu(k) for k, name in enumerate(("int32_xad", "mad_u64_u32", "add_u128_carry",
                                                                   "mix_7pct_int64"))}
     ceiling = {"probe_only_probes_per_s": probe_rate, "raw_random_8B_gathers_per_s": 3 * n / (rms / 1000.0),
-               "probe_only_transaction_GBps_at_64B": probe_rate * 3 * 64 / 1e9,
+               "probe_only_line_fill_GBps_at_128B": probe_rate * 3 * 128 / 1e9,
                "copy_GBps": 2 * (4 << 30) / (cms / 1000.0) / 1e9,
                "read_stream_GBps": read_bytes / (sms / 1000.0) / 1e9,
                "random_gathers_by_region": regions,

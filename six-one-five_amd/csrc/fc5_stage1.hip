@@ -14,7 +14,12 @@
 // progression member above the previous one), counting in the first pass and
 // writing at its exclusive-scan offset in the second, so the output is the
 // host emitter's list: B ascending, classes ascending, u ascending.
-#include <hipcub/hipcub.hpp>
+// The exclusive scan between the passes is this file's own: per block of
+// 2048 counts a wave-shuffle scan of each lane's 8-count run, the block
+// totals scanned by one block, then added back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "fc5_kernels.hpp"
 
@@ -94,8 +99,95 @@ int launch_stage1(const Stage1Args& a, int pass, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
+namespace {
+
+constexpr u32 kScanThreads = 256, kScanPer = 8, kScanBlock = kScanThreads * kScanPer;  // 2048 counts
+
+// exclusive prefix of v over the block's threads (wave shuffles, then the
+// four wave totals through LDS); *total gets the block's sum
+__device__ __forceinline__ u64 block_exclusive(u64 v, u64* s_wave, u64* total) {
+  const u32 lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  u64 x = v;
+  for (u32 o = 1; o < 64; o <<= 1) {
+    const u64 y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wave[w] = x;
+  __syncthreads();
+  u64 before = 0, all = 0;
+  for (u32 k = 0; k < kScanThreads / 64; ++k) {
+    if (k < w) before += s_wave[k];
+    all += s_wave[k];
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+// block b: exclusive offsets of counts [2048 b, 2048 (b + 1)) within the block, its total to sums[b]
+__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(const u64* in, u64* out, u64 n, u64* sums) {
+  __shared__ u64 s_wave[kScanThreads / 64];
+  const u64 base = (u64)blockIdx.x * kScanBlock + (u64)threadIdx.x * kScanPer;
+  u64 v[kScanPer], run = 0;
+#pragma unroll
+  for (u32 k = 0; k < kScanPer; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    run += v[k];
+  }
+  u64 total;
+  u64 off = block_exclusive(run, s_wave, &total);
+#pragma unroll
+  for (u32 k = 0; k < kScanPer; ++k) {
+    if (base + k < n) out[base + k] = off;
+    off += v[k];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// one block: exclusive scan of the m <= 2048 block totals, in place
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(u64* sums, u32 m) {
+  __shared__ u64 s_wave[kScanThreads / 64];
+  const u32 base = threadIdx.x * kScanPer;
+  u64 v[kScanPer], run = 0;
+#pragma unroll
+  for (u32 k = 0; k < kScanPer; ++k) {
+    v[k] = base + k < m ? sums[base + k] : 0;
+    run += v[k];
+  }
+  u64 total;
+  u64 off = block_exclusive(run, s_wave, &total);
+#pragma unroll
+  for (u32 k = 0; k < kScanPer; ++k) {
+    if (base + k < m) sums[base + k] = off;
+    off += v[k];
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_add(u64* out, u64 n, const u64* sums) {
+  const u64 add = sums[blockIdx.x];
+  const u64 base = (u64)blockIdx.x * kScanBlock;
+  for (u32 k = threadIdx.x; k < kScanBlock; k += kScanThreads)
+    if (base + k < n) out[base + k] += add;
+}
+
+}  // namespace
+
+// exclusive prefix sum of n u64 counts: temp holds one u64 per 2048 counts,
+// n up to 2048 * 2048 (temp == nullptr: temp_bytes gets the size)
 int stage1_scan(const u64* counts_in, u64* offsets_out, u64 n, void* temp, size_t& temp_bytes, hipStream_t s) {
-  return (int)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts_in, offsets_out, (int)n, s);
+  const u64 blocks = (n + kScanBlock - 1) / kScanBlock;
+  if (blocks > kScanBlock) return (int)hipErrorInvalidValue;
+  if (!temp) {
+    temp_bytes = (size_t)std::max<u64>(blocks, 1) * 8;
+    return (int)hipSuccess;
+  }
+  if (!n) return (int)hipSuccess;
+  if (temp_bytes < blocks * 8) return (int)hipErrorInvalidValue;
+  u64* sums = static_cast<u64*>(temp);
+  hipLaunchKernelGGL(k_scan_blocks, dim3((u32)blocks), dim3(kScanThreads), 0, s, counts_in, offsets_out, n, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, sums, (u32)blocks);
+  hipLaunchKernelGGL(k_scan_add, dim3((u32)blocks), dim3(kScanThreads), 0, s, offsets_out, n, sums);
+  return (int)hipGetLastError();
 }
 
 }  // namespace fc5
