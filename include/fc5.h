@@ -186,6 +186,31 @@ int fc5_last_class_result(fc5_ctx *ctx, int cls, fc5_result *out);
 /* hits and solutions of the last run of this context */
 int fc5_last_hits(fc5_ctx *ctx, fc5_hit *out, size_t cap, size_t *n);
 int fc5_last_solutions(fc5_ctx *ctx, fc5_solution *out, size_t cap, size_t *n);
+/* fc5_run_units (mode FC5_MODE_BATCHED) for callers that cannot allocate.
+ * No counterpart in the reference: it is for the callers that cannot call
+ * hipMalloc -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid device allocation.  Every device buffer beyond the inputs and outputs
+ * (packed units, launch segment, d table, counters, work-queue and hit counters,
+ * hit buffer) lives in one static device array of FC5_NOALLOC_BYTES per GPU,
+ * held until the process ends; the call allocates no device memory and no
+ * pinned host memory (its host staging is set up once, when a context is
+ * created on the device; unit packing uses ordinary host temporaries).
+ * Inputs: class-5 Stage-1 units (a line the packing refuses is skipped and
+ * counted; any other irregular line is refused with FC5_E_ARG), at most
+ * FC5_NOALLOC_UNITS of them, B <= FC5_NOALLOC_BMAX.  Outputs, in caller
+ * buffers: the counters (res), the maybe-hits sorted as fc5_last_hits (up to
+ * hit_cap; *n_hits gets the number found, at most FC5_NOALLOC_HITS: beyond that
+ * the rest are dropped, counted in hits_dropped, and FC5_E_OVERFLOW is
+ * returned), and with the option "verify" the solutions (up to sol_cap;
+ * *n_sols gets the count) from an exact check of the hits on the host.  Calls
+ * on one GPU share its array and are serialized by a lock in the process;
+ * the counters are cleared at the start of every call. */
+#define FC5_NOALLOC_UNITS 65536
+#define FC5_NOALLOC_HITS 65536
+#define FC5_NOALLOC_BMAX 5505024 /* free term d = (B - 1) / 21 below 2^18 */
+#define FC5_NOALLOC_BYTES 7340032 /* 7 MiB */
+int fc5_run_units_noalloc(fc5_ctx *ctx, const fc5_unit *units, size_t n, fc5_result *res, fc5_hit *hits,
+                          size_t hit_cap, size_t *n_hits, fc5_solution *sols, size_t sol_cap, size_t *n_sols);
 /* closed-form calls of a unit list (the counter `calls`, fourcore_cls5_gpu_v4.cu:383-386) */
 int fc5_units_calls(const fc5_unit *units, size_t n, uint64_t *calls);
 

@@ -243,6 +243,9 @@ int exact_table_pairs(fc5_ctx* ctx, const std::vector<PairTarget>& targets,
                       std::vector<std::vector<std::pair<u32, u32>>>& pairs, std::vector<u8>* extra,
                       hipStream_t st = nullptr);
 void free_exact(fc5_ctx* ctx);
+// the allocation-free entry point's per-device set-up (fc5_noalloc.cpp): the
+// device address of its scratch array, with `device` current; once per device
+int noalloc_setup(int device);
 // twin buffers (fc5_twin_api.cpp)
 void free_twin(fc5_ctx* ctx);
 }  // namespace fc5

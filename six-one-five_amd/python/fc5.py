@@ -27,6 +27,7 @@ _lib = C.CDLL(LIB_PATH)
 OK, E_ARG, E_IO, E_FORMAT, E_NMISMATCH, E_FPR, E_HIP, E_NOMEM, E_STATE, E_OVERFLOW, E_BUILD, E_PENDING = (
     0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10, -11)
 MODE_PLAIN, MODE_BATCHED, MODE_CHUNKED = 0, 1, 2
+NOALLOC_UNITS, NOALLOC_HITS, NOALLOC_BMAX = 65536, 65536, 5505024  # limits of fc5_run_units_noalloc (fc5.h)
 
 
 class StoreHeader(C.Structure):
@@ -124,6 +125,8 @@ _sig = {
     "fc5_autotune": (C.c_int, [_P, _P, C.c_int, C.POINTER(Tune)]),
     "fc5_band_run": (C.c_int, [_P, _P, C.c_int, C.POINTER(Result)]),
     "fc5_run_units": (C.c_int, [_P, _P, C.c_size_t, C.c_int, C.POINTER(Result)]),
+    "fc5_run_units_noalloc": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(Result), _P, C.c_size_t,
+                                        C.POINTER(C.c_size_t), _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_last_hits": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_last_solutions": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_units_calls": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_uint64)]),
@@ -414,6 +417,22 @@ class Context:
         if n.value:
             self._check(_lib.fc5_last_hits(self.h, out.ctypes.data_as(_P), n.value, C.byref(n)))
         return out
+
+    def run_noalloc(self, units, hit_cap=1 << 16, sol_cap=64, allow_overflow=False):
+        """fc5_run_units_noalloc: the batched class-5 search with no device allocation
+        (RunOutput; hits beyond hit_cap are not copied out, their count is in n_hits)"""
+        units = np.ascontiguousarray(units)
+        res = Result()
+        hits = np.zeros(hit_cap, HIT_DTYPE)
+        sols = (Solution * max(1, sol_cap))()
+        nh, ns = C.c_size_t(), C.c_size_t()
+        rc = _lib.fc5_run_units_noalloc(self.h, _ptr(units), len(units), C.byref(res), _ptr(hits), hit_cap,
+                                        C.byref(nh), C.cast(sols, _P), sol_cap, C.byref(ns))
+        if rc not in (OK, E_OVERFLOW) or (rc == E_OVERFLOW and not allow_overflow):
+            self._check(rc)
+        sl = [dict(B=x.B, u=x.u, cls=x.cls, d=x.d, e=x.e, a=x.a, b=x.b, c=x.c, terms=list(x.terms), gcd=x.gcd)
+              for x in sols[:min(ns.value, sol_cap)]]
+        return RunOutput(res.as_dict(), hits[:min(nh.value, hit_cap)], sl, rc)
 
     def last_solutions(self):
         n = C.c_size_t()

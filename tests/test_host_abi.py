@@ -240,3 +240,13 @@ def test_bench_runs_no_reference_binary():
     assert "subprocess" not in src and "solve_516_v2\"" not in src and "_ref\"" not in src
     ref = json.load(open(os.path.join(REPO, "oracle", "ref_cpu_timing.json")))
     assert set(ref["configs"]) == {"1", "2"} and ref["configs"]["1"]["grid_points"] == 10973276
+
+
+def test_noalloc_limits_match_header():
+    """the Python mirror's limits of fc5_run_units_noalloc are the header's"""
+    hdr = open(os.path.join(REPO, "include", "fc5.h")).read()
+    val = {k: int(v) for k, v in re.findall(r"#define (FC5_NOALLOC_\w+) (\d+)", hdr)}
+    assert (val["FC5_NOALLOC_UNITS"], val["FC5_NOALLOC_HITS"], val["FC5_NOALLOC_BMAX"]) == \
+        (fc5.NOALLOC_UNITS, fc5.NOALLOC_HITS, fc5.NOALLOC_BMAX)
+    assert (fc5.NOALLOC_BMAX - 1) // 21 == 2 ** 18 - 1 and fc5.NOALLOC_BMAX // 21 == 2 ** 18  # B + 1: d past 2^18 - 1
+    assert val["FC5_NOALLOC_BYTES"] <= 8 << 20
