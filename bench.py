@@ -808,9 +808,10 @@ def cpu_baseline(ctx, units, ucls, every, cfg, r, pkeys, offset=0):
 
 def cpu_reference(config):
     """The reference's own CPU class-5 search (solve_516_v2, every (e, d) with an
-    exact pair-index lookup, solve_516_v2.cpp:400-416) as timed in the build
-    container by oracle/time_ref_cpu.py -- it is never run on this host.  Its
-    figures for configs 1 and 2 (the solver stops at B <= 2.2M), or None."""
+    exact pair-index lookup, solve_516_v2.cpp:400-416) as timed by
+    oracle/time_ref_cpu.py where the reference sources are present -- this script
+    never runs it.  Its figures for the bands of --config 1 and 2 (the solver
+    stops at B <= 2.2M), or None."""
     p = os.path.join(REPO, "oracle", "ref_cpu_timing.json")
     if not os.path.exists(p):
         return None
@@ -820,7 +821,7 @@ def cpu_reference(config):
     if c is None:
         return None
     return {"value": c["points_per_s"], "unit": "(d,e) grid points/s", "cores": t["threads"], "kind": "reference",
-            "host": t["host"], "where": "build container (oracle/time_ref_cpu.py), not this host",
+            "host": t["host"], "where": "timed by oracle/time_ref_cpu.py where the reference sources are, not on this host",
             "sample": f"B in [{c['B_lo']},{c['B_hi']}] ({c['units']} class-5 units, {c['grid_points']} grid points): "
                       f"the reference's solve_516_v2 class-5 search built from its sources, {c['search_s']:.1f} s "
                       f"after its {c['index_build_s']:.1f} s pair-index build; {c['solutions']} solutions"}

@@ -1,16 +1,17 @@
 #!/usr/bin/env python3
-"""Time the reference's own CPU class-5 search in the build container.
+"""Time the reference's own CPU class-5 search where its sources are present.
 
 Test infrastructure, run by hand where the reference sources are present and
 `make -C oracle ref` has built oracle/_ref/solve_516_v2 from them: the
 reference's exact CPU engine (solve_516_v2.cpp:400-416: every (e, d) of a
-class-5 unit, an exact lookup of the target in its sorted pair index).  It is
-never run on the GPU host; bench.py reports the figures this script writes to
-oracle/ref_cpu_timing.json beside its own on-host CPU baseline (the oracle).
+class-5 unit, an exact lookup of the target in its sorted pair index).
+bench.py never runs it: it reports the figures this script writes to
+oracle/ref_cpu_timing.json beside its own CPU baseline (the oracle, timed where
+the bench runs).
 
-Workloads (the solver's i128 limit is B <= 2.2M):
-  config 1  B in [100001, 102000], every class-5 unit
-  config 2  B in [1000001, 1000100], the first 100 integers of [1000001, 1020000]
+Workloads (the solver's i128 limit is B <= 2.2M), keyed by bench.py's --config:
+  1  B in [100001, 102000], every class-5 unit
+  2  B in [1000001, 1000100], the first 100 integers of [1000001, 1020000]
 The rate is the closed-form grid-point count of the same Stage-1 units over the
 search time (the solver's pair-index build, which it reports, is excluded).
 """

@@ -7,7 +7,7 @@
 //                        [--no-gate] [--stop-first] [--device K] [--load-table F]
 //                        [--save-table F] [--allow-rebuild]
 //   fourcore_cls5_gpu_v4 --check-table --N K --load-table F [--r BITS] [--save-table F]
-// Additions: --mode plain|batched (default batched), --hit-cap N, --counters,
+// Additions: --mode plain|batched|chunked (default batched), --hit-cap N, --counters,
 // --exact-table (check filter survivors against the exact sorted pair table
 // of all a^6 + b^6, a <= b <= N, built on the device)
 // (one machine-readable counters line on stdout).
@@ -32,7 +32,7 @@ static void usage() {
          "                            [--N K] [--quiet|--no-quiet] [--no-gate]\n"
          "                            [--stop-first] [--device K]\n"
          "                            [--load-table F] [--save-table F]\n"
-         "                            [--allow-rebuild] [--mode plain|batched]\n"
+         "                            [--allow-rebuild] [--mode plain|batched|chunked]\n"
          "                            [--hit-cap N] [--counters] [--exact-table]\n"
          "   or: fourcore_cls5_gpu_v4 --check-table --N K --load-table F\n"
          "                            [--r BITS] [--save-table F] [--allow-rebuild]\n");
@@ -113,8 +113,11 @@ int main(int argc, char** argv) {
   }
   if (do_host || argc == 1) return selftest_host();
   if (xor_r < 8 || xor_r > 64) { fprintf(stderr, "--r must be 8..64\n"); return 1; }
-  const int mode = mode_s == "plain" ? FC5_MODE_PLAIN : FC5_MODE_BATCHED;
-  if (mode_s != "plain" && mode_s != "batched") { fprintf(stderr, "--mode must be plain or batched\n"); return 1; }
+  const int mode = mode_s == "plain" ? FC5_MODE_PLAIN : mode_s == "chunked" ? FC5_MODE_CHUNKED : FC5_MODE_BATCHED;
+  if (mode_s != "plain" && mode_s != "batched" && mode_s != "chunked") {
+    fprintf(stderr, "--mode must be plain, batched or chunked\n");
+    return 1;
+  }
 
   auto acquire = [&](int N, XorTable& t) -> bool {  // acquire_xor_filter (cls5v4:568-594)
     bool loaded = false;

@@ -1,7 +1,7 @@
 """Writes the golden fixtures in tests/golden/ from the reference pieces that
 compile from their own sources (oracle/_ref, built by `make -C oracle ref`).
 
-Run in the build container only (the reference is not present elsewhere):
+Run only where the reference sources are present (nowhere else has them):
     python tests/golden/make_golden.py
 Outputs (data only: inputs and expected outputs):
     xor_tables.json     SHA-256 and header of reference xor_build_save tables

@@ -233,9 +233,9 @@ def test_bench_default_config_finds_its_counter_summaries():
 
 
 def test_bench_runs_no_reference_binary():
-    """bench.py never executes a program built from the reference's sources (the
-    reference CPU figure is read from the container timing), so nothing built from
-    the reference runs on a GPU host"""
+    """bench.py never executes a program built from the reference's sources: the
+    reference CPU figure is read from oracle/ref_cpu_timing.json, written by
+    oracle/time_ref_cpu.py where the reference sources are"""
     src = open(os.path.join(REPO, "bench.py")).read()
     assert "subprocess" not in src and "solve_516_v2\"" not in src and "_ref\"" not in src
     ref = json.load(open(os.path.join(REPO, "oracle", "ref_cpu_timing.json")))
