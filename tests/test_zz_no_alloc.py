@@ -60,3 +60,28 @@ def test_noalloc_limits_refused(ctx):
     with pytest.raises(fc5.Fc5Error) as e:
         ctx.run_noalloc(fc5.units_array(Bs, us))
     assert e.value.code == fc5.E_ARG
+
+
+def test_noalloc_hit_overflow_reported(ctx):
+    """more maybe-hits than the array's hit buffer holds: FC5_E_OVERFLOW, the kept
+    hits are a subset of fc5_run_units' hits, and the counters are unaffected"""
+    c = fc5.Context(0)
+    try:
+        c.table_build(2428, 4)  # 4-bit cells: one false positive in 16 probes
+        c.set_option("use_gate", 0)
+        c.set_option("verify", 0)
+        _, Bs, us = fc5.emit_units(100001, 102000)
+        units = fc5.units_array(Bs, us)
+        alloc = c.run(units, fc5.MODE_BATCHED)
+        total = alloc.result["n_hits"]
+        assert total > fc5.NOALLOC_HITS and alloc.result["hits_dropped"] == 0
+        free = c.run_noalloc(units, hit_cap=fc5.NOALLOC_HITS, allow_overflow=True)
+        assert free.rc == fc5.E_OVERFLOW
+        assert free.result["n_hits"] == fc5.NOALLOC_HITS == len(free.hits)
+        assert free.result["hits_dropped"] == total - fc5.NOALLOC_HITS
+        for k in ("calls", "neg", "zero", "gated", "probes", "maybe"):
+            assert free.result[k] == alloc.result[k], k
+        key = lambda h: (int(h["unit"]), int(h["e"]), int(h["d"]), int(h["flags"]), int(h["fp"]))
+        assert {key(h) for h in free.hits} <= {key(h) for h in alloc.hits}
+    finally:
+        c.close()
