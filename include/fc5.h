@@ -112,8 +112,9 @@ void fc5_ctx_destroy(fc5_ctx *ctx);
 const char *fc5_last_error(const fc5_ctx *ctx);
 /* options: "hit_cap" (default 1<<20), "use_gate" (1; 0 = --no-gate,
  * fourcore_cls5_gpu_v4.cu:634), "blocks_per_cu" (0 = occupancy limit) for the
- * persistent grid, "rows_per_item" (4, 8, 16, 32; default 8: e- or d-rows per work
- * item), "use_graph" (0; 1 replays the batched pipeline as a captured hipGraph),
+ * persistent grid, "rows_per_item" (e- or d-rows per work item: 4, 8, 16, 32, or
+ * 0, the default: by band size, the largest of 32, 16, 8 that leaves every wave of
+ * the persistent grid at least 8 items, else 4), "use_graph" (0; 1 replays the batched pipeline as a captured hipGraph),
  * "load_policy" (-1..3, default -1: cache policy of the xor cell gathers:
  * -1 by table size (plain loads up to 1 GiB of cells, non-temporal above),
  * 0 default, 1 nontemporal, 2 device-scope, 3 system-scope), "verify" (1: exact
@@ -324,7 +325,7 @@ int fc5_twin_last_solutions(fc5_ctx *ctx, fc5_twin_solution *out, size_t cap, si
  *      bands whose B exceed what the table's N covers are refused. ---- */
 typedef struct {
   int max_bands;          /* bands admitted at once (counter slots), 1..64; default 16 */
-  int rows_per_item;      /* 4, 8, 16, 32; default: the context option */
+  int rows_per_item;      /* 4, 8, 16, 32; default: the context option, or 16 when that is 0 */
   uint64_t arena_units5;  /* resident class-5 unit capacity (64 B each); default 1<<23 */
   uint64_t arena_units3;  /* resident class 2-4 unit capacity (128 B each); default 1<<22 */
   double round_points;    /* grid points per scheduling round; default 2e10 (~30 ms) */

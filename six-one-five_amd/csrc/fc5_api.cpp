@@ -129,7 +129,8 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) try {
     ctx->main.dev.policy = table_policy(ctx->load_policy, ctx->main.hdr.m_cells);
     ctx->side.dev.policy = table_policy(ctx->load_policy, ctx->side.hdr.m_cells);
   } else if (k == "rows_per_item") {
-    if (v != 4 && v != 8 && v != 16 && v != 32) FAIL(ctx, FC5_E_ARG, "rows_per_item must be 4, 8, 16 or 32");
+    if (v != 0 && v != 4 && v != 8 && v != 16 && v != 32)
+      FAIL(ctx, FC5_E_ARG, "rows_per_item must be 0 (by band size), 4, 8, 16 or 32");
     ctx->rows_per_item = (int)v;
   } else if (k == "use_graph") {
     ctx->use_graph = v ? 1 : 0;
@@ -624,16 +625,34 @@ bool fc5::number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, 
   return true;
 }
 
+int fc5::auto_rows_per_item(int num_cu, u64 rows) {
+  for (int E : {32, 16, 8}) {
+    const u64 waves = (u64)num_cu * (u64)std::max(1, batched_blocks_per_cu(E)) * 4u;
+    if (rows / (u64)E >= 8 * waves) return E;
+  }
+  return 4;
+}
+
+// rows per work item for a band: the context option, or (option 0) by the band's size
+static int band_rows_per_item(const fc5_ctx* ctx, const std::vector<DevUnit5>& dev,
+                              const std::vector<DevUnit3>& dev3) {
+  if (ctx->rows_per_item > 0) return ctx->rows_per_item;
+  u64 rows = 0;
+  for (const DevUnit5& D : dev) rows += D.rows;
+  for (const DevUnit3& D : dev3) rows += D.rows;
+  return auto_rows_per_item(ctx->num_cu, rows);
+}
+
 static int renumber(fc5_ctx* ctx, fc5_band* b) {
-  if (b->rows == ctx->rows_per_item) return FC5_OK;
-  if (!number_items(b->dev_h, b->dev3_h, ctx->rows_per_item, b->n_items, b->n_items3))
-    FAIL(ctx, FC5_E_ARG, kTooManyItems);
+  const int E = band_rows_per_item(ctx, b->dev_h, b->dev3_h);
+  if (b->rows == E) return FC5_OK;
+  if (!number_items(b->dev_h, b->dev3_h, E, b->n_items, b->n_items3)) FAIL(ctx, FC5_E_ARG, kTooManyItems);
   if (!b->dev_h.empty())
     HIPCHK(ctx, hipMemcpy(b->d_units, b->dev_h.data(), b->dev_h.size() * sizeof(DevUnit5), hipMemcpyHostToDevice));
   if (!b->dev3_h.empty())
     HIPCHK(ctx, hipMemcpy(b->d_units3, b->dev3_h.data(), b->dev3_h.size() * sizeof(DevUnit3),
                           hipMemcpyHostToDevice));
-  b->rows = ctx->rows_per_item;
+  b->rows = E;
   return upload_segments(ctx, b);
 }
 
@@ -708,8 +727,9 @@ static int band_create(fc5_ctx* ctx, const uint32_t* cls, const fc5_unit* units,
   b->n_dev = (u32)dev.size();
   b->n_plain = (u32)plain.size();
   b->n_dev3 = (u32)dev3.size();
-  if (!number_items(dev, dev3, ctx->rows_per_item, b->n_items, b->n_items3)) FAIL(ctx, FC5_E_ARG, kTooManyItems);
-  b->rows = ctx->rows_per_item;
+  const int E = band_rows_per_item(ctx, dev, dev3);
+  if (!number_items(dev, dev3, E, b->n_items, b->n_items3)) FAIL(ctx, FC5_E_ARG, kTooManyItems);
+  b->rows = E;
   int rc = FC5_OK;
   auto upload = [&](void** dst, const void* src, size_t bytes) {
     if (!bytes || rc != FC5_OK) return;
@@ -915,7 +935,7 @@ static int ensure_graph(fc5_ctx* ctx, fc5_band* b, const BatchedArgs& a, int gri
   std::vector<u64> key(sizeof(BatchedArgs) / 8 + 2);
   std::memcpy(key.data(), &a, sizeof(BatchedArgs));
   key[key.size() - 2] = (u64)grid;
-  key[key.size() - 1] = (u64)ctx->rows_per_item;
+  key[key.size() - 1] = (u64)b->rows;
   if (b->graph_exec && key == b->graph_key) return FC5_OK;
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -923,7 +943,7 @@ static int ensure_graph(fc5_ctx* ctx, fc5_band* b, const BatchedArgs& a, int gri
   b->graph = nullptr;
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   int rc = reset_counters(ctx);
-  if (rc == FC5_OK && launch_cls5_batched(a, grid, ctx->rows_per_item, ctx->stream) != 0) rc = FC5_E_HIP;
+  if (rc == FC5_OK && launch_cls5_batched(a, grid, b->rows, ctx->stream) != 0) rc = FC5_E_HIP;
   if (rc == FC5_OK &&
       (hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, 16 * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream) !=
            hipSuccess ||
@@ -977,7 +997,7 @@ static int run_chunked(fc5_ctx* ctx, fc5_band* b, BatchedArgs a, int bpc) {
     a.total = items[i];
     const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)items[i]));
     if (launch_fill_u64((u64*)ctx->d_item_ctr, 1, 0, ctx->stream) != 0 ||
-        launch_cls5_batched(a, grid, ctx->rows_per_item, ctx->stream) != 0 ||
+        launch_cls5_batched(a, grid, b->rows, ctx->stream) != 0 ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
       rc = FC5_E_HIP;
   }
@@ -1007,7 +1027,7 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   const bool batched = mode != FC5_MODE_PLAIN;  // the T-space kernel, in one launch or in chunks
   const bool any_items = b->n_items + b->n_items3 > 0;
   BatchedArgs a = batched_args(ctx, b);
-  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(ctx->rows_per_item);
+  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(b->rows);
   const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)(b->n_items + b->n_items3)));
   const bool graph = mode == FC5_MODE_BATCHED && any_items && ctx->use_graph && b->irregular.empty() &&
                      b->irregular3.empty();
@@ -1023,7 +1043,7 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
       if (mode == FC5_MODE_CHUNKED) {
         rc = run_chunked(ctx, b, a, bpc);
         if (rc != FC5_OK) return rc;
-      } else if (any_items && launch_cls5_batched(a, grid, ctx->rows_per_item, ctx->stream) != 0) {
+      } else if (any_items && launch_cls5_batched(a, grid, b->rows, ctx->stream) != 0) {
         FAIL(ctx, FC5_E_HIP, "batched launch");
       }
       if (!b->irregular.empty()) {
@@ -1166,6 +1186,7 @@ extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tun
   int rc = band_create(ctx, sc.data(), su.data(), su.size(), &sb);
   if (rc != FC5_OK) return rc;
   const int keep_verify = ctx->verify, keep_rows = ctx->rows_per_item, keep_bpc = ctx->blocks_per_cu;
+  const int auto_E = band_rows_per_item(ctx, b->dev_h, b->dev3_h);  // the shape used without tuning
   ctx->verify = 0;
   std::memset(out, 0, sizeof *out);
   out->best_ms = 1e30;
@@ -1182,7 +1203,7 @@ extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tun
       }
       if (rc != FC5_OK) break;
       ++out->n_tried;
-      if (E == kRowsPerItem && bpc == occ) out->default_ms = ms;
+      if (E == auto_E && bpc == occ) out->default_ms = ms;
       if (ms < out->best_ms) {
         out->best_ms = ms;
         out->rows_per_item = E;

@@ -114,7 +114,7 @@ struct fc5_ctx {
   // options
   u64 hit_cap = 1u << 20;
   int use_gate = 1, verify = 1, stop_first = 0, blocks_per_cu = 0;
-  int rows_per_item = kRowsPerItem, use_graph = 0;
+  int rows_per_item = 0, use_graph = 0;  // rows_per_item 0: by band size (auto_rows_per_item)
   Counters* h_ctr = nullptr;  // pinned read-back of the counters
   u64* h_nhits = nullptr;
   int load_policy = -1;  // cache policy of the xor cell gathers; -1: by table size (table_policy)
@@ -218,6 +218,9 @@ constexpr u64 kMaxItems = (1ull << 32) - (1ull << 20);
 constexpr const char* kTooManyItems =
     "band too large for one launch (more than 2^32 - 2^20 work items at this rows_per_item): split it";
 bool number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3);
+// rows per work item for `rows` e- or d-rows in all: the largest of 32, 16, 8
+// that still gives every wave of the persistent grid 8 items or more, else 4
+int auto_rows_per_item(int num_cu, u64 rows);
 // serving loop (fc5_serve.cpp): stops and frees the server of a context
 void serve_shutdown(fc5_ctx* ctx);
 // Stage-1 units of classes in mask (bits 2..5) for B in [lo, hi] on the device,

@@ -179,7 +179,7 @@ struct Stage1Args {
   u64 *out_B, *out_u;
 };
 
-constexpr int kRowsPerItem = 8;  // default rows (e- or d-rows) per work item of the batched kernel
+constexpr int kRowsPerItem = 16;  // rows (e- or d-rows) per work item of a serving round, unless fixed
 
 // launchers (return hipError_t as int)
 int launch_cls5_batched(const BatchedArgs& a, int grid, int rows, hipStream_t s);  // rows: 4, 8, 16, 32

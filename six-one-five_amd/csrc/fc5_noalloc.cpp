@@ -165,7 +165,9 @@ extern "C" int fc5_run_units_noalloc(fc5_ctx* ctx, const fc5_unit* units, size_t
     dmax = std::max(dmax, U.fmax2);
     H.units[n_dev++] = dev_unit5(U, (u32)i, 5);
   }
-  const int E = ctx->rows_per_item;
+  u64 rows = 0;
+  for (u32 k = 0; k < n_dev; ++k) rows += H.units[k].rows;
+  const int E = ctx->rows_per_item > 0 ? ctx->rows_per_item : auto_rows_per_item(ctx->num_cu, rows);
   u64 items = 0;
   for (u32 k = 0; k < n_dev; ++k) {
     H.units[k].item_base = (u32)items;

@@ -580,7 +580,9 @@ extern "C" int fc5_serve_start(fc5_ctx* ctx, const fc5_serve_opts* opts) try {
   if (!ctx->main.present || !ctx->main.hdr.N) FAIL(ctx, FC5_E_STATE, "serving needs a pair table with its N");
   if (ctx->verify && ctx->verify_table && (!ctx->d_exact || ctx->exact_N < (int)ctx->main.hdr.N))
     FAIL(ctx, FC5_E_STATE, "verify_table needs an exact table with N at least the pair table's N");
-  fc5_serve_opts o{16, ctx->rows_per_item, 1ull << 23, 1ull << 22, 2e10, 288000000000ull};
+  // rounds hold a slice of each band: kRowsPerItem rows per item unless the context fixes it
+  fc5_serve_opts o{16, ctx->rows_per_item > 0 ? ctx->rows_per_item : kRowsPerItem, 1ull << 23, 1ull << 22, 2e10,
+                   288000000000ull};
   if (opts) {
     if (opts->max_bands) o.max_bands = opts->max_bands;
     if (opts->rows_per_item) o.rows_per_item = opts->rows_per_item;

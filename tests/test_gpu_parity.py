@@ -452,15 +452,15 @@ def test_graph_replays_match_eager(ctx):
 
 
 def test_tile_shapes_and_autotune_keep_results(ctx):
-    """every rows-per-item instantiation and the tuned choice give the same
-    counters and hits"""
+    """every rows-per-item instantiation, the choice by band size (0) and the
+    tuned choice give the same counters and hits"""
     say("tile shapes / autotune")
     cls, Bs, us = fc5.emit_units(2000001, 2000300, classes=(2, 3, 4, 5))
     _load(ctx, 2428, 16)
     b = ctx.band(fc5.units_array(Bs, us), cls)
     base = b.run()
     try:
-        for E in (4, 16, 32):
+        for E in (4, 8, 16, 32, 0):
             ctx.set_option("rows_per_item", E)
             _same_run(b.run(), base)
         tune = b.autotune(sample_every=3)
@@ -468,7 +468,7 @@ def test_tile_shapes_and_autotune_keep_results(ctx):
         assert tune["best_ms"] <= tune["default_ms"]
         _same_run(b.run(), base)
     finally:
-        ctx.set_option("rows_per_item", 8)
+        ctx.set_option("rows_per_item", 0)
         ctx.set_option("blocks_per_cu", 0)
         b.close()
 
@@ -741,4 +741,4 @@ def test_band_item_count_limit(ctx):
         ctx.set_option("rows_per_item", 8)  # 2132 items per unit: 2.3e9, accepted (not run)
         ctx.band(units).close()
     finally:
-        ctx.set_option("rows_per_item", 8)
+        ctx.set_option("rows_per_item", 0)
