@@ -313,6 +313,7 @@ def main():
                    "B_lo": cfg["lo"], "B_hi": cfg["hi"], "N": int(hdr.N), "r": int(hdr.r),
                    "units": int(len(units)), "parallelism": "single GPU"},
         "eligible_B_per_hour": n_B * 3600.0 / (ms_step / 1000.0),
+        "reference_gpu_projection": reference_projection(cfg, n_B * 3600.0 / (ms_step / 1000.0)),
         "filter_probes_per_s": r["probes"] / (ms_step / 1000.0),
         "cell_gathers_per_s": 3 * r["probes"] / (ms_step / 1000.0),
         "counters": {k: r[k] for k in COUNTER_KEYS + ("n_hits", "solutions")},
@@ -668,6 +669,29 @@ def table_via_file(ctx, hdr, where):
         f"{t_load:.1f}s (read, exact-N check N={hdr.N}, FPR smoke, upload, widen)")
     return {"loaded": True, "bytes": need, "save_s": t_save, "load_s": t_load, "dir": d,
             "load": "fc5_table_load: read, exact-N check, FPR smoke on the host bytes, H2D copy, cell widening"}
+
+
+# The reference's published class-5 band times with fourcore_cls5_gpu_v4 on an RTX PRO
+# 6000 Blackwell (README.md:146): (B_lo, B_hi, minutes) for 3.5M-4.0M, 4.0M-4.5M,
+# 4.5M-5.0M.  Work per B grows as B^3 (units as B, the (d,e) grid as B^2), so each band
+# projects to eligible B/hour at another B by (B_mid / B)^3.
+REFERENCE_BANDS = ((3500000, 4000000, 49.0), (4000000, 4500000, 71.0), (4500000, 5000000, 98.0))
+
+
+def reference_projection(cfg, b_per_hour):
+    """the reference GPU's eligible class-5 B per hour projected to this band's middle
+    from its published band times, and this run's ratio to it (class-5 bands only)"""
+    if cfg["classes"] != (5,):
+        return None
+    mid = (cfg["lo"] + cfg["hi"]) / 2.0
+    proj = []
+    for lo, hi, minutes in REFERENCE_BANDS:
+        per_hour = eligible_count(lo, hi) * 60.0 / minutes
+        proj.append(per_hour * (((lo + hi) / 2.0) / mid) ** 3)
+    p = sum(proj) / len(proj)
+    return {"eligible_B_per_hour": p, "ratio": b_per_hour / p, "derived": True,
+            "how": "published band times of fourcore_cls5_gpu_v4 on an RTX PRO 6000 (README.md:146), "
+                   "scaled to this band by B^3; a derived figure, not a published one"}
 
 
 def load_profile(name):

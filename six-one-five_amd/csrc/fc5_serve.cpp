@@ -443,7 +443,8 @@ int process_round(fc5_server* S, Round& R) {
     const Counters* c = R.h_ctr + (size_t)B->slot * kSlotStride;
     u64 recs = 0;
     for (int k = 2; k <= 5; ++k) recs += c[k].recs;
-    B->dropped += (recs - B->recs_seen) - kept[(size_t)B->slot];
+    const u64 produced = recs - B->recs_seen, got = kept[(size_t)B->slot];
+    B->dropped += produced > got ? produced - got : 0;  // the round's records of this band it did not keep
     B->recs_seen = recs;
     // running estimates of the work per item from the band's own counters
     if (B->done5) B->cost5 = std::max(1.0, (double)c[5].calls / B->done5);
