@@ -93,7 +93,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--plain-every", type=int, default=64, help="plain form on every k-th unit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-every", type=int, default=160, help="CPU oracle on every k-th unit")
+    ap.add_argument("--cpu-every", type=int, default=0,
+                    help="CPU oracle on every k-th unit (default: k for about 1e10 grid points, ~15 s on 16 threads)")
     ap.add_argument("--cpu-offset", type=int, default=0, help="first unit of the CPU oracle sample")
     ap.add_argument("--table-dir", default=None,
                     help="4: directory for the table file written and loaded back before timing (default: TMPDIR)")
@@ -293,7 +294,8 @@ def main():
 
     cpu = cpu_ref = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(ctx, units, ucls, args.cpu_every, cfg, r, pkeys, args.cpu_offset)
+        every = args.cpu_every or max(1, round(r["calls"] / 1e10))
+        cpu = cpu_baseline(ctx, units, ucls, every, cfg, r, pkeys, args.cpu_offset)
         if not args.band:
             cpu_ref = cpu_reference(args.config)
 

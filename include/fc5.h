@@ -174,7 +174,9 @@ int fc5_band_run(fc5_ctx *ctx, fc5_band *band, int mode, fc5_result *res);
  * and 256 threads, fourcore_cls5_gpu_v4.cu:468,496): times the persistent
  * kernel on every sample_every-th unit of the band for rows per work item in
  * {4, 8, 16, 32} and two persistent grid sizes, keeps the fastest in the
- * context options "rows_per_item" and "blocks_per_cu", and reports it */
+ * context options "rows_per_item" and "blocks_per_cu", and reports it
+ * (default_ms: the shape used without tuning -- the context's rows_per_item, or
+ * the one chosen by the band's size -- at the occupancy-limit grid) */
 typedef struct {
   int rows_per_item, blocks_per_cu, n_tried, pad;
   double best_ms, default_ms;
