@@ -129,7 +129,11 @@ int fc5_set_option(fc5_ctx *ctx, const char *key, int64_t value);
  *      fourcore_cls5_gpu_v4.cu:427-440,568-594) ---- */
 /* xor_load_file (fourcore_xor_store.hpp:147-181) with the exact-N check
  * (expected_N <= 0 accepts any N) and the FPR smoke refusal (> 16/200000),
- * then upload.  Returns FC5_E_IO / FC5_E_FORMAT / FC5_E_NMISMATCH / FC5_E_FPR. */
+ * then upload.  Returns FC5_E_IO / FC5_E_FORMAT / FC5_E_NMISMATCH / FC5_E_FPR;
+ * on any failure the current table stays.  Tables of more packed bytes than the
+ * option "stream_load_min" (4 GiB) are streamed from the file to the device in
+ * chunks (no host copy) and smoked there with the same keys; smaller ones are
+ * read whole and a host copy is kept. */
 int fc5_table_load(fc5_ctx *ctx, const char *path, int expected_N);
 /* upload a table already in host memory (header + packed bytes as on disk) */
 int fc5_table_upload(fc5_ctx *ctx, const fc5_store_header *hdr, const uint8_t *packed, size_t len);
@@ -174,7 +178,8 @@ int fc5_band_run(fc5_ctx *ctx, fc5_band *band, int mode, fc5_result *res);
  * and 256 threads, fourcore_cls5_gpu_v4.cu:468,496): times the persistent
  * kernel on every sample_every-th unit of the band for rows per work item in
  * {4, 8, 16, 32} and two persistent grid sizes, keeps the fastest in the
- * context options "rows_per_item" and "blocks_per_cu", and reports it
+ * context options "rows_per_item" and "blocks_per_cu" unless it is within 1% of
+ * the untuned shape (then the options are left as they were), and reports it
  * (default_ms: the shape used without tuning -- the context's rows_per_item, or
  * the one chosen by the band's size -- at the occupancy-limit grid) */
 typedef struct {

@@ -134,6 +134,9 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) try {
     ctx->rows_per_item = (int)v;
   } else if (k == "use_graph") {
     ctx->use_graph = v ? 1 : 0;
+  } else if (k == "stream_load_min") {
+    if (v < 0) FAIL(ctx, FC5_E_ARG, "stream_load_min must be >= 0");
+    ctx->stream_load_min = (u64)v;
   } else if (k == "twin_tri_gate") {
     ctx->twin_tri = v ? 1 : 0;
   } else if (k == "twin_plain") {
@@ -201,6 +204,90 @@ static int upload_table(fc5_ctx* ctx, const StoreHeader& h, const u8* packed, si
 
 static constexpr size_t kKeepHostTable = (size_t)4 << 30;
 
+// The file's packed cells straight to the device: read in chunks into two
+// pinned buffers, each copied up while the next is read, then widened into
+// dst.  No host copy of the table is made (fc5_table_load of tables above
+// kKeepHostTable; the reference reads the whole file into host memory first,
+// fourcore_xor_store.hpp:147-181).
+static int stream_table(fc5_ctx* ctx, std::FILE* f, const StoreHeader& h, DeviceTable& dst) {
+  constexpr size_t kChunk = (size_t)64 << 20;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  dst = DeviceTable{};
+  dst.hdr = h;
+  if (h.m_cells > (1ull << 40)) FAIL(ctx, FC5_E_FORMAT, "cell count out of range");
+  const size_t want = packed_bytes(h.m_cells, h.r);
+  if (h.n_keys == 0 || h.m_cells < 3) {
+    dst.present = true;
+    return FC5_OK;
+  }
+  // +16 bytes of zero pad for 3-word reads, rounded to whole 8-byte words for the fill below
+  const size_t words = ((want + 3) / 4 + 4 + 1) / 2 * 2;
+  u8* staging = nullptr;
+  u8* pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool recorded[2] = {false, false};
+  auto release = [&] {
+    if (staging) (void)hipFree(staging);
+    for (int k = 0; k < 2; ++k) {
+      if (pin[k]) (void)hipHostFree(pin[k]);
+      if (ev[k]) (void)hipEventDestroy(ev[k]);
+    }
+  };
+  if (hipMalloc(&staging, words * 4) != hipSuccess || hipMalloc(&dst.cells, h.m_cells * 8) != hipSuccess ||
+      hipHostMalloc((void**)&pin[0], kChunk, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&pin[1], kChunk, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess) {
+    release();
+    free_table(dst);
+    FAIL(ctx, FC5_E_NOMEM, "device table or staging allocation");
+  }
+  int rc = FC5_OK;
+  // the zero pad past the packed bytes (a fill kernel over the last 16-byte words)
+  const size_t pad_from = want / 8 * 8;
+  if (launch_fill_u64(reinterpret_cast<u64*>(staging + pad_from), (words * 4 - pad_from) / 8, 0, ctx->stream) != 0)
+    rc = FC5_E_HIP;
+  size_t off = 0;
+  for (int k = 0; off < want && rc == FC5_OK; k ^= 1) {
+    if (recorded[k] && hipEventSynchronize(ev[k]) != hipSuccess) {  // buffer k's last copy is done
+      rc = FC5_E_HIP;
+      break;
+    }
+    const size_t n = std::min(kChunk, want - off);
+    if (std::fread(pin[k], 1, n, f) != n) {
+      rc = FC5_E_IO;
+      break;
+    }
+    if (hipMemcpyAsync(staging + off, pin[k], n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipEventRecord(ev[k], ctx->stream) != hipSuccess) {
+      rc = FC5_E_HIP;
+      break;
+    }
+    recorded[k] = true;
+    off += n;
+  }
+  if (rc == FC5_OK &&
+      (launch_expand_cells(reinterpret_cast<const u32*>(staging), h.m_cells, h.r, dst.cells, ctx->stream) != 0 ||
+       hipStreamSynchronize(ctx->stream) != hipSuccess))
+    rc = FC5_E_HIP;
+  (void)hipStreamSynchronize(ctx->stream);
+  release();
+  if (rc != FC5_OK) {
+    free_table(dst);
+    ctx->err = rc == FC5_E_IO ? "short read of the table file" : "table upload failed";
+    return rc;
+  }
+  dst.dev.cells = dst.cells;
+  dst.dev.block = h.m_cells / 3;
+  dst.dev.mask = fp_mask(h.r);
+  dst.dev.seed = h.mix_seed;
+  dst.dev.md = make_moddiv(dst.dev.block);
+  dst.dev.present = dst.dev.block ? 1u : 0u;
+  dst.dev.policy = table_policy(ctx->load_policy, h.m_cells);
+  dst.present = true;
+  return FC5_OK;
+}
+
 static int adopt_main(fc5_ctx* ctx, XorTable&& t) {
   const int rc = upload_table(ctx, t.hdr, t.packed.data(), t.packed.size(), ctx->main);
   if (rc != FC5_OK) return rc;
@@ -210,12 +297,15 @@ static int adopt_main(fc5_ctx* ctx, XorTable&& t) {
   return FC5_OK;
 }
 
+static int fpr_device(fc5_ctx* ctx, const XorDev& x, int trials, int& hits);
+
 extern "C" int fc5_table_load(fc5_ctx* ctx, const char* path, int expected_N) try {
   if (!ctx || !path) return FC5_E_ARG;
   NOT_SERVING(ctx);
-  XorTable t;
+  StoreHeader h;
+  std::FILE* f = nullptr;
   std::string msg;
-  const int st = load_bin(path, expected_N, t, msg);
+  const int st = open_bin(path, expected_N, h, f, msg);
   switch (st) {
     case kLoadOk: break;
     case kLoadOpen:
@@ -223,10 +313,40 @@ extern "C" int fc5_table_load(fc5_ctx* ctx, const char* path, int expected_N) tr
     case kLoadNMismatch: FAIL(ctx, FC5_E_NMISMATCH, msg);
     default: FAIL(ctx, FC5_E_FORMAT, msg);
   }
-  const int fpr = fpr_smoke(t);
-  if (fpr < 0 || fpr > 16)
+  const size_t want = packed_bytes(h.m_cells, h.r);
+  if (want <= ctx->stream_load_min) {  // small tables: read whole, smoke on the host, keep a host copy
+    XorTable t;
+    t.hdr = h;
+    t.packed.assign(want, 0);
+    const bool ok = !want || std::fread(t.packed.data(), 1, want, f) == want;
+    std::fclose(f);
+    if (!ok) FAIL(ctx, FC5_E_IO, "short read " + std::string(path));
+    const int fpr = fpr_smoke(t);
+    if (fpr < 0 || fpr > 16)
+      FAIL(ctx, FC5_E_FPR, "xor FPR smoke too high (" + std::to_string(fpr) + ") - refuse to search");
+    return adopt_main(ctx, std::move(t));
+  }
+  // large tables: streamed to the device, the same FPR smoke run there on the
+  // new table before it replaces the current one (which stays on any failure)
+  DeviceTable nt;
+  int rc = stream_table(ctx, f, h, nt);
+  std::fclose(f);
+  if (rc != FC5_OK) return rc;
+  int fpr = 0;
+  rc = fpr_device(ctx, nt.dev, 200000, fpr);
+  if (rc != FC5_OK) {
+    free_table(nt);
+    return rc;
+  }
+  if (fpr > 16) {
+    free_table(nt);
     FAIL(ctx, FC5_E_FPR, "xor FPR smoke too high (" + std::to_string(fpr) + ") - refuse to search");
-  return adopt_main(ctx, std::move(t));
+  }
+  free_table(ctx->main);
+  ctx->main = nt;
+  ctx->host_main_valid = false;
+  ctx->host_main = XorTable{};
+  return FC5_OK;
 } catch (...) {
   return abi_exception(ctx);
 }
@@ -316,9 +436,78 @@ static int table_packed(fc5_ctx* ctx, XorTable& t, bool want_bytes) {
   return rc;
 }
 
+// A device table with no host copy to its file: packed on the device, then
+// copied down in 64 MiB chunks through two pinned buffers, each written to the
+// file while the next is copied (xor_save_file, fourcore_xor_store.hpp:126-143;
+// the same bytes as table_packed + save_bin without a host copy of the table).
+static int stream_save(fc5_ctx* ctx, const char* path) {
+  constexpr size_t kChunk = (size_t)64 << 20;
+  const StoreHeader& h = ctx->main.hdr;
+  const u64 m = h.m_cells;
+  const size_t want = packed_bytes(m, h.r);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t room = std::max<size_t>(want, (size_t)((m + 7) / 8) * h.r);
+  u8* d = nullptr;
+  u8* pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::FILE* f = nullptr;
+  auto release = [&] {
+    (void)hipStreamSynchronize(ctx->stream);
+    if (d) (void)hipFree(d);
+    for (int k = 0; k < 2; ++k) {
+      if (pin[k]) (void)hipHostFree(pin[k]);
+      if (ev[k]) (void)hipEventDestroy(ev[k]);
+    }
+    if (f) std::fclose(f);
+  };
+  if (hipMalloc(&d, room) != hipSuccess || hipHostMalloc((void**)&pin[0], kChunk, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&pin[1], kChunk, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess) {
+    release();
+    FAIL(ctx, FC5_E_NOMEM, "pack buffers");
+  }
+  if (hipMemsetAsync(d, 0, room, ctx->stream) != hipSuccess ||
+      gpu_pack_cells(ctx->main.cells, m, h.r, d, ctx->stream) != 0) {
+    release();
+    FAIL(ctx, FC5_E_HIP, "table pack failed");
+  }
+  f = std::fopen(path, "wb");
+  if (!f || std::fwrite(&h, sizeof h, 1, f) != 1) {
+    release();
+    FAIL(ctx, FC5_E_IO, std::string("cannot write ") + path);
+  }
+  const size_t n_chunks = (want + kChunk - 1) / kChunk;
+  auto enqueue = [&](size_t i) {
+    const size_t off = i * kChunk, n = std::min(kChunk, want - off);
+    return hipMemcpyAsync(pin[i % 2], d + off, n, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+           hipEventRecord(ev[i % 2], ctx->stream) == hipSuccess;
+  };
+  int rc = FC5_OK;
+  if (n_chunks && !enqueue(0)) rc = FC5_E_HIP;
+  for (size_t i = 0; i < n_chunks && rc == FC5_OK; ++i) {
+    if (i + 1 < n_chunks && !enqueue(i + 1)) {  // into the other buffer, written out in the last step
+      rc = FC5_E_HIP;
+      break;
+    }
+    const size_t n = std::min(kChunk, want - i * kChunk);
+    if (hipEventSynchronize(ev[i % 2]) != hipSuccess) rc = FC5_E_HIP;
+    else if (std::fwrite(pin[i % 2], 1, n, f) != n) rc = FC5_E_IO;
+  }
+  std::FILE* g = f;
+  f = nullptr;
+  if (std::fclose(g) != 0 && rc == FC5_OK) rc = FC5_E_IO;  // buffered bytes that could not be written
+  release();
+  if (rc == FC5_E_IO) FAIL(ctx, rc, std::string("short write ") + path);
+  if (rc != FC5_OK) FAIL(ctx, rc, "table copy-down failed");
+  return FC5_OK;
+}
+
 extern "C" int fc5_table_save(fc5_ctx* ctx, const char* path) try {
   if (!ctx || !path) return FC5_E_ARG;
   NOT_SERVING(ctx);
+  if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no table");
+  if (!ctx->host_main_valid && ctx->main.cells) return stream_save(ctx, path);
   XorTable t;
   const int rc = table_packed(ctx, t, true);
   if (rc != FC5_OK) return rc;
@@ -378,21 +567,25 @@ extern "C" int fc5_table_query(fc5_ctx* ctx, const uint64_t* keys, size_t n, uin
   return abi_exception(ctx);
 }
 
+// the FPR smoke of xor_fpr_smoke (fourcore_xor_store.hpp:54-65) on a device table
+static int fpr_device(fc5_ctx* ctx, const XorDev& x, int trials, int& hits) {
+  std::vector<u64> keys((size_t)trials);
+  u64 v = 0x123456789abcdef0ULL;
+  for (auto& k : keys) { v = v * kPhi + 1; k = v; }
+  std::vector<u8> out((size_t)trials);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int rc = device_query(ctx, x, keys.data(), keys.size(), out.data());
+  if (rc != FC5_OK) return rc;
+  hits = 0;
+  for (u8 b : out) hits += b;
+  return FC5_OK;
+}
+
 extern "C" int fc5_table_fpr_smoke(fc5_ctx* ctx, int trials, int* hits) try {
   if (!ctx || !hits || trials < 0) return FC5_E_ARG;
   NOT_SERVING(ctx);
   if (!ctx->main.present) FAIL(ctx, FC5_E_STATE, "no table");
-  std::vector<u64> keys((size_t)trials);
-  u64 x = 0x123456789abcdef0ULL;
-  for (auto& k : keys) { x = x * kPhi + 1; k = x; }
-  std::vector<u8> out((size_t)trials);
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  const int rc = device_query(ctx, ctx->main.dev, keys.data(), keys.size(), out.data());
-  if (rc != FC5_OK) return rc;
-  int h = 0;
-  for (u8 v : out) h += v;
-  *hits = h;
-  return FC5_OK;
+  return fpr_device(ctx, ctx->main.dev, trials, *hits);
 } catch (...) {
   return abi_exception(ctx);
 }
@@ -1218,6 +1411,16 @@ extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tun
     ctx->rows_per_item = keep_rows;
     ctx->blocks_per_cu = keep_bpc;
     return rc;
+  }
+  // a shape within 3% of the untuned one on the sample is no reliable gain on the whole
+  // band (the sample's launches are short, so their tails weigh more): keep the untuned shape
+  if (out->default_ms > 0 && out->best_ms > 0.97 * out->default_ms) {
+    out->rows_per_item = auto_E;
+    out->blocks_per_cu = batched_blocks_per_cu(auto_E);
+    out->best_ms = out->default_ms;
+    ctx->rows_per_item = keep_rows;
+    ctx->blocks_per_cu = keep_bpc;
+    return FC5_OK;
   }
   ctx->rows_per_item = out->rows_per_item;
   ctx->blocks_per_cu = out->blocks_per_cu;

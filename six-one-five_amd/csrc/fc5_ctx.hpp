@@ -122,6 +122,7 @@ struct fc5_ctx {
   DeviceTable main, side;
   XorTable host_main;  // host copy when small enough (save, host verify)
   bool host_main_valid = false;
+  u64 stream_load_min = (u64)4 << 30;  // option: fc5_table_load streams tables of more packed bytes
   // gate
   u32* d_gate = nullptr;
   u32* d_gate_ones = nullptr;

@@ -399,8 +399,8 @@ bool read_buc(const std::string& path, unsigned cls_mask, std::vector<BucLine>& 
   return true;
 }
 
-int load_bin(const std::string& path, int expected_N, XorTable& t, std::string& msg) {
-  t.packed.clear();
+int open_bin(const std::string& path, int expected_N, StoreHeader& hdr, std::FILE*& out, std::string& msg) {
+  out = nullptr;
   FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) { msg = "cannot open " + path + " for read"; return kLoadOpen; }
   StoreHeader h;
@@ -434,8 +434,17 @@ int load_bin(const std::string& path, int expected_N, XorTable& t, std::string& 
     msg = "short read " + path;
     return kLoadShort;
   }
-  const size_t want = packed_bytes(h.m_cells, h.r);
-  t.hdr = h;
+  hdr = h;
+  out = f;
+  return kLoadOk;
+}
+
+int load_bin(const std::string& path, int expected_N, XorTable& t, std::string& msg) {
+  t.packed.clear();
+  FILE* f = nullptr;
+  const int st = open_bin(path, expected_N, t.hdr, f, msg);
+  if (st != kLoadOk) return st;
+  const size_t want = packed_bytes(t.hdr.m_cells, t.hdr.r);
   t.packed.assign(want, 0);
   if (want && std::fread(t.packed.data(), 1, want, f) != want) {
     std::fclose(f);

@@ -139,6 +139,9 @@ struct XorTable {
 enum LoadStatus { kLoadOk = 0, kLoadOpen = -1, kLoadBad = -2, kLoadVersion = -3,
                   kLoadNMismatch = -4, kLoadShort = -5 };
 int load_bin(const std::string& path, int expected_N, XorTable& t, std::string& msg);
+// the checks of load_bin (header, version, exact N, file long enough for the
+// packed cells) without reading the cells: on kLoadOk *f is open just past the header
+int open_bin(const std::string& path, int expected_N, StoreHeader& hdr, std::FILE*& f, std::string& msg);
 bool save_bin(const std::string& path, const XorTable& t, std::string& msg);
 
 // ---------------------------------------------------------------------------

@@ -168,6 +168,46 @@ def test_hit_overflow_is_reported(ctx):
     assert len(out.hits) == 16
 
 
+def test_table_load_streamed_equals_whole_file(ctx, tmp_path):
+    """fc5_table_load of a table above "stream_load_min" streams the file to the
+    device in 64 MiB chunks (forced here on a 369 MB table: six chunks) and smokes
+    it there: the device bytes equal the file's, the search gives the counters and
+    hits of the table it was saved from, an FPR refusal and a wrong N keep the
+    current table"""
+    say("streamed table load")
+    N = 10000
+    ctx.table_build(N, 48, on_gpu=True)
+    p = str(tmp_path / "t.bin")
+    ctx.table_save(p)
+    _, Bs, us = fc5.emit_units(400001, 420000)
+    units = fc5.units_array(Bs[::5], us[::5])
+    before = ctx.run(units)
+    raw = open(p, "rb").read()
+    q = str(tmp_path / "r4.bin")
+    fc5.build_pairs_file(300, 4, q)  # far too many false positives: refused
+    ctx.set_option("stream_load_min", 0)
+    try:
+        ctx.table_load(p, N)
+        h, packed = ctx.table_download()
+        assert len(raw) - 64 == len(packed) > 5 * (64 << 20)
+        assert packed.tobytes() == raw[64:] and bytes(h) == raw[:64]
+        after = ctx.run(units)
+        for k in COUNTERS:
+            assert after.result[k] == before.result[k], k
+        assert np.array_equal(after.hits, before.hits)
+        with pytest.raises(fc5.Fc5Error) as e:
+            ctx.table_load(q, 300)
+        assert e.value.code == fc5.E_FPR
+        with pytest.raises(fc5.Fc5Error) as e:
+            ctx.table_load(p, N + 1)
+        assert e.value.code == fc5.E_NMISMATCH
+        assert bytes(ctx.table_info()) == raw[:64]  # the loaded table stayed
+        again = ctx.run(units)
+        assert np.array_equal(again.hits, before.hits) and again.result["probes"] == before.result["probes"]
+    finally:
+        ctx.set_option("stream_load_min", 4 << 30)
+
+
 def test_table_load_errors(ctx, tmp_path):
     say("table load errors")
     p = str(tmp_path / "t.bin")
