@@ -91,6 +91,8 @@ def main():
                     help="exact sorted pair table beside the xor filter, planted targets in both, hits "
                          "checked by lookup (auto: on for --config 2 and 4)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    ap.add_argument("--no-strict", action="store_true",
+                    help="skip the strict-gate leg (the same band with the option strict_gate on)")
     ap.add_argument("--plain-every", type=int, default=64, help="plain form on every k-th unit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-every", type=int, default=0,
@@ -139,7 +141,8 @@ def main():
     if args.load_policy is not None:
         ctx.set_option("load_policy", args.load_policy)
     use_exact = args.exact_table == "on" or (args.exact_table == "auto" and cfg.get("exact", False))
-    plants = fc5.plant_points(Bs, us, cfg["plants"], cfg["seed"], exact=True) if cfg["plants"] else []
+    # plants that pass the strict gate too, as a real pair sum does: both legs must find them all
+    plants = fc5.plant_points(Bs, us, cfg["plants"], cfg["seed"], exact=True, strict=True) if cfg["plants"] else []
     pkeys = np.array([p[3] for p in plants], np.uint64)
     t0 = time.time()
     ctx.table_build(cfg["N"], args.r, extra=pkeys if cfg["plant_mode"] == "main" else None, on_gpu=True)
@@ -241,6 +244,11 @@ def main():
         f"{value:.4e} points/s, "
         f"probes={r['probes']} maybe={r['maybe']} side_maybe={r['side_maybe']}")
 
+    strict = None
+    if not args.no_strict:
+        saved = (tune["rows_per_item"], tune["blocks_per_cu"]) if tune else (0, args.blocks_per_cu)
+        strict = strict_leg(ctx, band, outs[-1], Bs, us, ucls if mixed else None, args, check, value, saved)
+
     ctx.set_option("use_graph", 0)
     ctx.set_option("verify_table", 0)
     plain = None
@@ -337,6 +345,7 @@ def main():
         "autotune": tune,
         "graph": not args.no_graph,
         "plain_form": plain,
+        "strict_gate": strict,
         "cpu_baseline": cpu,
         "cpu_reference": cpu_ref,
         "table_build_s": t_build,
@@ -672,6 +681,68 @@ def table_via_file(ctx, hdr, where):
     return {"loaded": True, "bytes": need, "save_s": t_save, "load_s": t_load, "dir": d,
             "load": "fc5_table_load: exact-N check, file streamed to the device in 64 MiB pinned chunks, "
                     "cell widening, FPR smoke on the device"}
+
+
+def hit_T(cls, B, u, d, e):
+    """the exact T of a hit: class 5 T(e, d); classes 2-4 T(d) - c3^6 with c3 in e"""
+    if cls == 5:
+        return (B ** 6 - u ** 6 - (14 * e) ** 6 - (21 * d) ** 6) // 42 ** 6
+    f = {2: 14, 3: 21, 4: 7}[cls]
+    return (B ** 6 - u ** 6 - (f * d) ** 6) // 42 ** 6 - e ** 6
+
+
+def strict_leg(ctx, band, base, Bs, us, ucls, args, check, base_value, saved):
+    """The same band with the option strict_gate on (tuned and timed like the
+    default leg).  Checks: the gate counters equal the default run's, every
+    plant is found, and the hit list is exactly the default hit list restricted
+    to hits whose exact T passes the strict gate.  saved: the default leg's
+    (rows_per_item, blocks_per_cu), restored afterwards."""
+    ctx.set_option("strict_gate", 1)
+    ctx.set_option("rows_per_item", 0)  # the variant's own untuned shape
+    ctx.set_option("blocks_per_cu", args.blocks_per_cu)
+    tune = None
+    try:
+        if not args.no_autotune:
+            tune = band.autotune(sample_every=16)
+        check(band.run(fc5.MODE_BATCHED))
+        kms, outs = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            outs.append(band.run(fc5.MODE_BATCHED))
+            kms.append(outs[-1].result["kernel_ms"])
+        wall = time.perf_counter() - t0
+    finally:
+        ctx.set_option("strict_gate", 0)
+        ctx.set_option("rows_per_item", saved[0])
+        ctx.set_option("blocks_per_cu", saved[1])
+    for o in outs:
+        check(o)
+    r, r0 = outs[-1].result, base.result
+    for k in ("calls", "neg", "zero", "gated", "probes", "jobs", "fail_t"):
+        assert r[k] == r0[k], k
+    keep = set()
+    for h in base.hits:
+        i = int(h["unit"])
+        cls = int(ucls[i]) if ucls is not None else 5
+        if fc5.strict_gate_ok(hit_T(cls, int(Bs[i]), int(us[i]), int(h["d"]), int(h["e"]))):
+            keep.add((i, int(h["e"]), int(h["d"]), int(h["flags"]) & 3))
+    got = {(int(h["unit"]), int(h["e"]), int(h["d"]), int(h["flags"]) & 3) for h in outs[-1].hits}
+    assert got == keep, "strict hits differ from the strict-passing default hits"
+    ms = 1000.0 * wall / args.steps
+    value = r["calls"] / (ms / 1000.0)
+    queries = r["probes"] - r["strict_out"]
+    log(f"[bench] strict gate: {ms:.1f} ms/step, kernel {float(np.mean(kms)):.1f} ms, {value:.4e} points/s "
+        f"({value / base_value:.2f}x the default leg); queries {queries} of {r['probes']} probes; "
+        f"maybe={r['maybe']} side_maybe={r['side_maybe']} (default {r0['maybe']}, {r0['side_maybe']})")
+    return {"value": value, "unit": "(d,e) grid points/s", "ms_per_step": ms, "kernel_ms": float(np.mean(kms)),
+            "speedup_vs_default": value / base_value, "probes": r["probes"], "queries": queries,
+            "strict_out": r["strict_out"], "maybe": r["maybe"], "side_maybe": r["side_maybe"],
+            "default_maybe": r0["maybe"], "default_side_maybe": r0["side_maybe"],
+            "hits_equal_filtered_default": True, "autotune": tune,
+            "option": "strict_gate=1: before each xor query T mod 64, 81 and 31*37*43 must also be sums "
+                      "of two sixth powers (a necessary condition for a pair sum that the reference does "
+                      "not test); every counter but maybe/side_maybe is the reference's, the hit list is "
+                      "the default one without the filter's false positives the strict gate rejects"}
 
 
 # The reference's published class-5 band times with fourcore_cls5_gpu_v4 on an RTX PRO

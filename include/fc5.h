@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FC5_ABI_VERSION 2
+#define FC5_ABI_VERSION 3
 
 enum {
   FC5_OK = 0,
@@ -77,6 +77,8 @@ typedef struct {
   uint64_t units, units_skipped, units_irregular;
   uint64_t jobs, fail_t;
   double kernel_ms, wall_ms;
+  uint64_t strict_out;  /* option "strict_gate": gate survivors the strict gate rejected
+                           (no xor query made); the queries made are probes - strict_out */
 } fc5_result;
 
 /* a filter hit (fourcore_cls5_gpu_v4.cu:140).  unit indexes the band's units
@@ -111,7 +113,13 @@ void fc5_ctx_destroy(fc5_ctx *ctx);
  * next fc5_last_error call (safe while a server thread runs) */
 const char *fc5_last_error(const fc5_ctx *ctx);
 /* options: "hit_cap" (default 1<<20), "use_gate" (1; 0 = --no-gate,
- * fourcore_cls5_gpu_v4.cu:634), "blocks_per_cu" (0 = occupancy limit) for the
+ * fourcore_cls5_gpu_v4.cu:634), "strict_gate" (0; 1 = before each xor query, also
+ * require T mod 64, T mod 81 and T mod 31*37*43 to be sums of two sixth powers:
+ * a necessary condition for T = a^6 + b^6 that the reference does not test, so no
+ * solution is lost; counters and hits are the reference's except that the xor
+ * filter's false positives among the T it rejects are not reported, and
+ * strict_out counts the rejected gate survivors; classes 2-5, every mode, not the
+ * (6,1,7) twin), "blocks_per_cu" (0 = occupancy limit) for the
  * persistent grid, "rows_per_item" (e- or d-rows per work item: 4, 8, 16, 32, or
  * 0, the default: by band size, the largest of 32, 16, 8 that leaves every wave of
  * the persistent grid at least 8 items, else 4), "use_graph" (0; 1 replays the batched pipeline as a captured hipGraph),

@@ -57,6 +57,7 @@
 #include <math.h>
 #ifdef _OPENMP
 #include <omp.h>
+#include <pthread.h>
 #endif
 
 typedef unsigned __int128 u128;
@@ -806,6 +807,38 @@ int or_pack_unit5(u64 B, u64 u, or_unit5_t *U) {
 }
 
 /* ------------------------------------------------------------------ */
+/* the strict gate: sums of two sixth powers modulo 64, 81, 31, 37, 43   */
+/* by enumeration (a test of this build's option; the reference has     */
+/* no such gate)                                                        */
+/* ------------------------------------------------------------------ */
+static int sum2_pow6_mod(u32 t, u32 m) {
+  for (u32 a = 0; a < m; ++a) {
+    u64 pa = 1;
+    for (int k = 0; k < 6; ++k) pa = pa * a % m;
+    for (u32 b = a; b < m; ++b) {
+      u64 pb = 1;
+      for (int k = 0; k < 6; ++k) pb = pb * b % m;
+      if ((pa + pb) % m == t) return 1;
+    }
+  }
+  return 0;
+}
+
+static unsigned char g_strict[5][81];
+static void strict_init_once(void) {
+  static const u32 mods[5] = {64, 81, 31, 37, 43};
+  for (int i = 0; i < 5; ++i)
+    for (u32 t = 0; t < mods[i]; ++t) g_strict[i][t] = (unsigned char)sum2_pow6_mod(t, mods[i]);
+}
+static pthread_once_t g_strict_once = PTHREAD_ONCE_INIT;
+
+int or_strict_pass(u32 t64, u32 t81, u32 tA) {
+  pthread_once(&g_strict_once, strict_init_once);
+  return g_strict[0][t64 % 64] && g_strict[1][t81 % 81] && g_strict[2][tA % 31] && g_strict[3][tA % 37] &&
+         g_strict[4][tA % 43];
+}
+
+/* ------------------------------------------------------------------ */
 /* the peel (fourcore_cls5_gpu_v4.cu:353-410)                          */
 /* ------------------------------------------------------------------ */
 int or_cls5_peel(const or_unit5_t *units, int n, const or_xor_t *f, const or_xor_t *side,
@@ -820,11 +853,13 @@ int or_cls5_peel(const or_unit5_t *units, int n, const or_xor_t *f, const or_xor
   const u64 inv42_247 = (u64)mod_inv(or_mod_pow6(42, 247), 247);
   const u64 blk = (f && f->hdr.m_cells) ? f->hdr.m_cells / 3 : 0;
   const u64 sblk = (side && side->hdr.m_cells) ? side->hdr.m_cells / 3 : 0;
-  u64 calls = 0, neg = 0, zero = 0, gated = 0, probes = 0, maybe = 0, smaybe = 0;
+  const u64 mA = 31ULL * 37 * 43 * (u64)M42, m81 = 81ULL * (u64)M42;
+  const u64 c_mA = (u64)(((u128)1 << 64) % (u128)mA), c_m81 = (u64)(((u128)1 << 64) % (u128)m81);
+  u64 calls = 0, neg = 0, zero = 0, gated = 0, probes = 0, maybe = 0, smaybe = 0, queries = 0;
   u64 nh = 0;
   if (threads < 1) threads = 1;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads) \
-    reduction(+ : calls, neg, zero, gated, probes, maybe, smaybe)
+    reduction(+ : calls, neg, zero, gated, probes, maybe, smaybe, queries)
   for (int ui = 0; ui < n; ++ui) {
     const or_unit5_t *U = &units[ui];
     const u32 kmax1 = U->fmax1 / U->mod1 + 1;
@@ -845,7 +880,7 @@ int or_cls5_peel(const or_unit5_t *units, int n, const or_xor_t *f, const or_xor
         or_pow6_192(21u * d, D6);
         if (or_sub192(R, D6)) { ++neg; continue; }
         if ((R[0] | R[1] | R[2]) == 0) { ++zero; continue; }
-        if (use_gate) {
+        if (use_gate & 1) {
           const u64 rmod = or_mod192(R, m504, c_m504);
           const u32 t504 = (u32)((rmod / (u64)M42) % 504ULL);
           const u64 r247 = or_mod192(R, 247ULL, c_m247);
@@ -857,6 +892,12 @@ int or_cls5_peel(const or_unit5_t *units, int n, const or_xor_t *f, const or_xor
         }
         ++probes;
         const u64 fp = ((R[0] >> 6) | (R[1] << 58)) * inv216;
+        if (use_gate & 2) { /* T mod m = (R mod 42^6 m) / 42^6 */
+          const u32 tA = (u32)(or_mod192(R, mA, c_mA) / (u64)M42);
+          const u32 t81 = (u32)(or_mod192(R, m81, c_m81) / (u64)M42);
+          if (!or_strict_pass((u32)(fp & 63), t81, tA)) continue;
+          ++queries;
+        }
         u32 flags = 0;
         if (f && device_query(f->packed, blk, f->hdr.r, f->hdr.mix_seed, fp)) flags |= 1u;
         if (side && device_query(side->packed, sblk, side->hdr.r, side->hdr.mix_seed, fp))
@@ -881,6 +922,7 @@ int or_cls5_peel(const or_unit5_t *units, int n, const or_xor_t *f, const or_xor
   counts->calls = calls; counts->neg = neg; counts->zero = zero;
   counts->gated = gated; counts->probes = probes;
   counts->maybe = maybe; counts->side_maybe = smaybe;
+  counts->queries = queries;
   *n_hits = nh;
   return 0;
 }
@@ -985,10 +1027,10 @@ int or_find3(const int *cls, const u64 *Bv, const u64 *uv, int n, const or_xor_t
   or_gate_t g;
   or_build_gate(&g);
   const u64 blk = (f && f->hdr.m_cells) ? f->hdr.m_cells / 3 : 0;
-  u64 jobs = 0, failT = 0, calls = 0, gated = 0, probes = 0, maybe = 0, nh = 0;
+  u64 jobs = 0, failT = 0, calls = 0, gated = 0, probes = 0, maybe = 0, queries = 0, nh = 0;
   if (threads < 1) threads = 1;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads) \
-    reduction(+ : jobs, failT, calls, gated, probes, maybe)
+    reduction(+ : jobs, failT, calls, gated, probes, maybe, queries)
   for (int ui = 0; ui < n; ++ui) {
     const u64 B = Bv[ui], u = uv[ui];
     const int c = cls[ui];
@@ -1015,7 +1057,7 @@ int or_find3(const int *cls, const u64 *Bv, const u64 *uv, int n, const or_xor_t
         const u32 T504 = (u32)(T % 504), T247 = (u32)(T % 247);
         for (u32 c3 = lo; c3 <= hi; ++c3) {          /* k_find3 (fourcore_find_v4.cu:255-284) */
           ++calls;
-          if (use_gate) {
+          if (use_gate & 1) {
             u32 r504 = T504 + 504u - g.p6504[c3 % 504];
             if (r504 >= 504u) r504 -= 504u;
             u32 r247 = T247 + 247u - g.p6247[c3 % 247];
@@ -1025,6 +1067,11 @@ int or_find3(const int *cls, const u64 *Bv, const u64 *uv, int n, const or_xor_t
           ++probes;
           const u64 c2 = (u64)c3 * c3;
           const u64 fp = Tl - c2 * c2 * c2;
+          if (use_gate & 2) { /* V = T - c3^6 exactly */
+            const u128 V = T - pow6_u128(c3);
+            if (!or_strict_pass((u32)(V % 64), (u32)(V % 81), (u32)(V % (31 * 37 * 43)))) continue;
+            ++queries;
+          }
           if (f && device_query(f->packed, blk, f->hdr.r, f->hdr.mix_seed, fp)) {
             ++maybe;
             u64 slot;
@@ -1044,6 +1091,7 @@ int or_find3(const int *cls, const u64 *Bv, const u64 *uv, int n, const or_xor_t
   }
   counts->jobs = jobs; counts->failT = failT; counts->calls = calls;
   counts->gated = gated; counts->probes = probes; counts->maybe = maybe;
+  counts->queries = queries;
   *n_hits = nh;
   return 0;
 }

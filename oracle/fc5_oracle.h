@@ -106,9 +106,15 @@ typedef struct {
 } or_unit5_t;
 int or_pack_unit5(uint64_t B, uint64_t u, or_unit5_t *U);
 
+/* ---- the strict gate (no counterpart in the reference): 1 when T mod 64,
+ * 81, 31, 37 and 43 are each a sum of two sixth powers modulo that number;
+ * t64 = T mod 64, tA = T mod 31*37*43, t81 = T mod 81 ---- */
+int or_strict_pass(uint32_t t64, uint32_t t81, uint32_t tA);
+
 /* ---- the peel: k_cls5_192 semantics (fourcore_cls5_gpu_v4.cu:353-410) --- */
 typedef struct {
   uint64_t calls, neg, zero, gated, probes, maybe, side_maybe;
+  uint64_t queries; /* use_gate & 2 (strict gate): probes that pass it and are queried */
 } or_counts_t;
 
 typedef struct {
@@ -116,7 +122,9 @@ typedef struct {
   uint64_t fp;
 } or_hit_t;
 
-/* Runs the peel for units[0..n).  side may be NULL (planted-key side filter,
+/* Runs the peel for units[0..n).  use_gate bit 0: the reference gate; bit 1:
+ * the strict gate of this build (T mod 64, 81, 31, 37 and 43 a sum of two sixth
+ * powers) on every probe before its query.  side may be NULL (planted-key side filter,
  * probed with the same fingerprint as the main filter).  Hits are written to
  * hits (up to cap; *n_hits gets the true count).  OpenMP over units when
  * threads > 1.  Returns 0. */
@@ -127,6 +135,7 @@ int or_cls5_peel(const or_unit5_t *units, int n, const or_xor_t *f,
 /* ---- classes 2-4 find3 leaf (fourcore_find_v4.cu:165-205,255-284,401-425) ---- */
 typedef struct {
   uint64_t jobs, failT, calls, gated, probes, maybe;
+  uint64_t queries; /* use_gate & 2: probes that pass the strict gate */
 } or_f3counts_t;
 /* expand each (cls,B,u) unit into jobs T = (B^6-u^6-(f d)^6)/42^6 (failT when
  * not positive/divisible), window [lo,hi] per job, c3 sweep with the gate on

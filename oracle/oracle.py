@@ -34,7 +34,8 @@ class Unit5(C.Structure):
 
 
 class Counts(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe")]
+    _fields_ = [(n, C.c_uint64) for n in ("calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe",
+                                          "queries")]
 
 
 class Hit(C.Structure):
@@ -67,7 +68,7 @@ _lib.or_free_de_cls5.argtypes = [C.c_int64, C.c_int64, _P, C.POINTER(C.c_int), C
                                  C.POINTER(C.c_int64)]
 _lib.or_pow6_192.argtypes = [C.c_uint32, _P]
 class F3Counts(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in ("jobs", "failT", "calls", "gated", "probes", "maybe")]
+    _fields_ = [(n, C.c_uint64) for n in ("jobs", "failT", "calls", "gated", "probes", "maybe", "queries")]
 
 
 _lib.or_find3.argtypes = [_P, _P, _P, C.c_int, C.POINTER(Xor), C.c_int, C.POINTER(F3Counts), _P, C.c_uint64,
@@ -225,13 +226,15 @@ def pack_units(B, u):
     return arr, ok
 
 
-def peel(units_arr, n, table, side=None, use_gate=True, threads=8, cap=1 << 20):
-    """k_cls5_192 semantics over packed units: counters dict and sorted hits."""
+def peel(units_arr, n, table, side=None, use_gate=True, threads=8, cap=1 << 20, strict=False):
+    """k_cls5_192 semantics over packed units: counters dict and sorted hits.
+    strict: the strict residue gate before every query (an option of this build)."""
     c = Counts()
     hits = np.zeros(cap, HIT_DTYPE)
     nh = C.c_uint64()
     _lib.or_cls5_peel(C.cast(units_arr, _P), n, C.byref(table.x) if table is not None else None,
-                      C.byref(side.x) if side is not None else None, 1 if use_gate else 0, C.byref(c),
+                      C.byref(side.x) if side is not None else None, (1 if use_gate else 0) | (2 if strict else 0),
+                      C.byref(c),
                       hits.ctypes.data_as(_P), cap, C.byref(nh), threads)
     hits = hits[:min(nh.value, cap)]
     hits = np.sort(hits, order=["unit", "e", "d", "flags"])
@@ -245,8 +248,9 @@ def verify_hit(B, u, lim, d, e, fp, N, cap=16):
     return [(terms[5 * i:5 * i + 5].tolist(), int(g[i])) for i in range(min(n, cap))]
 
 
-def find3(cls, B, u, table, use_gate=True, threads=8, cap=1 << 20):
-    """classes 2-4 find3 semantics: counters dict and sorted hits (e holds c3)"""
+def find3(cls, B, u, table, use_gate=True, threads=8, cap=1 << 20, strict=False):
+    """classes 2-4 find3 semantics: counters dict and sorted hits (e holds c3);
+    strict: the strict residue gate before every query"""
     cl = np.ascontiguousarray(cls, dtype=np.int32)
     Bv = np.ascontiguousarray(B, dtype=np.uint64)
     uv = np.ascontiguousarray(u, dtype=np.uint64)
@@ -254,7 +258,8 @@ def find3(cls, B, u, table, use_gate=True, threads=8, cap=1 << 20):
     hits = np.zeros(cap, HIT_DTYPE)
     nh = C.c_uint64()
     _lib.or_find3(cl.ctypes.data_as(_P), Bv.ctypes.data_as(_P), uv.ctypes.data_as(_P), len(cl),
-                  C.byref(table.x) if table is not None else None, 1 if use_gate else 0, C.byref(c),
+                  C.byref(table.x) if table is not None else None, (1 if use_gate else 0) | (2 if strict else 0),
+                  C.byref(c),
                   hits.ctypes.data_as(_P), cap, C.byref(nh), threads)
     hits = np.sort(hits[:min(nh.value, cap)], order=["unit", "e", "d", "flags"])
     return {f: getattr(c, f) for f, _ in F3Counts._fields_}, hits, nh.value
@@ -384,3 +389,12 @@ def exact_range(entries, s, T):
     """equal_sum_range (quad_sum.hpp:37-45) over the entry form: [lo, hi)"""
     import bisect
     return bisect.bisect_left(entries, T << s), bisect.bisect_left(entries, (T + 1) << s)
+
+
+_lib.or_strict_pass.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+_lib.or_strict_pass.restype = C.c_int
+
+
+def strict_pass_T(T):
+    """the strict gate on an exact T (an option of this build, not the reference's)"""
+    return bool(_lib.or_strict_pass(T % 64, T % 81, T % (31 * 37 * 43)))

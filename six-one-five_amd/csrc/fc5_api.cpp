@@ -14,6 +14,7 @@
 
 #include "fc5_build.hpp"
 #include "fc5_ctx.hpp"
+#include "fc5_strict.hpp"
 
 using namespace fc5;
 
@@ -117,6 +118,8 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) try {
     ctx->hit_cap = (u64)v;
   } else if (k == "use_gate") {
     ctx->use_gate = v ? 1 : 0;
+  } else if (k == "strict_gate") {
+    ctx->strict_gate = v ? 1 : 0;
   } else if (k == "verify") {
     ctx->verify = v ? 1 : 0;
   } else if (k == "verify_table") {
@@ -818,9 +821,9 @@ bool fc5::number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, 
   return true;
 }
 
-int fc5::auto_rows_per_item(int num_cu, u64 rows) {
+int fc5::auto_rows_per_item(int num_cu, u64 rows, bool strict) {
   for (int E : {32, 16, 8}) {
-    const u64 waves = (u64)num_cu * (u64)std::max(1, batched_blocks_per_cu(E)) * 4u;
+    const u64 waves = (u64)num_cu * (u64)std::max(1, batched_blocks_per_cu(E, strict)) * 4u;
     if (rows / (u64)E >= 8 * waves) return E;
   }
   return 4;
@@ -833,7 +836,7 @@ static int band_rows_per_item(const fc5_ctx* ctx, const std::vector<DevUnit5>& d
   u64 rows = 0;
   for (const DevUnit5& D : dev) rows += D.rows;
   for (const DevUnit3& D : dev3) rows += D.rows;
-  return auto_rows_per_item(ctx->num_cu, rows);
+  return auto_rows_per_item(ctx->num_cu, rows, ctx->strict_gate != 0);
 }
 
 static int renumber(fc5_ctx* ctx, fc5_band* b) {
@@ -961,6 +964,7 @@ extern "C" int fc5_band_create_mixed(fc5_ctx* ctx, const uint32_t* cls, const fc
 static PlainArgs plain_args(fc5_ctx* ctx) {
   PlainArgs p{};
   p.use_gate = ctx->use_gate;
+  p.strict = ctx->strict_gate;
   p.gate = ctx->gw;
   p.main = ctx->main.dev;
   p.side = ctx->side.present ? ctx->side.dev : XorDev{};
@@ -1038,6 +1042,7 @@ static int run_plain_find3(fc5_ctx* ctx, fc5_band* b, const std::vector<u32>& wh
   }
   PlainF3Args a{};
   a.use_gate = ctx->use_gate;
+  a.strict = ctx->strict_gate;
   a.gate = ctx->gw;
   a.p6 = ctx->d_p6;
   a.main = ctx->main.dev;
@@ -1080,7 +1085,7 @@ static int run_plain_find3(fc5_ctx* ctx, fc5_band* b, const std::vector<u32>& wh
       C.uid = uid;
       C.d = d;
       C.band = (u32)U.cls;
-      C.pad = 0;
+      C.srow = strict_row((u64)(T >> 64), (u64)T);
       if (n == kChunk) flush();
     }
   }
@@ -1100,6 +1105,7 @@ static BatchedArgs batched_args(fc5_ctx* ctx, fc5_band* b) {
   a.ndg = ctx->d_ndg;
   a.ndg_K = ctx->ndg_K;
   a.use_gate = (u32)ctx->use_gate;
+  a.strict = (u32)ctx->strict_gate;
   a.main = ctx->main.dev;
   a.side = ctx->side.present ? ctx->side.dev : XorDev{};
   a.hits = ctx->d_hits;
@@ -1220,7 +1226,7 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   const bool batched = mode != FC5_MODE_PLAIN;  // the T-space kernel, in one launch or in chunks
   const bool any_items = b->n_items + b->n_items3 > 0;
   BatchedArgs a = batched_args(ctx, b);
-  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(b->rows);
+  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(b->rows, ctx->strict_gate != 0);
   const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)(b->n_items + b->n_items3)));
   const bool graph = mode == FC5_MODE_BATCHED && any_items && ctx->use_graph && b->irregular.empty() &&
                      b->irregular3.empty();
@@ -1282,6 +1288,7 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
     r.side_maybe = B.side_maybe + P.side_maybe;
     r.jobs = B.jobs + jobs_host[k];
     r.fail_t = B.failT + fail_host[k];
+    r.strict_out = ctx->strict_gate ? (B.probes + P.probes) - (B.queries + P.queries) : 0;
     res->calls += r.calls;
     res->neg += r.neg;
     res->zero += r.zero;
@@ -1291,6 +1298,7 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
     res->side_maybe += r.side_maybe;
     res->jobs += r.jobs;
     res->fail_t += r.fail_t;
+    res->strict_out += r.strict_out;
   }
   for (size_t i = 0; i < b->n_in; ++i) {
     const int k = b->cls_of[i];
@@ -1385,7 +1393,7 @@ extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tun
   out->best_ms = 1e30;
   fc5_result r;
   for (int E : {8, 4, 16, 32}) {
-    const int occ = batched_blocks_per_cu(E);
+    const int occ = batched_blocks_per_cu(E, ctx->strict_gate != 0);
     for (int bpc : {occ, std::max(1, occ / 2)}) {
       ctx->rows_per_item = E;
       ctx->blocks_per_cu = bpc;
@@ -1416,7 +1424,7 @@ extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tun
   // band (the sample's launches are short, so their tails weigh more): keep the untuned shape
   if (out->default_ms > 0 && out->best_ms > 0.97 * out->default_ms) {
     out->rows_per_item = auto_E;
-    out->blocks_per_cu = batched_blocks_per_cu(auto_E);
+    out->blocks_per_cu = batched_blocks_per_cu(auto_E, ctx->strict_gate != 0);
     out->best_ms = out->default_ms;
     ctx->rows_per_item = keep_rows;
     ctx->blocks_per_cu = keep_bpc;

@@ -8,12 +8,17 @@
 //                 and an LDS bitmap; survivors are compacted per wavefront
 //                 (ballot + mbcnt into an LDS ring) and probed 64 at a time:
 //                 fingerprint, 3 mixes, 3 exact remainders, 3 HBM gathers.
+//                 With the strict gate on (variant S), each batch of 64 gate
+//                 survivors first passes the strict residue gate
+//                 (fc5_strict.hpp) and is compacted again into a second ring;
+//                 only its survivors are probed.
 // k_cls5_plain    the reference kernel shape (fourcore_cls5_gpu_v4.cu:353-410):
 //                 block per (unit, e), threads over d, per-point 192-bit
 //                 sixth power, subtract-with-borrow and modular gate.
 // k_expand_cells  widens the r-bit packed table to one u64 per cell.
 // k_query         membership queries for a key list (tests, FPR smoke).
 #include "fc5_device.hpp"
+#include "fc5_strict.hpp"
 
 namespace fc5 {
 
@@ -32,13 +37,22 @@ __device__ __forceinline__ u128 dq_of(u32 x) {
 // ---------------------------------------------------------------------------
 constexpr u32 kFind3Bit = 0x80000000u;  // ring entry unit field: find3 row
 
-__device__ __forceinline__ void drain(const BatchedArgs& a, const u32* qu, const u32* qd,
-                                      const u32* qe, u32 head, u32 n, u32 lane) {
+struct Ring {
+  u32 *qu, *qd, *qe;  // unit (| kFind3Bit), d, e (or c3)
+  u32* qx;            // strict-gate row word (variant S, survivor ring only)
+  u32 head, tail;
+};
+
+// probe the ring's oldest n entries (n <= 64, one per lane).  S: also count
+// the queries per band (one atomic per batch unless it spans bands).
+template <bool S>
+__device__ __forceinline__ void drain(const BatchedArgs& a, const Ring& q, u32 n, u32 lane) {
+  u32 band = 0;
   if (lane < n) {
-    const u32 slot = (head + lane) & (kRing - 1);
-    const u32 tag = qu[slot], d = qd[slot], e = qe[slot];
+    const u32 slot = (q.head + lane) & (kRing - 1);
+    const u32 tag = q.qu[slot], d = q.qd[slot], e = q.qe[slot];
     u64 fp;
-    u32 uid, band;
+    u32 uid;
     if (tag & kFind3Bit) {  // classes 2-4: V = Q - floor(d^6/mod) - c^6, c in e
       const DevUnit3* U = a.units3 + (tag & ~kFind3Bit);
       const u64 d3 = (u64)d * d * d;
@@ -76,16 +90,64 @@ __device__ __forceinline__ void drain(const BatchedArgs& a, const u32* qu, const
       }
     }
   }
+  if (S) {
+    const u32 b0 = (u32)__builtin_amdgcn_readfirstlane((int)band);  // lane 0 holds an entry
+    const u64 same = __ballot(lane < n && band == b0);
+    if (lane == 0) atomicAdd((unsigned long long*)&a.ctr[b0].queries, (unsigned long long)__popcll(same));
+    if (lane < n && band != b0) atomicAdd((unsigned long long*)&a.ctr[band].queries, 1ULL);
+  }
 }
 
-struct Ring {
-  u32 *qu, *qd, *qe;
-  u32 head, tail;
-};
+// the ring's LDS writes by some lanes visible to the reads by others
+__device__ __forceinline__ void ring_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
-// append the lanes with `pass` set; drain full batches of 64
-__device__ __forceinline__ u32 ring_push(const BatchedArgs& a, Ring& q, bool pass, u32 tag, u32 x, u32 y,
-                                         u32 lane) {
+// variant S: append strict-gate survivors to the probe ring p; probe full batches of 64
+__device__ __forceinline__ void push_probe(const BatchedArgs& a, Ring& p, bool pass, u32 tag, u32 x, u32 y,
+                                           u32 lane) {
+  const u64 m = __ballot(pass);
+  if (!m) return;
+  if (pass) {
+    const u32 slot = (p.tail + lanes_below(m)) & (kRing - 1);
+    p.qu[slot] = tag;
+    p.qd[slot] = x;
+    p.qe[slot] = y;
+  }
+  p.tail += (u32)__popcll(m);
+  if (p.tail - p.head >= 64u) {
+    ring_sync();
+    drain<true>(a, p, 64u, lane);
+    p.head += 64u;
+  }
+}
+
+// variant S: the strict gate on the survivor ring's oldest n entries (n <= 64);
+// T = row part - lane part with L = floor(d^6/64) (class 5) or c3^6 (classes 2-4)
+__device__ __forceinline__ void strict_stage(const BatchedArgs& a, const Ring& q, Ring& p, u32 n, u32 lane) {
+  bool pass = false;
+  u32 tag = 0, x = 0, y = 0;
+  if (lane < n) {
+    const u32 slot = (q.head + lane) & (kRing - 1);
+    tag = q.qu[slot];
+    x = q.qd[slot];
+    y = q.qe[slot];
+    const u32 row = q.qx[slot];
+    u32 la, l81, l64;
+    if (tag & kFind3Bit) strict_lane_p6(y, la, l81, l64);
+    else strict_lane_q64(x, la, l81, l64);
+    pass = strict_pass(row, la, l81, l64);
+  }
+  push_probe(a, p, pass, tag, x, y, lane);
+}
+
+// append the lanes with `pass` set to the survivor ring q; pass full batches
+// of 64 to the probes (S: through the strict gate into the probe ring p)
+template <bool S>
+__device__ __forceinline__ u32 ring_push(const BatchedArgs& a, Ring& q, Ring& p, bool pass, u32 tag, u32 x,
+                                         u32 y, u32 xw, u32 lane) {
   const u64 m = __ballot(pass);
   if (!m) return 0;
   if (pass) {
@@ -93,14 +155,14 @@ __device__ __forceinline__ u32 ring_push(const BatchedArgs& a, Ring& q, bool pas
     q.qu[slot] = tag;
     q.qd[slot] = x;
     q.qe[slot] = y;
+    if (S) q.qx[slot] = xw;
   }
   const u32 cnt = (u32)__popcll(m);
   q.tail += cnt;
   if (q.tail - q.head >= 64u) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    drain(a, q.qu, q.qd, q.qe, q.head, 64u, lane);
+    ring_sync();
+    if (S) strict_stage(a, q, p, 64u, lane);
+    else drain<false>(a, q, 64u, lane);
     q.head += 64u;
   }
   return cnt;
@@ -131,9 +193,9 @@ __device__ __forceinline__ void flush_counters(Counters* c, u32 lane, u32 calls,
 }
 
 // one class-5 item: E e-rows of one unit, every d
-template <int E>
+template <int E, bool S>
 __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_units, const u32* s_gate, Ring& q,
-                                          u32 item, u32 lane) {
+                                          Ring& p, u32 item, u32 lane) {
   const u32 K = a.ndg_K;
   u32 lo = u0, hi = u0 + n_units;
   while (hi - lo > 1) {
@@ -148,7 +210,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
   const u32 minr = min(min(r0, r1), min(r2, r3));
 
   // per-row set-up on lanes 0..E-1: e, T-space row value, positive cut
-  u32 e = 0, dlim = 0, xg = 0;
+  u32 e = 0, dlim = 0, xg = 0, xs = 0;
   u32 c_calls = 0, c_neg = 0, c_zero = 0, c_pos = 0;
   if (lane < (u32)E) {
     const u32 r = row0 + lane;
@@ -170,6 +232,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
           dlim = min(dc, U.fmax2);
           pos = cnt(r0, dlim) + cnt(r1, dlim) + cnt(r2, dlim) + cnt(r3, dlim);
           xg = mod_gate(X);
+          if (S) xs = strict_row((u64)(X >> 64), (u64)X);
           const u32 z = dc + 1;  // the only d that can give T == 0
           if (z <= U.fmax2 && dq_of(z) == X) {
             const u32 zr = z & 63u;
@@ -204,12 +267,13 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
       if (dl < dmin_blk) continue;
       const u32 xgi = rdl(xg, i);
       const u32 ei = rdl(e, i);
+      const u32 xsi = S ? rdl(xs, i) : 0u;
       const u32 ds[4] = {d0, d1, d2, d3};
       const u32 ns[4] = {n0, n1, n2, n3};
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const bool pass = gate_pass(s_gate, xgi, ns[j]) && ds[j] <= dl;
-        probes_item += ring_push(a, q, pass, ui, ds[j], ei, lane);
+        probes_item += ring_push<S>(a, q, p, pass, ui, ds[j], ei, xsi, lane);
       }
     }
   }
@@ -218,9 +282,9 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
 
 // one class 2-4 item: E d-rows of one unit, every c3 of each row's window
 // (fourcore_find_v4.cu:165-205 expansion, :401-425 window, :255-284 sweep)
-template <int E>
+template <int E, bool S>
 __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_units, const u32* s_gate, Ring& q,
-                                           u32 item, u32 lane) {
+                                           Ring& p, u32 item, u32 lane) {
   u32 lo = u0, hi = u0 + n_units;
   while (hi - lo > 1) {
     const u32 mid = (lo + hi) >> 1;
@@ -230,7 +294,7 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_u
   const u32 ui = lo;
   const DevUnit3* Up = a.units3 + ui;
   const u32 row0 = (item - Up->item_base) * (u32)E;
-  u32 d = 0, wlo = 1, whi = 0, xg = 0;
+  u32 d = 0, wlo = 1, whi = 0, xg = 0, xs = 0;
   u32 c_calls = 0, c_jobs = 0, c_fail = 0;
   if (lane < (u32)E) {
     const u32 r = row0 + lane;
@@ -253,6 +317,7 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_u
             c_calls = h - l + 1;
           }
           xg = mod_gate(X);
+          if (S) xs = strict_row((u64)(X >> 64), (u64)X);
         } else {
           c_fail = 1;  // not positive: compute_T fails (fourcore_gmp.hpp:47)
         }
@@ -264,28 +329,33 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_u
     const u32 l = rdl(wlo, i), h = rdl(whi, i);
     if (l > h) continue;
     const u32 xgi = rdl(xg, i), di = rdl(d, i);
+    const u32 xsi = S ? rdl(xs, i) : 0u;
     for (u32 cb = l; cb <= h; cb += 64u) {
       const u32 c = cb + lane;
       const bool in = c <= h;
       const u32 n = in ? neg_p6_gate(c) : 0u;  // c <= lim < 2^21
       const bool pass = in && gate_pass(s_gate, xgi, n);
-      probes_item += ring_push(a, q, pass, ui | kFind3Bit, di, c, lane);
+      probes_item += ring_push<S>(a, q, p, pass, ui | kFind3Bit, di, c, xsi, lane);
       if (h - cb < 64u) break;  // cb + 64 could wrap past 2^32
     }
   }
   flush_counters<E>(a.ctr + Up->band, lane, c_calls, 0u, 0u, c_calls, probes_item, c_jobs, c_fail);
 }
 
-template <int E>
+template <int E, bool S>
 __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
+  constexpr u32 kSW = S ? kWaves : 1, kSR = S ? kRing : 1;  // variant S only: row words, probe ring
   __shared__ u32 s_gate[kGateWords];
   __shared__ u32 s_qu[kWaves][kRing], s_qd[kWaves][kRing], s_qe[kWaves][kRing];
+  __shared__ u32 s_qx[kSW][kSR], s_pu[kSW][kSR], s_pd[kSW][kSR], s_pe[kSW][kSR];
   for (u32 i = threadIdx.x; i < kGateWords; i += blockDim.x) s_gate[i] = a.gate[i];
   __syncthreads();
 
   const u32 lane = threadIdx.x & 63u;
   const u32 w = threadIdx.x >> 6;
-  Ring q{s_qu[w], s_qd[w], s_qe[w], 0u, 0u};
+  const u32 ws = S ? w : 0u;
+  Ring q{s_qu[w], s_qd[w], s_qe[w], s_qx[ws], 0u, 0u};
+  Ring p{s_pu[ws], s_pd[ws], s_pe[ws], nullptr, 0u, 0u};
   for (;;) {
     u32 item = 0;
     if (lane == 0) item = atomicAdd(a.item_ctr, 1u);
@@ -297,16 +367,23 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
       if (a.segs[mid].g0 <= item) lo = mid;
       else hi = mid;
     }
-    const Segment S = a.segs[lo];
-    const u32 local = S.item0 + (item - S.g0);
-    if (S.kind == 0) cls5_item<E>(a, S.u0, S.n_units, s_gate, q, local, lane);
-    else find3_item<E>(a, S.u0, S.n_units, s_gate, q, local, lane);
+    const Segment G = a.segs[lo];
+    const u32 local = G.item0 + (item - G.g0);
+    if (G.kind == 0) cls5_item<E, S>(a, G.u0, G.n_units, s_gate, q, p, local, lane);
+    else find3_item<E, S>(a, G.u0, G.n_units, s_gate, q, p, local, lane);
   }
   if (q.tail != q.head) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    drain(a, q.qu, q.qd, q.qe, q.head, q.tail - q.head, lane);
+    ring_sync();
+    if (S) strict_stage(a, q, p, q.tail - q.head, lane);
+    else drain<false>(a, q, q.tail - q.head, lane);
+  }
+  if (S) {
+    while (p.tail != p.head) {
+      const u32 n = min(64u, p.tail - p.head);
+      ring_sync();
+      drain<true>(a, p, n, lane);
+      p.head += n;
+    }
   }
 }
 
@@ -346,6 +423,10 @@ __device__ __forceinline__ u64 mod192(u64 a2, u64 a1, u64 a0, u64 m, u64 two64) 
   return r;
 }
 
+// T mod m from R = 42^6 T: (R mod 42^6 m) / 42^6 (strict gate of the plain kernel)
+constexpr u64 kStrictA42 = (u64)kStrictA * kM42, kStrict81_42 = 81ULL * kM42;
+constexpr u64 kTwo64A42 = (u64)(((u128)1 << 64) % kStrictA42), kTwo64_81_42 = (u64)(((u128)1 << 64) % kStrict81_42);
+
 __global__ __launch_bounds__(256) void k_cls5_plain(PlainArgs a) {
   const int ui = blockIdx.x;
   if (ui >= a.n_units) return;
@@ -359,7 +440,7 @@ __global__ __launch_bounds__(256) void k_cls5_plain(PlainArgs a) {
   if (threadIdx.x == 0) pow6_192(14u * e, s_e[2], s_e[1], s_e[0]);
   __syncthreads();
   const u32 n2 = U.nres2, total = n2 * (U.fmax2 / U.mod2 + 1);
-  u64 calls = 0, neg = 0, zero = 0, gated = 0, probes = 0;
+  u64 calls = 0, neg = 0, zero = 0, gated = 0, probes = 0, queries = 0;
   for (u32 t2 = threadIdx.x; t2 < total; t2 += blockDim.x) {
     const u32 d = U.res[U.nres1 + t2 % n2] + (t2 / n2) * U.mod2;
     if (d == 0 || d > U.fmax2) continue;
@@ -381,6 +462,12 @@ __global__ __launch_bounds__(256) void k_cls5_plain(PlainArgs a) {
     }
     ++probes;
     const u64 fp = ((x0 >> 6) | (x1 << 58)) * a.inv216;
+    if (a.strict) {
+      const u32 ta = (u32)(mod192(x2, x1, x0, kStrictA42, kTwo64A42) / kM42);
+      const u32 t81 = (u32)(mod192(x2, x1, x0, kStrict81_42, kTwo64_81_42) / kM42);
+      if (!strict_pass_t(ta, t81, (u32)fp & 63u)) continue;
+      ++queries;
+    }
     u32 flags = d_query(a.main, fp) ? 1u : 0u;
     if (a.side.present && d_query(a.side, fp)) flags |= 2u;
     if (flags) {
@@ -403,12 +490,14 @@ __global__ __launch_bounds__(256) void k_cls5_plain(PlainArgs a) {
   zero = wave_sum(zero);
   gated = wave_sum(gated);
   probes = wave_sum(probes);
+  queries = wave_sum(queries);
   if ((threadIdx.x & 63u) == 0) {
     atomicAdd((unsigned long long*)&a.ctr->calls, (unsigned long long)calls);
     atomicAdd((unsigned long long*)&a.ctr->neg, (unsigned long long)neg);
     atomicAdd((unsigned long long*)&a.ctr->zero, (unsigned long long)zero);
     atomicAdd((unsigned long long*)&a.ctr->gated, (unsigned long long)gated);
     atomicAdd((unsigned long long*)&a.ctr->probes, (unsigned long long)probes);
+    if (queries) atomicAdd((unsigned long long*)&a.ctr->queries, (unsigned long long)queries);
   }
 }
 
@@ -427,7 +516,7 @@ __global__ __launch_bounds__(256) void k_find3_plain(PlainF3Args a) {
   if (threadIdx.x < 8) g504[threadIdx.x] = a.gate.g504[threadIdx.x];
   if (threadIdx.x < 4) g247[threadIdx.x] = a.gate.g247[threadIdx.x];
   __syncthreads();
-  u64 calls = 0, gated = 0, probes = 0;
+  u64 calls = 0, gated = 0, probes = 0, queries = 0;
   for (u32 c3 = C.lo + threadIdx.x; c3 <= C.hi && c3 >= C.lo; c3 += blockDim.x) {
     ++calls;
     if (a.use_gate) {
@@ -441,6 +530,12 @@ __global__ __launch_bounds__(256) void k_find3_plain(PlainF3Args a) {
       }
     }
     ++probes;
+    if (a.strict) {
+      u32 la, l81, l64;
+      strict_lane_p6(c3, la, l81, l64);
+      if (!strict_pass(C.srow, la, l81, l64)) continue;
+      ++queries;
+    }
     const u64 c2 = (u64)c3 * c3;
     const u64 fp = C.q_lo - c2 * c2 * c2;
     u32 flags = d_query(a.main, fp) ? 1u : 0u;
@@ -465,11 +560,13 @@ __global__ __launch_bounds__(256) void k_find3_plain(PlainF3Args a) {
   calls = wave_sum(calls);
   gated = wave_sum(gated);
   probes = wave_sum(probes);
+  queries = wave_sum(queries);
   if ((threadIdx.x & 63u) == 0) {
     Counters* c = a.ctr + C.band;
     atomicAdd((unsigned long long*)&c->calls, (unsigned long long)calls);
     atomicAdd((unsigned long long*)&c->gated, (unsigned long long)gated);
     atomicAdd((unsigned long long*)&c->probes, (unsigned long long)probes);
+    if (queries) atomicAdd((unsigned long long*)&c->queries, (unsigned long long)queries);
   }
 }
 
@@ -503,15 +600,20 @@ static int grid_for(u64 n, int threads) {
   return (int)(g < 65536 ? (g ? g : 1) : 65536);
 }
 
-int launch_cls5_batched(const BatchedArgs& a, int grid, int rows, hipStream_t s) {
+template <bool S>
+static int launch_batched_variant(const BatchedArgs& a, int grid, int rows, hipStream_t s) {
   switch (rows) {
-    case 4: hipLaunchKernelGGL(k_cls5_batched<4>, dim3(grid), dim3(64 * kWaves), 0, s, a); break;
-    case 8: hipLaunchKernelGGL(k_cls5_batched<8>, dim3(grid), dim3(64 * kWaves), 0, s, a); break;
-    case 16: hipLaunchKernelGGL(k_cls5_batched<16>, dim3(grid), dim3(64 * kWaves), 0, s, a); break;
-    case 32: hipLaunchKernelGGL(k_cls5_batched<32>, dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_cls5_batched<4, S>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_cls5_batched<8, S>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_cls5_batched<16, S>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_cls5_batched<32, S>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
     default: return (int)hipErrorInvalidValue;
   }
   return (int)hipGetLastError();
+}
+
+int launch_cls5_batched(const BatchedArgs& a, int grid, int rows, hipStream_t s) {
+  return a.strict ? launch_batched_variant<true>(a, grid, rows, s) : launch_batched_variant<false>(a, grid, rows, s);
 }
 
 int launch_cls5_plain(const PlainArgs& a, int n_units, int ymax, hipStream_t s) {
@@ -539,16 +641,20 @@ int launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
-int batched_blocks_per_cu(int rows) {
-  int nb = 0;
-  hipError_t e = hipErrorInvalidValue;
+template <bool S>
+static hipError_t occupancy_variant(int& nb, int rows) {
   switch (rows) {
-    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<4>, 64 * kWaves, 0); break;
-    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<8>, 64 * kWaves, 0); break;
-    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<16>, 64 * kWaves, 0); break;
-    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<32>, 64 * kWaves, 0); break;
-    default: break;
+    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<4, S>, 64 * kWaves, 0);
+    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<8, S>, 64 * kWaves, 0);
+    case 16: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<16, S>, 64 * kWaves, 0);
+    case 32: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<32, S>, 64 * kWaves, 0);
+    default: return hipErrorInvalidValue;
   }
+}
+
+int batched_blocks_per_cu(int rows, bool strict) {
+  int nb = 0;
+  const hipError_t e = strict ? occupancy_variant<true>(nb, rows) : occupancy_variant<false>(nb, rows);
   if (e != hipSuccess) return 4;
   return nb > 0 ? nb : 1;
 }

@@ -83,7 +83,8 @@ struct PlainUnit {
 
 struct Counters {
   u64 calls, neg, zero, gated, probes, maybe, side_maybe, pos, jobs, failT;
-  u64 recs;  // hit records produced (written or dropped at hit_cap), batched kernel
+  u64 recs;     // hit records produced (written or dropped at hit_cap), batched kernel
+  u64 queries;  // strict gate on: xor queries made (gate survivors that pass the strict gate)
 };
 
 struct HitRec {
@@ -110,6 +111,7 @@ struct BatchedArgs {
   const u32* gate;  // 124488-bit bitmap
   const u32* ndg;   // [64][ndg_K]
   u32 ndg_K, use_gate;
+  u32 strict, pad_;  // strict: the strict residue gate before every xor query
   XorDev main, side;
   HitRec* hits;
   u64 hit_cap;
@@ -126,13 +128,14 @@ struct PlainCand3 {
   u32 lo, hi;
   u32 r504, r247;
   u32 uid, d;
-  u32 band, pad;
+  u32 band, srow;  // srow: strict-gate row word of the candidate's T (strict_row)
 };
 
 struct PlainF3Args {
   const PlainCand3* cands;
   int n;
   int use_gate;
+  int strict, pad_;
   GateWords gate;
   const u16* p6;  // [504] x^6 mod 504, then [247] x^6 mod 247 (GateData.p6504/p6247)
   XorDev main, side;
@@ -146,6 +149,7 @@ struct PlainArgs {
   const PlainUnit* units;
   int n_units;
   int use_gate;
+  int strict, pad_;
   GateWords gate;
   XorDev main, side;
   u64 inv216, c_m504, c_m247, inv42_247;
@@ -182,13 +186,14 @@ struct Stage1Args {
 constexpr int kRowsPerItem = 16;  // rows (e- or d-rows) per work item of a serving round, unless fixed
 
 // launchers (return hipError_t as int)
-int launch_cls5_batched(const BatchedArgs& a, int grid, int rows, hipStream_t s);  // rows: 4, 8, 16, 32
+// rows: 4, 8, 16, 32; a.strict picks the variant with the strict gate stage
+int launch_cls5_batched(const BatchedArgs& a, int grid, int rows, hipStream_t s);
 int launch_cls5_plain(const PlainArgs& a, int n_units, int ymax, hipStream_t s);
 int launch_find3_plain(const PlainF3Args& a, hipStream_t s);
 int launch_expand_cells(const u32* packed_words, u64 m_cells, u32 r, u64* cells, hipStream_t s);
 int launch_query(const XorDev& x, const u64* keys, u64 n, u8* out, hipStream_t s);
 int launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s);
-int batched_blocks_per_cu(int rows);
+int batched_blocks_per_cu(int rows, bool strict);  // occupancy limit of the kernel variant
 int launch_probe_only(const XorDev& x, u64 n, u64 salt, unsigned long long* sink, int grid, hipStream_t s);
 int launch_gather_raw(const u64* cells, u64 m, u64 n, u64 salt, u32 policy, unsigned long long* sink, int grid,
                       hipStream_t s);

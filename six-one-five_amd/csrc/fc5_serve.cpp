@@ -196,7 +196,7 @@ void free_server_buffers(fc5_server* S) {
 }
 
 // the band's result from its counter slots (same reduction as fc5_band_run)
-void collect(SBand& B) {
+void collect(SBand& B, bool strict) {
   const Counters* c = B.ctr;
   std::memset(&B.res, 0, sizeof B.res);
   std::memset(B.cls_res, 0, sizeof B.cls_res);
@@ -212,6 +212,7 @@ void collect(SBand& B) {
     r.side_maybe = C.side_maybe;
     r.jobs = C.jobs;
     r.fail_t = C.failT;
+    r.strict_out = strict ? C.probes - C.queries : 0;
     B.res.calls += r.calls;
     B.res.neg += r.neg;
     B.res.zero += r.zero;
@@ -221,6 +222,7 @@ void collect(SBand& B) {
     B.res.side_maybe += r.side_maybe;
     B.res.jobs += r.jobs;
     B.res.fail_t += r.fail_t;
+    B.res.strict_out += r.strict_out;
   }
   for (size_t i = 0; i < B.n_in; ++i)
     if (B.cls_of[i]) B.cls_res[B.cls_of[i]].units++;
@@ -231,7 +233,7 @@ void collect(SBand& B) {
 // under the lock afterwards, and nothing else touches an open band's results.
 void finalize(fc5_server* S, SBand& B) {
   fc5_ctx* ctx = S->ctx;
-  collect(B);
+  collect(B, ctx->strict_gate != 0);
   std::sort(B.hits.begin(), B.hits.end(), [](const fc5_hit& x, const fc5_hit& y) {
     if (x.unit != y.unit) return x.unit < y.unit;
     if (x.e != y.e) return x.e < y.e;
@@ -388,6 +390,7 @@ int plan_round(fc5_server* S, const std::vector<SBand*>& bands, Round& R) {
   a.ndg = ctx->d_ndg;
   a.ndg_K = ctx->ndg_K;
   a.use_gate = (u32)ctx->use_gate;
+  a.strict = (u32)ctx->strict_gate;
   a.main = ctx->main.dev;
   a.side = ctx->side.present ? ctx->side.dev : XorDev{};
   a.hits = R.d_hits;
@@ -396,7 +399,7 @@ int plan_round(fc5_server* S, const std::vector<SBand*>& bands, Round& R) {
   a.ctr = S->d_ctr;
   a.item_ctr = ctx->d_item_ctr;  // rounds run in stream order, so one work counter serves both
   a.inv729 = inv_mod_2_64(729);
-  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(S->rows);
+  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(S->rows, ctx->strict_gate != 0);
   const int grid = (int)std::max<u64>(1, std::min<u64>((u64)ctx->num_cu * (u64)bpc, total));
   const size_t nslots = S->slots.size() * kSlotStride;
   HIPCHK(ctx, hipMemcpyAsync(R.d_segs, R.h_segs, n_segs * sizeof(Segment), hipMemcpyHostToDevice, ctx->stream));

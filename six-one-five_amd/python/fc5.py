@@ -44,7 +44,7 @@ class Result(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "calls", "neg", "zero", "gated", "probes", "maybe", "side_maybe", "n_hits",
         "hits_dropped", "solutions", "units", "units_skipped", "units_irregular", "jobs", "fail_t")] + \
-               [("kernel_ms", C.c_double), ("wall_ms", C.c_double)]
+               [("kernel_ms", C.c_double), ("wall_ms", C.c_double), ("strict_out", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -671,15 +671,24 @@ def _pair_sum_set(m):
 
 
 _S504, _S247 = _pair_sum_set(504), _pair_sum_set(247)
+# the strict gate's sets (option "strict_gate")
+_STRICT = {m: _pair_sum_set(m) for m in (64, 81, 31, 37, 43)}
 
 
-def plant_points(Bs, us, n, seed, exact=False):
+def strict_gate_ok(T):
+    """T passes the strict gate: T mod 64, 81, 31, 37, 43 each a sum of two sixth powers"""
+    return all(T % m in s for m, s in _STRICT.items())
+
+
+def plant_points(Bs, us, n, seed, exact=False, strict=False):
     """n distinct planted grid points (unit index, e, d, key) drawn with splitmix64:
     the unit uniformly over the band, then (e, d) uniformly over the unit's
     valid values, redrawn while the residual R is not positive or T = R/42^6
     fails the sixth-power residue gate (a real solution always passes it: T is
     a pair sum), at most 2000 draws per unit; key = T mod 2^64.  exact=True
-    appends T itself (unit index, e, d, key, T) for the exact table."""
+    appends T itself (unit index, e, d, key, T) for the exact table.  strict=True
+    also redraws while T fails the strict gate (a pair sum passes that too), so
+    the plants are found with the option strict_gate on as well."""
     s = seed & _M64
     out, seen = [], set()
     while len(out) < n:
@@ -697,7 +706,7 @@ def plant_points(Bs, us, n, seed, exact=False):
             if R <= 0:
                 continue
             T = R // 42 ** 6
-            if T % 504 in _S504 and T % 247 in _S247:
+            if T % 504 in _S504 and T % 247 in _S247 and (not strict or strict_gate_ok(T)):
                 break
         else:
             continue

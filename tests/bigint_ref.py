@@ -14,6 +14,12 @@ def pair_sum_set(m):
 
 S504 = pair_sum_set(504)
 S247 = pair_sum_set(247)
+# the strict gate (an option of this build): the same condition modulo 64, 81, 31, 37, 43
+STRICT = {m: pair_sum_set(m) for m in (64, 81, 31, 37, 43)}
+
+
+def strict_ok(T):
+    return all(T % m in s for m, s in STRICT.items())
 
 
 def residues(t, m):
@@ -32,9 +38,10 @@ def cls5_grid(B, u, fmax1=None, fmax2=None):
     return es, ds
 
 
-def peel_counts(B, u, es, ds, contains=None, use_gate=True):
-    """counters (calls, neg, zero, gated, probes, maybe) and maybe-hit list"""
-    c = dict(calls=0, neg=0, zero=0, gated=0, probes=0, maybe=0)
+def peel_counts(B, u, es, ds, contains=None, use_gate=True, strict=False):
+    """counters (calls, neg, zero, gated, probes, maybe, queries) and maybe-hit
+    list; strict: the strict gate before every query"""
+    c = dict(calls=0, neg=0, zero=0, gated=0, probes=0, maybe=0, queries=0)
     hits = []
     base = B ** 6 - u ** 6
     d6 = [(21 * d) ** 6 for d in ds]
@@ -55,6 +62,10 @@ def peel_counts(B, u, es, ds, contains=None, use_gate=True):
                 c["gated"] += 1
                 continue
             c["probes"] += 1
+            if strict:
+                if not strict_ok(T):
+                    continue
+                c["queries"] += 1
             fp = T % 2 ** 64
             if contains is not None and contains(fp):
                 c["maybe"] += 1
@@ -115,11 +126,11 @@ def iroot6(n):
     return a
 
 
-def find3_counts(cls, B, u, contains=None, use_gate=True):
+def find3_counts(cls, B, u, contains=None, use_gate=True, strict=False):
     f, m = F_OF[cls], MOD_OF[cls]
     fmax = (B - 1) // f
     lim = (B - 1) // 42
-    c = dict(jobs=0, failT=0, calls=0, gated=0, probes=0, maybe=0)
+    c = dict(jobs=0, failT=0, calls=0, gated=0, probes=0, maybe=0, queries=0)
     hits = []
     base = B ** 6 - u ** 6
     for r in cls234_residues(cls, B, u):
@@ -141,6 +152,10 @@ def find3_counts(cls, B, u, contains=None, use_gate=True):
                     c["gated"] += 1
                     continue
                 c["probes"] += 1
+                if strict:
+                    if not strict_ok(V):
+                        continue
+                    c["queries"] += 1
                 fp = V % 2 ** 64
                 if contains is not None and contains(fp):
                     c["maybe"] += 1

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(lib, name), name
     assert declared <= set(fc5.EXPORTS)
-    assert lib.fc5_abi_version() == 2
+    assert lib.fc5_abi_version() == 3
 
 
 def test_emit_units_matches_reference_golden():
