@@ -119,8 +119,10 @@ const char *fc5_last_error(const fc5_ctx *ctx);
  * solution is lost; counters and hits are the reference's except that the xor
  * filter's false positives among the T it rejects are not reported, and
  * strict_out counts the rejected gate survivors; classes 2-5, every mode, not the
- * (6,1,7) twin), "blocks_per_cu" (0 = occupancy limit) for the
- * persistent grid, "rows_per_item" (e- or d-rows per work item: 4, 8, 16, 32, or
+ * (6,1,7) twin), "sweep" (1: the class-5 rows of the batched kernel sieve 32 d
+ * values per LDS word, the reference gate split into its conditions mod 8, 9, 7,
+ * 13 and 19; 0: one bitmap test of T mod 124488 per grid point; same counters and
+ * hits), "blocks_per_cu" (0 = occupancy limit) for the persistent grid, "rows_per_item" (e- or d-rows per work item: 4, 8, 16, 32, or
  * 0, the default: by band size, the largest of 32, 16, 8 that leaves every wave of
  * the persistent grid at least 8 items, else 4), "use_graph" (0; 1 replays the batched pipeline as a captured hipGraph),
  * "load_policy" (-1..3, default -1: cache policy of the xor cell gathers:

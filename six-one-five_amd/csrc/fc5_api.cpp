@@ -120,6 +120,9 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) try {
     ctx->use_gate = v ? 1 : 0;
   } else if (k == "strict_gate") {
     ctx->strict_gate = v ? 1 : 0;
+  } else if (k == "sweep") {
+    if (v != 0 && v != 1) FAIL(ctx, FC5_E_ARG, "sweep must be 0 (per-point bitmap test) or 1 (bit sieve)");
+    ctx->sweep = (int)v;
   } else if (k == "verify") {
     ctx->verify = v ? 1 : 0;
   } else if (k == "verify_table") {
@@ -821,9 +824,9 @@ bool fc5::number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, 
   return true;
 }
 
-int fc5::auto_rows_per_item(int num_cu, u64 rows, bool strict) {
+int fc5::auto_rows_per_item(int num_cu, u64 rows, bool strict, bool sieve) {
   for (int E : {32, 16, 8}) {
-    const u64 waves = (u64)num_cu * (u64)std::max(1, batched_blocks_per_cu(E, strict)) * 4u;
+    const u64 waves = (u64)num_cu * (u64)std::max(1, batched_blocks_per_cu(E, strict, sieve)) * 4u;
     if (rows / (u64)E >= 8 * waves) return E;
   }
   return 4;
@@ -836,7 +839,7 @@ static int band_rows_per_item(const fc5_ctx* ctx, const std::vector<DevUnit5>& d
   u64 rows = 0;
   for (const DevUnit5& D : dev) rows += D.rows;
   for (const DevUnit3& D : dev3) rows += D.rows;
-  return auto_rows_per_item(ctx->num_cu, rows, ctx->strict_gate != 0);
+  return auto_rows_per_item(ctx->num_cu, rows, ctx->strict_gate != 0, ctx->sweep != 0);
 }
 
 static int renumber(fc5_ctx* ctx, fc5_band* b) {
@@ -1106,6 +1109,7 @@ static BatchedArgs batched_args(fc5_ctx* ctx, fc5_band* b) {
   a.ndg_K = ctx->ndg_K;
   a.use_gate = (u32)ctx->use_gate;
   a.strict = (u32)ctx->strict_gate;
+  a.sieve = (u32)ctx->sweep;
   a.main = ctx->main.dev;
   a.side = ctx->side.present ? ctx->side.dev : XorDev{};
   a.hits = ctx->d_hits;
@@ -1226,7 +1230,7 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   const bool batched = mode != FC5_MODE_PLAIN;  // the T-space kernel, in one launch or in chunks
   const bool any_items = b->n_items + b->n_items3 > 0;
   BatchedArgs a = batched_args(ctx, b);
-  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(b->rows, ctx->strict_gate != 0);
+  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(b->rows, ctx->strict_gate != 0, ctx->sweep != 0);
   const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)(b->n_items + b->n_items3)));
   const bool graph = mode == FC5_MODE_BATCHED && any_items && ctx->use_graph && b->irregular.empty() &&
                      b->irregular3.empty();
@@ -1393,7 +1397,7 @@ extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tun
   out->best_ms = 1e30;
   fc5_result r;
   for (int E : {8, 4, 16, 32}) {
-    const int occ = batched_blocks_per_cu(E, ctx->strict_gate != 0);
+    const int occ = batched_blocks_per_cu(E, ctx->strict_gate != 0, ctx->sweep != 0);
     for (int bpc : {occ, std::max(1, occ / 2)}) {
       ctx->rows_per_item = E;
       ctx->blocks_per_cu = bpc;
@@ -1424,7 +1428,7 @@ extern "C" int fc5_autotune(fc5_ctx* ctx, fc5_band* b, int sample_every, fc5_tun
   // band (the sample's launches are short, so their tails weigh more): keep the untuned shape
   if (out->default_ms > 0 && out->best_ms > 0.97 * out->default_ms) {
     out->rows_per_item = auto_E;
-    out->blocks_per_cu = batched_blocks_per_cu(auto_E, ctx->strict_gate != 0);
+    out->blocks_per_cu = batched_blocks_per_cu(auto_E, ctx->strict_gate != 0, ctx->sweep != 0);
     out->best_ms = out->default_ms;
     ctx->rows_per_item = keep_rows;
     ctx->blocks_per_cu = keep_bpc;

@@ -115,6 +115,7 @@ struct fc5_ctx {
   u64 hit_cap = 1u << 20;
   int use_gate = 1, verify = 1, stop_first = 0, blocks_per_cu = 0;
   int strict_gate = 0;  // 1: the strict residue gate before every xor query (fc5_strict.hpp)
+  int sweep = 1;        // class-5 sweep of the batched kernel: 1 bit sieve, 0 per-point bitmap test
   int rows_per_item = 0, use_graph = 0;  // rows_per_item 0: by band size (auto_rows_per_item)
   Counters* h_ctr = nullptr;  // pinned read-back of the counters
   u64* h_nhits = nullptr;
@@ -222,7 +223,7 @@ constexpr const char* kTooManyItems =
 bool number_items(std::vector<DevUnit5>& dev, std::vector<DevUnit3>& dev3, int E, u32& n5, u32& n3);
 // rows per work item for `rows` e- or d-rows in all: the largest of 32, 16, 8
 // that still gives every wave of the persistent grid 8 items or more, else 4
-int auto_rows_per_item(int num_cu, u64 rows, bool strict);
+int auto_rows_per_item(int num_cu, u64 rows, bool strict, bool sieve);
 // serving loop (fc5_serve.cpp): stops and frees the server of a context
 void serve_shutdown(fc5_ctx* ctx);
 // Stage-1 units of classes in mask (bits 2..5) for B in [lo, hi] on the device,

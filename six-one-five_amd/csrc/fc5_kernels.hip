@@ -18,6 +18,7 @@
 // k_expand_cells  widens the r-bit packed table to one u64 per cell.
 // k_query         membership queries for a key list (tests, FPR smoke).
 #include "fc5_device.hpp"
+#include "fc5_sieve.hpp"
 #include "fc5_strict.hpp"
 
 namespace fc5 {
@@ -25,6 +26,8 @@ namespace fc5 {
 constexpr u64 kM42 = 5489031744ULL;
 constexpr int kWaves = 4;   // wavefronts per workgroup
 constexpr u32 kRing = 128;  // per-wave survivor ring (entries)
+
+__constant__ SieveTab kSieveDev = make_sieve();  // staged into LDS by every workgroup
 
 // floor(x^6 / 64) for x < 2^21
 __device__ __forceinline__ u128 dq_of(u32 x) {
@@ -37,14 +40,44 @@ __device__ __forceinline__ u128 dq_of(u32 x) {
 // ---------------------------------------------------------------------------
 constexpr u32 kFind3Bit = 0x80000000u;  // ring entry unit field: find3 row
 
+// A wavefront's counter sums not yet added to its band's counters: one band at
+// a time, flushed when the band changes and at the end of the launch, so the
+// counters take a few atomics per wavefront instead of several per work item.
+struct WaveAcc {
+  u64 v[8];  // calls, neg, zero, pos, probes, jobs, failT, queries
+  u32 band, live;
+};
+
+// lane 0: add the sums to the band's counters and clear them
+__device__ __forceinline__ void acc_flush(const BatchedArgs& a, WaveAcc* A) {
+  if (!A->live) return;
+  Counters* c = a.ctr + A->band;
+  u64* dst[8] = {&c->calls, &c->neg, &c->zero, &c->pos, &c->probes, &c->jobs, &c->failT, &c->queries};
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    if (A->v[i]) atomicAdd((unsigned long long*)dst[i], (unsigned long long)A->v[i]);
+    A->v[i] = 0;
+  }
+  A->live = 0;
+}
+
+// lane 0: make `band` the accumulated one
+__device__ __forceinline__ void acc_band(const BatchedArgs& a, WaveAcc* A, u32 band) {
+  if (A->live && A->band != band) acc_flush(a, A);
+  A->band = band;
+  A->live = 1;
+}
+
 struct Ring {
   u32 *qu, *qd, *qe;  // unit (| kFind3Bit), d, e (or c3)
   u32* qx;            // strict-gate row word (variant S, survivor ring only)
   u32 head, tail;
+  WaveAcc* acc;       // the wavefront's counter sums
 };
 
 // probe the ring's oldest n entries (n <= 64, one per lane).  S: also count
-// the queries per band (one atomic per batch unless it spans bands).
+// the queries per band (into the wavefront's sums; entries of a second band
+// in one batch by an atomic each).
 template <bool S>
 __device__ __forceinline__ void drain(const BatchedArgs& a, const Ring& q, u32 n, u32 lane) {
   u32 band = 0;
@@ -93,7 +126,10 @@ __device__ __forceinline__ void drain(const BatchedArgs& a, const Ring& q, u32 n
   if (S) {
     const u32 b0 = (u32)__builtin_amdgcn_readfirstlane((int)band);  // lane 0 holds an entry
     const u64 same = __ballot(lane < n && band == b0);
-    if (lane == 0) atomicAdd((unsigned long long*)&a.ctr[b0].queries, (unsigned long long)__popcll(same));
+    if (lane == 0) {
+      acc_band(a, q.acc, b0);
+      q.acc->v[7] += (u64)__popcll(same);
+    }
     if (lane < n && band != b0) atomicAdd((unsigned long long*)&a.ctr[band].queries, 1ULL);
   }
 }
@@ -168,9 +204,10 @@ __device__ __forceinline__ u32 ring_push(const BatchedArgs& a, Ring& q, Ring& p,
   return cnt;
 }
 
+// an item's counters (per-row values on lanes 0..E-1; probes uniform) into the wavefront's sums
 template <int E>
-__device__ __forceinline__ void flush_counters(Counters* c, u32 lane, u32 calls, u32 neg, u32 zero, u32 pos,
-                                               u32 probes, u32 jobs, u32 failT) {
+__device__ __forceinline__ void flush_counters(const BatchedArgs& a, WaveAcc* A, u32 band, u32 lane, u32 calls,
+                                               u32 neg, u32 zero, u32 pos, u32 probes, u32 jobs, u32 failT) {
   u64 s_calls = 0, s_neg = 0, s_zero = 0, s_pos = 0, s_jobs = 0, s_fail = 0;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
@@ -182,20 +219,21 @@ __device__ __forceinline__ void flush_counters(Counters* c, u32 lane, u32 calls,
     s_fail += rdl(failT, i);
   }
   if (lane == 0) {
-    atomicAdd((unsigned long long*)&c->calls, (unsigned long long)s_calls);
-    if (s_neg) atomicAdd((unsigned long long*)&c->neg, (unsigned long long)s_neg);
-    if (s_zero) atomicAdd((unsigned long long*)&c->zero, (unsigned long long)s_zero);
-    atomicAdd((unsigned long long*)&c->pos, (unsigned long long)s_pos);
-    atomicAdd((unsigned long long*)&c->probes, (unsigned long long)probes);
-    if (s_jobs) atomicAdd((unsigned long long*)&c->jobs, (unsigned long long)s_jobs);
-    if (s_fail) atomicAdd((unsigned long long*)&c->failT, (unsigned long long)s_fail);
+    acc_band(a, A, band);
+    A->v[0] += s_calls;
+    A->v[1] += s_neg;
+    A->v[2] += s_zero;
+    A->v[3] += s_pos;
+    A->v[4] += probes;
+    A->v[5] += s_jobs;
+    A->v[6] += s_fail;
   }
 }
 
 // one class-5 item: E e-rows of one unit, every d
-template <int E, bool S>
-__device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_units, const u32* s_gate, Ring& q,
-                                          Ring& p, u32 item, u32 lane) {
+template <int E, bool S, bool V>
+__device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_units, const u32* s_gate,
+                                          const u32* s_sieve, Ring& q, Ring& p, u32 item, u32 lane) {
   const u32 K = a.ndg_K;
   u32 lo = u0, hi = u0 + n_units;
   while (hi - lo > 1) {
@@ -210,7 +248,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
   const u32 minr = min(min(r0, r1), min(r2, r3));
 
   // per-row set-up on lanes 0..E-1: e, T-space row value, positive cut
-  u32 e = 0, dlim = 0, xg = 0, xs = 0;
+  u32 e = 0, dlim = 0, xg = 0, xs = 0, sa = 0, sb = 0, sc = 0;
   u32 c_calls = 0, c_neg = 0, c_zero = 0, c_pos = 0;
   if (lane < (u32)E) {
     const u32 r = row0 + lane;
@@ -232,6 +270,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
           dlim = min(dc, U.fmax2);
           pos = cnt(r0, dlim) + cnt(r1, dlim) + cnt(r2, dlim) + cnt(r3, dlim);
           xg = mod_gate(X);
+          if (V) sieve_row_bases(xg, U.td, sa, sb, sc);
           if (S) xs = strict_row((u64)(X >> 64), (u64)X);
           const u32 z = dc + 1;  // the only d that can give T == 0
           if (z <= U.fmax2 && dq_of(z) == X) {
@@ -252,8 +291,42 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
 #pragma unroll
   for (int i = 0; i < E; ++i) maxdl = max(maxdl, rdl(dlim, i));
 
-  // gate over the positive part of the rows; lanes run over k
   u32 probes_item = 0;
+  if (V) {
+    // bit sieve: lane slot = (word w, residue index) over the item's d range,
+    // one word = 32 consecutive k of one residue; per row five LDS words AND-ed,
+    // the cut applied, survivors taken lowest bit first
+    const u32 nslots = maxdl >= minr ? 4u * ((maxdl - minr) / 2048u + 1u) : 0u;
+    for (u32 s0 = 0; s0 < nslots; s0 += 64u) {
+      const u32 slot = s0 + lane, ri = slot & 3u, w = slot >> 2;
+      const u32 r = ri == 0u ? r0 : ri == 1u ? r1 : ri == 2u ? r2 : r3;
+      const u32 d0 = slot < nslots ? r + 2048u * w : 0xFFFFFFFFu;  // past the slots: no grid point
+      const u32 j9 = d0 % 3u, j7 = d0 % 7u, j13 = d0 % 13u, j19 = d0 % 19u;
+      const u32 dminp = minr + 2048u * (s0 >> 2);  // the pass's least d0
+      for (int i = 0; i < E; ++i) {
+        const u32 dl = rdl(dlim, i);
+        if (dl < dminp) continue;
+        u32 P = sieve_cut(d0, dl);
+        if (a.use_gate) {
+          const u32 ra = rdl(sa, i), rb = rdl(sb, i), rc = rdl(sc, i);
+          P &= s_sieve[(ra & 0xFFFFu) + r] & s_sieve[(ra >> 16) + j9] & s_sieve[(rb & 0xFFFFu) + j7] &
+               s_sieve[(rb >> 16) + j13] & s_sieve[rc + j19];
+        }
+        probes_item += (u32)__popc(P);
+        const u32 ei = rdl(e, i);
+        const u32 xsi = S ? rdl(xs, i) : 0u;
+        while (__ballot(P != 0u)) {
+          const bool has = P != 0u;
+          const u32 d = d0 + 64u * (has ? (u32)__builtin_ctz(P) : 0u);
+          ring_push<S>(a, q, p, has, ui, d, ei, xsi, lane);
+          P &= P - 1u;
+        }
+      }
+    }
+    flush_counters<E>(a, q.acc, U.band, lane, c_calls, c_neg, c_zero, c_pos, wave_sum_u32(probes_item), 0u, 0u);
+    return;
+  }
+  // gate over the positive part of the rows; lanes run over k
   for (u32 kb = 0; minr + 64u * kb <= maxdl; kb += 64u) {
     const u32 k = kb + lane;
     const u32 d0 = r0 + 64u * k, d1 = r1 + 64u * k, d2 = r2 + 64u * k, d3 = r3 + 64u * k;
@@ -277,7 +350,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
       }
     }
   }
-  flush_counters<E>(a.ctr + U.band, lane, c_calls, c_neg, c_zero, c_pos, probes_item, 0u, 0u);
+  flush_counters<E>(a, q.acc, U.band, lane, c_calls, c_neg, c_zero, c_pos, probes_item, 0u, 0u);
 }
 
 // one class 2-4 item: E d-rows of one unit, every c3 of each row's window
@@ -339,38 +412,51 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_u
       if (h - cb < 64u) break;  // cb + 64 could wrap past 2^32
     }
   }
-  flush_counters<E>(a.ctr + Up->band, lane, c_calls, 0u, 0u, c_calls, probes_item, c_jobs, c_fail);
+  flush_counters<E>(a, q.acc, Up->band, lane, c_calls, 0u, 0u, c_calls, probes_item, c_jobs, c_fail);
 }
 
-template <int E, bool S>
+template <int E, bool S, bool V>
 __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
   constexpr u32 kSW = S ? kWaves : 1, kSR = S ? kRing : 1;  // variant S only: row words, probe ring
   __shared__ u32 s_gate[kGateWords];
+  __shared__ u32 s_sieve[V ? kSieveWords : 1];
   __shared__ u32 s_qu[kWaves][kRing], s_qd[kWaves][kRing], s_qe[kWaves][kRing];
   __shared__ u32 s_qx[kSW][kSR], s_pu[kSW][kSR], s_pd[kSW][kSR], s_pe[kSW][kSR];
+  __shared__ WaveAcc s_acc[kWaves];
   for (u32 i = threadIdx.x; i < kGateWords; i += blockDim.x) s_gate[i] = a.gate[i];
+  if (V)
+    for (u32 i = threadIdx.x; i < kSieveWords; i += blockDim.x) s_sieve[i] = kSieveDev.w[i];
   __syncthreads();
 
   const u32 lane = threadIdx.x & 63u;
   const u32 w = threadIdx.x >> 6;
   const u32 ws = S ? w : 0u;
-  Ring q{s_qu[w], s_qd[w], s_qe[w], s_qx[ws], 0u, 0u};
-  Ring p{s_pu[ws], s_pd[ws], s_pe[ws], nullptr, 0u, 0u};
+  WaveAcc* acc = &s_acc[w];
+  if (lane == 0) {
+    acc->live = 0;
+    for (int i = 0; i < 8; ++i) acc->v[i] = 0;
+  }
+  Ring q{s_qu[w], s_qd[w], s_qe[w], s_qx[ws], 0u, 0u, acc};
+  Ring p{s_pu[ws], s_pd[ws], s_pe[ws], nullptr, 0u, 0u, acc};
+  const u32 grab = a.grab ? a.grab : 1u;  // consecutive items per work-counter update
   for (;;) {
-    u32 item = 0;
-    if (lane == 0) item = atomicAdd(a.item_ctr, 1u);
-    item = __shfl(item, 0, 64);
-    if (item >= a.total) break;
-    u32 lo = 0, hi = a.n_segs;  // the segment holding this launch item
-    while (hi - lo > 1) {
-      const u32 mid = (lo + hi) >> 1;
-      if (a.segs[mid].g0 <= item) lo = mid;
-      else hi = mid;
+    u32 item0 = 0;
+    if (lane == 0) item0 = atomicAdd(a.item_ctr, grab);
+    item0 = __shfl(item0, 0, 64);
+    if (item0 >= a.total) break;
+    const u32 iend = a.total - item0 > grab ? item0 + grab : a.total;
+    for (u32 item = item0; item < iend; ++item) {
+      u32 lo = 0, hi = a.n_segs;  // the segment holding this launch item
+      while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (a.segs[mid].g0 <= item) lo = mid;
+        else hi = mid;
+      }
+      const Segment G = a.segs[lo];
+      const u32 local = G.item0 + (item - G.g0);
+      if (G.kind == 0) cls5_item<E, S, V>(a, G.u0, G.n_units, s_gate, s_sieve, q, p, local, lane);
+      else find3_item<E, S>(a, G.u0, G.n_units, s_gate, q, p, local, lane);
     }
-    const Segment G = a.segs[lo];
-    const u32 local = G.item0 + (item - G.g0);
-    if (G.kind == 0) cls5_item<E, S>(a, G.u0, G.n_units, s_gate, q, p, local, lane);
-    else find3_item<E, S>(a, G.u0, G.n_units, s_gate, q, p, local, lane);
   }
   if (q.tail != q.head) {
     ring_sync();
@@ -385,6 +471,7 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
       p.head += n;
     }
   }
+  if (lane == 0) acc_flush(a, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -600,20 +687,28 @@ static int grid_for(u64 n, int threads) {
   return (int)(g < 65536 ? (g ? g : 1) : 65536);
 }
 
-template <bool S>
+template <bool S, bool V>
 static int launch_batched_variant(const BatchedArgs& a, int grid, int rows, hipStream_t s) {
   switch (rows) {
-    case 4: hipLaunchKernelGGL((k_cls5_batched<4, S>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((k_cls5_batched<8, S>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((k_cls5_batched<16, S>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((k_cls5_batched<32, S>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_cls5_batched<4, S, V>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_cls5_batched<8, S, V>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_cls5_batched<16, S, V>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_cls5_batched<32, S, V>), dim3(grid), dim3(64 * kWaves), 0, s, a); break;
     default: return (int)hipErrorInvalidValue;
   }
   return (int)hipGetLastError();
 }
 
-int launch_cls5_batched(const BatchedArgs& a, int grid, int rows, hipStream_t s) {
-  return a.strict ? launch_batched_variant<true>(a, grid, rows, s) : launch_batched_variant<false>(a, grid, rows, s);
+int launch_cls5_batched(const BatchedArgs& args, int grid, int rows, hipStream_t s) {
+  // items per work-counter update: 4 when every wavefront has at least 256
+  // items to take, fewer for short launches (the tail stays balanced)
+  BatchedArgs a = args;
+  const u64 per_wave = (u64)a.total / ((u64)(grid > 0 ? grid : 1) * kWaves);
+  a.grab = per_wave >= 256 ? 4u : per_wave >= 64 ? 2u : 1u;
+  if (a.strict) return a.sieve ? launch_batched_variant<true, true>(a, grid, rows, s)
+                               : launch_batched_variant<true, false>(a, grid, rows, s);
+  return a.sieve ? launch_batched_variant<false, true>(a, grid, rows, s)
+                 : launch_batched_variant<false, false>(a, grid, rows, s);
 }
 
 int launch_cls5_plain(const PlainArgs& a, int n_units, int ymax, hipStream_t s) {
@@ -641,20 +736,21 @@ int launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
-template <bool S>
+template <bool S, bool V>
 static hipError_t occupancy_variant(int& nb, int rows) {
   switch (rows) {
-    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<4, S>, 64 * kWaves, 0);
-    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<8, S>, 64 * kWaves, 0);
-    case 16: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<16, S>, 64 * kWaves, 0);
-    case 32: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<32, S>, 64 * kWaves, 0);
+    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<4, S, V>, 64 * kWaves, 0);
+    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<8, S, V>, 64 * kWaves, 0);
+    case 16: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<16, S, V>, 64 * kWaves, 0);
+    case 32: return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cls5_batched<32, S, V>, 64 * kWaves, 0);
     default: return hipErrorInvalidValue;
   }
 }
 
-int batched_blocks_per_cu(int rows, bool strict) {
+int batched_blocks_per_cu(int rows, bool strict, bool sieve) {
   int nb = 0;
-  const hipError_t e = strict ? occupancy_variant<true>(nb, rows) : occupancy_variant<false>(nb, rows);
+  const hipError_t e = strict ? (sieve ? occupancy_variant<true, true>(nb, rows) : occupancy_variant<true, false>(nb, rows))
+                              : (sieve ? occupancy_variant<false, true>(nb, rows) : occupancy_variant<false, false>(nb, rows));
   if (e != hipSuccess) return 4;
   return nb > 0 ? nb : 1;
 }

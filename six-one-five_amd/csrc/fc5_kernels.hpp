@@ -111,7 +111,8 @@ struct BatchedArgs {
   const u32* gate;  // 124488-bit bitmap
   const u32* ndg;   // [64][ndg_K]
   u32 ndg_K, use_gate;
-  u32 strict, pad_;  // strict: the strict residue gate before every xor query
+  u32 strict;  // the strict residue gate before every xor query
+  u32 sieve;   // class-5 sweep: 1 bit sieve over d (fc5_sieve.hpp), 0 one LDS bitmap test per point
   XorDev main, side;
   HitRec* hits;
   u64 hit_cap;
@@ -120,6 +121,7 @@ struct BatchedArgs {
   u32* item_ctr;
   u64 inv729;  // 729^-1 mod 2^64
   const DevUnit3* units3;  // class 2-4 unit arena
+  u32 grab, pad_;           // work items per work-counter update (set by launch_cls5_batched)
 };
 
 // reference-shaped find3 candidate (Cand, fourcore_find_v4.cu:102-109)
@@ -193,7 +195,7 @@ int launch_find3_plain(const PlainF3Args& a, hipStream_t s);
 int launch_expand_cells(const u32* packed_words, u64 m_cells, u32 r, u64* cells, hipStream_t s);
 int launch_query(const XorDev& x, const u64* keys, u64 n, u8* out, hipStream_t s);
 int launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s);
-int batched_blocks_per_cu(int rows, bool strict);  // occupancy limit of the kernel variant
+int batched_blocks_per_cu(int rows, bool strict, bool sieve);  // occupancy limit of the kernel variant
 int launch_probe_only(const XorDev& x, u64 n, u64 salt, unsigned long long* sink, int grid, hipStream_t s);
 int launch_gather_raw(const u64* cells, u64 m, u64 n, u64 salt, u32 policy, unsigned long long* sink, int grid,
                       hipStream_t s);

@@ -167,7 +167,7 @@ extern "C" int fc5_run_units_noalloc(fc5_ctx* ctx, const fc5_unit* units, size_t
   }
   u64 rows = 0;
   for (u32 k = 0; k < n_dev; ++k) rows += H.units[k].rows;
-  const int E = ctx->rows_per_item > 0 ? ctx->rows_per_item : auto_rows_per_item(ctx->num_cu, rows, ctx->strict_gate != 0);
+  const int E = ctx->rows_per_item > 0 ? ctx->rows_per_item : auto_rows_per_item(ctx->num_cu, rows, ctx->strict_gate != 0, ctx->sweep != 0);
   u64 items = 0;
   for (u32 k = 0; k < n_dev; ++k) {
     H.units[k].item_base = (u32)items;
@@ -210,6 +210,7 @@ extern "C" int fc5_run_units_noalloc(fc5_ctx* ctx, const fc5_unit* units, size_t
     a.ndg_K = ndgK;
     a.use_gate = (u32)ctx->use_gate;
     a.strict = (u32)ctx->strict_gate;
+    a.sieve = (u32)ctx->sweep;
     a.main = ctx->main.dev;
     a.side = ctx->side.present ? ctx->side.dev : XorDev{};
     a.hits = S->hits;
@@ -219,7 +220,7 @@ extern "C" int fc5_run_units_noalloc(fc5_ctx* ctx, const fc5_unit* units, size_t
     a.item_ctr = reinterpret_cast<u32*>(&S->item_ctr);
     a.inv729 = inv_mod_2_64(729);
     a.units3 = nullptr;
-    const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(E, ctx->strict_gate != 0);
+    const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(E, ctx->strict_gate != 0, ctx->sweep != 0);
     const int grid = (int)std::max<u64>(1, std::min<u64>((u64)ctx->num_cu * (u64)bpc, items));
     if (launch_cls5_batched(a, grid, E, ctx->stream) != 0) FAIL(ctx, FC5_E_HIP, "batched launch");
   }

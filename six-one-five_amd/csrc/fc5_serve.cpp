@@ -391,6 +391,7 @@ int plan_round(fc5_server* S, const std::vector<SBand*>& bands, Round& R) {
   a.ndg_K = ctx->ndg_K;
   a.use_gate = (u32)ctx->use_gate;
   a.strict = (u32)ctx->strict_gate;
+  a.sieve = (u32)ctx->sweep;
   a.main = ctx->main.dev;
   a.side = ctx->side.present ? ctx->side.dev : XorDev{};
   a.hits = R.d_hits;
@@ -399,7 +400,7 @@ int plan_round(fc5_server* S, const std::vector<SBand*>& bands, Round& R) {
   a.ctr = S->d_ctr;
   a.item_ctr = ctx->d_item_ctr;  // rounds run in stream order, so one work counter serves both
   a.inv729 = inv_mod_2_64(729);
-  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(S->rows, ctx->strict_gate != 0);
+  const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(S->rows, ctx->strict_gate != 0, ctx->sweep != 0);
   const int grid = (int)std::max<u64>(1, std::min<u64>((u64)ctx->num_cu * (u64)bpc, total));
   const size_t nslots = S->slots.size() * kSlotStride;
   HIPCHK(ctx, hipMemcpyAsync(R.d_segs, R.h_segs, n_segs * sizeof(Segment), hipMemcpyHostToDevice, ctx->stream));

@@ -93,6 +93,44 @@ def test_band_100001_102000_no_gate(ctx):
     assert out.result["gated"] == 0
 
 
+@pytest.mark.parametrize("sweep", [0, 1], ids=["bitmap", "sieve"])
+def test_sweep_forms_match_oracle(ctx, sweep):
+    """both class-5 sweeps of the batched kernel (one LDS bitmap test per grid
+    point; the bit sieve over d, 32 points a word) against the oracle: the full
+    [100001,102000] band at r=16, sampled 3.2M units (B^6 > 2^128), without the
+    gate, and with the strict gate"""
+    say(f"sweep form {sweep}")
+    t = _load(ctx, 2428, 16)
+    ctx.set_option("sweep", sweep)
+    try:
+        for lo, hi, every in ((100001, 102000, 1), (3200001, 3210000, 499)):
+            Bs, us = _units(lo, hi, every=every)
+            out = ctx.run(fc5.units_array(Bs, us))
+            oc, oh = _oracle_run(Bs, us, t)
+            _compare(out, oc, oh)
+            assert len(oh) > 0
+        Bs, us = _units(100001, 100600)
+        ctx.set_option("use_gate", 0)
+        try:
+            out = ctx.run(fc5.units_array(Bs, us))
+        finally:
+            ctx.set_option("use_gate", 1)
+        oc, oh = _oracle_run(Bs, us, t, use_gate=False)
+        _compare(out, oc, oh)
+        Bs, us = _units(3200001, 3210000, every=499)
+        arr, ok = O.pack_units(Bs, us)
+        oc, oh, _ = O.peel(arr, len(Bs), t, strict=True, threads=min(16, os.cpu_count() or 8))
+        ctx.set_option("strict_gate", 1)
+        try:
+            out = ctx.run(fc5.units_array(Bs, us))
+        finally:
+            ctx.set_option("strict_gate", 0)
+        _compare(out, oc, oh)
+        assert out.result["strict_out"] == oc["probes"] - oc["queries"]
+    finally:
+        ctx.set_option("sweep", 1)
+
+
 @pytest.mark.parametrize("lo,hi,every", [(1000001, 1020000, 97), (2000001, 2010000, 211),
                                          (3200001, 3210000, 997)])
 def test_sampled_units_match_oracle(ctx, lo, hi, every):

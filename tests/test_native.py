@@ -35,3 +35,13 @@ def test_solution_record_fields(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout
     assert "solution ok" in out.stdout
+
+
+def test_sieve_words_equal_reference_gate(tmp_path):
+    """the class-5 bit sieve (five small moduli, 32 grid points a word) is the reference gate"""
+    exe = str(tmp_path / "sieve_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(PKG, "csrc"),
+                    os.path.join(REPO, "tests", "native", "sieve_check.cpp"), "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout
+    assert "sieve ok" in out.stdout
