@@ -75,6 +75,7 @@ extern "C" int fc5_ctx_create(int device, fc5_ctx** out) try {
       hipMemcpy(ctx->d_gate, bitmap.data(), kGateWordsU * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(ctx->d_gate_ones, ones.data(), kGateWordsU * 4, hipMemcpyHostToDevice) != hipSuccess)
     return FC5_E_HIP;
+  if (upload_sieve_tables() != 0) return FC5_E_HIP;
   (void)noalloc_setup(device);  // its failure is reported by fc5_run_units_noalloc alone
   *out = ctx.release();
   return FC5_OK;

@@ -35,6 +35,11 @@ __device__ __forceinline__ u32 wave_sum_u32(u32 v) {
   return v;
 }
 
+__device__ __forceinline__ u32 wave_max_u32(u32 v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (u32)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
 __device__ __forceinline__ u32 rdl(u32 v, int lane) {
   return (u32)__builtin_amdgcn_readlane((int)v, lane);
 }

@@ -17,6 +17,8 @@
 //                 sixth power, subtract-with-borrow and modular gate.
 // k_expand_cells  widens the r-bit packed table to one u64 per cell.
 // k_query         membership queries for a key list (tests, FPR smoke).
+#include <vector>
+
 #include "fc5_device.hpp"
 #include "fc5_sieve.hpp"
 #include "fc5_strict.hpp"
@@ -28,6 +30,7 @@ constexpr int kWaves = 4;   // wavefronts per workgroup
 constexpr u32 kRing = 128;  // per-wave survivor ring (entries)
 
 __constant__ SieveTab kSieveDev = make_sieve();  // staged into LDS by every workgroup
+__device__ u32 g_strict_sieve[kStrictWords];     // fill_strict_sieve, uploaded per device (upload_sieve_tables)
 
 // floor(x^6 / 64) for x < 2^21
 __device__ __forceinline__ u128 dq_of(u32 x) {
@@ -208,16 +211,9 @@ __device__ __forceinline__ u32 ring_push(const BatchedArgs& a, Ring& q, Ring& p,
 template <int E>
 __device__ __forceinline__ void flush_counters(const BatchedArgs& a, WaveAcc* A, u32 band, u32 lane, u32 calls,
                                                u32 neg, u32 zero, u32 pos, u32 probes, u32 jobs, u32 failT) {
-  u64 s_calls = 0, s_neg = 0, s_zero = 0, s_pos = 0, s_jobs = 0, s_fail = 0;
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    s_calls += rdl(calls, i);
-    s_neg += rdl(neg, i);
-    s_zero += rdl(zero, i);
-    s_pos += rdl(pos, i);
-    s_jobs += rdl(jobs, i);
-    s_fail += rdl(failT, i);
-  }
+  // lanes E.. hold zeros; an item's sums stay below 2^32 (at most 32 rows of 2^26 values)
+  const u64 s_calls = wave_sum_u32(calls), s_neg = wave_sum_u32(neg), s_zero = wave_sum_u32(zero);
+  const u64 s_pos = wave_sum_u32(pos), s_jobs = wave_sum_u32(jobs), s_fail = wave_sum_u32(failT);
   if (lane == 0) {
     acc_band(a, A, band);
     A->v[0] += s_calls;
@@ -248,7 +244,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
   const u32 minr = min(min(r0, r1), min(r2, r3));
 
   // per-row set-up on lanes 0..E-1: e, T-space row value, positive cut
-  u32 e = 0, dlim = 0, xg = 0, xs = 0, sa = 0, sb = 0, sc = 0;
+  u32 e = 0, dlim = 0, xg = 0, xs = 0, sa = 0, sb = 0, sc = 0, ta = 0, tb = 0, tc = 0;
   u32 c_calls = 0, c_neg = 0, c_zero = 0, c_pos = 0;
   if (lane < (u32)E) {
     const u32 r = row0 + lane;
@@ -272,6 +268,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
           xg = mod_gate(X);
           if (V) sieve_row_bases(xg, U.td, sa, sb, sc);
           if (S) xs = strict_row((u64)(X >> 64), (u64)X);
+          if (V && S) strict_sieve_bases(xs, U.td, ta, tb, tc);
           const u32 z = dc + 1;  // the only d that can give T == 0
           if (z <= U.fmax2 && dq_of(z) == X) {
             const u32 zr = z & 63u;
@@ -287,21 +284,21 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
       }
     }
   }
-  u32 maxdl = 0;
-#pragma unroll
-  for (int i = 0; i < E; ++i) maxdl = max(maxdl, rdl(dlim, i));
+  const u32 maxdl = wave_max_u32(dlim);
 
   u32 probes_item = 0;
   if (V) {
     // bit sieve: lane slot = (word w, residue index) over the item's d range,
     // one word = 32 consecutive k of one residue; per row five LDS words AND-ed,
-    // the cut applied, survivors taken lowest bit first
+    // the cut applied, survivors taken lowest bit first.  S: five strict words
+    // more (global memory), and the survivors go straight to the probe ring.
     const u32 nslots = maxdl >= minr ? 4u * ((maxdl - minr) / 2048u + 1u) : 0u;
     for (u32 s0 = 0; s0 < nslots; s0 += 64u) {
       const u32 slot = s0 + lane, ri = slot & 3u, w = slot >> 2;
       const u32 r = ri == 0u ? r0 : ri == 1u ? r1 : ri == 2u ? r2 : r3;
       const u32 d0 = slot < nslots ? r + 2048u * w : 0xFFFFFFFFu;  // past the slots: no grid point
       const u32 j9 = d0 % 3u, j7 = d0 % 7u, j13 = d0 % 13u, j19 = d0 % 19u;
+      const u32 j27 = S ? d0 % 27u : 0u, j31 = S ? d0 % 31u : 0u, j37 = S ? d0 % 37u : 0u, j43 = S ? d0 % 43u : 0u;
       const u32 dminp = minr + 2048u * (s0 >> 2);  // the pass's least d0
       for (int i = 0; i < E; ++i) {
         const u32 dl = rdl(dlim, i);
@@ -313,12 +310,18 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
                s_sieve[(rb >> 16) + j13] & s_sieve[rc + j19];
         }
         probes_item += (u32)__popc(P);
+        if (S) {
+          const u32 xa = rdl(ta, i), xb = rdl(tb, i), xc = rdl(tc, i);
+          if (P)
+            P &= g_strict_sieve[(xa & 0xFFFFu) + r] & g_strict_sieve[(xa >> 16) + j27] &
+                 g_strict_sieve[(xb & 0xFFFFu) + j31] & g_strict_sieve[(xb >> 16) + j37] & g_strict_sieve[xc + j43];
+        }
         const u32 ei = rdl(e, i);
-        const u32 xsi = S ? rdl(xs, i) : 0u;
         while (__ballot(P != 0u)) {
           const bool has = P != 0u;
           const u32 d = d0 + 64u * (has ? (u32)__builtin_ctz(P) : 0u);
-          ring_push<S>(a, q, p, has, ui, d, ei, xsi, lane);
+          if (S) push_probe(a, p, has, ui, d, ei, lane);
+          else ring_push<false>(a, q, p, has, ui, d, ei, 0u, lane);
           P &= P - 1u;
         }
       }
@@ -697,6 +700,13 @@ static int launch_batched_variant(const BatchedArgs& a, int grid, int rows, hipS
     default: return (int)hipErrorInvalidValue;
   }
   return (int)hipGetLastError();
+}
+
+int upload_sieve_tables() {
+  std::vector<u32> w(kStrictWords);
+  fill_strict_sieve(w.data());
+  return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_strict_sieve), w.data(), kStrictWords * sizeof(u32), 0,
+                                hipMemcpyHostToDevice);
 }
 
 int launch_cls5_batched(const BatchedArgs& args, int grid, int rows, hipStream_t s) {

@@ -2,10 +2,12 @@
 // random rows X, residues r and grid positions k, the sieve bit of
 // d = r + 64k equals the reference gate on the exact T = X - floor(d^6/64)
 // (T mod 504 and T mod 247 sums of two sixth powers, sets by brute force),
-// and the cut mask keeps exactly the d <= dl of a word.
+// the strict sieve bit equals the strict gate (fc5_strict.hpp) on that T, and
+// the cut mask keeps exactly the d <= dl of a word.
 #include <cstdio>
 
 #include "fc5_sieve.hpp"
+#include "fc5_strict.hpp"
 
 using namespace fc5;
 using u128 = unsigned __int128;
@@ -26,7 +28,9 @@ int main() {
   for (uint32_t a = 0; a < 247; ++a)
     for (uint32_t b = 0; b < 247; ++b) g247[(sieve_pow6(a, 247) + sieve_pow6(b, 247)) % 247] = true;
   constexpr SieveTab tab = make_sieve();
-  long bad = 0, pass = 0, n = 0;
+  static uint32_t stab[kStrictWords];
+  fill_strict_sieve(stab);
+  long bad = 0, pass = 0, n = 0, spass = 0;
   for (int it = 0; it < 4000000; ++it) {
     const uint32_t r = (uint32_t)(next() & 63u);
     const uint32_t k = (uint32_t)(next() % 32768u);  // d < 2^21
@@ -44,6 +48,12 @@ int main() {
     if (got != want) ++bad;
     pass += want;
     ++n;
+    const bool swant = strict_pass_t((uint32_t)(T % kStrictA), (uint32_t)(T % 81u), (uint32_t)(T % 64u));
+    uint32_t sa, sb, sc;
+    strict_sieve_bases(strict_row((uint64_t)(X >> 64), (uint64_t)X), td, sa, sb, sc);
+    const bool sgot = (strict_sieve_word(stab, sa, sb, sc, r, d0) >> j) & 1u;
+    if (sgot != swant) ++bad;
+    spass += swant;
   }
   // the cut: bit j of sieve_cut(d0, dl) is set iff d0 + 64j <= dl
   for (int it = 0; it < 1000000; ++it) {
@@ -52,7 +62,7 @@ int main() {
     for (uint32_t j = 0; j < 32; ++j)
       if ((((m >> j) & 1u) != 0) != ((uint64_t)d0 + 64u * j <= dl)) ++bad;
   }
-  std::printf("gate pass fraction %.5f over %ld points\n", (double)pass / n, n);
+  std::printf("gate pass fraction %.5f, strict %.5f over %ld points\n", (double)pass / n, (double)spass / n, n);
   if (bad) {
     std::printf("sieve mismatches %ld\n", bad);
     return 1;
