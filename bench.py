@@ -247,7 +247,7 @@ def main():
     strict = None
     if not args.no_strict:
         saved = (tune["rows_per_item"], tune["blocks_per_cu"]) if tune else (0, args.blocks_per_cu)
-        strict = strict_leg(ctx, band, outs[-1], Bs, us, ucls if mixed else None, args, check, value, saved)
+        strict = strict_leg(ctx, band, outs[-1], Bs, us, ucls if mixed else None, args, check, value, saved, n_B)
 
     ctx.set_option("use_graph", 0)
     ctx.set_option("verify_table", 0)
@@ -691,7 +691,7 @@ def hit_T(cls, B, u, d, e):
     return (B ** 6 - u ** 6 - (f * d) ** 6) // 42 ** 6 - e ** 6
 
 
-def strict_leg(ctx, band, base, Bs, us, ucls, args, check, base_value, saved):
+def strict_leg(ctx, band, base, Bs, us, ucls, args, check, base_value, saved, n_B):
     """The same band with the option strict_gate on (tuned and timed like the
     default leg).  Checks: the gate counters equal the default run's, every
     plant is found, and the hit list is exactly the default hit list restricted
@@ -735,6 +735,7 @@ def strict_leg(ctx, band, base, Bs, us, ucls, args, check, base_value, saved):
         f"({value / base_value:.2f}x the default leg); queries {queries} of {r['probes']} probes; "
         f"maybe={r['maybe']} side_maybe={r['side_maybe']} (default {r0['maybe']}, {r0['side_maybe']})")
     return {"value": value, "unit": "(d,e) grid points/s", "ms_per_step": ms, "kernel_ms": float(np.mean(kms)),
+            "eligible_B_per_hour": n_B * 3600.0 / (ms / 1000.0),
             "speedup_vs_default": value / base_value, "probes": r["probes"], "queries": queries,
             "strict_out": r["strict_out"], "maybe": r["maybe"], "side_maybe": r["side_maybe"],
             "default_maybe": r0["maybe"], "default_side_maybe": r0["side_maybe"],
