@@ -740,7 +740,7 @@ def strict_leg(ctx, band, base, Bs, us, ucls, args, check, base_value, saved, n_
             "strict_out": r["strict_out"], "maybe": r["maybe"], "side_maybe": r["side_maybe"],
             "default_maybe": r0["maybe"], "default_side_maybe": r0["side_maybe"],
             "hits_equal_filtered_default": True, "autotune": tune,
-            "option": "strict_gate=1: before each xor query T mod 64, 81 and 31*37*43 must also be sums "
+            "option": "strict_gate=1: before each xor query T mod 64, 81, 31, 37, 43, 49, 61, 67, 73, 79 must also be sums "
                       "of two sixth powers (a necessary condition for a pair sum that the reference does "
                       "not test); every counter but maybe/side_maybe is the reference's, the hit list is "
                       "the default one without the filter's false positives the strict gate rejects"}

@@ -114,7 +114,7 @@ void fc5_ctx_destroy(fc5_ctx *ctx);
 const char *fc5_last_error(const fc5_ctx *ctx);
 /* options: "hit_cap" (default 1<<20), "use_gate" (1; 0 = --no-gate,
  * fourcore_cls5_gpu_v4.cu:634), "strict_gate" (0; 1 = before each xor query, also
- * require T mod 64, T mod 81 and T mod 31*37*43 to be sums of two sixth powers:
+ * require T mod 64, 81, 31, 37, 43, 49, 61, 67, 73 and 79 to be sums of two sixth powers:
  * a necessary condition for T = a^6 + b^6 that the reference does not test, so no
  * solution is lost; counters and hits are the reference's except that the xor
  * filter's false positives among the T it rejects are not reported, and

@@ -807,7 +807,8 @@ int or_pack_unit5(u64 B, u64 u, or_unit5_t *U) {
 }
 
 /* ------------------------------------------------------------------ */
-/* the strict gate: sums of two sixth powers modulo 64, 81, 31, 37, 43   */
+/* the strict gate: sums of two sixth powers modulo 64, 81, 31, 37, 43,  */
+/* 49, 61, 67, 73, 79                                                   */
 /* by enumeration (a test of this build's option; the reference has     */
 /* no such gate)                                                        */
 /* ------------------------------------------------------------------ */
@@ -824,18 +825,19 @@ static int sum2_pow6_mod(u32 t, u32 m) {
   return 0;
 }
 
-static unsigned char g_strict[5][81];
+static unsigned char g_strict[10][81];
+static const u32 g_strict_mods[10] = {64, 81, 31, 37, 43, 49, 61, 67, 73, 79};
 static void strict_init_once(void) {
-  static const u32 mods[5] = {64, 81, 31, 37, 43};
-  for (int i = 0; i < 5; ++i)
-    for (u32 t = 0; t < mods[i]; ++t) g_strict[i][t] = (unsigned char)sum2_pow6_mod(t, mods[i]);
+  for (int i = 0; i < 10; ++i)
+    for (u32 t = 0; t < g_strict_mods[i]; ++t) g_strict[i][t] = (unsigned char)sum2_pow6_mod(t, g_strict_mods[i]);
 }
 static pthread_once_t g_strict_once = PTHREAD_ONCE_INIT;
 
-int or_strict_pass(u32 t64, u32 t81, u32 tA) {
+int or_strict_pass(u32 t64, u32 t81, u32 tA, u32 tB, u32 tC) {
   pthread_once(&g_strict_once, strict_init_once);
   return g_strict[0][t64 % 64] && g_strict[1][t81 % 81] && g_strict[2][tA % 31] && g_strict[3][tA % 37] &&
-         g_strict[4][tA % 43];
+         g_strict[4][tA % 43] && g_strict[5][tB % 49] && g_strict[6][tB % 61] && g_strict[7][tB % 67] &&
+         g_strict[8][tC % 73] && g_strict[9][tC % 79];
 }
 
 /* ------------------------------------------------------------------ */
@@ -854,7 +856,9 @@ int or_cls5_peel(const or_unit5_t *units, int n, const or_xor_t *f, const or_xor
   const u64 blk = (f && f->hdr.m_cells) ? f->hdr.m_cells / 3 : 0;
   const u64 sblk = (side && side->hdr.m_cells) ? side->hdr.m_cells / 3 : 0;
   const u64 mA = 31ULL * 37 * 43 * (u64)M42, m81 = 81ULL * (u64)M42;
+  const u64 mB = 49ULL * 61 * 67 * (u64)M42, mC = 73ULL * 79 * (u64)M42;
   const u64 c_mA = (u64)(((u128)1 << 64) % (u128)mA), c_m81 = (u64)(((u128)1 << 64) % (u128)m81);
+  const u64 c_mB = (u64)(((u128)1 << 64) % (u128)mB), c_mC = (u64)(((u128)1 << 64) % (u128)mC);
   u64 calls = 0, neg = 0, zero = 0, gated = 0, probes = 0, maybe = 0, smaybe = 0, queries = 0;
   u64 nh = 0;
   if (threads < 1) threads = 1;
@@ -895,7 +899,9 @@ int or_cls5_peel(const or_unit5_t *units, int n, const or_xor_t *f, const or_xor
         if (use_gate & 2) { /* T mod m = (R mod 42^6 m) / 42^6 */
           const u32 tA = (u32)(or_mod192(R, mA, c_mA) / (u64)M42);
           const u32 t81 = (u32)(or_mod192(R, m81, c_m81) / (u64)M42);
-          if (!or_strict_pass((u32)(fp & 63), t81, tA)) continue;
+          const u32 tB = (u32)(or_mod192(R, mB, c_mB) / (u64)M42);
+          const u32 tC = (u32)(or_mod192(R, mC, c_mC) / (u64)M42);
+          if (!or_strict_pass((u32)(fp & 63), t81, tA, tB, tC)) continue;
           ++queries;
         }
         u32 flags = 0;
@@ -1069,7 +1075,9 @@ int or_find3(const int *cls, const u64 *Bv, const u64 *uv, int n, const or_xor_t
           const u64 fp = Tl - c2 * c2 * c2;
           if (use_gate & 2) { /* V = T - c3^6 exactly */
             const u128 V = T - pow6_u128(c3);
-            if (!or_strict_pass((u32)(V % 64), (u32)(V % 81), (u32)(V % (31 * 37 * 43)))) continue;
+            if (!or_strict_pass((u32)(V % 64), (u32)(V % 81), (u32)(V % (31 * 37 * 43)), (u32)(V % (49 * 61 * 67)),
+                                (u32)(V % (73 * 79))))
+              continue;
             ++queries;
           }
           if (f && device_query(f->packed, blk, f->hdr.r, f->hdr.mix_seed, fp)) {

@@ -107,9 +107,10 @@ typedef struct {
 int or_pack_unit5(uint64_t B, uint64_t u, or_unit5_t *U);
 
 /* ---- the strict gate (no counterpart in the reference): 1 when T mod 64,
- * 81, 31, 37 and 43 are each a sum of two sixth powers modulo that number;
- * t64 = T mod 64, tA = T mod 31*37*43, t81 = T mod 81 ---- */
-int or_strict_pass(uint32_t t64, uint32_t t81, uint32_t tA);
+ * 81, 31, 37, 43, 49, 61, 67, 73 and 79 are each a sum of two sixth powers
+ * modulo that number; t64 = T mod 64, t81 = T mod 81, tA = T mod 31*37*43,
+ * tB = T mod 49*61*67, tC = T mod 73*79 ---- */
+int or_strict_pass(uint32_t t64, uint32_t t81, uint32_t tA, uint32_t tB, uint32_t tC);
 
 /* ---- the peel: k_cls5_192 semantics (fourcore_cls5_gpu_v4.cu:353-410) --- */
 typedef struct {
@@ -123,8 +124,8 @@ typedef struct {
 } or_hit_t;
 
 /* Runs the peel for units[0..n).  use_gate bit 0: the reference gate; bit 1:
- * the strict gate of this build (T mod 64, 81, 31, 37 and 43 a sum of two sixth
- * powers) on every probe before its query.  side may be NULL (planted-key side filter,
+ * the strict gate of this build (T mod 64, 81, 31, 37, 43, 49, 61, 67, 73 and 79
+ * a sum of two sixth powers) on every probe before its query.  side may be NULL (planted-key side filter,
  * probed with the same fingerprint as the main filter).  Hits are written to
  * hits (up to cap; *n_hits gets the true count).  OpenMP over units when
  * threads > 1.  Returns 0. */

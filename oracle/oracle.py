@@ -391,10 +391,10 @@ def exact_range(entries, s, T):
     return bisect.bisect_left(entries, T << s), bisect.bisect_left(entries, (T + 1) << s)
 
 
-_lib.or_strict_pass.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+_lib.or_strict_pass.argtypes = [C.c_uint32] * 5
 _lib.or_strict_pass.restype = C.c_int
 
 
 def strict_pass_T(T):
     """the strict gate on an exact T (an option of this build, not the reference's)"""
-    return bool(_lib.or_strict_pass(T % 64, T % 81, T % (31 * 37 * 43)))
+    return bool(_lib.or_strict_pass(T % 64, T % 81, T % (31 * 37 * 43), T % (49 * 61 * 67), T % (73 * 79)))

@@ -1089,7 +1089,9 @@ static int run_plain_find3(fc5_ctx* ctx, fc5_band* b, const std::vector<u32>& wh
       C.uid = uid;
       C.d = d;
       C.band = (u32)U.cls;
-      C.srow = strict_row((u64)(T >> 64), (u64)T);
+      const StrictRow sr = strict_row((u64)(T >> 64), (u64)T);
+      C.srow = sr.w0;
+      C.srow1 = sr.w1;
       if (n == kChunk) flush();
     }
   }

@@ -73,7 +73,7 @@ __device__ __forceinline__ void acc_band(const BatchedArgs& a, WaveAcc* A, u32 b
 
 struct Ring {
   u32 *qu, *qd, *qe;  // unit (| kFind3Bit), d, e (or c3)
-  u32* qx;            // strict-gate row word (variant S, survivor ring only)
+  u32 *qx, *qy;       // strict-gate row words (variant S, survivor ring only)
   u32 head, tail;
   WaveAcc* acc;       // the wavefront's counter sums
 };
@@ -173,11 +173,8 @@ __device__ __forceinline__ void strict_stage(const BatchedArgs& a, const Ring& q
     tag = q.qu[slot];
     x = q.qd[slot];
     y = q.qe[slot];
-    const u32 row = q.qx[slot];
-    u32 la, l81, l64;
-    if (tag & kFind3Bit) strict_lane_p6(y, la, l81, l64);
-    else strict_lane_q64(x, la, l81, l64);
-    pass = strict_pass(row, la, l81, l64);
+    const StrictRow row{q.qx[slot], q.qy[slot]};
+    pass = strict_pass(row, (tag & kFind3Bit) ? strict_lane_p6(y) : strict_lane_q64(x));
   }
   push_probe(a, p, pass, tag, x, y, lane);
 }
@@ -186,7 +183,7 @@ __device__ __forceinline__ void strict_stage(const BatchedArgs& a, const Ring& q
 // of 64 to the probes (S: through the strict gate into the probe ring p)
 template <bool S>
 __device__ __forceinline__ u32 ring_push(const BatchedArgs& a, Ring& q, Ring& p, bool pass, u32 tag, u32 x,
-                                         u32 y, u32 xw, u32 lane) {
+                                         u32 y, StrictRow xw, u32 lane) {
   const u64 m = __ballot(pass);
   if (!m) return 0;
   if (pass) {
@@ -194,7 +191,10 @@ __device__ __forceinline__ u32 ring_push(const BatchedArgs& a, Ring& q, Ring& p,
     q.qu[slot] = tag;
     q.qd[slot] = x;
     q.qe[slot] = y;
-    if (S) q.qx[slot] = xw;
+    if (S) {
+      q.qx[slot] = xw.w0;
+      q.qy[slot] = xw.w1;
+    }
   }
   const u32 cnt = (u32)__popcll(m);
   q.tail += cnt;
@@ -244,7 +244,9 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
   const u32 minr = min(min(r0, r1), min(r2, r3));
 
   // per-row set-up on lanes 0..E-1: e, T-space row value, positive cut
-  u32 e = 0, dlim = 0, xg = 0, xs = 0, sa = 0, sb = 0, sc = 0, ta = 0, tb = 0, tc = 0;
+  u32 e = 0, dlim = 0, xg = 0, sa = 0, sb = 0, sc = 0;
+  StrictRow xs{0u, 0u};
+  StrictSieveRow st{};
   u32 c_calls = 0, c_neg = 0, c_zero = 0, c_pos = 0;
   if (lane < (u32)E) {
     const u32 r = row0 + lane;
@@ -268,7 +270,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
           xg = mod_gate(X);
           if (V) sieve_row_bases(xg, U.td, sa, sb, sc);
           if (S) xs = strict_row((u64)(X >> 64), (u64)X);
-          if (V && S) strict_sieve_bases(xs, U.td, ta, tb, tc);
+          if (V && S) st = strict_sieve_bases(xs, U.td);
           const u32 z = dc + 1;  // the only d that can give T == 0
           if (z <= U.fmax2 && dq_of(z) == X) {
             const u32 zr = z & 63u;
@@ -298,7 +300,8 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
       const u32 r = ri == 0u ? r0 : ri == 1u ? r1 : ri == 2u ? r2 : r3;
       const u32 d0 = slot < nslots ? r + 2048u * w : 0xFFFFFFFFu;  // past the slots: no grid point
       const u32 j9 = d0 % 3u, j7 = d0 % 7u, j13 = d0 % 13u, j19 = d0 % 19u;
-      const u32 j27 = S ? d0 % 27u : 0u, j31 = S ? d0 % 31u : 0u, j37 = S ? d0 % 37u : 0u, j43 = S ? d0 % 43u : 0u;
+      StrictSieveLane sj{};
+      if (S) sj = strict_sieve_lane(d0);
       const u32 dminp = minr + 2048u * (s0 >> 2);  // the pass's least d0
       for (int i = 0; i < E; ++i) {
         const u32 dl = rdl(dlim, i);
@@ -311,17 +314,17 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
         }
         probes_item += (u32)__popc(P);
         if (S) {
-          const u32 xa = rdl(ta, i), xb = rdl(tb, i), xc = rdl(tc, i);
-          if (P)
-            P &= g_strict_sieve[(xa & 0xFFFFu) + r] & g_strict_sieve[(xa >> 16) + j27] &
-                 g_strict_sieve[(xb & 0xFFFFu) + j31] & g_strict_sieve[(xb >> 16) + j37] & g_strict_sieve[xc + j43];
+          StrictSieveRow b;
+#pragma unroll
+          for (int k = 0; k < kStrictRowWords; ++k) b.w[k] = rdl(st.w[k], i);
+          if (P) P &= strict_sieve_word(g_strict_sieve, b, sj, r);
         }
         const u32 ei = rdl(e, i);
         while (__ballot(P != 0u)) {
           const bool has = P != 0u;
           const u32 d = d0 + 64u * (has ? (u32)__builtin_ctz(P) : 0u);
           if (S) push_probe(a, p, has, ui, d, ei, lane);
-          else ring_push<false>(a, q, p, has, ui, d, ei, 0u, lane);
+          else ring_push<false>(a, q, p, has, ui, d, ei, StrictRow{0u, 0u}, lane);
           P &= P - 1u;
         }
       }
@@ -343,7 +346,7 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
       if (dl < dmin_blk) continue;
       const u32 xgi = rdl(xg, i);
       const u32 ei = rdl(e, i);
-      const u32 xsi = S ? rdl(xs, i) : 0u;
+      const StrictRow xsi = S ? StrictRow{rdl(xs.w0, i), rdl(xs.w1, i)} : StrictRow{0u, 0u};
       const u32 ds[4] = {d0, d1, d2, d3};
       const u32 ns[4] = {n0, n1, n2, n3};
 #pragma unroll
@@ -370,7 +373,8 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_u
   const u32 ui = lo;
   const DevUnit3* Up = a.units3 + ui;
   const u32 row0 = (item - Up->item_base) * (u32)E;
-  u32 d = 0, wlo = 1, whi = 0, xg = 0, xs = 0;
+  u32 d = 0, wlo = 1, whi = 0, xg = 0;
+  StrictRow xs{0u, 0u};
   u32 c_calls = 0, c_jobs = 0, c_fail = 0;
   if (lane < (u32)E) {
     const u32 r = row0 + lane;
@@ -405,7 +409,7 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_u
     const u32 l = rdl(wlo, i), h = rdl(whi, i);
     if (l > h) continue;
     const u32 xgi = rdl(xg, i), di = rdl(d, i);
-    const u32 xsi = S ? rdl(xs, i) : 0u;
+    const StrictRow xsi = S ? StrictRow{rdl(xs.w0, i), rdl(xs.w1, i)} : StrictRow{0u, 0u};
     for (u32 cb = l; cb <= h; cb += 64u) {
       const u32 c = cb + lane;
       const bool in = c <= h;
@@ -424,7 +428,7 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
   __shared__ u32 s_gate[kGateWords];
   __shared__ u32 s_sieve[V ? kSieveWords : 1];
   __shared__ u32 s_qu[kWaves][kRing], s_qd[kWaves][kRing], s_qe[kWaves][kRing];
-  __shared__ u32 s_qx[kSW][kSR], s_pu[kSW][kSR], s_pd[kSW][kSR], s_pe[kSW][kSR];
+  __shared__ u32 s_qx[kSW][kSR], s_qy[kSW][kSR], s_pu[kSW][kSR], s_pd[kSW][kSR], s_pe[kSW][kSR];
   __shared__ WaveAcc s_acc[kWaves];
   for (u32 i = threadIdx.x; i < kGateWords; i += blockDim.x) s_gate[i] = a.gate[i];
   if (V)
@@ -439,8 +443,8 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
     acc->live = 0;
     for (int i = 0; i < 8; ++i) acc->v[i] = 0;
   }
-  Ring q{s_qu[w], s_qd[w], s_qe[w], s_qx[ws], 0u, 0u, acc};
-  Ring p{s_pu[ws], s_pd[ws], s_pe[ws], nullptr, 0u, 0u, acc};
+  Ring q{s_qu[w], s_qd[w], s_qe[w], s_qx[ws], s_qy[ws], 0u, 0u, acc};
+  Ring p{s_pu[ws], s_pd[ws], s_pe[ws], nullptr, nullptr, 0u, 0u, acc};
   const u32 grab = a.grab ? a.grab : 1u;  // consecutive items per work-counter update
   for (;;) {
     u32 item0 = 0;
@@ -515,7 +519,9 @@ __device__ __forceinline__ u64 mod192(u64 a2, u64 a1, u64 a0, u64 m, u64 two64) 
 
 // T mod m from R = 42^6 T: (R mod 42^6 m) / 42^6 (strict gate of the plain kernel)
 constexpr u64 kStrictA42 = (u64)kStrictA * kM42, kStrict81_42 = 81ULL * kM42;
+constexpr u64 kStrictB42 = (u64)kStrictB * kM42, kStrictC42 = (u64)kStrictC * kM42;
 constexpr u64 kTwo64A42 = (u64)(((u128)1 << 64) % kStrictA42), kTwo64_81_42 = (u64)(((u128)1 << 64) % kStrict81_42);
+constexpr u64 kTwo64B42 = (u64)(((u128)1 << 64) % kStrictB42), kTwo64C42 = (u64)(((u128)1 << 64) % kStrictC42);
 
 __global__ __launch_bounds__(256) void k_cls5_plain(PlainArgs a) {
   const int ui = blockIdx.x;
@@ -555,7 +561,9 @@ __global__ __launch_bounds__(256) void k_cls5_plain(PlainArgs a) {
     if (a.strict) {
       const u32 ta = (u32)(mod192(x2, x1, x0, kStrictA42, kTwo64A42) / kM42);
       const u32 t81 = (u32)(mod192(x2, x1, x0, kStrict81_42, kTwo64_81_42) / kM42);
-      if (!strict_pass_t(ta, t81, (u32)fp & 63u)) continue;
+      const u32 tb = (u32)(mod192(x2, x1, x0, kStrictB42, kTwo64B42) / kM42);
+      const u32 tc = (u32)(mod192(x2, x1, x0, kStrictC42, kTwo64C42) / kM42);
+      if (!strict_pass_t(ta, t81, (u32)fp & 63u, tb, tc)) continue;
       ++queries;
     }
     u32 flags = d_query(a.main, fp) ? 1u : 0u;
@@ -621,9 +629,7 @@ __global__ __launch_bounds__(256) void k_find3_plain(PlainF3Args a) {
     }
     ++probes;
     if (a.strict) {
-      u32 la, l81, l64;
-      strict_lane_p6(c3, la, l81, l64);
-      if (!strict_pass(C.srow, la, l81, l64)) continue;
+      if (!strict_pass(StrictRow{C.srow, C.srow1}, strict_lane_p6(c3))) continue;
       ++queries;
     }
     const u64 c2 = (u64)c3 * c3;

@@ -130,7 +130,8 @@ struct PlainCand3 {
   u32 lo, hi;
   u32 r504, r247;
   u32 uid, d;
-  u32 band, srow;  // srow: strict-gate row word of the candidate's T (strict_row)
+  u32 band, srow;  // srow, srow1: strict-gate row words of the candidate's T (strict_row)
+  u32 srow1, pad;
 };
 
 struct PlainF3Args {

@@ -22,6 +22,7 @@
 #include <cstdint>
 
 #include "fc5_moddiv.hpp"  // FC5_HD
+#include "fc5_strict.hpp"  // StrictRow
 
 namespace fc5 {
 
@@ -129,19 +130,25 @@ FC5_HD uint32_t sieve_cut(uint32_t d0, uint32_t dl) {
 }
 
 // ---------------------------------------------------------------------------
-// The strict residue gate (fc5_strict.hpp: T mod 64, 81, 31, 37 and 43 sums of
-// two sixth powers) as sieve words in the same form, for the class-5 rows:
+// The strict residue gate (fc5_strict.hpp: T mod 64, 81, 31, 37, 43, 49, 61,
+// 67, 73 and 79 sums of two sixth powers) as sieve words in the same form, for
+// the class-5 rows:
 //   m = 64  T mod 64 = X mod 64 - ((d^6 mod 4096) >> 6), a function of d mod
 //           2048, so j0 = r (2048w = 0 mod 2048): [A = X mod 64][r]
 //   m = 81  x^6 mod 81 depends on x mod 27: [A = X + td/64 mod 81][d0 mod 27]
-//   m = 31, 37, 43: [A = X + td/64 mod m][d0 mod m]
-// 10462 words, read from global memory (the class-5 rows of the strict variant).
-constexpr uint32_t kStrictOff64 = 0;     // 64 x 64
-constexpr uint32_t kStrictOff81 = 4096;  // 81 x 27
-constexpr uint32_t kStrictOff31 = 6283;  // 31 x 31
-constexpr uint32_t kStrictOff37 = 7244;  // 37 x 37
-constexpr uint32_t kStrictOff43 = 8613;  // 43 x 43
-constexpr uint32_t kStrictWords = 10462;
+//   other m [A = X + td/64 mod m][d0 mod m]
+// 32643 words, read from global memory (the class-5 rows of the strict variant).
+constexpr int kStrictMods = 10;
+constexpr uint32_t kStrictMod[kStrictMods] = {64, 81, 31, 37, 43, 49, 61, 67, 73, 79};
+constexpr uint32_t kStrictNj[kStrictMods] = {64, 27, 31, 37, 43, 49, 61, 67, 73, 79};  // j0 classes
+
+constexpr uint32_t strict_sieve_off(int i) {
+  uint32_t o = 0;
+  for (int k = 0; k < i; ++k) o += kStrictMod[k] * kStrictNj[k];
+  return o;
+}
+constexpr uint32_t kStrictWords = strict_sieve_off(kStrictMods);
+static_assert(kStrictWords == 32643 && kStrictWords < (1u << 16), "table bases fit 16 bits");
 
 // out[t]: t is a sum of two sixth powers mod m (m <= 128)
 inline void sieve_sum_set(uint32_t m, bool* out) {
@@ -165,12 +172,10 @@ inline void fill_strict_sieve(uint32_t* w) {
         for (int i = 0; i < 6; ++i) x6 = x6 * x % 4096u;
         if (S[(A + 64u - (uint32_t)(x6 >> 6)) & 63u]) word |= 1u << j;
       }
-      w[kStrictOff64 + A * 64u + r] = word;
+      w[A * 64u + r] = word;
     }
-  const uint32_t mods[4] = {81, 31, 37, 43}, nj[4] = {27, 31, 37, 43};
-  const uint32_t offs[4] = {kStrictOff81, kStrictOff31, kStrictOff37, kStrictOff43};
-  for (int t = 0; t < 4; ++t) {
-    const uint32_t m = mods[t], n = nj[t], inv = sieve_inv64(m);
+  for (int t = 1; t < kStrictMods; ++t) {
+    const uint32_t m = kStrictMod[t], n = kStrictNj[t], inv = sieve_inv64(m), off = strict_sieve_off(t);
     sieve_sum_set(m, S);
     for (uint32_t A = 0; A < m; ++A)
       for (uint32_t j0 = 0; j0 < n; ++j0) {
@@ -180,33 +185,53 @@ inline void fill_strict_sieve(uint32_t* w) {
           const uint32_t D = sieve_pow6(x, m) * inv % m;
           if (S[(A + m - D) % m]) word |= 1u << j;
         }
-        w[offs[t] + A * n + j0] = word;
+        w[off + A * n + j0] = word;
       }
   }
 }
 
-static_assert(sieve_inv64(81) == 19 && sieve_inv64(31) == 16 && sieve_inv64(37) == 11 && sieve_inv64(43) == 41,
-              "inverses of 64 used by strict_sieve_bases");
+// a row's ten table bases, two to a word (b[2k] | b[2k+1] << 16), and a
+// lane's nine j0 = d0 mod kStrictNj[1..9]
+constexpr int kStrictRowWords = kStrictMods / 2;
+struct StrictSieveRow {
+  uint32_t w[kStrictRowWords];
+};
+struct StrictSieveLane {
+  uint32_t j[kStrictMods - 1];
+};
 
-// The row's strict table bases from its strict row word xs (X mod 31*37*43 in
-// bits 0-15, X mod 81 in 16-22, X mod 64 in 23-28) and td:
-// sa = b64 | b81 << 16, sb = b31 | b37 << 16, sc = b43.
-FC5_HD void strict_sieve_bases(uint32_t xs, uint32_t td, uint32_t& sa, uint32_t& sb, uint32_t& sc) {
-  const uint32_t xa = xs & 0xFFFFu, x81 = (xs >> 16) & 127u, x64 = (xs >> 23) & 63u;
-  const uint32_t a81 = (x81 + 19u * td) % 81u;
-  const uint32_t a31 = (xa % 31u + 16u * td) % 31u;
-  const uint32_t a37 = (xa % 37u + 11u * td) % 37u;
-  const uint32_t a43 = (xa % 43u + 41u * td) % 43u;
-  sa = (kStrictOff64 + x64 * 64u) | (kStrictOff81 + a81 * 27u) << 16;
-  sb = (kStrictOff31 + a31 * 31u) | (kStrictOff37 + a37 * 37u) << 16;
-  sc = kStrictOff43 + a43 * 43u;
+// The row's strict table bases from its strict row words (fc5_strict.hpp) and td.
+FC5_HD StrictSieveRow strict_sieve_bases(const StrictRow& xs, uint32_t td) {
+  const uint32_t xa = xs.w0 & 0xFFFFu, xb = xs.w1 & 0x3FFFFu, xc = xs.w1 >> 18;
+  const uint32_t xm[kStrictMods] = {(xs.w0 >> 23) & 63u, (xs.w0 >> 16) & 127u, xa % 31u, xa % 37u, xa % 43u,
+                                    xb % 49u, xb % 61u, xb % 67u, xc % 73u, xc % 79u};
+  uint32_t b[kStrictMods];
+  b[0] = xm[0] * 64u;
+#pragma unroll
+  for (int t = 1; t < kStrictMods; ++t) {
+    const uint32_t m = kStrictMod[t];
+    b[t] = strict_sieve_off(t) + (xm[t] + td * sieve_inv64(m)) % m * kStrictNj[t];
+  }
+  StrictSieveRow r;
+#pragma unroll
+  for (int k = 0; k < kStrictRowWords; ++k) r.w[k] = b[2 * k] | b[2 * k + 1] << 16;
+  return r;
+}
+
+FC5_HD StrictSieveLane strict_sieve_lane(uint32_t d0) {
+  StrictSieveLane l;
+#pragma unroll
+  for (int t = 1; t < kStrictMods; ++t) l.j[t - 1] = d0 % kStrictNj[t];
+  return l;
 }
 
 // The strict survivors among a word's 32 grid points (before the cut).
-FC5_HD uint32_t strict_sieve_word(const uint32_t* tab, uint32_t sa, uint32_t sb, uint32_t sc, uint32_t r,
-                                  uint32_t d0) {
-  return tab[(sa & 0xFFFFu) + r] & tab[(sa >> 16) + d0 % 27u] & tab[(sb & 0xFFFFu) + d0 % 31u] &
-         tab[(sb >> 16) + d0 % 37u] & tab[sc + d0 % 43u];
+FC5_HD uint32_t strict_sieve_word(const uint32_t* tab, const StrictSieveRow& b, const StrictSieveLane& l, uint32_t r) {
+  uint32_t w = tab[(b.w[0] & 0xFFFFu) + r] & tab[(b.w[0] >> 16) + l.j[0]];
+#pragma unroll
+  for (int k = 1; k < kStrictRowWords; ++k)
+    w &= tab[(b.w[k] & 0xFFFFu) + l.j[2 * k - 1]] & tab[(b.w[k] >> 16) + l.j[2 * k]];
+  return w;
 }
 
 }  // namespace fc5

@@ -672,11 +672,12 @@ def _pair_sum_set(m):
 
 _S504, _S247 = _pair_sum_set(504), _pair_sum_set(247)
 # the strict gate's sets (option "strict_gate")
-_STRICT = {m: _pair_sum_set(m) for m in (64, 81, 31, 37, 43)}
+_STRICT = {m: _pair_sum_set(m) for m in (64, 81, 31, 37, 43, 49, 61, 67, 73, 79)}
 
 
 def strict_gate_ok(T):
-    """T passes the strict gate: T mod 64, 81, 31, 37, 43 each a sum of two sixth powers"""
+    """T passes the strict gate: T mod 64, 81, 31, 37, 43, 49, 61, 67, 73, 79 each a sum
+    of two sixth powers"""
     return all(T % m in s for m, s in _STRICT.items())
 
 

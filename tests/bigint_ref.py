@@ -14,8 +14,9 @@ def pair_sum_set(m):
 
 S504 = pair_sum_set(504)
 S247 = pair_sum_set(247)
-# the strict gate (an option of this build): the same condition modulo 64, 81, 31, 37, 43
-STRICT = {m: pair_sum_set(m) for m in (64, 81, 31, 37, 43)}
+# the strict gate (an option of this build): the same condition modulo 64, 81, 31, 37, 43,
+# 49, 61, 67, 73, 79
+STRICT = {m: pair_sum_set(m) for m in (64, 81, 31, 37, 43, 49, 61, 67, 73, 79)}
 
 
 def strict_ok(T):

@@ -48,10 +48,10 @@ int main() {
     if (got != want) ++bad;
     pass += want;
     ++n;
-    const bool swant = strict_pass_t((uint32_t)(T % kStrictA), (uint32_t)(T % 81u), (uint32_t)(T % 64u));
-    uint32_t sa, sb, sc;
-    strict_sieve_bases(strict_row((uint64_t)(X >> 64), (uint64_t)X), td, sa, sb, sc);
-    const bool sgot = (strict_sieve_word(stab, sa, sb, sc, r, d0) >> j) & 1u;
+    const bool swant = strict_pass_t((uint32_t)(T % kStrictA), (uint32_t)(T % 81u), (uint32_t)(T % 64u),
+                                     (uint32_t)(T % kStrictB), (uint32_t)(T % kStrictC));
+    const StrictSieveRow sb = strict_sieve_bases(strict_row((uint64_t)(X >> 64), (uint64_t)X), td);
+    const bool sgot = (strict_sieve_word(stab, sb, strict_sieve_lane(d0), r) >> j) & 1u;
     if (sgot != swant) ++bad;
     spass += swant;
   }
