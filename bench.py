@@ -406,6 +406,31 @@ def bench_twin(args):
                    "kernel_call_rate_frac_of_probe_only": res["calls"] / kern_s / probe_rate}
         log(f"[bench] gather ceiling on this table: {probe_rate:.4e} probes/s; kernel at "
             f"{res['calls'] / kern_s / probe_rate:.1%}")
+    strict = None
+    if not args.no_strict:
+        # the same candidates with the option strict_gate on: same counters but maybe, same solutions
+        ctx.set_option("strict_gate", 1)
+        try:
+            ctx.twin_run(cands)
+            skms = []
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                sres, shits, ssols = ctx.twin_run(cands)
+                skms.append(sres["kernel_ms"])
+            swall = time.perf_counter() - t1
+        finally:
+            ctx.set_option("strict_gate", 0)
+        for k in ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "probes"):
+            assert sres[k] == res[k], k
+        assert {(x["B"], tuple(x["terms"])) for x in ssols} == {(x["B"], tuple(x["terms"])) for x in sols}
+        sms = 1000.0 * swall / args.steps
+        strict = {"calls_per_s": sres["calls"] / (sms / 1000.0), "ms_per_step": sms, "kernel_ms": float(np.mean(skms)),
+                  "speedup_vs_default": ms_step / sms, "queries": sres["calls"] - sres["strict_out"],
+                  "strict_out": sres["strict_out"], "maybe": sres["maybe"], "default_maybe": res["maybe"],
+                  "solutions_equal": True,
+                  "option": "strict_gate=1: the strict residue gate on every key T - c4^6 - c3^6 before its query"}
+        log(f"[bench] strict gate: {sms:.1f} ms/step, kernel {strict['kernel_ms']:.1f} ms ({ms_step / sms:.2f}x "
+            f"the default); queries {strict['queries']} of {sres['calls']} calls; same solutions")
     plain = None
     if not args.no_plain:
         sub = cands[:: args.twin_plain_every]
@@ -478,6 +503,7 @@ def bench_twin(args):
                      "traffic_GBps": traffic / kern_s / 1e9 if traffic else None,
                      "traffic_frac": traffic / kern_s / 1e9 / HBM_PEAK_GBS if traffic else None},
         "gather_ceiling": ceiling,
+        "strict_gate": strict,
         "plain_form": plain,
         "cpu_baseline": cpu,
     }

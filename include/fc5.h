@@ -118,8 +118,8 @@ const char *fc5_last_error(const fc5_ctx *ctx);
  * a necessary condition for T = a^6 + b^6 that the reference does not test, so no
  * solution is lost; counters and hits are the reference's except that the xor
  * filter's false positives among the T it rejects are not reported, and
- * strict_out counts the rejected gate survivors; classes 2-5, every mode, not the
- * (6,1,7) twin), "sweep" (1: the class-5 rows of the batched kernel sieve 32 d
+ * strict_out counts the rejected gate survivors; classes 2-5, every mode, and the
+ * (6,1,7) twin's batched kernel on its keys T - c4^6 - c3^6), "sweep" (1: the class-5 rows of the batched kernel sieve 32 d
  * values per LDS word, the reference gate split into its conditions mod 8, 9, 7,
  * 13 and 19; 0: one bitmap test of T mod 124488 per grid point; same counters and
  * hits), "blocks_per_cu" (0 = occupancy limit) for the persistent grid, "rows_per_item" (e- or d-rows per work item: 4, 8, 16, 32, or
@@ -297,6 +297,8 @@ typedef struct {
   uint64_t jobs, c4_total, c4_tri_skipped, calls, gated, probes, maybe;
   uint64_t n_hits, hits_dropped, solutions, cands, items;
   double kernel_ms, wall_ms;
+  uint64_t strict_out;  /* option "strict_gate": calls whose key V = T - c4^6 - c3^6 the strict
+                           gate rejected (no query made); 0 without the option */
 } fc5_twin_result;
 
 /* a maybe-hit (Hit, solve_617_v1.cu:365: c = c3, d = c4, e = grid d) */

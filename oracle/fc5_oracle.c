@@ -1261,7 +1261,7 @@ int or_twin_run(const or_twin_cand_t *cands, int n, const or_xor_t *f, int use_g
   const u64 m504 = 504ULL * (u64)M42;
   const u64 inv42_247 = (u64)mod_inv(or_mod_pow6(42, 247), 247);
   const u64 blk = (f && f->hdr.m_cells) ? f->hdr.m_cells / 3 : 0;
-  u64 jobs = 0, c4t = 0, c4s = 0, calls = 0, gated = 0, probes = 0, maybe = 0, nh = 0;
+  u64 jobs = 0, c4t = 0, c4s = 0, calls = 0, gated = 0, probes = 0, maybe = 0, queries = 0, nh = 0;
   if (threads < 1) threads = 1;
   if (c4_div != 4) c4_div = 3;
   /* tile > 0: the reference launch's c4 tiles, zmax = ceil(max lim / tile) blocks
@@ -1286,7 +1286,7 @@ int or_twin_run(const or_twin_cand_t *cands, int n, const or_xor_t *f, int use_g
   }
   const long total = (long)row0[n];
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads) \
-    reduction(+ : jobs, c4t, c4s, calls, gated, probes, maybe)
+    reduction(+ : jobs, c4t, c4s, calls, gated, probes, maybe, queries)
   for (long row = 0; row < total; ++row) {
     int lo_c = 0, hi_c = n;
     while (hi_c - lo_c > 1) { const int mid = (lo_c + hi_c) / 2; if (row0[mid] <= (u64)row) lo_c = mid; else hi_c = mid; }
@@ -1345,6 +1345,13 @@ int or_twin_run(const or_twin_cand_t *cands, int n, const or_xor_t *f, int use_g
           probes += 3;
           const u64 c2 = (u64)c3 * c3;
           const u64 fp = tfp - c2 * c2 * c2;
+          if (use_gate & 2) { /* the strict gate on V = T4 - c3^6, exactly */
+            const u128 V = R4 / (u128)M42 - pow6_u128(c3);
+            if (!or_strict_pass((u32)(V % 64), (u32)(V % 81), (u32)(V % (31 * 37 * 43)), (u32)(V % (49 * 61 * 67)),
+                                (u32)(V % (73 * 79))))
+              continue;
+            ++queries;
+          }
           if (f && device_query(f->packed, blk, f->hdr.r, f->hdr.mix_seed, fp)) {
             ++maybe;
             u64 slot;
@@ -1365,6 +1372,7 @@ int or_twin_run(const or_twin_cand_t *cands, int n, const or_xor_t *f, int use_g
   free(row0);
   counts->jobs = jobs; counts->c4_total = c4t; counts->c4_tri_skipped = c4s;
   counts->calls = calls; counts->gated = gated; counts->probes = probes; counts->maybe = maybe;
+  counts->queries = queries;
   *n_hits = nh;
   return 0;
 }

@@ -184,6 +184,7 @@ typedef struct {
   uint64_t gated;           /* c3 values the pair gate removed */
   uint64_t probes;          /* table cell reads, 3 per call */
   uint64_t maybe;           /* maybe-hits of the table */
+  uint64_t queries;         /* use_gate & 2 (strict gate): calls that pass it and are queried */
 } or_twin_counts_t;
 typedef struct { uint32_t cand, d, c4, c3; uint64_t fp; } or_twin_hit_t;
 /* k_find4_cls5 semantics (solve_617_v1.cu:563-657).  c4_div: 3 = the reference

@@ -296,7 +296,7 @@ assert TWIN_CAND_DTYPE.itemsize == 88 and TWIN_HIT_DTYPE.itemsize == 24
 
 class TwinCounts(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "probes",
-                                          "maybe")]
+                                          "maybe", "queries")]
 
 
 _lib.or_twin_cands.argtypes = [C.c_int64, C.c_double, C.c_double, _P, C.c_int]
@@ -332,15 +332,17 @@ def tri_sets():
     return t504, t247, a.value, b.value
 
 
-def twin_run(cands, table, use_gate=True, use_tri=True, c4_div=3, threads=8, cap=1 << 20, tile=0):
+def twin_run(cands, table, use_gate=True, use_tri=True, c4_div=3, threads=8, cap=1 << 20, tile=0, strict=False):
     """k_find4_cls5 semantics: counters dict and sorted hits (tile > 0: the
-    reference launch's unbounded c4 tiles of that width)"""
+    reference launch's unbounded c4 tiles of that width; strict: the strict
+    residue gate of this build before every query)"""
     cands = np.ascontiguousarray(cands, dtype=TWIN_CAND_DTYPE)
     c = TwinCounts()
     hits = np.zeros(cap, TWIN_HIT_DTYPE)
     nh = C.c_uint64()
     _lib.or_twin_run(cands.ctypes.data_as(_P), len(cands), C.byref(table.x) if table is not None else None,
-                     1 if use_gate else 0, 1 if use_tri else 0, c4_div, C.byref(c), hits.ctypes.data_as(_P),
+                     (1 if use_gate else 0) | (2 if strict else 0), 1 if use_tri else 0, c4_div, C.byref(c),
+                     hits.ctypes.data_as(_P),
                      cap, C.byref(nh), threads, tile)
     hits = np.sort(hits[:min(nh.value, cap)], order=["cand", "d", "c4", "c3"])
     return {f: getattr(c, f) for f, _ in TwinCounts._fields_}, hits, nh.value

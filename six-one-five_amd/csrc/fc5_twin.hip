@@ -17,8 +17,11 @@
 // visits it: its c4 tiles start at lo4 + z * 2048 (z < zmax <= 32) and have
 // no end (solve_617_v1.cu:607-608,961-979), so a c4 at offset k from lo4 is
 // searched min(32, k / 2048 + 1) times; the counters and hits are multiplied
-// accordingly.
+// accordingly.  Variant S (option strict_gate): each probe batch first tests
+// the strict residue gate on V = T - c4^6 - c3^6 (the row words of T - c4^6 ride
+// in the ring with the entry) and queries only the keys that pass.
 #include "fc5_device.hpp"
+#include "fc5_strict.hpp"
 #include "fc5_twin.hpp"
 
 namespace fc5 {
@@ -30,16 +33,26 @@ constexpr u32 kTwRing = 128;
 
 struct TwRing {
   u32 *qc, *qd, *qx;  // candidate index, d, c3 | c4 << 16
+  u32 *qs0, *qs1;     // variant S: strict row words of T - c4^6
   u32 head, tail;
+  u64 nq;             // variant S: queries of this wavefront not yet added to the counters
 };
 
 // visits of a c4 at offset k from the window start under the reference tiles
 __device__ __forceinline__ u32 tile_visits(u32 k, u32 tile) { return tile ? min(32u, k / tile + 1u) : 1u; }
 
-__device__ __forceinline__ void tw_drain(const TwinArgs& a, const TwRing& q, u32 n, u32 lane) {
-  if (lane >= n) return;
-  const u32 slot = (q.head + lane) & (kTwRing - 1);
-  const u32 ci = q.qc[slot], d = q.qd[slot], x = q.qx[slot];
+template <bool S>
+__device__ __forceinline__ void tw_drain(const TwinArgs& a, TwRing& q, u32 n, u32 lane) {
+  bool go = lane < n;
+  u32 slot = 0, x = 0;
+  if (go) {
+    slot = (q.head + lane) & (kTwRing - 1);
+    x = q.qx[slot];
+    if (S) go = strict_pass(StrictRow{q.qs0[slot], q.qs1[slot]}, strict_lane_p6(x & 0xffffu));
+  }
+  if (S) q.nq += (u64)__popcll(__ballot(go));
+  if (!go) return;
+  const u32 ci = q.qc[slot], d = q.qd[slot];
   const u32 c3 = x & 0xffffu, c4 = x >> 16;
   const DevTwinCand* C = a.cands + ci;
   const u64 d3 = (u64)d * d * d;
@@ -67,8 +80,9 @@ __device__ __forceinline__ void tw_drain(const TwinArgs& a, const TwRing& q, u32
   }
 }
 
+template <bool S>
 __device__ __forceinline__ void tw_push(const TwinArgs& a, TwRing& q, bool pass, u32 ci, u32 d, u32 x,
-                                        u32 lane) {
+                                        StrictRow sr, u32 lane) {
   const u64 m = __ballot(pass);
   if (!m) return;
   if (pass) {
@@ -76,17 +90,22 @@ __device__ __forceinline__ void tw_push(const TwinArgs& a, TwRing& q, bool pass,
     q.qc[slot] = ci;
     q.qd[slot] = d;
     q.qx[slot] = x;
+    if (S) {
+      q.qs0[slot] = sr.w0;
+      q.qs1[slot] = sr.w1;
+    }
   }
   q.tail += (u32)__popcll(m);
   if (q.tail - q.head >= 64u) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    tw_drain(a, q, 64u, lane);
+    tw_drain<S>(a, q, 64u, lane);
     q.head += 64u;
   }
 }
 
+template <bool S>
 __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, const u32* s_tri, TwRing& q,
                                           u32 item, u32 lane) {
   u32 lo = 0, hi = a.n_cands;
@@ -132,6 +151,7 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
       }
     }
     u32 l3 = 1, h3 = 0, x4 = 0;
+    StrictRow s4{0u, 0u};
     if (live) {
       const u128 c6 = p6_u128(c4);
       if (T > c6) {
@@ -143,6 +163,7 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
             l3 = l;
             h3 = h;
             x4 = t4;
+            if (S) s4 = strict_row((u64)(T4 >> 64), (u64)T4);
           }
         }
       }
@@ -152,6 +173,7 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
       const int i = __builtin_ctzll(m);
       m &= m - 1;
       const u32 l = rdl(l3, i), h = rdl(h3, i), xi = rdl(x4, i), vi = rdl(visits, i);
+      const StrictRow si = S ? StrictRow{rdl(s4.w0, i), rdl(s4.w1, i)} : StrictRow{0u, 0u};
       const u32 tag = (c4b + (u32)i) << 16;
       for (u32 cb = l; cb <= h; cb += 64u) {
         const u32 c = cb + lane;
@@ -160,7 +182,7 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
         const u64 pm = __ballot(pass), im = __ballot(inn);
         n_calls += (u64)__popcll(pm) * vi;
         n_gated += (u64)__popcll(im & ~pm) * vi;
-        tw_push(a, q, pass, ci, d, c | tag, lane);
+        tw_push<S>(a, q, pass, ci, d, c | tag, si, lane);
         if (h - cb < 64u) break;  // cb + 64 could wrap past 2^32
       }
     }
@@ -176,28 +198,32 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
   }
 }
 
+template <bool S>
 __global__ __launch_bounds__(256) void k_twin_batched(TwinArgs a) {
+  constexpr u32 kSR = S ? kTwRing : 1;
   __shared__ u32 s_gate[kGateWords], s_tri[24];
   __shared__ u32 s_qc[kTwWaves][kTwRing], s_qd[kTwWaves][kTwRing], s_qx[kTwWaves][kTwRing];
+  __shared__ u32 s_qs0[kTwWaves][kSR], s_qs1[kTwWaves][kSR];
   for (u32 i = threadIdx.x; i < kGateWords; i += blockDim.x) s_gate[i] = a.gate[i];
   if (threadIdx.x < 24) s_tri[threadIdx.x] = a.tri[threadIdx.x];
   __syncthreads();
   const u32 lane = threadIdx.x & 63u;
   const u32 w = threadIdx.x >> 6;
-  TwRing q{s_qc[w], s_qd[w], s_qx[w], 0u, 0u};
+  TwRing q{s_qc[w], s_qd[w], s_qx[w], s_qs0[w], s_qs1[w], 0u, 0u, 0u};
   for (;;) {
     u32 item = 0;
     if (lane == 0) item = atomicAdd(a.item_ctr, 1u);
     item = __shfl(item, 0, 64);
     if (item >= a.n_items) break;
-    twin_item(a, s_gate, s_tri, q, item, lane);
+    twin_item<S>(a, s_gate, s_tri, q, item, lane);
   }
   if (q.tail != q.head) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    tw_drain(a, q, q.tail - q.head, lane);
+    tw_drain<S>(a, q, q.tail - q.head, lane);
   }
+  if (S && lane == 0 && q.nq) atomicAdd((unsigned long long*)&a.ctr->queries, (unsigned long long)q.nq);
 }
 
 // ---------------------------------------------------------------------------
@@ -368,13 +394,17 @@ int launch_twin_plain(const PlainTwinArgs& a, u32 ymax, u32 zmax, hipStream_t s)
 }
 
 int launch_twin_batched(const TwinArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_twin_batched, dim3(grid), dim3(64 * kTwWaves), 0, s, a);
+  if (a.strict) hipLaunchKernelGGL(k_twin_batched<true>, dim3(grid), dim3(64 * kTwWaves), 0, s, a);
+  else hipLaunchKernelGGL(k_twin_batched<false>, dim3(grid), dim3(64 * kTwWaves), 0, s, a);
   return (int)hipGetLastError();
 }
 
-int twin_blocks_per_cu() {
+int twin_blocks_per_cu(bool strict) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_twin_batched, 64 * kTwWaves, 0) != hipSuccess) return 2;
+  const hipError_t e = strict
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_twin_batched<true>, 64 * kTwWaves, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_twin_batched<false>, 64 * kTwWaves, 0);
+  if (e != hipSuccess) return 2;
   return nb > 0 ? nb : 1;
 }
 

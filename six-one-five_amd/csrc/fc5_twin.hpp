@@ -26,6 +26,7 @@ static_assert(sizeof(DevTwinCand) == 64, "DevTwinCand is 64 bytes");
 
 struct TwinCounters {
   u64 jobs, c4_total, c4_tri_skipped, calls, gated, maybe;
+  u64 queries;  // strict gate on: calls that pass it and are queried
 };
 
 struct TwinHit {
@@ -42,6 +43,7 @@ struct TwinArgs {
   u32 use_gate, use_tri;
   u32 c4_div;       // 3: the reference kernel's c4 bound; 4: exact bound
   u32 tile;         // 0: every c4 once; > 0: the reference launch's unbounded c4 tiles of this width
+  u32 strict, pad_; // strict: the strict residue gate (fc5_strict.hpp) on V = T - c4^6 - c3^6 before each query
   XorDev main;
   TwinHit* hits;
   u64 hit_cap;
@@ -73,8 +75,8 @@ struct PlainTwinArgs {
   TwinCounters* ctr;
 };
 
-int launch_twin_batched(const TwinArgs& a, int grid, hipStream_t s);
-int twin_blocks_per_cu();
+int launch_twin_batched(const TwinArgs& a, int grid, hipStream_t s);  // a.strict picks the variant
+int twin_blocks_per_cu(bool strict);
 // the reference launch shape: grid (candidate, d-row, c4 tile of 2048 or a.tile)
 int launch_twin_plain(const PlainTwinArgs& a, u32 ymax, u32 zmax, hipStream_t s);
 

@@ -114,6 +114,8 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
     items += D.rows;
   }
   if (items > kMaxItems) FAIL(ctx, FC5_E_ARG, "too many twin work items in one launch (more than 2^32 - 2^20)");
+  if (ctx->strict_gate && ctx->twin_plain)
+    FAIL(ctx, FC5_E_ARG, "the strict gate runs in the batched twin kernel only (twin_plain = 1 with strict_gate = 1)");
   int rc = ensure_twin(ctx, dev.size());
   if (rc != FC5_OK) return rc;
   if ((rc = ensure_hits(ctx)) != FC5_OK) return rc;
@@ -129,6 +131,7 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
   a.use_tri = (u32)ctx->twin_tri;
   a.c4_div = (u32)ctx->twin_c4_div;
   a.tile = (u32)ctx->twin_tile;
+  a.strict = (u32)ctx->strict_gate;
   a.main = ctx->main.dev;
   a.hits = reinterpret_cast<TwinHit*>(ctx->d_hits);
   a.hit_cap = ctx->hit_cap;
@@ -196,7 +199,7 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
         launch_twin_plain(p, ymax, zmax, ctx->stream) != 0)
       FAIL(ctx, FC5_E_HIP, "plain twin launch");
   } else if (items) {
-    const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : twin_blocks_per_cu();
+    const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : twin_blocks_per_cu(ctx->strict_gate != 0);
     const u64 want = (items + 3) / 4;  // 4 waves per workgroup
     const int grid = (int)std::max<u64>(1, std::min<u64>((u64)ctx->num_cu * (u64)bpc, want));
     if (launch_twin_batched(a, grid, ctx->stream) != 0) FAIL(ctx, FC5_E_HIP, "twin launch");
@@ -222,6 +225,7 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
   res->gated += c.gated;
   res->probes += 3 * c.calls;
   res->maybe += c.maybe;
+  if (ctx->strict_gate) res->strict_out += c.calls - c.queries;
   res->n_hits += nh;
   res->hits_dropped += nh - stored;
   res->cands += cands.size();

@@ -68,7 +68,8 @@ class Solution(C.Structure):
 class TwinResult(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "probes", "maybe", "n_hits", "hits_dropped",
-        "solutions", "cands", "items")] + [("kernel_ms", C.c_double), ("wall_ms", C.c_double)]
+        "solutions", "cands", "items")] + [("kernel_ms", C.c_double), ("wall_ms", C.c_double),
+                                           ("strict_out", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -176,14 +176,14 @@ T504 = triple_sum_set(504)
 T247 = triple_sum_set(247)
 
 
-def twin_counts(cand, contains=None, c4_div=3, use_gate=True, use_tri=True, tile=0, zmax=1):
+def twin_counts(cand, contains=None, c4_div=3, use_gate=True, use_tri=True, tile=0, zmax=1, strict=False):
     """k_find4_cls5 from its definition: every grid d, every c4 of the window,
     every c3 of the inner window; Python integers throughout.  tile = 0 visits
     each c4 once; tile > 0 runs the reference launch's zmax c4 tiles, tile z
     from lo4 + z * tile to hi4 (no tile end)."""
     base = cand["base"]
     lim, fmax1 = cand["lim"], cand["fmax1"]
-    c = dict(jobs=0, c4_total=0, c4_tri_skipped=0, calls=0, gated=0, probes=0, maybe=0)
+    c = dict(jobs=0, c4_total=0, c4_tri_skipped=0, calls=0, gated=0, probes=0, maybe=0, queries=0)
     hits = []
     ds = sorted(d for r in cand["res"] for d in range(r, fmax1 + 1, 64) if d)
     for d in ds:
@@ -224,6 +224,10 @@ def twin_counts(cand, contains=None, c4_div=3, use_gate=True, use_tri=True, tile
                     continue
                 c["calls"] += 1
                 c["probes"] += 3
+                if strict:
+                    if not strict_ok(V):
+                        continue
+                    c["queries"] += 1
                 fp = V % 2 ** 64
                 if contains is not None and contains(fp):
                     c["maybe"] += 1

@@ -129,6 +129,38 @@ def test_twin_recovers_every_known_solution(ctx, c4_div):
     assert got == sorted(tuple(k["terms"]) for k in KNOWN)
 
 
+def test_twin_strict_gate_matches_oracle_and_keeps_the_known(ctx):
+    """option strict_gate on the twin's keys V = T - c4^6 - c3^6: two whole B
+    against the oracle's strict mode (r = 8, so that false positives pass the
+    strict gate too), the 54 known solutions all recovered with it on, and the
+    plain form refusing it"""
+    say("twin strict gate")
+    t = device_table(ctx, 1500, 8)
+    cands = np.concatenate([O.twin_cands(54347), O.twin_cands(59819)])
+    ctx.set_option("strict_gate", 1)
+    try:
+        res, hits, sols = ctx.twin_run(cands.view(fc5.TWIN_CAND_DTYPE))
+        oc, oh, _ = O.twin_run(cands, t, strict=True, threads=THREADS)
+        compare(res, hits, oc, oh)
+        assert res["strict_out"] == oc["calls"] - oc["queries"] and 0 < oc["queries"] < oc["calls"]
+        assert len(oh) > 0
+        N = max(k["B"] for k in KNOWN) // 42
+        ctx.table_build(N, 48, on_gpu=True)
+        rows = []
+        for k in KNOWN:
+            c = fc5.twin_cands(k["B"])
+            rows.append(c[(c["u"] == k["u"]) & (c["w"] == k["w"])])
+        res, hits, sols = ctx.twin_run(np.concatenate(rows))
+        assert sorted(tuple(s["terms"]) for s in sols if s["gcd"] == 1) == sorted(tuple(k["terms"]) for k in KNOWN)
+        assert res["strict_out"] > 0
+        ctx.set_option("twin_plain", 1)
+        with pytest.raises(fc5.Fc5Error):
+            ctx.twin_run(np.concatenate(rows))
+    finally:
+        ctx.set_option("strict_gate", 0)
+        ctx.set_option("twin_plain", 0)
+
+
 def test_twin_whole_B_ranges_find_exactly_the_known(ctx):
     """every candidate of every Branch-A B of small ranges around known B"""
     say("twin whole-B ranges")

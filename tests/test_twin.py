@@ -103,6 +103,41 @@ def test_twin_oracle_matches_bigint_restatement(c4_div, use_gate, use_tri):
         assert [(int(h["d"]), int(h["c4"]), int(h["c3"]), int(h["fp"])) for h in hits] == sorted(whits)
 
 
+def test_twin_oracle_strict_matches_bigint_restatement():
+    """the strict residue gate on the twin's probe keys V = T4 - c3^6 (an option of
+    this build): the oracle against the big-integer restatement, and against the
+    default run (same counters but queries; hits = the default hits whose exact V
+    passes the strict gate; the known solution's key passes)"""
+    N = 700
+    tab = O.Table.build_pairs(N, 8)  # r = 8: false positives survive the strict gate too
+    rnd = random.Random(6171)
+    k = KNOWN[0]
+    cases = [small_cand(k["B"], k["u"], k["w"], fmax1=k["d"], res=[k["d"] % 64])]
+    for B in (54347, 63631):
+        c = O.twin_cands(B)
+        x = c[rnd.randrange(len(c))]
+        cases.append(small_cand(B, int(x["u"]), int(x["w"]), fmax1=rnd.randrange(100, 160)))
+    for cand, spec in cases:
+        got, hits, _ = O.twin_run(cand, tab, strict=True)
+        want, whits = br.twin_counts(spec, contains=tab.contains, strict=True)
+        assert got == want
+        assert [(int(h["d"]), int(h["c4"]), int(h["c3"]), int(h["fp"])) for h in hits] == sorted(whits)
+        base, bhits, _ = O.twin_run(cand, tab)
+        for key in ("jobs", "c4_total", "c4_tri_skipped", "calls", "gated", "probes"):
+            assert base[key] == got[key], key
+        assert base["queries"] == 0 and 0 < got["queries"] < got["calls"]
+
+        def V(h):
+            T = (spec["base"] - (21 * int(h["d"])) ** 6) // br.M42
+            return T - int(h["c4"]) ** 6 - int(h["c3"]) ** 6
+        keep = [(int(h["d"]), int(h["c4"]), int(h["c3"])) for h in bhits if br.strict_ok(V(h))]
+        assert keep == [(int(h["d"]), int(h["c4"]), int(h["c3"])) for h in hits]
+    # the known solution: its pair key is found under the strict gate
+    kc, _ = cases[0]
+    found = O.twin_run(kc, O.Table.build_pairs(N, 48), strict=True)[1]
+    assert any((int(h["d"]), int(h["c4"]), int(h["c3"])) == (k["d"], k["cores"][3], k["cores"][2]) for h in found)
+
+
 @pytest.mark.parametrize("tile", [16, 60, 2048])
 def test_twin_oracle_reference_tiles_match_bigint_restatement(tile):
     """the reference launch's unbounded c4 tiles (solve_617_v1.cu:607-608,961-979):
