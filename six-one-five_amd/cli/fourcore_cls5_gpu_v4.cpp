@@ -7,10 +7,12 @@
 //                        [--no-gate] [--stop-first] [--device K] [--load-table F]
 //                        [--save-table F] [--allow-rebuild]
 //   fourcore_cls5_gpu_v4 --check-table --N K --load-table F [--r BITS] [--save-table F]
-// Additions: --mode plain|batched|chunked (default batched), --hit-cap N, --counters,
-// --exact-table (check filter survivors against the exact sorted pair table
-// of all a^6 + b^6, a <= b <= N, built on the device)
-// (one machine-readable counters line on stdout).
+// Additions: --mode plain|batched|chunked (default batched), --hit-cap N, --counters
+// (one machine-readable counters line on stdout), --exact-table (check filter
+// survivors against the exact sorted pair table of all a^6 + b^6, a <= b <= N,
+// built on the device), --strict-gate (before each xor query also require T mod
+// 64, 81, 31, 37, 43, 49, 61, 67, 73, 79 to be sums of two sixth powers: the same
+// solutions, fewer queries, the filter false positives it rejects not reported).
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -33,7 +35,7 @@ static void usage() {
          "                            [--stop-first] [--device K]\n"
          "                            [--load-table F] [--save-table F]\n"
          "                            [--allow-rebuild] [--mode plain|batched|chunked]\n"
-         "                            [--hit-cap N] [--counters] [--exact-table]\n"
+         "                            [--hit-cap N] [--counters] [--exact-table] [--strict-gate]\n"
          "   or: fourcore_cls5_gpu_v4 --check-table --N K --load-table F\n"
          "                            [--r BITS] [--save-table F] [--allow-rebuild]\n");
 }
@@ -86,7 +88,7 @@ int main(int argc, char** argv) {
   int device = 0, N_override = 0, xor_r = 48;
   long long hit_cap = 1 << 20;
   bool do_host = false, quiet = true, use_gate = true, stop_first = false;
-  bool allow_rebuild = false, check_table = false, counters = false, exact_table = false;
+  bool allow_rebuild = false, check_table = false, counters = false, exact_table = false, strict = false;
   for (int i = 1; i < argc; ++i) {
     const std::string s = argv[i];
     auto next = [&]() { return (i + 1 < argc) ? std::string(argv[++i]) : std::string(); };
@@ -108,6 +110,7 @@ int main(int argc, char** argv) {
     else if (s == "--hit-cap") hit_cap = std::atoll(next().c_str());
     else if (s == "--counters") counters = true;
     else if (s == "--exact-table") exact_table = true;
+    else if (s == "--strict-gate") strict = true;
     else if (s == "-h" || s == "--help") { usage(); return 0; }
     else { fprintf(stderr, "unknown %s\n", s.c_str()); usage(); return 1; }
   }
@@ -195,6 +198,7 @@ int main(int argc, char** argv) {
     return 1;
   }
   fc5_set_option(ctx, "use_gate", use_gate ? 1 : 0);
+  fc5_set_option(ctx, "strict_gate", strict ? 1 : 0);
   fc5_set_option(ctx, "stop_first", stop_first ? 1 : 0);
   fc5_set_option(ctx, "hit_cap", hit_cap);
   fc5_store_header h;
@@ -244,11 +248,11 @@ int main(int argc, char** argv) {
             (unsigned long long)r.solutions, sec);
   if (counters)
     printf("COUNTERS calls=%llu neg=%llu zero=%llu gated=%llu probes=%llu maybe=%llu side_maybe=%llu "
-           "hits=%llu dropped=%llu solutions=%llu kernel_ms=%.3f\n",
+           "hits=%llu dropped=%llu solutions=%llu kernel_ms=%.3f strict_out=%llu\n",
            (unsigned long long)r.calls, (unsigned long long)r.neg, (unsigned long long)r.zero,
            (unsigned long long)r.gated, (unsigned long long)r.probes, (unsigned long long)r.maybe,
            (unsigned long long)r.side_maybe, (unsigned long long)r.n_hits, (unsigned long long)r.hits_dropped,
-           (unsigned long long)r.solutions, r.kernel_ms);
+           (unsigned long long)r.solutions, r.kernel_ms, (unsigned long long)r.strict_out);
   printf("---- done cls5-gpu: solutions=%llu units=%zu calls=%llu probes=%llu time=%.1fs ----\n",
          (unsigned long long)r.solutions, units.size(), (unsigned long long)r.calls, (unsigned long long)r.probes,
          sec);

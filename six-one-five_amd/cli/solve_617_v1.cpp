@@ -10,7 +10,9 @@
 // kernel's 3 c4^6 >= T), --ref-tiles (count and report as the reference launch
 // does, whose unbounded 2048-wide c4 tiles search part of a window more than
 // once past B ~ 515k), --gpu-build (build the table on the device instead of
-// the host's byte-identical builder), --counters (one machine-readable line).
+// the host's byte-identical builder), --counters (one machine-readable line),
+// --strict-gate (the strict residue gate on every key T - c4^6 - c3^6 before its
+// query: the same solutions, fewer queries).
 // Branch B, classes 1-4 of the (6,1,7) system, --xcheck and --bench are not
 // provided here: asking for them is an error (exit 1).
 #include <algorithm>
@@ -35,7 +37,7 @@ static void usage(const char* a0) {
           "  options: --branch-a-only --chunk K --device K --r R --hit-cap N\n"
           "           --save-table F --load-table F --quiet --no-gate --no-tri-gate\n"
           "           --check-known [--known-file F]  diff vs database after run\n"
-          "           --exact-c4 --ref-tiles --gpu-build --counters\n",
+          "           --exact-c4 --ref-tiles --gpu-build --counters --strict-gate\n",
           a0);
 }
 
@@ -52,7 +54,7 @@ int main(int argc, char** argv) {
   long long hit_cap = 1 << 20;
   std::string save_table, load_table, known_file = "617-solutions-clean.txt";
   bool quiet = false, no_gate = false, no_tri = false, check_known = false, exact_c4 = false;
-  bool gpu_build = false, counters = false, branch_b = true, ref_tiles = false;
+  bool gpu_build = false, counters = false, branch_b = true, ref_tiles = false, strict = false;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     if (a == "--quiet") quiet = true;
@@ -65,6 +67,7 @@ int main(int argc, char** argv) {
     else if (a == "--ref-tiles") ref_tiles = true;
     else if (a == "--gpu-build") gpu_build = true;
     else if (a == "--counters") counters = true;
+    else if (a == "--strict-gate") strict = true;
     else if (a == "--known-file" && i + 1 < argc) known_file = argv[++i];
     else if (a == "--chunk" && i + 1 < argc) chunk = atoll(argv[++i]);
     else if (a == "--device" && i + 1 < argc) device = atoi(argv[++i]);
@@ -159,6 +162,7 @@ int main(int argc, char** argv) {
   fc5_set_option(ctx, "twin_tri_gate", no_tri ? 0 : 1);
   fc5_set_option(ctx, "twin_c4_div", exact_c4 ? 4 : 3);
   fc5_set_option(ctx, "twin_ref_tiles", ref_tiles ? 2048 : 0);
+  fc5_set_option(ctx, "strict_gate", strict ? 1 : 0);
   bool loaded = false;
   if (!load_table.empty()) {
     const int rc = fc5_table_load(ctx, load_table.c_str(), N);
