@@ -29,8 +29,10 @@ constexpr u64 kM42 = 5489031744ULL;
 constexpr int kWaves = 4;   // wavefronts per workgroup
 constexpr u32 kRing = 128;  // per-wave survivor ring (entries)
 
-__constant__ SieveTab kSieveDev = make_sieve();  // staged into LDS by every workgroup
-__device__ u32 g_strict_sieve[kStrictWords];     // fill_strict_sieve, uploaded per device (upload_sieve_tables)
+__constant__ SieveTab kSieveDev = make_sieve();     // staged into LDS by every workgroup
+__constant__ CSieveTab kCSieveDev = make_csieve();  // the same, consecutive c (class 2-4 rows)
+__device__ u32 g_strict_sieve[kStrictWords];        // fill_strict_sieve, uploaded per device (upload_sieve_tables)
+__device__ u32 g_strict_csieve[kStrictCWords];      // fill_strict_csieve, likewise
 
 // floor(x^6 / 64) for x < 2^21
 __device__ __forceinline__ u128 dq_of(u32 x) {
@@ -361,9 +363,9 @@ __device__ __forceinline__ void cls5_item(const BatchedArgs& a, u32 u0, u32 n_un
 
 // one class 2-4 item: E d-rows of one unit, every c3 of each row's window
 // (fourcore_find_v4.cu:165-205 expansion, :401-425 window, :255-284 sweep)
-template <int E, bool S>
-__device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_units, const u32* s_gate, Ring& q,
-                                           Ring& p, u32 item, u32 lane) {
+template <int E, bool S, bool V>
+__device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_units, const u32* s_gate,
+                                           const u32* s_csieve, Ring& q, Ring& p, u32 item, u32 lane) {
   u32 lo = u0, hi = u0 + n_units;
   while (hi - lo > 1) {
     const u32 mid = (lo + hi) >> 1;
@@ -373,8 +375,9 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_u
   const u32 ui = lo;
   const DevUnit3* Up = a.units3 + ui;
   const u32 row0 = (item - Up->item_base) * (u32)E;
-  u32 d = 0, wlo = 1, whi = 0, xg = 0;
+  u32 d = 0, wlo = 1, whi = 0, xg = 0, ca = 0, cb = 0, cc = 0;
   StrictRow xs{0u, 0u};
+  StrictSieveRow cs{};
   u32 c_calls = 0, c_jobs = 0, c_fail = 0;
   if (lane < (u32)E) {
     const u32 r = row0 + lane;
@@ -397,7 +400,9 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_u
             c_calls = h - l + 1;
           }
           xg = mod_gate(X);
+          if (V) csieve_row_bases(xg, ca, cb, cc);
           if (S) xs = strict_row((u64)(X >> 64), (u64)X);
+          if (V && S) cs = strict_csieve_bases(xs);
         } else {
           c_fail = 1;  // not positive: compute_T fails (fourcore_gmp.hpp:47)
         }
@@ -405,6 +410,43 @@ __device__ __forceinline__ void find3_item(const BatchedArgs& a, u32 u0, u32 n_u
     }
   }
   u32 probes_item = 0;
+  if (V) {
+    // bit sieve over c: a lane takes 32 consecutive c of the row's window, the
+    // five reference words (LDS) and, S, the ten strict words (global memory)
+    // AND-ed; the survivors (S: strict survivors, straight to the probe ring)
+    // taken lowest bit first
+    for (int i = 0; i < E; ++i) {
+      const u32 l = rdl(wlo, i), h = rdl(whi, i);
+      if (l > h) continue;
+      const u32 di = rdl(d, i), ra = rdl(ca, i), rb = rdl(cb, i), rc = rdl(cc, i);
+      StrictSieveRow sb{};
+      if (S) {
+#pragma unroll
+        for (int k = 0; k < kStrictRowWords; ++k) sb.w[k] = rdl(cs.w[k], i);
+      }
+      const u32 nw = (h - l) / 32u + 1u;
+      for (u32 w0 = 0; w0 < nw; w0 += 64u) {
+        const u32 w = w0 + lane;
+        const u32 c0 = l + 32u * w;  // below 2^22: l, h < 2^21
+        u32 P = 0;
+        if (w < nw) {
+          P = csieve_cut(c0, h);
+          if (a.use_gate) P &= csieve_word(s_csieve, ra, rb, rc, c0);
+        }
+        probes_item += (u32)__popc(P);
+        if (S && P) P &= strict_csieve_word(g_strict_csieve, sb, c0);
+        while (__ballot(P != 0u)) {
+          const bool has = P != 0u;
+          const u32 c = c0 + (has ? (u32)__builtin_ctz(P) : 0u);
+          if (S) push_probe(a, p, has, ui | kFind3Bit, di, c, lane);
+          else ring_push<false>(a, q, p, has, ui | kFind3Bit, di, c, StrictRow{0u, 0u}, lane);
+          P &= P - 1u;
+        }
+      }
+    }
+    flush_counters<E>(a, q.acc, Up->band, lane, c_calls, 0u, 0u, c_calls, wave_sum_u32(probes_item), c_jobs, c_fail);
+    return;
+  }
   for (int i = 0; i < E; ++i) {
     const u32 l = rdl(wlo, i), h = rdl(whi, i);
     if (l > h) continue;
@@ -426,13 +468,15 @@ template <int E, bool S, bool V>
 __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
   constexpr u32 kSW = S ? kWaves : 1, kSR = S ? kRing : 1;  // variant S only: row words, probe ring
   __shared__ u32 s_gate[kGateWords];
-  __shared__ u32 s_sieve[V ? kSieveWords : 1];
+  __shared__ u32 s_sieve[V ? kSieveWords : 1], s_csieve[V ? kCSieveWords : 1];
   __shared__ u32 s_qu[kWaves][kRing], s_qd[kWaves][kRing], s_qe[kWaves][kRing];
   __shared__ u32 s_qx[kSW][kSR], s_qy[kSW][kSR], s_pu[kSW][kSR], s_pd[kSW][kSR], s_pe[kSW][kSR];
   __shared__ WaveAcc s_acc[kWaves];
   for (u32 i = threadIdx.x; i < kGateWords; i += blockDim.x) s_gate[i] = a.gate[i];
-  if (V)
+  if (V) {
     for (u32 i = threadIdx.x; i < kSieveWords; i += blockDim.x) s_sieve[i] = kSieveDev.w[i];
+    for (u32 i = threadIdx.x; i < kCSieveWords; i += blockDim.x) s_csieve[i] = kCSieveDev.w[i];
+  }
   __syncthreads();
 
   const u32 lane = threadIdx.x & 63u;
@@ -462,7 +506,7 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
       const Segment G = a.segs[lo];
       const u32 local = G.item0 + (item - G.g0);
       if (G.kind == 0) cls5_item<E, S, V>(a, G.u0, G.n_units, s_gate, s_sieve, q, p, local, lane);
-      else find3_item<E, S>(a, G.u0, G.n_units, s_gate, q, p, local, lane);
+      else find3_item<E, S, V>(a, G.u0, G.n_units, s_gate, s_csieve, q, p, local, lane);
     }
   }
   if (q.tail != q.head) {
@@ -709,10 +753,20 @@ static int launch_batched_variant(const BatchedArgs& a, int grid, int rows, hipS
 }
 
 int upload_sieve_tables() {
-  std::vector<u32> w(kStrictWords);
+  std::vector<u32> w(kStrictWords), c(kStrictCWords);
   fill_strict_sieve(w.data());
-  return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_strict_sieve), w.data(), kStrictWords * sizeof(u32), 0,
-                                hipMemcpyHostToDevice);
+  fill_strict_csieve(c.data());
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(g_strict_sieve), w.data(), kStrictWords * sizeof(u32), 0,
+                                   hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpyToSymbol(HIP_SYMBOL(g_strict_csieve), c.data(), kStrictCWords * sizeof(u32), 0,
+                          hipMemcpyHostToDevice);
+  return (int)e;
+}
+
+const u32* strict_csieve_table() {
+  void* p = nullptr;
+  return hipGetSymbolAddress(&p, HIP_SYMBOL(g_strict_csieve)) == hipSuccess ? static_cast<const u32*>(p) : nullptr;
 }
 
 int launch_cls5_batched(const BatchedArgs& args, int grid, int rows, hipStream_t s) {

@@ -191,8 +191,10 @@ constexpr int kRowsPerItem = 16;  // rows (e- or d-rows) per work item of a serv
 // launchers (return hipError_t as int)
 // rows: 4, 8, 16, 32; a.strict picks the variant with the strict gate stage
 int launch_cls5_batched(const BatchedArgs& a, int grid, int rows, hipStream_t s);
-// the strict sieve table of the batched kernel into the current device (once per device)
+// the strict sieve tables of the batched kernel into the current device (once per device)
 int upload_sieve_tables();
+// the current device's strict sieve table over consecutive c (for the twin kernel)
+const u32* strict_csieve_table();
 int launch_cls5_plain(const PlainArgs& a, int n_units, int ymax, hipStream_t s);
 int launch_find3_plain(const PlainF3Args& a, hipStream_t s);
 int launch_expand_cells(const u32* packed_words, u64 m_cells, u32 r, u64* cells, hipStream_t s);

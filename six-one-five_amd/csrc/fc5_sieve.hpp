@@ -234,4 +234,105 @@ FC5_HD uint32_t strict_sieve_word(const uint32_t* tab, const StrictSieveRow& b, 
   return w;
 }
 
+// ---------------------------------------------------------------------------
+// The same sieves over consecutive c: a class 2-4 row (T(d) - c^6, c in the
+// row's window) and a (6,1,7) twin row (T - c4^6 - c^6).  A word covers
+// c = c0 .. c0+31; T - c^6 mod m depends on the row through A = X mod m and on
+// c through c mod P, P the period of c^6 mod m: 2 for m = 8, 3 for 9, 32 for 64,
+// 27 for 81, m for the others.  Reference moduli (LDS, 622 words) and strict
+// moduli (global memory, 30595 words), tables [A][c0 mod P].
+constexpr uint32_t kCSieveOff8 = 0;     // [A = X mod 8][c0 mod 2]
+constexpr uint32_t kCSieveOff9 = 16;    // [A][c0 mod 3]
+constexpr uint32_t kCSieveOff7 = 43;    // [A][c0 mod 7]
+constexpr uint32_t kCSieveOff13 = 92;   // [A][c0 mod 13]
+constexpr uint32_t kCSieveOff19 = 261;  // [A][c0 mod 19]
+constexpr uint32_t kCSieveWords = 622;
+
+struct CSieveTab {
+  uint32_t w[kCSieveWords];
+};
+
+constexpr CSieveTab make_csieve() {
+  CSieveTab s{};
+  const uint32_t mods[5] = {8, 9, 7, 13, 19}, per[5] = {2, 3, 7, 13, 19};
+  const uint32_t offs[5] = {kCSieveOff8, kCSieveOff9, kCSieveOff7, kCSieveOff13, kCSieveOff19};
+  for (int t = 0; t < 5; ++t) {
+    const uint32_t m = mods[t], P = per[t], g = sieve_sum_mask(m);
+    for (uint32_t A = 0; A < m; ++A)
+      for (uint32_t j0 = 0; j0 < P; ++j0) {
+        uint32_t word = 0;
+        for (uint32_t j = 0; j < 32; ++j)
+          if ((g >> ((A + m - sieve_pow6((j0 + j) % P, m)) % m)) & 1u) word |= 1u << j;
+        s.w[offs[t] + A * P + j0] = word;
+      }
+  }
+  return s;
+}
+
+// the row's bases from xg = X mod 124488: ra = b8 | b9 << 16, rb = b7 | b13 << 16, rc = b19
+FC5_HD void csieve_row_bases(uint32_t xg, uint32_t& ra, uint32_t& rb, uint32_t& rc) {
+  ra = (kCSieveOff8 + (xg & 7u) * 2u) | (kCSieveOff9 + xg % 9u * 3u) << 16;
+  rb = (kCSieveOff7 + xg % 7u * 7u) | (kCSieveOff13 + xg % 13u * 13u) << 16;
+  rc = kCSieveOff19 + xg % 19u * 19u;
+}
+
+// the gate survivors among c = c0 .. c0+31 (before the window cut)
+FC5_HD uint32_t csieve_word(const uint32_t* tab, uint32_t ra, uint32_t rb, uint32_t rc, uint32_t c0) {
+  return tab[(ra & 0xFFFFu) + (c0 & 1u)] & tab[(ra >> 16) + c0 % 3u] & tab[(rb & 0xFFFFu) + c0 % 7u] &
+         tab[(rb >> 16) + c0 % 13u] & tab[rc + c0 % 19u];
+}
+
+constexpr uint32_t kStrictCPer[kStrictMods] = {32, 27, 31, 37, 43, 49, 61, 67, 73, 79};
+
+constexpr uint32_t strict_csieve_off(int i) {
+  uint32_t o = 0;
+  for (int k = 0; k < i; ++k) o += kStrictMod[k] * kStrictCPer[k];
+  return o;
+}
+constexpr uint32_t kStrictCWords = strict_csieve_off(kStrictMods);
+static_assert(kStrictCWords == 30595, "strict c-sieve size");
+
+inline void fill_strict_csieve(uint32_t* w) {
+  bool S[128];
+  for (int t = 0; t < kStrictMods; ++t) {
+    const uint32_t m = kStrictMod[t], P = kStrictCPer[t], off = strict_csieve_off(t);
+    sieve_sum_set(m, S);
+    for (uint32_t A = 0; A < m; ++A)
+      for (uint32_t j0 = 0; j0 < P; ++j0) {
+        uint32_t word = 0;
+        for (uint32_t j = 0; j < 32; ++j)
+          if (S[(A + m - sieve_pow6((j0 + j) % P, m)) % m]) word |= 1u << j;
+        w[off + A * P + j0] = word;
+      }
+  }
+}
+
+// the row's strict c-sieve bases from its strict row words (X mod each modulus)
+FC5_HD StrictSieveRow strict_csieve_bases(const StrictRow& xs) {
+  const uint32_t xa = xs.w0 & 0xFFFFu, xb = xs.w1 & 0x3FFFFu, xc = xs.w1 >> 18;
+  const uint32_t xm[kStrictMods] = {(xs.w0 >> 23) & 63u, (xs.w0 >> 16) & 127u, xa % 31u, xa % 37u, xa % 43u,
+                                    xb % 49u, xb % 61u, xb % 67u, xc % 73u, xc % 79u};
+  StrictSieveRow r;
+#pragma unroll
+  for (int k = 0; k < kStrictRowWords; ++k)
+    r.w[k] = (strict_csieve_off(2 * k) + xm[2 * k] * kStrictCPer[2 * k]) |
+             (strict_csieve_off(2 * k + 1) + xm[2 * k + 1] * kStrictCPer[2 * k + 1]) << 16;
+  return r;
+}
+
+// the strict survivors among c = c0 .. c0+31
+FC5_HD uint32_t strict_csieve_word(const uint32_t* tab, const StrictSieveRow& b, uint32_t c0) {
+  uint32_t w = ~0u;
+#pragma unroll
+  for (int k = 0; k < kStrictRowWords; ++k)
+    w &= tab[(b.w[k] & 0xFFFFu) + c0 % kStrictCPer[2 * k]] & tab[(b.w[k] >> 16) + c0 % kStrictCPer[2 * k + 1]];
+  return w;
+}
+
+// the window cut: bits j with c0 + j <= h (c0 <= h)
+FC5_HD uint32_t csieve_cut(uint32_t c0, uint32_t h) {
+  const uint32_t jm = h - c0;
+  return jm >= 31u ? 0xFFFFFFFFu : (2u << jm) - 1u;
+}
+
 }  // namespace fc5

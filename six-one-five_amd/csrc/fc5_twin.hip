@@ -17,10 +17,12 @@
 // visits it: its c4 tiles start at lo4 + z * 2048 (z < zmax <= 32) and have
 // no end (solve_617_v1.cu:607-608,961-979), so a c4 at offset k from lo4 is
 // searched min(32, k / 2048 + 1) times; the counters and hits are multiplied
-// accordingly.  Variant S (option strict_gate): each probe batch first tests
-// the strict residue gate on V = T - c4^6 - c3^6 (the row words of T - c4^6 ride
-// in the ring with the entry) and queries only the keys that pass.
+// accordingly.  Variant S (option strict_gate): the c3 rows run as a bit sieve
+// (fc5_sieve.hpp: 32 consecutive c3 per lane, the pair-sum gate's five table
+// words from LDS and the strict gate's ten from global memory, AND-ed), and only
+// the strict survivors enter the ring.
 #include "fc5_device.hpp"
+#include "fc5_sieve.hpp"
 #include "fc5_strict.hpp"
 #include "fc5_twin.hpp"
 
@@ -29,30 +31,21 @@ namespace fc5 {
 namespace {
 
 constexpr int kTwWaves = 4;
+__constant__ CSieveTab kTwCSieve = make_csieve();  // staged into LDS by the strict variant
 constexpr u32 kTwRing = 128;
 
 struct TwRing {
   u32 *qc, *qd, *qx;  // candidate index, d, c3 | c4 << 16
-  u32 *qs0, *qs1;     // variant S: strict row words of T - c4^6
   u32 head, tail;
-  u64 nq;             // variant S: queries of this wavefront not yet added to the counters
 };
 
 // visits of a c4 at offset k from the window start under the reference tiles
 __device__ __forceinline__ u32 tile_visits(u32 k, u32 tile) { return tile ? min(32u, k / tile + 1u) : 1u; }
 
-template <bool S>
 __device__ __forceinline__ void tw_drain(const TwinArgs& a, TwRing& q, u32 n, u32 lane) {
-  bool go = lane < n;
-  u32 slot = 0, x = 0;
-  if (go) {
-    slot = (q.head + lane) & (kTwRing - 1);
-    x = q.qx[slot];
-    if (S) go = strict_pass(StrictRow{q.qs0[slot], q.qs1[slot]}, strict_lane_p6(x & 0xffffu));
-  }
-  if (S) q.nq += (u64)__popcll(__ballot(go));
-  if (!go) return;
-  const u32 ci = q.qc[slot], d = q.qd[slot];
+  if (lane >= n) return;
+  const u32 slot = (q.head + lane) & (kTwRing - 1);
+  const u32 ci = q.qc[slot], d = q.qd[slot], x = q.qx[slot];
   const u32 c3 = x & 0xffffu, c4 = x >> 16;
   const DevTwinCand* C = a.cands + ci;
   const u64 d3 = (u64)d * d * d;
@@ -80,9 +73,7 @@ __device__ __forceinline__ void tw_drain(const TwinArgs& a, TwRing& q, u32 n, u3
   }
 }
 
-template <bool S>
-__device__ __forceinline__ void tw_push(const TwinArgs& a, TwRing& q, bool pass, u32 ci, u32 d, u32 x,
-                                        StrictRow sr, u32 lane) {
+__device__ __forceinline__ void tw_push(const TwinArgs& a, TwRing& q, bool pass, u32 ci, u32 d, u32 x, u32 lane) {
   const u64 m = __ballot(pass);
   if (!m) return;
   if (pass) {
@@ -90,24 +81,25 @@ __device__ __forceinline__ void tw_push(const TwinArgs& a, TwRing& q, bool pass,
     q.qc[slot] = ci;
     q.qd[slot] = d;
     q.qx[slot] = x;
-    if (S) {
-      q.qs0[slot] = sr.w0;
-      q.qs1[slot] = sr.w1;
-    }
   }
   q.tail += (u32)__popcll(m);
   if (q.tail - q.head >= 64u) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    tw_drain<S>(a, q, 64u, lane);
+    tw_drain(a, q, 64u, lane);
     q.head += 64u;
   }
 }
 
+// variant S: the sieve's per-lane counts (calls, gated, queries), reduced at the end of the launch
+struct TwLaneCounts {
+  u64 calls, gated, queries;
+};
+
 template <bool S>
-__device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, const u32* s_tri, TwRing& q,
-                                          u32 item, u32 lane) {
+__device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, const u32* s_tri,
+                                          const u32* s_csieve, TwRing& q, TwLaneCounts& lc, u32 item, u32 lane) {
   u32 lo = 0, hi = a.n_cands;
   while (hi - lo > 1) {
     const u32 mid = (lo + hi) >> 1;
@@ -173,8 +165,33 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
       const int i = __builtin_ctzll(m);
       m &= m - 1;
       const u32 l = rdl(l3, i), h = rdl(h3, i), xi = rdl(x4, i), vi = rdl(visits, i);
-      const StrictRow si = S ? StrictRow{rdl(s4.w0, i), rdl(s4.w1, i)} : StrictRow{0u, 0u};
       const u32 tag = (c4b + (u32)i) << 16;
+      if (S) {
+        u32 ra, rb, rc;
+        csieve_row_bases(xi, ra, rb, rc);
+        const StrictSieveRow sb = strict_csieve_bases(StrictRow{rdl(s4.w0, i), rdl(s4.w1, i)});
+        const u32 nw = (h - l) / 32u + 1u;
+        for (u32 w0 = 0; w0 < nw; w0 += 64u) {
+          const u32 w = w0 + lane;
+          const u32 c0 = l + 32u * w;  // below 2^17: l, h <= lim < 2^16
+          u32 P = 0, cut = 0;
+          if (w < nw) {
+            cut = csieve_cut(c0, h);
+            P = a.use_gate ? cut & csieve_word(s_csieve, ra, rb, rc, c0) : cut;
+          }
+          lc.calls += (u64)__popc(P) * vi;
+          lc.gated += (u64)(__popc(cut) - __popc(P)) * vi;
+          if (P) P &= strict_csieve_word(a.scsieve, sb, c0);
+          lc.queries += (u64)__popc(P) * vi;
+          while (__ballot(P != 0u)) {
+            const bool has = P != 0u;
+            const u32 c = c0 + (has ? (u32)__builtin_ctz(P) : 0u);
+            tw_push(a, q, has, ci, d, c | tag, lane);
+            P &= P - 1u;
+          }
+        }
+        continue;
+      }
       for (u32 cb = l; cb <= h; cb += 64u) {
         const u32 c = cb + lane;
         const bool inn = c <= h;
@@ -182,7 +199,7 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
         const u64 pm = __ballot(pass), im = __ballot(inn);
         n_calls += (u64)__popcll(pm) * vi;
         n_gated += (u64)__popcll(im & ~pm) * vi;
-        tw_push<S>(a, q, pass, ci, d, c | tag, si, lane);
+        tw_push(a, q, pass, ci, d, c | tag, lane);
         if (h - cb < 64u) break;  // cb + 64 could wrap past 2^32
       }
     }
@@ -200,30 +217,38 @@ __device__ __forceinline__ void twin_item(const TwinArgs& a, const u32* s_gate, 
 
 template <bool S>
 __global__ __launch_bounds__(256) void k_twin_batched(TwinArgs a) {
-  constexpr u32 kSR = S ? kTwRing : 1;
-  __shared__ u32 s_gate[kGateWords], s_tri[24];
+  __shared__ u32 s_gate[kGateWords], s_tri[24], s_csieve[S ? kCSieveWords : 1];
   __shared__ u32 s_qc[kTwWaves][kTwRing], s_qd[kTwWaves][kTwRing], s_qx[kTwWaves][kTwRing];
-  __shared__ u32 s_qs0[kTwWaves][kSR], s_qs1[kTwWaves][kSR];
   for (u32 i = threadIdx.x; i < kGateWords; i += blockDim.x) s_gate[i] = a.gate[i];
   if (threadIdx.x < 24) s_tri[threadIdx.x] = a.tri[threadIdx.x];
+  if (S)
+    for (u32 i = threadIdx.x; i < kCSieveWords; i += blockDim.x) s_csieve[i] = kTwCSieve.w[i];
   __syncthreads();
   const u32 lane = threadIdx.x & 63u;
   const u32 w = threadIdx.x >> 6;
-  TwRing q{s_qc[w], s_qd[w], s_qx[w], s_qs0[w], s_qs1[w], 0u, 0u, 0u};
+  TwRing q{s_qc[w], s_qd[w], s_qx[w], 0u, 0u};
+  TwLaneCounts lc{0, 0, 0};
   for (;;) {
     u32 item = 0;
     if (lane == 0) item = atomicAdd(a.item_ctr, 1u);
     item = __shfl(item, 0, 64);
     if (item >= a.n_items) break;
-    twin_item<S>(a, s_gate, s_tri, q, item, lane);
+    twin_item<S>(a, s_gate, s_tri, s_csieve, q, lc, item, lane);
   }
   if (q.tail != q.head) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    tw_drain<S>(a, q, q.tail - q.head, lane);
+    tw_drain(a, q, q.tail - q.head, lane);
   }
-  if (S && lane == 0 && q.nq) atomicAdd((unsigned long long*)&a.ctr->queries, (unsigned long long)q.nq);
+  if (S) {
+    const u64 calls = wave_sum(lc.calls), gated = wave_sum(lc.gated), queries = wave_sum(lc.queries);
+    if (lane == 0) {
+      if (calls) atomicAdd((unsigned long long*)&a.ctr->calls, (unsigned long long)calls);
+      if (gated) atomicAdd((unsigned long long*)&a.ctr->gated, (unsigned long long)gated);
+      if (queries) atomicAdd((unsigned long long*)&a.ctr->queries, (unsigned long long)queries);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -44,6 +44,7 @@ struct TwinArgs {
   u32 c4_div;       // 3: the reference kernel's c4 bound; 4: exact bound
   u32 tile;         // 0: every c4 once; > 0: the reference launch's unbounded c4 tiles of this width
   u32 strict, pad_; // strict: the strict residue gate (fc5_strict.hpp) on V = T - c4^6 - c3^6 before each query
+  const u32* scsieve;  // strict: the strict sieve table over consecutive c (strict_csieve_table)
   XorDev main;
   TwinHit* hits;
   u64 hit_cap;

@@ -132,6 +132,8 @@ int twin_launch(fc5_ctx* ctx, const std::vector<TwinCand>& cands, const std::vec
   a.c4_div = (u32)ctx->twin_c4_div;
   a.tile = (u32)ctx->twin_tile;
   a.strict = (u32)ctx->strict_gate;
+  a.scsieve = strict_csieve_table();
+  if (a.strict && !a.scsieve) FAIL(ctx, FC5_E_HIP, "strict sieve table not on the device");
   a.main = ctx->main.dev;
   a.hits = reinterpret_cast<TwinHit*>(ctx->d_hits);
   a.hit_cap = ctx->hit_cap;

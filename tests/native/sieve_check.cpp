@@ -2,8 +2,9 @@
 // random rows X, residues r and grid positions k, the sieve bit of
 // d = r + 64k equals the reference gate on the exact T = X - floor(d^6/64)
 // (T mod 504 and T mod 247 sums of two sixth powers, sets by brute force),
-// the strict sieve bit equals the strict gate (fc5_strict.hpp) on that T, and
-// the cut mask keeps exactly the d <= dl of a word.
+// the strict sieve bit equals the strict gate (fc5_strict.hpp) on that T, the
+// consecutive-c sieves (class 2-4 and twin rows, T = X - c^6) do the same for
+// their words, and the cut masks keep exactly the grid points inside.
 #include <cstdio>
 
 #include "fc5_sieve.hpp"
@@ -55,6 +56,29 @@ int main() {
     if (sgot != swant) ++bad;
     spass += swant;
   }
+  // consecutive c: T = X - c^6, X random, c = c0 + j
+  constexpr CSieveTab ctab = make_csieve();
+  static uint32_t sctab[kStrictCWords];
+  fill_strict_csieve(sctab);
+  for (int it = 0; it < 2000000; ++it) {
+    const uint32_t c = (uint32_t)(next() % (1u << 21));
+    const uint64_t c3 = (uint64_t)c * c * c;
+    const u128 T = (((u128)(next() >> 12) << 64) | next()) + 1u;
+    const u128 X = T + (u128)c3 * c3;
+    const uint32_t j = (uint32_t)(next() % 32u) % (c + 1u), c0 = c - j;  // c is bit j of the word at c0
+    const bool want = g504[(uint32_t)(T % 504u)] && g247[(uint32_t)(T % 247u)];
+    uint32_t ra, rb, rc;
+    csieve_row_bases((uint32_t)(X % 124488u), ra, rb, rc);
+    if ((((csieve_word(ctab.w, ra, rb, rc, c0) >> j) & 1u) != 0) != want) ++bad;
+    const bool swant = strict_pass_t((uint32_t)(T % kStrictA), (uint32_t)(T % 81u), (uint32_t)(T % 64u),
+                                     (uint32_t)(T % kStrictB), (uint32_t)(T % kStrictC));
+    const StrictSieveRow sb = strict_csieve_bases(strict_row((uint64_t)(X >> 64), (uint64_t)X));
+    if ((((strict_csieve_word(sctab, sb, c0) >> j) & 1u) != 0) != swant) ++bad;
+  }
+  for (uint32_t c0 = 0; c0 < 1000; c0 += 7)
+    for (uint32_t h = c0; h < c0 + 80; ++h)
+      for (uint32_t j = 0; j < 32; ++j)
+        if ((((csieve_cut(c0, h) >> j) & 1u) != 0) != (c0 + j <= h)) ++bad;
   // the cut: bit j of sieve_cut(d0, dl) is set iff d0 + 64j <= dl
   for (int it = 0; it < 1000000; ++it) {
     const uint32_t d0 = (uint32_t)(next() % 3000000u), dl = (uint32_t)(next() % 3000000u);

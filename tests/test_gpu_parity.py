@@ -620,7 +620,8 @@ def test_cli_cls5_units_bbj_buc_and_exact_table(tmp_path):
 
 def test_cli_cls5_strict_gate(tmp_path):
     """the cls5 CLI with --strict-gate: its counters are the oracle's strict peel
-    on the same units and table bytes (r=8, so that false positives pass)"""
+    on the same units and table bytes (r=16, the lowest width the CLI's FPR smoke
+    accepts)"""
     import re
     import subprocess
     from conftest import BIN
@@ -628,18 +629,18 @@ def test_cli_cls5_strict_gate(tmp_path):
     p = str(tmp_path / "u.buc")
     subprocess.run([os.path.join(BIN, "fourcore_hunt_v3"), "--lo", "100001", "--hi", "102000", "--classes", "5",
                     "--emit-units", p], check=True, capture_output=True)
-    r = subprocess.run([os.path.join(BIN, "fourcore_cls5_gpu_v4"), "--units", p, "--N", "2428", "--r", "8",
+    r = subprocess.run([os.path.join(BIN, "fourcore_cls5_gpu_v4"), "--units", p, "--N", "2428", "--r", "16",
                         "--quiet", "--counters", "--strict-gate"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-1000:]
     line = [l for l in r.stdout.splitlines() if l.startswith("COUNTERS")][0]
     got = {k: int(float(v)) for k, v in re.findall(r"(\w+)=([0-9.]+)", line)}
     Bs, us = _units(100001, 102000)
     arr, ok = O.pack_units(Bs, us)
-    oc, oh, _ = O.peel(arr, len(Bs), O.Table.build_pairs(2428, 8), strict=True,
+    oc, oh, _ = O.peel(arr, len(Bs), O.Table.build_pairs(2428, 16), strict=True,
                        threads=min(16, os.cpu_count() or 8))
     for k in ("calls", "neg", "zero", "gated", "probes", "maybe"):
         assert got[k] == oc[k], k
-    assert got["strict_out"] == oc["probes"] - oc["queries"] > 0 and got["hits"] == len(oh) > 0
+    assert got["strict_out"] == oc["probes"] - oc["queries"] > 0 and got["hits"] == len(oh)
 
 
 def test_cli_xor_build_save_on_device(ctx, tmp_path):
