@@ -3,15 +3,17 @@
 // k_cls5_batched  persistent kernel fed by a device work queue (one item =
 //                 one unit x E e-rows x all d, or E d-rows of a class 2-4
 //                 unit; the launch's items are segments of unit arenas, so
-//                 one launch can serve several bands).  Per point it tests the
-//                 sixth-power residue gate in T-space with 32-bit VALU work
-//                 and an LDS bitmap; survivors are compacted per wavefront
+//                 one launch can serve several bands).  Variant V (option
+//                 sweep = 1, the default) tests the sixth-power residue gate in
+//                 T-space as a bit sieve, 32 grid points per lane and LDS table
+//                 word (fc5_sieve.hpp); variant !V tests one point per lane
+//                 against an LDS bitmap.  Survivors are compacted per wavefront
 //                 (ballot + mbcnt into an LDS ring) and probed 64 at a time:
 //                 fingerprint, 3 mixes, 3 exact remainders, 3 HBM gathers.
-//                 With the strict gate on (variant S), each batch of 64 gate
-//                 survivors first passes the strict residue gate
-//                 (fc5_strict.hpp) and is compacted again into a second ring;
-//                 only its survivors are probed.
+//                 Variant S (option strict_gate): V applies the strict gate as
+//                 ten more sieve words and pushes its survivors straight to
+//                 the probe ring; !V tests each batch of 64 gate survivors
+//                 (fc5_strict.hpp) and compacts the passing ones into it.
 // k_cls5_plain    the reference kernel shape (fourcore_cls5_gpu_v4.cu:353-410):
 //                 block per (unit, e), threads over d, per-point 192-bit
 //                 sixth power, subtract-with-borrow and modular gate.

@@ -189,3 +189,31 @@ def test_serving_hit_overflow_is_charged_per_band(ctx):
             ctx.serve_stop()
     finally:
         ctx.set_option("hit_cap", 1 << 20)
+
+
+def test_serving_with_the_strict_gate_matches_alone(ctx):
+    """the option strict_gate on a served mix: every band's counters (strict_out
+    included) and hits are its result alone, and its hit list is the default
+    one restricted to the keys that pass the strict gate"""
+    say("serving: strict gate, 6 bands alone vs mixed")
+    bands = make_bands(6150, 6, 1000000, 1200000, 200, 2000)
+    base = [ctx.run(u, fc5.MODE_BATCHED, cls=c) for c, u in bands]
+    ctx.set_option("strict_gate", 1)
+    try:
+        alone = [ctx.run(u, fc5.MODE_BATCHED, cls=c) for c, u in bands]
+        ctx.serve_start(max_bands=3, round_points=1e9)
+        try:
+            ids = [ctx.serve_submit(u, c) for c, u in bands]
+            for i in ids:
+                assert ctx.serve_wait(i, 120.0)
+            for i, a, b0 in zip(ids, alone, base):
+                r = ctx.serve_result(i)
+                same(a, r)
+                assert r.result["strict_out"] == a.result["strict_out"] > 0
+                assert r.result["probes"] == b0.result["probes"] and r.result["n_hits"] <= b0.result["n_hits"]
+                assert set(r.hits.tolist()) <= set(b0.hits.tolist())
+                ctx.serve_release(i)
+        finally:
+            ctx.serve_stop()
+    finally:
+        ctx.set_option("strict_gate", 0)
