@@ -406,6 +406,19 @@ int fc5_measure_read(fc5_ctx *ctx, uint64_t bytes, double *ms);
  * occupancy: kind 0 = 32-bit (v_xad_u32), 1 = v_mad_u64_u32, 2 = 128-bit
  * add-with-carry chains, 3 = one v_mad_u64_u32 per 13 32-bit operations */
 int fc5_measure_valu(fc5_ctx *ctx, int kind, double *instr_per_s);
+/* partitioned probing of n <= 2^32-1 pseudo-random keys (those of
+ * fc5_measure_gathers) against the main table: keys radix-partitioned by table
+ * slice, each slice's low 16 cell bits in LDS, candidates fully queried.
+ * ms[9]: first partition, fine partition and lookup of cells 0, 1, 2 (six
+ * stages), full queries, then the direct queries of the same keys.
+ * counts[8]: accepted and the sum of their hashes, the same for the direct
+ * queries, full queries made, records lost (must be 0), slices per third,
+ * coarse buckets per third */
+int fc5_measure_partitioned(fc5_ctx *ctx, uint64_t n, double *ms, uint64_t *counts);
+/* the last batched run with partitioned probing (option probe_mode):
+ * ms[3] host-timed emit launches, partition pipelines and identify launch;
+ * counts[4] pieces, accepted hashes, full queries, emit chunks filled */
+int fc5_part_stats(fc5_ctx *ctx, double *ms, uint64_t *counts);
 
 /* ---- host helpers shared by the CLIs (no device use) ---- */
 /* Stage-1 emission over [lo,hi] for classes in cls_mask (bit c = class c),

@@ -88,6 +88,7 @@ extern "C" void fc5_ctx_destroy(fc5_ctx* ctx) {
   serve_shutdown(ctx);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  free_part(ctx);
   free_table(ctx->main);
   free_table(ctx->side);
   for (void* p : {(void*)ctx->d_gate, (void*)ctx->d_gate_ones, (void*)ctx->d_ndg, (void*)ctx->d_hits,
@@ -154,6 +155,13 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) try {
   } else if (k == "twin_c4_div") {
     if (v != 3 && v != 4) FAIL(ctx, FC5_E_ARG, "twin_c4_div must be 3 or 4");
     ctx->twin_c4_div = (int)v;
+  } else if (k == "probe_mode") {
+    if (v < -1 || v > 1) FAIL(ctx, FC5_E_ARG, "probe_mode must be -1 (by table), 0 (gathers) or 1 (partitioned)");
+    ctx->probe_mode = (int)v;
+  } else if (k == "part_records") {
+    if (v < (int64_t)kEmitChunk || v > (int64_t)0xFFFFFFFF) FAIL(ctx, FC5_E_ARG, "part_records must be 4096..2^32-1");
+    ctx->part_records = (u64)v;
+    free_part(ctx);
   } else if (k == "blocks_per_cu") {
     if (v < 0 || v > 64) FAIL(ctx, FC5_E_ARG, "blocks_per_cu out of range");
     ctx->blocks_per_cu = (int)v;
@@ -349,6 +357,7 @@ extern "C" int fc5_table_load(fc5_ctx* ctx, const char* path, int expected_N) tr
     free_table(nt);
     FAIL(ctx, FC5_E_FPR, "xor FPR smoke too high (" + std::to_string(fpr) + ") - refuse to search");
   }
+  free_part(ctx);
   free_table(ctx->main);
   ctx->main = nt;
   ctx->host_main_valid = false;
@@ -378,6 +387,7 @@ extern "C" int fc5_table_build(fc5_ctx* ctx, int N, int r, const uint64_t* extra
   NOT_SERVING(ctx);
   if (on_gpu) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    free_part(ctx);
     free_table(ctx->main);
     ctx->host_main = XorTable{};
     ctx->host_main_valid = false;
@@ -1235,7 +1245,8 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   BatchedArgs a = batched_args(ctx, b);
   const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(b->rows, ctx->strict_gate != 0, ctx->sweep != 0);
   const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)(b->n_items + b->n_items3)));
-  const bool graph = mode == FC5_MODE_BATCHED && any_items && ctx->use_graph && b->irregular.empty() &&
+  const bool part = mode == FC5_MODE_BATCHED && any_items && use_partitioned(ctx);
+  const bool graph = mode == FC5_MODE_BATCHED && any_items && ctx->use_graph && !part && b->irregular.empty() &&
                      b->irregular3.empty();
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (graph) {
@@ -1248,6 +1259,9 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
     if (batched) {
       if (mode == FC5_MODE_CHUNKED) {
         rc = run_chunked(ctx, b, a, bpc);
+        if (rc != FC5_OK) return rc;
+      } else if (part) {
+        rc = run_partitioned(ctx, a, grid, b->rows);
         if (rc != FC5_OK) return rc;
       } else if (any_items && launch_cls5_batched(a, grid, b->rows, ctx->stream) != 0) {
         FAIL(ctx, FC5_E_HIP, "batched launch");

@@ -18,6 +18,8 @@
 
 namespace fc5 {
 
+struct PartBufs;  // partitioned probing buffers (fc5_part_api.cpp)
+
 constexpr u64 kGateModU = 124488;
 constexpr u32 kGateWordsU = (124488 + 31) / 32;
 
@@ -161,6 +163,16 @@ struct fc5_ctx {
   int verify_table = 0;  // option: answer the exact check from d_exact
   // resident serving loop (fc5_serve.cpp); while set, only fc5_serve_* may run
   struct fc5_server* server = nullptr;
+  // partitioned probing (fc5_part_api.cpp): option probe_mode (0 a gather per
+  // cell of every probe, 1 partitioned, -1 by table: partitioned for tables of
+  // 2^28 cells or more with r >= 24), the emit buffer's capacity in hashes,
+  // the buffers (the low-bits table belongs to the current main table), and
+  // the last partitioned run's stage times and counts
+  int probe_mode = -1;
+  u64 part_records = 3ull << 30;
+  PartBufs* part = nullptr;
+  double part_ms[3] = {0, 0, 0};  // emit launches, partition pipeline, identify launch
+  u64 part_counts[4] = {0, 0, 0, 0};  // pieces, accepted hashes, full queries, emit chunks filled
 };
 
 // No C++ exception may cross the C ABI: every entry point catches what its
@@ -254,4 +266,11 @@ void free_exact(fc5_ctx* ctx);
 int noalloc_setup(int device);
 // twin buffers (fc5_twin_api.cpp)
 void free_twin(fc5_ctx* ctx);
+// partitioned probing (fc5_part_api.cpp): whether a batched run of this
+// context probes by partition, the run itself (counters and hits land where
+// the persistent launch puts them; counters, hit slot and work counter zero
+// on entry), and the buffers' release (also when the main table changes)
+bool use_partitioned(const fc5_ctx* ctx);
+int run_partitioned(fc5_ctx* ctx, const BatchedArgs& a, int grid, int rows);
+void free_part(fc5_ctx* ctx);
 }  // namespace fc5

@@ -53,6 +53,7 @@ constexpr u32 kFind3Bit = 0x80000000u;  // ring entry unit field: find3 row
 struct WaveAcc {
   u64 v[8];  // calls, neg, zero, pos, probes, jobs, failT, queries
   u32 band, live;
+  u32 chunk, cfill;  // probe mode 1: the wavefront's emit chunk (~0u: none yet) and its fill
 };
 
 // lane 0: add the sums to the band's counters and clear them
@@ -82,17 +83,62 @@ struct Ring {
   WaveAcc* acc;       // the wavefront's counter sums
 };
 
+// probe mode 1: append the hashes of lanes < n to the wavefront's emit chunk
+// (a new chunk when this one cannot take them all); without a free chunk,
+// query here and keep the accepted hashes
+__device__ __forceinline__ void emit_hashes(const BatchedArgs& a, WaveAcc* A, u64 fp, u32 n, u32 lane) {
+  const u64 h = mix64(fp + a.main.seed);
+  u32 ch = 0, cf = 0;
+  if (lane == 0) {
+    if (A->chunk == ~0u || (A->chunk < a.n_chunks && A->cfill + n > kEmitChunk)) {
+      if (A->chunk != ~0u) a.chunk_fill[A->chunk] = A->cfill;
+      A->chunk = atomicAdd(a.chunk_ctr, 1u);
+      A->cfill = 0;
+    }
+    ch = A->chunk;
+    cf = A->cfill;
+    if (ch < a.n_chunks) A->cfill = cf + n;
+  }
+  ch = (u32)__shfl((int)ch, 0, 64);
+  cf = (u32)__shfl((int)cf, 0, 64);
+  if (ch < a.n_chunks) {
+    if (lane < n) a.flat[(u64)ch * kEmitChunk + cf + lane] = h;
+  } else if (lane < n && d_query(a.main, fp)) {
+    const u32 k = atomicAdd(a.acc_n, 1u);
+    if (k < a.acc_cap)
+      a.acc_h[k] = h;
+    else
+      atomicAdd(a.acc_lost, 1u);
+  }
+}
+
+// probe mode 2: is hash h in the accepted set
+__device__ __forceinline__ bool in_set(const BatchedArgs& a, u64 h) {
+  const u32 bit = (u32)(h >> 49);
+  if (a.set_n == 0 || !((a.set_bits[bit >> 5] >> (bit & 31u)) & 1u)) return false;
+  u32 lo = 0, hi = a.set_n;
+  while (lo < hi) {
+    const u32 mid = (lo + hi) >> 1;
+    if (a.set_h[mid] < h)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < a.set_n && a.set_h[lo] == h;
+}
+
 // probe the ring's oldest n entries (n <= 64, one per lane).  S: also count
 // the queries per band (into the wavefront's sums; entries of a second band
 // in one batch by an atomic each).
 template <bool S>
 __device__ __forceinline__ void drain(const BatchedArgs& a, const Ring& q, u32 n, u32 lane) {
-  u32 band = 0;
+  u32 band = 0, uid = 0, d = 0, e = 0;
+  u64 fp = 0;
   if (lane < n) {
     const u32 slot = (q.head + lane) & (kRing - 1);
-    const u32 tag = q.qu[slot], d = q.qd[slot], e = q.qe[slot];
-    u64 fp;
-    u32 uid;
+    const u32 tag = q.qu[slot];
+    d = q.qd[slot];
+    e = q.qe[slot];
     if (tag & kFind3Bit) {  // classes 2-4: V = Q - floor(d^6/mod) - c^6, c in e
       const DevUnit3* U = a.units3 + (tag & ~kFind3Bit);
       const u64 d3 = (u64)d * d * d;
@@ -111,7 +157,14 @@ __device__ __forceinline__ void drain(const BatchedArgs& a, const Ring& q, u32 n
       uid = U->uid;
       band = U->band;
     }
-    u32 flags = d_query(a.main, fp) ? 1u : 0u;
+  }
+  if (!S && a.pmode == 1u) {
+    emit_hashes(a, q.acc, fp, n, lane);
+    return;
+  }
+  if (lane < n) {
+    const bool main_hit = a.pmode == 2u ? in_set(a, mix64(fp + a.main.seed)) : d_query(a.main, fp);
+    u32 flags = main_hit ? 1u : 0u;
     if (a.side.present && d_query(a.side, fp)) flags |= 2u;
     if (flags) {
       Counters* c = a.ctr + band;
@@ -487,6 +540,8 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
   WaveAcc* acc = &s_acc[w];
   if (lane == 0) {
     acc->live = 0;
+    acc->chunk = ~0u;
+    acc->cfill = 0;
     for (int i = 0; i < 8; ++i) acc->v[i] = 0;
   }
   Ring q{s_qu[w], s_qd[w], s_qe[w], s_qx[ws], s_qy[ws], 0u, 0u, acc};
@@ -494,7 +549,11 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
   const u32 grab = a.grab ? a.grab : 1u;  // consecutive items per work-counter update
   for (;;) {
     u32 item0 = 0;
-    if (lane == 0) item0 = atomicAdd(a.item_ctr, grab);
+    // probe mode 1: once this wavefront's latest chunk (handed out in order)
+    // is past the limit, the emit chunks are nearly used up: the work items
+    // left go to the next launch
+    if (lane == 0)
+      item0 = (a.pmode == 1u && acc->chunk != ~0u && acc->chunk >= a.chunk_limit) ? ~0u : atomicAdd(a.item_ctr, grab);
     item0 = __shfl(item0, 0, 64);
     if (item0 >= a.total) break;
     const u32 iend = a.total - item0 > grab ? item0 + grab : a.total;
@@ -524,7 +583,10 @@ __global__ __launch_bounds__(256) void k_cls5_batched(BatchedArgs a) {
       p.head += n;
     }
   }
-  if (lane == 0) acc_flush(a, acc);
+  if (lane == 0) {
+    acc_flush(a, acc);
+    if (a.pmode == 1u && acc->chunk < a.n_chunks) a.chunk_fill[acc->chunk] = acc->cfill;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -777,6 +839,7 @@ int launch_cls5_batched(const BatchedArgs& args, int grid, int rows, hipStream_t
   BatchedArgs a = args;
   const u64 per_wave = (u64)a.total / ((u64)(grid > 0 ? grid : 1) * kWaves);
   a.grab = per_wave >= 256 ? 4u : per_wave >= 64 ? 2u : 1u;
+  if (a.pmode > 2u || (a.pmode && a.strict)) return (int)hipErrorInvalidValue;
   if (a.strict) return a.sieve ? launch_batched_variant<true, true>(a, grid, rows, s)
                                : launch_batched_variant<true, false>(a, grid, rows, s);
   return a.sieve ? launch_batched_variant<false, true>(a, grid, rows, s)

@@ -122,7 +122,28 @@ struct BatchedArgs {
   u64 inv729;  // 729^-1 mod 2^64
   const DevUnit3* units3;  // class 2-4 unit arena
   u32 grab, pad_;           // work items per work-counter update (set by launch_cls5_batched)
+  // Probe mode (fc5_part.hip).  0: every probe is an xor query of the main
+  // table.  1 (emit): each probe's hash h = mix64(key + seed) is appended to
+  // the wavefront's chunk of `flat` (chunk c holds chunk_fill[c] hashes at
+  // c * kEmitChunk); no hit records; a probe that finds no free chunk is
+  // queried at once and, if accepted, its hash pushed to acc_h; a wavefront
+  // takes no work item once its latest chunk is >= chunk_limit.  2 (identify): the main table's
+  // answer for a probe is whether h is in the sorted set set_h[0, set_n)
+  // (set_bits: 2^15-bit filter of the hashes' top 15 bits).  Modes 1 and 2
+  // need the strict gate off.
+  u32 pmode, chunk_limit;
+  u64* flat;
+  u32* chunk_ctr;
+  u32* chunk_fill;
+  u32 n_chunks, set_n;
+  u64* acc_h;
+  u32* acc_n;
+  u32* acc_lost;
+  u32 acc_cap, pad2_;
+  const u64* set_h;
+  const u32* set_bits;
 };
+constexpr u32 kEmitChunk = 4096;  // hashes per emit chunk (one partition tile)
 
 // reference-shaped find3 candidate (Cand, fourcore_find_v4.cu:102-109)
 struct PlainCand3 {

@@ -136,6 +136,8 @@ _sig = {
     "fc5_measure_gathers_region": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
     "fc5_measure_read": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double)]),
     "fc5_measure_valu": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
+    "fc5_measure_partitioned": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "fc5_part_stats": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "fc5_emit_units": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint, C.c_double, C.c_double,
                                  _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "fc5_build_pairs_file": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.POINTER(StoreHeader)]),
@@ -395,6 +397,27 @@ class Context:
         t = C.c_double()
         self._check(_lib.fc5_measure_valu(self.h, kind, C.byref(t)))
         return t.value
+
+    def measure_partitioned(self, n):
+        """partitioned probing of n pseudo-random keys: (stage ms, counts) as
+        named in fc5_measure_partitioned"""
+        ms = (C.c_double * 9)()
+        cnt = (C.c_uint64 * 8)()
+        self._check(_lib.fc5_measure_partitioned(self.h, n, ms, cnt))
+        names = ("first", "fine0", "lookup0", "fine1", "lookup1", "fine2", "last", "check", "direct")
+        cn = ("accepted", "accepted_hsum", "direct_accepted", "direct_hsum", "full_queries", "lost", "slices",
+              "coarse")
+        return dict(zip(names, list(ms))), dict(zip(cn, list(cnt)))
+
+    def part_stats(self):
+        """the last partitioned batched run: host ms of the emit launches, the
+        partition pipelines and the identify launch; pieces, accepted hashes,
+        full queries, emit chunks filled"""
+        ms = (C.c_double * 3)()
+        cnt = (C.c_uint64 * 4)()
+        self._check(_lib.fc5_part_stats(self.h, ms, cnt))
+        return (dict(zip(("emit", "pipeline", "identify"), list(ms))),
+                dict(zip(("pieces", "accepted", "full_queries", "chunks"), list(cnt))))
 
     def band(self, units, cls=None):
         return Band(self, units, cls)
