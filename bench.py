@@ -106,6 +106,10 @@ def main():
     ap.add_argument("--no-measure", action="store_true", help="skip the gather/copy microbenchmarks")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
+    ap.add_argument("--probe", choices=["auto", "gathers", "partitioned"], default="auto",
+                    help="filter probes: three gathers each, or partitioned by table slice (auto: by table)")
+    ap.add_argument("--no-gathers", action="store_true",
+                    help="skip the comparison leg with three gathers per probe (partitioned runs)")
     ap.add_argument("--load-policy", type=int, default=None, choices=[0, 1, 2, 3],
                     help="cache policy of the cell gathers (library default: by table size)")
     args = ap.parse_args()
@@ -219,6 +223,7 @@ def main():
             f"({tune['best_ms']:.1f} ms vs default {tune['default_ms']:.1f} ms on 1/{every} of the band)")
     if not args.no_graph:
         ctx.set_option("use_graph", 1)
+    ctx.set_option("probe_mode", PROBE_MODES[args.probe])
     for _ in range(args.warmup):
         check(band.run(fc5.MODE_BATCHED))
     log(f"[bench] setup {time.time() - t_setup:.1f}s; timing {args.steps} steps")
@@ -240,9 +245,18 @@ def main():
     kern_s = float(np.mean(kms)) / 1000.0
     cell_bytes = math.ceil(args.r / 8)
     alg_bytes = r["probes"] * 3 * cell_bytes
+    pst = ctx.part_stats()
+    partitioned = pst[1]["pieces"] > 0
     log(f"[bench] {ms_step:.1f} ms/step (library {float(np.mean(wms)):.1f} ms), kernel {1000 * kern_s:.1f} ms, "
         f"{value:.4e} points/s, "
-        f"probes={r['probes']} maybe={r['maybe']} side_maybe={r['side_maybe']}")
+        f"probes={r['probes']} maybe={r['maybe']} side_maybe={r['side_maybe']}"
+        + (f"; partitioned probing: emit {pst[0]['emit']:.0f} ms, pipeline {pst[0]['pipeline']:.0f} ms, "
+           f"identify {pst[0]['identify']:.0f} ms, {pst[1]['pieces']} pieces" if partitioned else ""))
+    part = partition_figures(pst, r, hdr, ms_step) if partitioned else None
+
+    gathers = None
+    if partitioned and not args.no_gathers:
+        gathers = gathers_leg(ctx, band, outs[-1], check, ms_step, cell_bytes, cfg)
 
     strict = None
     if not args.no_strict:
@@ -253,6 +267,9 @@ def main():
     ctx.set_option("verify_table", 0)
     plain = None
     if not args.no_plain:
+        # three gathers per probe in all three forms: batched/chunked isolates
+        # the persistent single launch of the same kernel
+        ctx.set_option("probe_mode", 0)
         sub = units[:: args.plain_every]
         b2 = ctx.band(sub, ucls[:: args.plain_every] if mixed else None)
         ob = b2.run(fc5.MODE_BATCHED)
@@ -286,6 +303,7 @@ def main():
                             "(the reference's launch pattern): batched/chunked isolates the persistent "
                             "single launch, chunked/plain the T-space arithmetic",
                  "counters_identical": True}
+        ctx.set_option("probe_mode", -1)
         log(f"[bench] plain form {pv:.4e} points/s on the subset; batched/plain = {value / pv:.1f}x; "
             f"same subset: batched {bv:.4e}, chunked {cv:.4e} (batched/chunked {bv / cv:.2f}x, "
             f"chunked/plain {cv / pv:.1f}x)")
@@ -299,6 +317,30 @@ def main():
         tprof = load_profile(f"traffic_cls5_batched_B{cfg['lo']}-{cfg['hi']}.json")
         traffic = tprof.get("traffic_bytes_per_launch") if tprof else None
         valu = valu_roofline(load_profile(f"valu_cls5_batched_B{cfg['lo']}-{cfg['hi']}.json"), ceiling)
+    # the gather kernel's roofline: the launch timed here (three gathers per
+    # probe), or the gathers leg's launch when this run probed by partition
+    g_s = (gathers["kernel_ms"] / 1000.0) if gathers else (None if partitioned else kern_s)
+    gather_roof = None
+    if g_s:
+        gather_roof = {"bound": "hbm", "achieved": alg_bytes / g_s / 1e9, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": alg_bytes / g_s / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+                       "kernel": "k_cls5_batched",
+                       "algorithmic_bytes_per_launch": alg_bytes,
+                       "per_unit": f"{3 * cell_bytes} B per filter probe (3 cells x {cell_bytes} B)",
+                       "line_fill_GBps_at_128B": r["probes"] * 3 * 128 / g_s / 1e9,
+                       # measured HBM traffic (128-byte lines per random gather) over the same launch time
+                       "traffic_GBps": traffic / g_s / 1e9 if traffic else None,
+                       "traffic_frac": traffic / g_s / 1e9 / HBM_PEAK_GBS if traffic else None,
+                       "valu": valu}
+    if partitioned:
+        lt = None if args.band else load_profile(f"traffic_part_lookup_B{cfg['lo']}-{cfg['hi']}.json")
+        part["roofline"]["traffic"] = lt.get("traffic_bytes_per_launch") if lt else None
+        if part["roofline"]["traffic"]:
+            lk_s = part["roofline"]["avg_launch_ms"] / 1000.0
+            part["roofline"]["traffic_GBps"] = part["roofline"]["traffic"] / lk_s / 1e9
+            part["roofline"]["traffic_frac"] = part["roofline"]["traffic_GBps"] / HBM_PEAK_GBS
+        if gathers:
+            gathers["roofline"] = gather_roof
 
     cpu = cpu_ref = None
     if not args.no_cpu_baseline:
@@ -313,7 +355,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (deterministic Stage-1 enumeration of the band"
                 + ("; seeded planted keys)" if plants else ")"),
-        "config": {"workload": "batched persistent cls5 peel, eligible B in "
+        "config": {"workload": "batched persistent cls5 peel"
+                               + (" with partitioned filter probing" if partitioned else "") + ", eligible B in "
                                f"[{cfg['lo']},{cfg['hi']}] ({n_B} B, {len(units)} units, "
                                f"{r['calls']} grid points), xor pair table N={hdr.N} r={hdr.r} "
                                f"({fc5_packed_gb(hdr):.2f} GB packed, "
@@ -331,16 +374,14 @@ def main():
                               for cl in cfg["classes"]} if mixed else None,
         "plants_recovered": f"{min(recovered)}/{len(plants)}",
         "kernel_ms": float(np.mean(kms)),
-        "roofline": {"bound": "hbm", "achieved": alg_bytes / kern_s / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": alg_bytes / kern_s / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_cls5_batched",
-                     "algorithmic_bytes_per_launch": alg_bytes,
-                     "per_unit": f"{3 * cell_bytes} B per filter probe (3 cells x {cell_bytes} B)",
-                     "line_fill_GBps_at_128B": r["probes"] * 3 * 128 / kern_s / 1e9,
-                     # measured HBM traffic (128-byte lines per random gather) over the same launch time
-                     "traffic_GBps": traffic / kern_s / 1e9 if traffic else None,
-                     "traffic_frac": traffic / kern_s / 1e9 / HBM_PEAK_GBS if traffic else None,
-                     "valu": valu},
+        "roofline": part["roofline"] if partitioned else gather_roof,
+        # the filter's algorithmic bytes (3 cells of every probe) over the whole step
+        "probe_roofline": {"achieved": alg_bytes / (ms_step / 1000.0) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": alg_bytes / (ms_step / 1000.0) / 1e9 / HBM_PEAK_GBS,
+                           "per_unit": f"{3 * cell_bytes} B per filter probe (3 cells x {cell_bytes} B)",
+                           "probes_per_s": r["probes"] / (ms_step / 1000.0)},
+        "partitioned_probing": part,
+        "gathers": gathers,
         "gather_ceiling": ceiling,
         "autotune": tune,
         "graph": not args.no_graph,
@@ -793,6 +834,78 @@ def reference_projection(cfg, b_per_hour):
     return {"eligible_B_per_hour": p, "ratio": b_per_hour / p, "derived": True,
             "how": "published band times of fourcore_cls5_gpu_v4 on an RTX PRO 6000 (README.md:146), "
                    "scaled to this band by B^3; a derived figure, not a published one"}
+
+
+PROBE_MODES = {"auto": -1, "gathers": 0, "partitioned": 1}
+
+# algorithmic bytes per probe of each pipeline stage (records: 8-byte hash,
+# then hash + 16-bit partial xor): read + written
+PART_STAGES = (("first", "k_part_first", 8 + 8), ("fine0", "k_part_fine", 8 + 8),
+               ("lookup0", "k_part_lookup", 8 + 10), ("fine1", "k_part_fine", 10 + 10),
+               ("lookup1", "k_part_lookup", 10 + 10), ("fine2", "k_part_fine", 10 + 10),
+               ("last", "k_part_last", 10), ("check", "k_part_check", 0))
+
+
+def partition_figures(pst, r, hdr, ms_step):
+    """Stage times of the last partitioned run (event-timed, summed over its
+    pieces), each stage's algorithmic bytes and rate, and the roofline of the
+    dominant kernel: k_part_lookup, two launches per piece."""
+    ms, cnt = pst
+    pieces = cnt["pieces"]
+    probes = r["probes"]
+    block = hdr.m_cells // 3
+    slice_bytes = -(-block // 32768) * 65536  # one third of the low-bits table, whole 64 KiB slices
+    stages = {}
+    for key, kern, per in PART_STAGES:
+        b = probes * per + (slice_bytes * pieces if kern in ("k_part_lookup", "k_part_last") else 0)
+        stages[key] = {"kernel": kern, "ms": ms[key], "bytes_per_probe": per, "bytes": b,
+                       "GBps": b / (ms[key] / 1000.0) / 1e9 if ms[key] > 0 else None}
+    lk_ms = (ms["lookup0"] + ms["lookup1"]) / (2 * pieces)
+    lk_bytes = (probes * (18 + 20) + 2 * pieces * slice_bytes) / (2 * pieces)
+    achieved = lk_bytes / (lk_ms / 1000.0) / 1e9
+    pipe_bytes = sum(v["bytes"] for v in stages.values())
+    pipe_ms = sum(v["ms"] for v in stages.values())
+    log(f"[bench] pipeline stages (ms over {pieces} pieces): "
+        + ", ".join(f"{k} {v['ms']:.0f} ({v['GBps'] or 0:.0f} GB/s)" for k, v in stages.items()))
+    return {"pieces": pieces, "accepted_hashes": cnt["accepted"], "full_queries": cnt["full_queries"],
+            "emit_chunks": cnt["chunks"], "host_ms": {k: ms[k] for k in ("emit", "pipeline", "identify")},
+            "stages": stages,
+            "pipeline": {"ms": pipe_ms, "bytes": pipe_bytes, "GBps": pipe_bytes / (pipe_ms / 1000.0) / 1e9,
+                         "frac": pipe_bytes / (pipe_ms / 1000.0) / 1e9 / HBM_PEAK_GBS},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_part_lookup",
+                         "algorithmic_bytes_per_launch": lk_bytes, "avg_launch_ms": lk_ms,
+                         "launches_per_step": 2 * pieces,
+                         "per_unit": "38 B per probe over a piece's two lookup launches (cell 0: 8 B read, "
+                                     "10 B written; cell 1: 10 B read, 10 B written) plus 64 KiB of low cell "
+                                     "bits per slice staged in LDS",
+                         "traffic_GBps": None, "traffic_frac": None}}
+
+
+def gathers_leg(ctx, band, base, check, ms_part, cell_bytes, cfg):
+    """The same band with three gathers per probe (probe_mode 0, the batched
+    kernel alone): its step time, and counters and hit records identical to
+    the partitioned run's."""
+    ctx.set_option("probe_mode", 0)
+    try:
+        check(band.run(fc5.MODE_BATCHED))
+        t0 = time.perf_counter()
+        outs = [band.run(fc5.MODE_BATCHED) for _ in range(2)]
+        ms = 1000.0 * (time.perf_counter() - t0) / 2
+    finally:
+        ctx.set_option("probe_mode", -1)
+    for o in outs:
+        check(o)
+        for k in COUNTER_KEYS:
+            assert o.result[k] == base.result[k], k
+        ga = np.sort(o.hits, order=["unit", "e", "d", "flags"])
+        gb = np.sort(base.hits, order=["unit", "e", "d", "flags"])
+        assert np.array_equal(ga, gb), "gathers and partitioned hit lists differ"
+    kms = float(np.mean([o.result["kernel_ms"] for o in outs]))
+    log(f"[bench] three gathers per probe: {ms:.1f} ms/step (kernel {kms:.1f} ms); partitioned is "
+        f"{ms / ms_part:.2f}x faster, counters and hits identical")
+    return {"ms_per_step": ms, "kernel_ms": kms, "points_per_s": base.result["calls"] / (ms / 1000.0),
+            "speedup_partitioned_vs_gathers": ms / ms_part, "counters_and_hits_identical": True}
 
 
 def load_profile(name):

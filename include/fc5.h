@@ -416,8 +416,11 @@ int fc5_measure_valu(fc5_ctx *ctx, int kind, double *instr_per_s);
  * coarse buckets per third */
 int fc5_measure_partitioned(fc5_ctx *ctx, uint64_t n, double *ms, uint64_t *counts);
 /* the last batched run with partitioned probing (option probe_mode):
- * ms[3] host-timed emit launches, partition pipelines and identify launch;
- * counts[4] pieces, accepted hashes, full queries, emit chunks filled */
+ * ms[11]: host-timed emit launches, partition pipelines and identify launch,
+ * then the pipeline's stages timed by events and summed over the pieces
+ * (first partition, fine partition and lookup of cell 0, of cell 1, fine
+ * partition and last lookup of cell 2, full queries);
+ * counts[4]: pieces, accepted hashes, full queries, emit chunks filled */
 int fc5_part_stats(fc5_ctx *ctx, double *ms, uint64_t *counts);
 
 /* ---- host helpers shared by the CLIs (no device use) ---- */

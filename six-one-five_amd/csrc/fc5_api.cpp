@@ -1246,6 +1246,11 @@ extern "C" int fc5_band_run(fc5_ctx* ctx, fc5_band* b, int mode, fc5_result* res
   const int bpc = ctx->blocks_per_cu > 0 ? ctx->blocks_per_cu : batched_blocks_per_cu(b->rows, ctx->strict_gate != 0, ctx->sweep != 0);
   const int grid = std::max(1, std::min<int>(ctx->num_cu * bpc, (int)(b->n_items + b->n_items3)));
   const bool part = mode == FC5_MODE_BATCHED && any_items && use_partitioned(ctx);
+  if (!part) {  // the partitioned-probing figures describe the last run only
+    std::fill(std::begin(ctx->part_ms), std::end(ctx->part_ms), 0.0);
+    std::fill(std::begin(ctx->part_stage_ms), std::end(ctx->part_stage_ms), 0.0);
+    std::fill(std::begin(ctx->part_counts), std::end(ctx->part_counts), 0);
+  }
   const bool graph = mode == FC5_MODE_BATCHED && any_items && ctx->use_graph && !part && b->irregular.empty() &&
                      b->irregular3.empty();
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));

@@ -171,7 +171,10 @@ struct fc5_ctx {
   int probe_mode = -1;
   u64 part_records = 3ull << 30;
   PartBufs* part = nullptr;
-  double part_ms[3] = {0, 0, 0};  // emit launches, partition pipeline, identify launch
+  double part_ms[3] = {0, 0, 0};  // emit launches, partition pipeline, identify launch (host clock)
+  // event-timed pipeline stages summed over the pieces: first partition, fine
+  // partition, lookup (cell 0), fine, lookup (cell 1), fine, last lookup, full queries
+  double part_stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   u64 part_counts[4] = {0, 0, 0, 0};  // pieces, accepted hashes, full queries, emit chunks filled
 };
 

@@ -62,9 +62,12 @@ struct PartBufs {
   u32 *cfill = nullptr, *ffill = nullptr, *set_bits = nullptr;
   u64 cap_c = 0, cap_f = 0;
   u32 side_cap = 0, acc_cap = 0, set_cap = 0;
+  hipEvent_t ev[16] = {};  // start and end of each of the pipeline's eight stages
   std::vector<void*> owned;
   ~PartBufs() {
     for (void* q : owned) (void)hipFree(q);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
   }
   template <class T>
   bool get(T*& p, u64 count) {
@@ -124,6 +127,11 @@ int ensure_part(fc5_ctx* ctx, PartBufs*& out) {
     FAIL(ctx, FC5_E_NOMEM, "partitioned probing buffers (lower the option part_records)");
   }
   P->g.lo16 = P->lo16;
+  for (hipEvent_t& e : P->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete P;
+      FAIL(ctx, FC5_E_HIP, "pipeline events");
+    }
   if (launch_part_lo16(P->g, P->lo16, ctx->stream) != 0) {
     delete P;
     FAIL(ctx, FC5_E_HIP, "low-bits table");
@@ -145,21 +153,38 @@ int run_pipeline(fc5_ctx* ctx, PartBufs* P) {
   const PartSet fine{P->fh, P->fa, P->ffill, P->cap_f, nf, 1, 1};
   const PartSide side{P->side_h, P->ctr + 1, P->ctr + 2, P->side_cap};
   const PartSide acc{P->acc_h, P->ctr + 3, P->ctr + 4, P->acc_cap};
+  // stage k (first, fine 0, lookup 0, fine 1, lookup 1, fine 2, last, check)
+  // between events 2k and 2k + 1; the counters a stage appends to are
+  // cleared before its start event, so the events bracket the kernel alone
+  auto stage = [&](int k, bool launched) {
+    return launched && hipEventRecord(P->ev[2 * k + 1], s) == hipSuccess;
+  };
+  auto start = [&](int k) { return hipEventRecord(P->ev[2 * k], s) == hipSuccess; };
   bool ok = hipMemsetAsync(P->ctr + 1, 0, 4, s) == hipSuccess &&
-            hipMemsetAsync(P->cfill, 0, ncr * kCoarseFillStride * 4, s) == hipSuccess &&
-            launch_part_first(P->flat, P->records, P->chunk_fill, g, coarse0, side, s) == 0;
+            hipMemsetAsync(P->cfill, 0, ncr * kCoarseFillStride * 4, s) == hipSuccess && start(0) &&
+            stage(0, launch_part_first(P->flat, P->records, P->chunk_fill, g, coarse0, side, s) == 0);
   for (int p = 0; p < 3 && ok; ++p) {
     const PartSet& fout = p == 0 ? fine0 : fine;
-    ok = hipMemsetAsync(P->ffill, 0, (u64)nf * 4, s) == hipSuccess &&
-         launch_part_fine(p, g, p == 0 ? coarse0 : coarse, fout, side, s) == 0;
+    ok = hipMemsetAsync(P->ffill, 0, (u64)nf * 4, s) == hipSuccess && start(1 + 2 * p) &&
+         stage(1 + 2 * p, launch_part_fine(p, g, p == 0 ? coarse0 : coarse, fout, side, s) == 0);
     if (ok && p < 2)
-      ok = hipMemsetAsync(P->cfill, 0, ncr * kCoarseFillStride * 4, s) == hipSuccess &&
-           launch_part_lookup(p, g, fout, coarse, side, s) == 0;
+      ok = hipMemsetAsync(P->cfill, 0, ncr * kCoarseFillStride * 4, s) == hipSuccess && start(2 + 2 * p) &&
+           stage(2 + 2 * p, launch_part_lookup(p, g, fout, coarse, side, s) == 0);
     else if (ok)
-      ok = launch_part_last(g, fine, side, s) == 0;
+      ok = start(6) && stage(6, launch_part_last(g, fine, side, s) == 0);
   }
-  ok = ok && launch_part_check(g, side, acc, ctx->num_cu * 4, s) == 0;
+  ok = ok && start(7) && stage(7, launch_part_check(g, side, acc, ctx->num_cu * 4, s) == 0);
   if (!ok) FAIL(ctx, FC5_E_HIP, "partition pipeline launch");
+  return FC5_OK;
+}
+
+// after the pipeline's synchronize: its stage times added to the run's
+int add_stage_times(fc5_ctx* ctx, PartBufs* P) {
+  for (int k = 0; k < 8; ++k) {
+    float t = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&t, P->ev[2 * k], P->ev[2 * k + 1]));
+    ctx->part_stage_ms[k] += t;
+  }
   return FC5_OK;
 }
 
@@ -186,6 +211,7 @@ int run_partitioned(fc5_ctx* ctx, const BatchedArgs& a0, int grid, int rows) {
   a.acc_lost = P->ctr + 4;
   a.acc_cap = P->acc_cap;
   std::fill(std::begin(ctx->part_ms), std::end(ctx->part_ms), 0.0);
+  std::fill(std::begin(ctx->part_stage_ms), std::end(ctx->part_stage_ms), 0.0);
   std::fill(std::begin(ctx->part_counts), std::end(ctx->part_counts), 0);
   HIPCHK(ctx, hipMemsetAsync(P->ctr, 0, 8 * 4, s));
   u32 h_ctr[8];
@@ -205,6 +231,8 @@ int run_partitioned(fc5_ctx* ctx, const BatchedArgs& a0, int grid, int rows) {
     HIPCHK(ctx, hipMemcpyAsync(&item_ctr, a.item_ctr, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
     ctx->part_ms[1] += ms_since(t0);
+    rc = add_stage_times(ctx, P);
+    if (rc != FC5_OK) return rc;
     ctx->part_counts[0] = piece + 1;
     ctx->part_counts[2] += std::min(h_ctr[1], P->side_cap);
     ctx->part_counts[3] += (u64)std::min(h_ctr[0], P->n_chunks);
@@ -249,6 +277,7 @@ int run_partitioned(fc5_ctx* ctx, const BatchedArgs& a0, int grid, int rows) {
 extern "C" int fc5_part_stats(fc5_ctx* ctx, double* ms, uint64_t* counts) try {
   if (!ctx || !ms || !counts) return FC5_E_ARG;
   for (int i = 0; i < 3; ++i) ms[i] = ctx->part_ms[i];
+  for (int i = 0; i < 8; ++i) ms[3 + i] = ctx->part_stage_ms[i];
   for (int i = 0; i < 4; ++i) counts[i] = ctx->part_counts[i];
   return FC5_OK;
 } catch (...) {

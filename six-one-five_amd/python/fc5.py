@@ -411,12 +411,14 @@ class Context:
 
     def part_stats(self):
         """the last partitioned batched run: host ms of the emit launches, the
-        partition pipelines and the identify launch; pieces, accepted hashes,
-        full queries, emit chunks filled"""
-        ms = (C.c_double * 3)()
+        partition pipelines and the identify launch, event ms of the pipeline
+        stages summed over the pieces; pieces, accepted hashes, full queries,
+        emit chunks filled"""
+        ms = (C.c_double * 11)()
         cnt = (C.c_uint64 * 4)()
         self._check(_lib.fc5_part_stats(self.h, ms, cnt))
-        return (dict(zip(("emit", "pipeline", "identify"), list(ms))),
+        return (dict(zip(("emit", "pipeline", "identify", "first", "fine0", "lookup0", "fine1", "lookup1", "fine2",
+                          "last", "check"), list(ms))),
                 dict(zip(("pieces", "accepted", "full_queries", "chunks"), list(cnt))))
 
     def band(self, units, cls=None):

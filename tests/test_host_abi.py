@@ -232,6 +232,32 @@ def test_bench_default_config_finds_its_counter_summaries():
     assert 0 < roof["frac"] < 1 and 0 < roof["int64_share"] < 0.2
 
 
+def test_bench_partition_figures_and_lookup_traffic_summary():
+    """the partitioned run's roofline (k_part_lookup: event-timed stage sums over
+    the pieces, 38 B per probe over a piece's two lookup launches plus the
+    staged slices) and the committed traffic summary it reads"""
+    import bench
+
+    class H:
+        m_cells = 3585713259
+
+    stage = dict(first=128.0, fine0=137.0, lookup0=274.0, fine1=191.0, lookup1=299.0, fine2=191.0, last=65.0,
+                 check=0.1, emit=190.0, pipeline=1287.0, identify=257.0)
+    cnt = dict(pieces=12, accepted=18, full_queries=518257, chunks=8334171)
+    probes = 34008059307
+    f = bench.partition_figures((stage, cnt), {"probes": probes}, H(), 1737.0)
+    roof = f["roofline"]
+    assert roof["kernel"] == "k_part_lookup" and roof["launches_per_step"] == 24
+    assert abs(roof["avg_launch_ms"] - (274.0 + 299.0) / 24) < 1e-9
+    slices = -(-(H.m_cells // 3) // 32768)
+    assert roof["algorithmic_bytes_per_launch"] == (probes * 38 + 24 * slices * 65536) / 24
+    assert 0 < roof["frac"] < 1 and f["stages"]["fine1"]["bytes"] == probes * 20
+    cfg = bench.CONFIGS[4]
+    t = bench.load_profile(f"traffic_part_lookup_B{cfg['lo']}-{cfg['hi']}.json")
+    assert t and 1.0 <= t["traffic_bytes_per_launch"] / t["algorithmic_bytes_per_launch"] < 1.2
+    assert set(t["pipeline_kernels"]) == {"k_part_first", "k_part_fine", "k_part_lookup", "k_part_last"}
+
+
 def test_bench_runs_no_reference_binary():
     """bench.py never executes a program built from the reference's sources: the
     reference CPU figure is read from oracle/ref_cpu_timing.json, written by
