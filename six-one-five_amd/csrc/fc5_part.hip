@@ -92,17 +92,10 @@ __device__ __forceinline__ void load_tile(T (&v)[R], const T* __restrict__ src, 
 // dig[j] (dig[j] < nbins <= kPartMaxBins).  Thread t holds records j < R, bit
 // j of `valid` set for the real ones.  Through LDS, so each bucket's records
 // leave as one contiguous run.  Every thread of the block must call it.
-struct NoHook {
-  __device__ void operator()() const {}
-};
-
-// after_reserve runs between the reservation and the write-out (a block's
-// prefetch of its next tile, so that the loads overlap the scatter)
-template <int NT, int R, int NB, class Hook = NoHook>
+template <int NT, int R, int NB>
 __device__ __forceinline__ void tile_scatter(TileSmem<NB>& s, const u64 (&h)[R], const u16 (&a)[R],
                                              const u32 (&dig)[R], u32 valid, u32 nbins, u32 bucket0, u32 sub,
-                                             const PartSet& out, const PartSide& side,
-                                             const Hook& after_reserve = Hook()) {
+                                             const PartSet& out, const PartSide& side) {
   static_assert(NT * R == kTileRecs, "tile shape");
   const u32 t = threadIdx.x;
   for (u32 b = t; b < nbins; b += NT) s.hist[b] = 0;
@@ -138,7 +131,6 @@ __device__ __forceinline__ void tile_scatter(TileSmem<NB>& s, const u64 (&h)[R],
     s.gbase[b] = c ? atomicAdd(out.fill + ((u64)(bucket0 + b) * out.subs + sub) * out.fill_stride, c) : 0u;
   }
   __syncthreads();
-  after_reserve();
 #pragma unroll
   for (int j = 0; j < R; ++j)
     if ((valid >> j) & 1u) {
@@ -249,15 +241,92 @@ __device__ __forceinline__ u32 stage_slice(u16* sl, const PartGeom& g, int p, u3
   return (u32)len;
 }
 
+// LDS of the lookup kernel's tile pipeline: one staging area, rankings of two
+// tiles (the one being written out and the next)
+template <int NB>
+struct LookSmem {
+  u64 h[kTileRecs];
+  u16 a[kTileRecs];
+  u16 d[kTileRecs];
+  u32 hist[2][NB];
+  u32 lofs[2][NB];
+  u32 gbase[NB];
+  u32 total[2];
+};
+
+// Rank a tile's valid records by bucket: zero the histogram, LDS atomics
+// (rank = place within the bucket), exclusive scan into lofs and total.
+// Every thread of the block calls it; it ends synchronised.
+template <int NT, int R>
+__device__ __forceinline__ void rank_tile(u32* hist, u32* lofs, u32* total, const u32 (&dig)[R], u32 valid,
+                                          u32 nbins, u32 (&rank)[R]) {
+  const u32 t = threadIdx.x;
+  for (u32 b = t; b < nbins; b += NT) hist[b] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R; ++j) rank[j] = ((valid >> j) & 1u) ? atomicAdd(&hist[dig[j]], 1u) : 0u;
+  __syncthreads();
+  if (t < 64) {
+    const u32 per = (nbins + 63) / 64;
+    u32 sum = 0;
+    for (u32 k = 0; k < per; ++k) {
+      const u32 b = t * per + k;
+      if (b < nbins) sum += hist[b];
+    }
+    u32 incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+      const u32 v = (u32)__shfl_up((int)incl, o, 64);
+      if ((int)t >= o) incl += v;
+    }
+    u32 run = incl - sum;
+    for (u32 k = 0; k < per; ++k) {
+      const u32 b = t * per + k;
+      if (b < nbins) {
+        lofs[b] = run;
+        run += hist[b];
+      }
+    }
+    if (t == 63) *total = incl;
+  }
+  __syncthreads();
+}
+
+// Each loaded record's cell of pass p (in slice [c0, c0 + len)) from LDS into
+// its partial xor, then the coarse bucket of its next cell; returns the valid
+// mask.  A record whose cell is not in the slice (not produced by the
+// partition kernels) gets the full query instead.
+template <int NT, int R>
+__device__ __forceinline__ u32 look_tile(const PartGeom& g, int p, const u16* sl, u64 c0, u32 len, u32 cnt,
+                                         const u64 (&h)[R], u16 (&a)[R], u32 (&dig)[R], const PartSide& side) {
+  u32 valid = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    dig[j] = 0;
+    if (threadIdx.x + NT * j < cnt) {
+      const u64 li = seg_cell(h[j], p, g.x) - c0;
+      if (li < len) {
+        a[j] ^= sl[li];
+        dig[j] = (u32)(seg_cell(h[j], p + 1, g.x) >> (g.slice_shift + g.fine_shift));
+        valid |= 1u << j;
+      } else {
+        side_push(side, h[j]);
+      }
+    }
+  }
+  return valid;
+}
+
 // One block per fine bucket (= slice) of pass p < 2: each record's cell of
 // this pass from LDS into its partial xor, then on to the coarse bucket of
-// its next cell.  A record whose cell is not in the slice (not produced by
-// the partition kernels) gets the full query instead.
+// its next cell.  The tiles of the bucket go through a software pipeline, so
+// that no load or reservation round trip is waited for in full: while tile k
+// is staged and written out, tile k+1 is looked up, ranked and reserved and
+// tile k+2 is loaded.
 template <int NB>
 __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g, PartSet in, PartSet out,
                                                                PartSide side) {
   constexpr int NT = kLookThreads, R = kTileRecs / NT;
-  __shared__ TileSmem<NB> s;
+  __shared__ LookSmem<NB> s;
   __shared__ __align__(16) u16 sl[kSliceCells];
   const u32 f = blockIdx.x;
   const u64 n = min((u64)in.fill[(u64)f * in.fill_stride], in.cap);
@@ -265,45 +334,86 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
   const u32 len = stage_slice<NT>(sl, g, p, f);
   const u64 c0 = (u64)f << g.slice_shift;
   const u64 off0 = (u64)f * in.cap;
+  const u32 t = threadIdx.x, nbins = g.nc, sub = f % out.subs;
+  const u64 ntiles = (n + kTileRecs - 1) / kTileRecs;
+  auto cnt_of = [&](u64 k) { return (u32)min((u64)kTileRecs, n - k * kTileRecs); };
+  auto reserve = [&](u32 b, u32 c) {
+    return c ? atomicAdd(out.fill + ((u64)b * out.subs + sub) * out.fill_stride, c) : 0u;
+  };
+  // tile 0: loaded, looked up, ranked and reserved; tile 1 loading
+  u64 hc[R];
+  u16 ac[R];
+  u32 dc[R], rc[R];
+  load_tile<NT, R>(hc, in.h + off0, cnt_of(0));
+  load_tile<NT, R>(ac, in.a ? in.a + off0 : nullptr, cnt_of(0));
+  u32 vc = look_tile<NT, R>(g, p, sl, c0, len, cnt_of(0), hc, ac, dc, side);
+  rank_tile<NT, R>(s.hist[0], s.lofs[0], &s.total[0], dc, vc, nbins, rc);
+  u32 gb = t < nbins ? reserve(t, s.hist[0][t]) : 0u;
   u64 hn[R];
   u16 an[R];
-  load_tile<NT, R>(hn, in.h + off0, (u32)min((u64)kTileRecs, n));
-  load_tile<NT, R>(an, in.a ? in.a + off0 : nullptr, (u32)min((u64)kTileRecs, n));
-  for (u64 base = 0; base < n; base += kTileRecs) {
-    const u32 cnt = (u32)min((u64)kTileRecs, n - base);
-    u64 h[R];
-    u16 a[R];
-    u32 dig[R];
-    u32 valid = 0;
+  if (ntiles > 1) {
+    load_tile<NT, R>(hn, in.h + off0 + kTileRecs, cnt_of(1));
+    load_tile<NT, R>(an, in.a ? in.a + off0 + kTileRecs : nullptr, cnt_of(1));
+  }
+  for (u64 k = 0; k < ntiles; ++k) {
+    const u32 cur = (u32)(k & 1), nxt = cur ^ 1u;
+    const bool more = k + 1 < ntiles;  // uniform over the block
+    u64 hx[R];
+    u16 ax[R];
+    u32 dx[R], rx[R], vx = 0;
+    if (more) {
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-      h[j] = hn[j];
-      a[j] = an[j];
-    }
-    const u64 nb = base + kTileRecs;
-    auto prefetch = [&]() {
-      if (nb < n) {
-        const u32 ncnt = (u32)min((u64)kTileRecs, n - nb);
-        load_tile<NT, R>(hn, in.h + off0 + nb, ncnt);
-        load_tile<NT, R>(an, in.a ? in.a + off0 + nb : nullptr, ncnt);
+      for (int j = 0; j < R; ++j) {
+        hx[j] = hn[j];
+        ax[j] = an[j];
       }
-    };
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const bool ok = threadIdx.x + NT * j < cnt;
-      dig[j] = 0;
-      if (ok) {
-        const u64 li = seg_cell(h[j], p, g.x) - c0;
-        if (li < len) {
-          a[j] ^= sl[li];
-          dig[j] = (u32)(seg_cell(h[j], p + 1, g.x) >> (g.slice_shift + g.fine_shift));
-          valid |= 1u << j;
-        } else {
-          side_push(side, h[j]);
-        }
+      vx = look_tile<NT, R>(g, p, sl, c0, len, cnt_of(k + 1), hx, ax, dx, side);
+      if (k + 2 < ntiles) {
+        const u64 o = off0 + (k + 2) * kTileRecs;
+        load_tile<NT, R>(hn, in.h + o, cnt_of(k + 2));
+        load_tile<NT, R>(an, in.a ? in.a + o : nullptr, cnt_of(k + 2));
       }
     }
-    tile_scatter<NT, R, NB>(s, h, a, dig, valid, g.nc, 0, f % out.subs, out, side, prefetch);
+    // tile k: reservation and staging (published by the next barrier)
+    if (t < nbins) s.gbase[t] = gb;
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if ((vc >> j) & 1u) {
+        const u32 pos = s.lofs[cur][dc[j]] + rc[j];
+        s.h[pos] = hc[j];
+        s.a[pos] = ac[j];
+        s.d[pos] = (u16)dc[j];
+      }
+    if (more)
+      rank_tile<NT, R>(s.hist[nxt], s.lofs[nxt], &s.total[nxt], dx, vx, nbins, rx);
+    else
+      __syncthreads();
+    // tile k: write-out in runs per bucket
+    const u32 total = s.total[cur];
+    for (u32 i = t; i < total; i += NT) {
+      const u32 b = s.d[i];
+      const u64 kk = (u64)s.gbase[b] + (i - s.lofs[cur][b]);
+      const u64 at = ((u64)b * out.subs + sub) * out.cap + kk;
+      if (kk < out.cap) {
+        out.h[at] = s.h[i];
+        if (out.a) out.a[at] = s.a[i];
+      } else {
+        side_push(side, s.h[i]);
+      }
+    }
+    // tile k+1: reservation, waited for only in the next iteration
+    if (more && t < nbins) gb = reserve(t, s.hist[nxt][t]);
+    __syncthreads();  // staging and gbase are reused by the next iteration
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        hc[j] = hx[j];
+        ac[j] = ax[j];
+        dc[j] = dx[j];
+        rc[j] = rx[j];
+      }
+      vc = vx;
+    }
   }
 }
 
