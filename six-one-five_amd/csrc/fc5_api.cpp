@@ -160,8 +160,8 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) try {
     ctx->probe_mode = (int)v;
   } else if (k == "part_records") {
     if (v < (int64_t)kEmitChunk || v > (int64_t)0xFFFFFFFF) FAIL(ctx, FC5_E_ARG, "part_records must be 4096..2^32-1");
+    if ((u64)v != ctx->part_records) free_part(ctx);  // the buffers are made for one capacity
     ctx->part_records = (u64)v;
-    free_part(ctx);
   } else if (k == "blocks_per_cu") {
     if (v < 0 || v > 64) FAIL(ctx, FC5_E_ARG, "blocks_per_cu out of range");
     ctx->blocks_per_cu = (int)v;

@@ -169,7 +169,7 @@ struct fc5_ctx {
   // the buffers (the low-bits table belongs to the current main table), and
   // the last partitioned run's stage times and counts
   int probe_mode = -1;
-  u64 part_records = 3ull << 30;
+  u64 part_records = (1ull << 32) - kEmitChunk;  // 1,048,575 emit chunks
   PartBufs* part = nullptr;
   double part_ms[3] = {0, 0, 0};  // emit launches, partition pipeline, identify launch (host clock)
   // event-timed pipeline stages summed over the pieces: first partition, fine
