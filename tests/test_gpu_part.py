@@ -136,3 +136,20 @@ def test_headline_sample_partitioned_equals_gathers(ctx):
     say(f"part/3.2M sample: {st}, probes={part.result['probes']}, maybe={part.result['maybe']}")
     assert st[1]["chunks"] > 0
     _same(direct, part)
+
+
+def test_flagship_table_512_coarse_buckets_partitioned_equals_gathers(ctx):
+    """the reference's flagship table, N = 119,047 (8.7e9 cells: 346 coarse
+    buckets per third, so the 512-bucket kernel variants), every 4000th unit of
+    the 4.5M -> 5.0M band; partitioned against gathers"""
+    _, Bs, us = fc5.emit_units(4500001, 5000000)
+    units = fc5.units_array(Bs[::4000], us[::4000])
+    ctx.table_build(119047, 48, on_gpu=True)
+    hdr = ctx.table_info()
+    assert -(-(hdr.m_cells // 3) // (32768 * 256)) > 256
+    part = ctx.run(units, fc5.MODE_BATCHED)  # probe_mode -1: partitioned for this table
+    st = ctx.part_stats()
+    direct = _run(ctx, units, probe_mode=0)
+    say(f"part/flagship table: {st[1]}, probes={part.result['probes']}")
+    assert st[1]["chunks"] > 0
+    _same(direct, part)
