@@ -114,7 +114,7 @@ __device__ __forceinline__ void emit_hashes(const BatchedArgs& a, WaveAcc* A, u6
 
 // probe mode 2: is hash h in the accepted set
 __device__ __forceinline__ bool in_set(const BatchedArgs& a, u64 h) {
-  const u32 bit = (u32)(h >> 49);
+  const u32 bit = (u32)(h >> (64 - kSetFilterBits));
   if (a.set_n == 0 || !((a.set_bits[bit >> 5] >> (bit & 31u)) & 1u)) return false;
   u32 lo = 0, hi = a.set_n;
   while (lo < hi) {

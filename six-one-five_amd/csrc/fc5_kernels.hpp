@@ -129,7 +129,7 @@ struct BatchedArgs {
   // queried at once and, if accepted, its hash pushed to acc_h; a wavefront
   // takes no work item once its latest chunk is >= chunk_limit.  2 (identify): the main table's
   // answer for a probe is whether h is in the sorted set set_h[0, set_n)
-  // (set_bits: 2^15-bit filter of the hashes' top 15 bits).  Modes 1 and 2
+  // (set_bits: 2^kSetFilterBits-bit filter of the hashes' top bits).  Modes 1 and 2
   // need the strict gate off.
   u32 pmode, chunk_limit;
   u64* flat;
@@ -144,6 +144,9 @@ struct BatchedArgs {
   const u32* set_bits;
 };
 constexpr u32 kEmitChunk = 4096;  // hashes per emit chunk (one partition tile)
+// probe mode 2: the accepted set's filter, 2^20 bits (128 KiB, L2-resident): a
+// probe passes it with odds (accepted hashes) / 2^20 before the binary search
+constexpr u32 kSetFilterBits = 20;
 
 // reference-shaped find3 candidate (Cand, fourcore_find_v4.cu:102-109)
 struct PlainCand3 {

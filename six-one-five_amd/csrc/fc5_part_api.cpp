@@ -121,7 +121,7 @@ int ensure_part(fc5_ctx* ctx, PartBufs*& out) {
                   P->get(P->chunk_fill, P->n_chunks) && P->get(P->ctr, 8) && P->get(P->ch, P->cap_c * ncr) &&
                   P->get(P->ca, P->cap_c * ncr) && P->get(P->cfill, ncr * kCoarseFillStride) &&
                   P->get(P->fh, P->cap_f * nf) && P->get(P->fa, P->cap_f * nf) && P->get(P->ffill, nf) &&
-                  P->get(P->side_h, P->side_cap) && P->get(P->acc_h, P->acc_cap) && P->get(P->set_bits, 1024);
+                  P->get(P->side_h, P->side_cap) && P->get(P->acc_h, P->acc_cap) && P->get(P->set_bits, (1u << kSetFilterBits) / 32);
   if (!ok) {
     delete P;
     FAIL(ctx, FC5_E_NOMEM, "partitioned probing buffers (lower the option part_records)");
@@ -240,14 +240,17 @@ int run_partitioned(fc5_ctx* ctx, const BatchedArgs& a0, int grid, int rows) {
       FAIL(ctx, FC5_E_OVERFLOW, "partitioned probing: candidate or accepted list full (use probe_mode 0)");
     if ((u32)item_ctr >= a.total) break;
   }
-  // the accepted set, sorted and unique, with its 2^15-bit filter
+  // the accepted set, sorted and unique, with its filter of top bits
   const u32 n_acc = h_ctr[3];
   std::vector<u64> acc(n_acc);
   if (n_acc) HIPCHK(ctx, hipMemcpy(acc.data(), P->acc_h, (u64)n_acc * 8, hipMemcpyDeviceToHost));
   std::sort(acc.begin(), acc.end());
   acc.erase(std::unique(acc.begin(), acc.end()), acc.end());
-  std::vector<u32> bits(1024, 0);
-  for (u64 h : acc) bits[(u32)(h >> 49) >> 5] |= 1u << ((u32)(h >> 49) & 31u);
+  std::vector<u32> bits((1u << kSetFilterBits) / 32, 0);
+  for (u64 h : acc) {
+    const u32 b = (u32)(h >> (64 - kSetFilterBits));
+    bits[b >> 5] |= 1u << (b & 31u);
+  }
   if (acc.size() > P->set_cap) {
     u64* nh = nullptr;
     if (!P->get(nh, acc.size())) FAIL(ctx, FC5_E_NOMEM, "accepted set");
