@@ -839,11 +839,12 @@ def reference_projection(cfg, b_per_hour):
 PROBE_MODES = {"auto": -1, "gathers": 0, "partitioned": 1}
 
 # algorithmic bytes per probe of each pipeline stage (records: 8-byte hash,
-# then hash + 16-bit partial xor): read + written
+# then hash + 8-bit partial xor): read + written
 PART_STAGES = (("first", "k_part_first", 8 + 8), ("fine0", "k_part_fine", 8 + 8),
-               ("lookup0", "k_part_lookup", 8 + 10), ("fine1", "k_part_fine", 10 + 10),
-               ("lookup1", "k_part_lookup", 10 + 10), ("fine2", "k_part_fine", 10 + 10),
-               ("last", "k_part_last", 10), ("check", "k_part_check", 0))
+               ("lookup0", "k_part_lookup", 8 + 9), ("fine1", "k_part_fine", 9 + 9),
+               ("lookup1", "k_part_lookup", 9 + 9), ("fine2", "k_part_fine", 9 + 9),
+               ("last", "k_part_last", 9), ("check", "k_part_check", 0))
+PART_SLICE_CELLS = 65536  # cells of one third per slice, staged as their low 8 bits
 
 
 def partition_figures(pst, r, hdr, ms_step):
@@ -854,14 +855,14 @@ def partition_figures(pst, r, hdr, ms_step):
     pieces = cnt["pieces"]
     probes = r["probes"]
     block = hdr.m_cells // 3
-    slice_bytes = -(-block // 32768) * 65536  # one third of the low-bits table, whole 64 KiB slices
+    slice_bytes = -(-block // PART_SLICE_CELLS) * PART_SLICE_CELLS  # one third of the low-bits table
     stages = {}
     for key, kern, per in PART_STAGES:
         b = probes * per + (slice_bytes * pieces if kern in ("k_part_lookup", "k_part_last") else 0)
         stages[key] = {"kernel": kern, "ms": ms[key], "bytes_per_probe": per, "bytes": b,
                        "GBps": b / (ms[key] / 1000.0) / 1e9 if ms[key] > 0 else None}
     lk_ms = (ms["lookup0"] + ms["lookup1"]) / (2 * pieces)
-    lk_bytes = (probes * (18 + 20) + 2 * pieces * slice_bytes) / (2 * pieces)
+    lk_bytes = (probes * (17 + 18) + 2 * pieces * slice_bytes) / (2 * pieces)
     achieved = lk_bytes / (lk_ms / 1000.0) / 1e9
     pipe_bytes = sum(v["bytes"] for v in stages.values())
     pipe_ms = sum(v["ms"] for v in stages.values())
@@ -876,8 +877,8 @@ def partition_figures(pst, r, hdr, ms_step):
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_part_lookup",
                          "algorithmic_bytes_per_launch": lk_bytes, "avg_launch_ms": lk_ms,
                          "launches_per_step": 2 * pieces,
-                         "per_unit": "38 B per probe over a piece's two lookup launches (cell 0: 8 B read, "
-                                     "10 B written; cell 1: 10 B read, 10 B written) plus 64 KiB of low cell "
+                         "per_unit": "35 B per probe over a piece's two lookup launches (cell 0: 8 B read, "
+                                     "9 B written; cell 1: 9 B read, 9 B written) plus 64 KiB of low cell "
                                      "bits per slice staged in LDS",
                          "traffic_GBps": None, "traffic_frac": None}}
 

@@ -5,21 +5,21 @@
 // 53-64), and each of those 8-byte gathers fills a whole 128-byte L2 line.
 // Here a batch of queries is taken to the table instead: the queries' hashes
 // are radix-partitioned, in two levels, by the slice (kSliceCells cells of one
-// third) that holds the current cell; a workgroup stages its slice's low 16
+// third) that holds the current cell; a workgroup stages its slice's low 8
 // cell bits in LDS, looks each query of its bucket up there, and the query's
-// 16-bit partial xor travels with its hash into the partition of the next
-// cell.  After the third cell a query whose partial xor equals the low 16
-// bits of its fingerprint is a candidate (about 2^-16 of the rest); every
+// 8-bit partial xor travels with its hash into the partition of the next
+// cell.  After the third cell a query whose partial xor equals the low 8
+// bits of its fingerprint is a candidate (about 2^-8 of the rest); every
 // candidate gets the full query of all r bits against the expanded cells, so
 // the accepted set is exactly the filter's.  A record that finds its bucket
 // full also gets the full query at once (nothing is dropped).
 //
-// Records: the 64-bit hash h = mix64(key + seed) and the partial xor (u16),
+// Records: the 64-bit hash h = mix64(key + seed) and the partial xor (a byte),
 // in separate arrays; buckets have a fixed capacity, so a partition is one
 // pass (per-tile histogram in LDS, one reservation atomic per non-empty
 // bucket and tile, runs written contiguously).
 //
-// k_part_lo16    the low 16 bits of every cell (the lookups' table)
+// k_part_low     the low 8 bits of every cell (the lookups' table)
 // k_part_first   hashes (in any order) -> coarse buckets of the first cell
 // k_part_fine    coarse bucket -> fine buckets (= slices) of the same cell
 // k_part_lookup  fine bucket: slice in LDS, xor, coarse buckets of the next cell
@@ -45,7 +45,7 @@ constexpr int kTileRecs = 4096;
 template <int NB>
 struct TileSmem {
   u64 h[kTileRecs];
-  u16 a[kTileRecs];
+  low_t a[kTileRecs];
   u16 d[kTileRecs];
   u32 hist[NB];
   u32 lofs[NB];
@@ -93,7 +93,7 @@ __device__ __forceinline__ void load_tile(T (&v)[R], const T* __restrict__ src, 
 // j of `valid` set for the real ones.  Through LDS, so each bucket's records
 // leave as one contiguous run.  Every thread of the block must call it.
 template <int NT, int R, int NB>
-__device__ __forceinline__ void tile_scatter(TileSmem<NB>& s, const u64 (&h)[R], const u16 (&a)[R],
+__device__ __forceinline__ void tile_scatter(TileSmem<NB>& s, const u64 (&h)[R], const low_t (&a)[R],
                                              const u32 (&dig)[R], u32 valid, u32 nbins, u32 bucket0, u32 sub,
                                              const PartSet& out, const PartSide& side) {
   static_assert(NT * R == kTileRecs, "tile shape");
@@ -156,10 +156,10 @@ __device__ __forceinline__ void tile_scatter(TileSmem<NB>& s, const u64 (&h)[R],
 }
 
 // cell c of third p -> lo[p lo_third + c]; the padding past each third stays unset
-__global__ void k_part_lo16(const u64* __restrict__ cells, u64 block, u64 lo_third, u16* __restrict__ lo) {
+__global__ void k_part_low(const u64* __restrict__ cells, u64 block, u64 lo_third, low_t* __restrict__ lo) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * block; i += (u64)gridDim.x * blockDim.x) {
     const u64 p = i / block;
-    lo[p * lo_third + (i - p * block)] = (u16)cells[i];
+    lo[p * lo_third + (i - p * block)] = (low_t)cells[i];
   }
 }
 
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kPartThreads) void k_part_first(const u64* __restri
                             : (base < n ? (u32)min((u64)kTileRecs, n - base) : 0u);
   if (cnt == 0) return;
   u64 h[R];
-  u16 a[R];
+  low_t a[R];
   u32 dig[R];
   u32 valid = 0;
   load_tile<NT, R>(h, src + base, cnt);
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kPartThreads) void k_part_fine(int p, PartGeom g, P
   const u64 off = (u64)reg * in.cap + base;
   const u32 fmask = (1u << g.fine_shift) - 1u;
   u64 h[R];
-  u16 a[R];
+  low_t a[R];
   u32 dig[R];
   u32 valid = 0;
   load_tile<NT, R>(h, in.h + off, cnt);
@@ -219,18 +219,18 @@ __global__ __launch_bounds__(kPartThreads) void k_part_fine(int p, PartGeom g, P
   tile_scatter<NT, R, 1 << kFineShift>(s, h, a, dig, valid, fmask + 1u, b << g.fine_shift, 0, out, side);
 }
 
-// Stage slice f of third p (its low 16 cell bits, padded to kSliceCells) in
+// Stage slice f of third p (its low cell bits, padded to kSliceCells) in
 // LDS with 16-byte loads, all of a thread's in flight at once; returns the
 // slice's length in real cells.  The block has NT threads.
 typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
 template <int NT>
-__device__ __forceinline__ u32 stage_slice(u16* sl, const PartGeom& g, int p, u32 f) {
-  constexpr int V = kSliceCells * 2 / 16 / NT;  // 16-byte words per thread
-  static_assert(V * NT * 16 == kSliceCells * 2, "slice shape");
+__device__ __forceinline__ u32 stage_slice(low_t* sl, const PartGeom& g, int p, u32 f) {
+  constexpr int V = kSliceCells * sizeof(low_t) / 16 / NT;  // 16-byte words per thread
+  static_assert(V * NT * 16 == kSliceCells * sizeof(low_t), "slice shape");
   const u64 c0 = (u64)f << g.slice_shift;
   const u64 len = c0 < g.x.block ? min((u64)kSliceCells, g.x.block - c0) : 0;
   if (len) {
-    const u32x4v* src = (const u32x4v*)(g.lo16 + (u64)p * g.lo_third + c0);
+    const u32x4v* src = (const u32x4v*)(g.lo + (u64)p * g.lo_third + c0);
     u32x4v v[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) v[k] = __builtin_nontemporal_load(src + threadIdx.x + NT * k);
@@ -246,7 +246,7 @@ __device__ __forceinline__ u32 stage_slice(u16* sl, const PartGeom& g, int p, u3
 template <int NB>
 struct LookSmem {
   u64 h[kTileRecs];
-  u16 a[kTileRecs];
+  low_t a[kTileRecs];
   u16 d[kTileRecs];
   u32 hist[2][NB];
   u32 lofs[2][NB];
@@ -296,8 +296,8 @@ __device__ __forceinline__ void rank_tile(u32* hist, u32* lofs, u32* total, cons
 // mask.  A record whose cell is not in the slice (not produced by the
 // partition kernels) gets the full query instead.
 template <int NT, int R>
-__device__ __forceinline__ u32 look_tile(const PartGeom& g, int p, const u16* sl, u64 c0, u32 len, u32 cnt,
-                                         const u64 (&h)[R], u16 (&a)[R], u32 (&dig)[R], const PartSide& side) {
+__device__ __forceinline__ u32 look_tile(const PartGeom& g, int p, const low_t* sl, u64 c0, u32 len, u32 cnt,
+                                         const u64 (&h)[R], low_t (&a)[R], u32 (&dig)[R], const PartSide& side) {
   u32 valid = 0;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
                                                                PartSide side) {
   constexpr int NT = kLookThreads, R = kTileRecs / NT;
   __shared__ LookSmem<NB> s;
-  __shared__ __align__(16) u16 sl[kSliceCells];
+  __shared__ __align__(16) low_t sl[kSliceCells];
   const u32 f = blockIdx.x;
   const u64 n = min((u64)in.fill[(u64)f * in.fill_stride], in.cap);
   if (n == 0) return;
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
   };
   // tile 0: loaded, looked up, ranked and reserved; tile 1 loading
   u64 hc[R];
-  u16 ac[R];
+  low_t ac[R];
   u32 dc[R], rc[R];
   load_tile<NT, R>(hc, in.h + off0, cnt_of(0));
   load_tile<NT, R>(ac, in.a ? in.a + off0 : nullptr, cnt_of(0));
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
   rank_tile<NT, R>(s.hist[0], s.lofs[0], &s.total[0], dc, vc, nbins, rc);
   u32 gb = t < nbins ? reserve(t, s.hist[0][t]) : 0u;
   u64 hn[R];
-  u16 an[R];
+  low_t an[R];
   if (ntiles > 1) {
     load_tile<NT, R>(hn, in.h + off0 + kTileRecs, cnt_of(1));
     load_tile<NT, R>(an, in.a ? in.a + off0 + kTileRecs : nullptr, cnt_of(1));
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
     const u32 cur = (u32)(k & 1), nxt = cur ^ 1u;
     const bool more = k + 1 < ntiles;  // uniform over the block
     u64 hx[R];
-    u16 ax[R];
+    low_t ax[R];
     u32 dx[R], rx[R], vx = 0;
     if (more) {
 #pragma unroll
@@ -418,29 +418,63 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
 }
 
 // One block per fine bucket of the third cell: a record whose partial xor
-// matches the low 16 fingerprint bits is a candidate for the full query.
+// matches the low fingerprint bits is a candidate for the full query.  The
+// candidates (2^-8 of the records) gather in an LDS buffer, and the block
+// reserves room in the side list once per flush: one global atomic per
+// block and flush instead of one per candidate (a single counter serialises
+// its atomics).
+constexpr u32 kCandBuf = 1024;
 __global__ __launch_bounds__(kLookThreads) void k_part_last(PartGeom g, PartSet in, PartSide side) {
   constexpr int NT = kLookThreads, R = 8;
-  __shared__ __align__(16) u16 sl[kSliceCells];
+  __shared__ __align__(16) low_t sl[kSliceCells];
+  __shared__ u64 cand[kCandBuf];
+  __shared__ u32 ncand, cbase;
   const u32 f = blockIdx.x;
   const u64 n = min((u64)in.fill[(u64)f * in.fill_stride], in.cap);
   if (n == 0) return;
-  const u32 len = stage_slice<NT>(sl, g, 2, f);
+  if (threadIdx.x == 0) ncand = 0;
+  const u32 len = stage_slice<NT>(sl, g, 2, f);  // its barrier also publishes ncand
   const u64 c0 = (u64)f << g.slice_shift;
   const u64 off0 = (u64)f * in.cap;
+  // the buffered candidates to the side list (every thread calls it)
+  auto flush = [&]() {
+    __syncthreads();
+    const u32 m = min(ncand, kCandBuf);
+    if (threadIdx.x == 0) cbase = m ? atomicAdd(side.n, m) : 0u;
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < m; i += NT) {
+      if (cbase + i < side.cap)
+        side.h[cbase + i] = cand[i];
+      else
+        atomicAdd(side.lost, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ncand = 0;
+    __syncthreads();
+  };
   for (u64 base = 0; base < n; base += NT * R) {
     const u32 cnt = (u32)min((u64)NT * R, n - base);
     u64 h[R];
-    u16 a[R];
+    low_t a[R];
     load_tile<NT, R>(h, in.h + off0 + base, cnt);
     load_tile<NT, R>(a, in.a + off0 + base, cnt);
 #pragma unroll
     for (int j = 0; j < R; ++j)
       if (threadIdx.x + NT * j < cnt) {
         const u64 li = seg_cell(h[j], 2, g.x) - c0;
-        if (li >= len || (((u32)(a[j] ^ sl[li]) ^ (u32)h[j]) & g.mask16) == 0u) side_push(side, h[j]);
+        if (li >= len || (((u32)(a[j] ^ sl[li]) ^ (u32)h[j]) & g.mask_lo) == 0u) {
+          const u32 k = atomicAdd(&ncand, 1u);  // LDS
+          if (k < kCandBuf)
+            cand[k] = h[j];
+          else
+            side_push(side, h[j]);  // more than the buffer in one round (small r): one atomic each
+        }
       }
+    // flush once the buffer is half full, so a round (NT * R records) cannot overrun it at r >= 8
+    __syncthreads();
+    if (ncand >= kCandBuf / 2) flush();
   }
+  flush();
 }
 
 // full queries of the side list; the accepted hashes go to acc
@@ -492,9 +526,9 @@ __global__ void k_part_sum(const u64* __restrict__ h, const u32* n, u32 cap, uns
   }
 }
 
-int launch_part_lo16(const PartGeom& g, u16* lo, hipStream_t s) {
+int launch_part_low(const PartGeom& g, low_t* lo, hipStream_t s) {
   if (g.lo_third < g.x.block || (g.lo_third & (kSliceCells - 1))) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_part_lo16, dim3(4096), dim3(256), 0, s, g.x.cells, g.x.block, g.lo_third, lo);
+  hipLaunchKernelGGL(k_part_low, dim3(4096), dim3(256), 0, s, g.x.cells, g.x.block, g.lo_third, lo);
   return (int)hipGetLastError();
 }
 

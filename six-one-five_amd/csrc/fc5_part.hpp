@@ -6,8 +6,12 @@
 
 namespace fc5 {
 
-constexpr u32 kSliceShift = 15;                 // slice = 32768 cells of one third
-constexpr u32 kSliceCells = 1u << kSliceShift;  // staged in LDS as u16: 64 KiB
+// The lookups use the low kLowBits bits of each cell: a slice of 65,536 cells
+// of one third fits LDS in 64 KiB, and a record's partial xor is one byte.
+using low_t = u8;
+constexpr u32 kLowBits = 8;
+constexpr u32 kSliceShift = 16;                 // slice = 65536 cells of one third
+constexpr u32 kSliceCells = 1u << kSliceShift;  // staged in LDS as low_t: 64 KiB
 constexpr u32 kFineShift = 8;                   // 256 fine buckets (slices) per coarse bucket
 constexpr u32 kPartMaxBins = 512;               // coarse buckets per third at most
 constexpr u32 kCoarseFillStride = 64;           // coarse fill counters 256 bytes apart
@@ -15,12 +19,12 @@ constexpr u32 kCoarseSubs = 32;                 // regions per coarse bucket
 
 struct PartGeom {
   XorDev x;          // the table: block, divisor, expanded cells (full queries)
-  const u16* lo16;   // low 16 bits of every cell, each third padded to whole slices
-  u64 lo_third;      // cells per third in lo16 (ns slices: the slice (p, f) starts at (p ns + f) S)
+  const low_t* lo;   // low kLowBits bits of every cell, each third padded to whole slices
+  u64 lo_third;      // cells per third in lo (ns slices: the slice (p, f) starts at (p ns + f) S)
   u32 slice_shift;   // log2 of the slice length
   u32 fine_shift;    // log2 of the fine buckets per coarse bucket
   u32 nc;            // coarse buckets per third
-  u32 mask16;        // low 16 bits of the fingerprint mask
+  u32 mask_lo;       // low kLowBits bits of the fingerprint mask
 };
 
 // nb buckets, each split into `subs` regions of cap records (region
@@ -31,7 +35,7 @@ struct PartGeom {
 // counter serialises its atomics (about 35 ns each, measured).
 struct PartSet {
   u64* h;
-  u16* a;
+  low_t* a;
   u32* fill;
   u64 cap;
   u32 nb, subs, fill_stride;
@@ -47,21 +51,21 @@ struct PartSide {
 };
 
 // geometry for a table of `block` cells per third
-inline PartGeom make_part_geom(const XorDev& x, const u16* lo16) {
+inline PartGeom make_part_geom(const XorDev& x, const low_t* lo) {
   PartGeom g{};
   g.x = x;
-  g.lo16 = lo16;
+  g.lo = lo;
   g.slice_shift = kSliceShift;
   g.fine_shift = kFineShift;
   const u64 ns = (x.block + kSliceCells - 1) >> kSliceShift;
   g.lo_third = ns << kSliceShift;
   g.nc = (u32)((ns + (1u << kFineShift) - 1) >> kFineShift);
-  g.mask16 = (u32)(x.mask & 0xFFFFu);
+  g.mask_lo = (u32)(x.mask & ((1u << kLowBits) - 1u));
   return g;
 }
 
-// lo16 of g (3 * g.lo_third entries) from the expanded cells of g.x
-int launch_part_lo16(const PartGeom& g, u16* lo, hipStream_t s);
+// the low-bits table of g (3 * g.lo_third entries) from the expanded cells of g.x
+int launch_part_low(const PartGeom& g, low_t* lo, hipStream_t s);
 // n hashes at h (tile_fill: tile t holds tile_fill[t] <= 4096 hashes at h + 4096 t)
 int launch_part_first(const u64* h, u64 n, const u32* tile_fill, const PartGeom& g, const PartSet& out,
                       const PartSide& side, hipStream_t s);

@@ -53,12 +53,12 @@ struct PartBufs {
   u64 records = 0;  // emit capacity in hashes: n_chunks * kEmitChunk
   u32 n_chunks = 0;
   PartGeom g{};
-  u16* lo16 = nullptr;
+  low_t* lo = nullptr;
   u64* flat = nullptr;
   u32* chunk_fill = nullptr;
   u32* ctr = nullptr;  // [0] chunk counter, [1] side n, [2] side lost, [3] acc n, [4] acc lost
   u64 *ch = nullptr, *fh = nullptr, *side_h = nullptr, *acc_h = nullptr, *set_h = nullptr;
-  u16 *ca = nullptr, *fa = nullptr;
+  low_t *ca = nullptr, *fa = nullptr;
   u32 *cfill = nullptr, *ffill = nullptr, *set_bits = nullptr;
   u64 cap_c = 0, cap_f = 0;
   u32 side_cap = 0, acc_cap = 0, set_cap = 0;
@@ -115,9 +115,11 @@ int ensure_part(fc5_ctx* ctx, PartBufs*& out) {
   const double per_slice = (double)P->records * (double)kSliceCells / (double)x.block;
   P->cap_c = bucket_cap(per_slice * (1u << kFineShift) / kCoarseSubs);
   P->cap_f = bucket_cap(per_slice);
-  P->side_cap = (u32)std::min<u64>(0x7FFFFFFFull, P->records / 4096 + (1u << 20));
+  // candidates: about 2^-min(r, kLowBits) of the records, so the list holds twice that
+  const u32 lowb = std::min<u32>(ctx->main.hdr.r, kLowBits);
+  P->side_cap = (u32)std::min<u64>(0x7FFFFFFFull, (P->records >> lowb) * 2 + (1u << 20));
   P->acc_cap = 1u << 22;
-  const bool ok = P->get(P->lo16, 3 * P->g.lo_third) && P->get(P->flat, P->records) &&
+  const bool ok = P->get(P->lo, 3 * P->g.lo_third) && P->get(P->flat, P->records) &&
                   P->get(P->chunk_fill, P->n_chunks) && P->get(P->ctr, 8) && P->get(P->ch, P->cap_c * ncr) &&
                   P->get(P->ca, P->cap_c * ncr) && P->get(P->cfill, ncr * kCoarseFillStride) &&
                   P->get(P->fh, P->cap_f * nf) && P->get(P->fa, P->cap_f * nf) && P->get(P->ffill, nf) &&
@@ -126,13 +128,13 @@ int ensure_part(fc5_ctx* ctx, PartBufs*& out) {
     delete P;
     FAIL(ctx, FC5_E_NOMEM, "partitioned probing buffers (lower the option part_records)");
   }
-  P->g.lo16 = P->lo16;
+  P->g.lo = P->lo;
   for (hipEvent_t& e : P->ev)
     if (hipEventCreate(&e) != hipSuccess) {
       delete P;
       FAIL(ctx, FC5_E_HIP, "pipeline events");
     }
-  if (launch_part_lo16(P->g, P->lo16, ctx->stream) != 0) {
+  if (launch_part_low(P->g, P->lo, ctx->stream) != 0) {
     delete P;
     FAIL(ctx, FC5_E_HIP, "low-bits table");
   }
@@ -295,9 +297,9 @@ extern "C" int fc5_measure_partitioned(fc5_ctx* ctx, uint64_t n, double* ms, uin
   const XorDev& x = ctx->main.dev;
   DevArena A;
   const PartGeom g0 = make_part_geom(x, nullptr);
-  u16* lo16 = A.get<u16>(3 * g0.lo_third);
-  if (!lo16) FAIL(ctx, FC5_E_NOMEM, "low-bits table");
-  const PartGeom g = make_part_geom(x, lo16);
+  low_t* lo = A.get<low_t>(3 * g0.lo_third);
+  if (!lo) FAIL(ctx, FC5_E_NOMEM, "low-bits table");
+  const PartGeom g = make_part_geom(x, lo);
   const u64 ns = (x.block + kSliceCells - 1) >> kSliceShift;
   if (g.nc == 0 || g.nc > kPartMaxBins) FAIL(ctx, FC5_E_ARG, "table too large for two partition levels");
   const u32 nf = g.nc << kFineShift;
@@ -305,14 +307,15 @@ extern "C" int fc5_measure_partitioned(fc5_ctx* ctx, uint64_t n, double* ms, uin
   const double per_slice = (double)n * (double)kSliceCells / (double)x.block;
   const u64 cap_c = bucket_cap(per_slice * (1u << kFineShift) / kCoarseSubs);  // per region
   const u64 cap_f = bucket_cap(per_slice);
-  const u32 side_cap = (u32)std::min<u64>(0x7FFFFFFFull, n / 4096 + (1u << 20));
+  const u32 side_cap = (u32)std::min<u64>(
+      0x7FFFFFFFull, (n >> std::min<u32>(ctx->main.hdr.r, kLowBits)) * 2 + (1u << 20));
   u64* flat = A.get<u64>(n);
   const u64 ncr = (u64)g.nc * kCoarseSubs;  // coarse regions
   u64* ch = A.get<u64>(cap_c * ncr);
-  u16* ca = A.get<u16>(cap_c * ncr);
+  low_t* ca = A.get<low_t>(cap_c * ncr);
   u32* cfill = A.get<u32>(ncr * kCoarseFillStride);
   u64* fh = A.get<u64>(cap_f * nf);
-  u16* fa = A.get<u16>(cap_f * nf);
+  low_t* fa = A.get<low_t>(cap_f * nf);
   u32* ffill = A.get<u32>(nf);
   u64* sh = A.get<u64>(side_cap);
   u64* ah = A.get<u64>(side_cap);
@@ -337,7 +340,7 @@ extern "C" int fc5_measure_partitioned(fc5_ctx* ctx, uint64_t n, double* ms, uin
     }
   } ev_free{ev};
   auto zero = [&](u32* p, u64 words) { return hipMemsetAsync(p, 0, words * 4, s) == hipSuccess; };
-  bool ok = launch_part_lo16(g, lo16, s) == 0 && launch_part_gen(x, n, 7, flat, s) == 0 &&
+  bool ok = launch_part_low(g, lo, s) == 0 && launch_part_gen(x, n, 7, flat, s) == 0 &&
             zero(ctr, 4) && hipMemsetAsync(sums, 0, 32, s) == hipSuccess;
   // the timed pipeline (events between the stages)
   ok = ok && hipEventRecord(ev[0], s) == hipSuccess && zero(cfill, ncr * kCoarseFillStride) &&

@@ -234,7 +234,7 @@ def test_bench_default_config_finds_its_counter_summaries():
 
 def test_bench_partition_figures_and_lookup_traffic_summary():
     """the partitioned run's roofline (k_part_lookup: event-timed stage sums over
-    the pieces, 38 B per probe over a piece's two lookup launches plus the
+    the pieces, 35 B per probe over a piece's two lookup launches plus the
     staged slices) and the committed traffic summary it reads"""
     import bench
 
@@ -249,9 +249,9 @@ def test_bench_partition_figures_and_lookup_traffic_summary():
     roof = f["roofline"]
     assert roof["kernel"] == "k_part_lookup" and roof["launches_per_step"] == 24
     assert abs(roof["avg_launch_ms"] - (274.0 + 299.0) / 24) < 1e-9
-    slices = -(-(H.m_cells // 3) // 32768)
-    assert roof["algorithmic_bytes_per_launch"] == (probes * 38 + 24 * slices * 65536) / 24
-    assert 0 < roof["frac"] < 1 and f["stages"]["fine1"]["bytes"] == probes * 20
+    slices = -(-(H.m_cells // 3) // 65536)
+    assert roof["algorithmic_bytes_per_launch"] == (probes * 35 + 24 * slices * 65536) / 24
+    assert 0 < roof["frac"] < 1 and f["stages"]["fine1"]["bytes"] == probes * 18
     cfg = bench.CONFIGS[4]
     t = bench.load_profile(f"traffic_part_lookup_B{cfg['lo']}-{cfg['hi']}.json")
     assert t and 1.0 <= t["traffic_bytes_per_launch"] / t["algorithmic_bytes_per_launch"] < 1.2
