@@ -408,7 +408,7 @@ int fc5_measure_read(fc5_ctx *ctx, uint64_t bytes, double *ms);
 int fc5_measure_valu(fc5_ctx *ctx, int kind, double *instr_per_s);
 /* partitioned probing of n <= 2^32-1 pseudo-random keys (those of
  * fc5_measure_gathers) against the main table: keys radix-partitioned by table
- * slice, each slice's low 16 cell bits in LDS, candidates fully queried.
+ * slice, each slice's low 8 cell bits in LDS, candidates fully queried.
  * ms[9]: first partition, fine partition and lookup of cells 0, 1, 2 (six
  * stages), full queries, then the direct queries of the same keys.
  * counts[8]: accepted and the sum of their hashes, the same for the direct

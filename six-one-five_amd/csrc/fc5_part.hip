@@ -40,7 +40,7 @@ constexpr int kPartThreads = 256;
 constexpr int kLookThreads = 1024;
 constexpr int kTileRecs = 4096;
 
-// LDS of one tile scatter into at most NB buckets (NB = 256: 52 KiB, three
+// LDS of one tile scatter into at most NB buckets (NB = 256: 47 KiB, three
 // blocks per CU)
 template <int NB>
 struct TileSmem {

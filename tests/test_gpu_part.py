@@ -6,7 +6,7 @@ with three gathers per probe (probe_mode 0), which the other GPU tests pin
 to the oracle; one band is also compared with the oracle directly.  Small
 emit buffers force many pieces and the direct-query fallback of probes that
 find no free chunk; r = 16 and r = 24 tables make the accepted sets and the
-16-bit candidate stage non-trivial.
+8-bit candidate stage non-trivial.
 """
 import os
 import sys
@@ -69,7 +69,7 @@ def test_measure_partitioned_equals_direct_queries(ctx, N, r):
     assert c["lost"] == 0
     assert c["accepted"] == c["direct_accepted"]
     assert c["accepted_hsum"] == c["direct_hsum"]
-    # r = 16: about 2^8 accepted keys; r = 24: about 2^8 16-bit candidates, each fully queried
+    # r = 16: about 2^8 accepted keys; r = 24: about 2^16 8-bit candidates, each fully queried
     assert (c["accepted"] if r == 16 else c["full_queries"]) > 100
 
 
