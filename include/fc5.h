@@ -131,7 +131,13 @@ const char *fc5_last_error(const fc5_ctx *ctx);
  * check of hits: device pair search + host identity check), "verify_table" (0;
  * 1 answers the exact check from the exact sorted pair table below),
  * "twin_*" (the (6,1,7) twin options below; "twin_plain" 1 runs its
- * reference-shaped kernel),
+ * reference-shaped kernel), "probe_mode" (-1, the default: the batched search
+ * probes the filter by partition for tables of 2^28 cells or more with r >= 24,
+ * else with three gathers per probe; 0 gathers; 1 by partition; same counters
+ * and hits), "part_records" (emit buffer of the partitioned probe, 4096..2^32-1
+ * hashes, default 2^32 - 4096), "part_hbm_limit" (bytes the partitioned probe's
+ * buffers may take, 0 = no limit; beyond it, or when they cannot be allocated,
+ * probe_mode -1 probes with gathers and probe_mode 1 fails with FC5_E_NOMEM),
  * "stop_first" (0, fourcore_cls5_gpu_v4.cu:635) */
 int fc5_set_option(fc5_ctx *ctx, const char *key, int64_t value);
 

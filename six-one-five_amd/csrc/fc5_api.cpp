@@ -162,6 +162,10 @@ extern "C" int fc5_set_option(fc5_ctx* ctx, const char* key, int64_t v) try {
     if (v < (int64_t)kEmitChunk || v > (int64_t)0xFFFFFFFF) FAIL(ctx, FC5_E_ARG, "part_records must be 4096..2^32-1");
     if ((u64)v != ctx->part_records) free_part(ctx);  // the buffers are made for one capacity
     ctx->part_records = (u64)v;
+  } else if (k == "part_hbm_limit") {
+    if (v < 0) FAIL(ctx, FC5_E_ARG, "part_hbm_limit must be >= 0 (bytes; 0: no limit)");
+    if ((u64)v != ctx->part_hbm_limit) free_part(ctx);
+    ctx->part_hbm_limit = (u64)v;
   } else if (k == "blocks_per_cu") {
     if (v < 0 || v > 64) FAIL(ctx, FC5_E_ARG, "blocks_per_cu out of range");
     ctx->blocks_per_cu = (int)v;

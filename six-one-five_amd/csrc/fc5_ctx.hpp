@@ -170,6 +170,8 @@ struct fc5_ctx {
   // the last partitioned run's stage times and counts
   int probe_mode = -1;
   u64 part_records = (1ull << 32) - kEmitChunk;  // 1,048,575 emit chunks
+  u64 part_hbm_limit = 0;  // bytes the buffers may take (0: no limit); beyond it, as on a failed allocation,
+                           // probe_mode -1 probes with gathers and probe_mode 1 fails
   PartBufs* part = nullptr;
   double part_ms[3] = {0, 0, 0};  // emit launches, partition pipeline, identify launch (host clock)
   // event-timed pipeline stages summed over the pieces: first partition, fine

@@ -119,6 +119,13 @@ int ensure_part(fc5_ctx* ctx, PartBufs*& out) {
   const u32 lowb = std::min<u32>(ctx->main.hdr.r, kLowBits);
   P->side_cap = (u32)std::min<u64>(0x7FFFFFFFull, (P->records >> lowb) * 2 + (1u << 20));
   P->acc_cap = 1u << 22;
+  const u64 need = 3 * P->g.lo_third * sizeof(low_t) + P->records * 8 + (u64)P->n_chunks * 4 + 64 +
+                   (P->cap_c * ncr + P->cap_f * nf) * (8 + sizeof(low_t)) + (ncr * kCoarseFillStride + nf) * 4 +
+                   ((u64)P->side_cap + P->acc_cap) * 8 + (1u << kSetFilterBits) / 8;
+  if (ctx->part_hbm_limit && need > ctx->part_hbm_limit) {
+    delete P;
+    FAIL(ctx, FC5_E_NOMEM, "partitioned probing buffers exceed the option part_hbm_limit");
+  }
   const bool ok = P->get(P->lo, 3 * P->g.lo_third) && P->get(P->flat, P->records) &&
                   P->get(P->chunk_fill, P->n_chunks) && P->get(P->ctr, 8) && P->get(P->ch, P->cap_c * ncr) &&
                   P->get(P->ca, P->cap_c * ncr) && P->get(P->cfill, ncr * kCoarseFillStride) &&
@@ -199,6 +206,12 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 int run_partitioned(fc5_ctx* ctx, const BatchedArgs& a0, int grid, int rows) {
   PartBufs* P = nullptr;
   int rc = ensure_part(ctx, P);
+  if (rc == FC5_E_NOMEM && ctx->probe_mode < 0) {
+    // chosen by table size only: without room for the buffers, probe with gathers
+    (void)hipGetLastError();
+    if (launch_cls5_batched(a0, grid, rows, ctx->stream) != 0) FAIL(ctx, FC5_E_HIP, "batched launch");
+    return FC5_OK;
+  }
   if (rc != FC5_OK) return rc;
   hipStream_t s = ctx->stream;
   BatchedArgs a = a0;

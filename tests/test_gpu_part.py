@@ -153,3 +153,24 @@ def test_flagship_table_512_coarse_buckets_partitioned_equals_gathers(ctx):
     say(f"part/flagship table: {st[1]}, probes={part.result['probes']}")
     assert st[1]["chunks"] > 0
     _same(direct, part)
+
+
+def test_buffers_over_the_hbm_limit_fall_back_to_gathers(ctx):
+    """option part_hbm_limit: by table size (-1) the search probes with gathers
+    when the buffers would not fit, with the same results; forced (1) it fails"""
+    _, Bs, us = fc5.emit_units(3200001, 3210000)
+    units = fc5.units_array(Bs[::997], us[::997])
+    ctx.table_build(76428, 48, on_gpu=True)
+    ctx.set_option("part_hbm_limit", 1 << 30)
+    try:
+        out = ctx.run(units, fc5.MODE_BATCHED)
+        assert ctx.part_stats()[1]["pieces"] == 0  # probed with gathers
+        ctx.set_option("probe_mode", 1)
+        with pytest.raises(fc5.Fc5Error):
+            ctx.run(units, fc5.MODE_BATCHED)
+    finally:
+        ctx.set_option("probe_mode", -1)
+        ctx.set_option("part_hbm_limit", 0)
+    part = ctx.run(units, fc5.MODE_BATCHED)
+    assert ctx.part_stats()[1]["pieces"] > 0
+    _same(out, part)
