@@ -254,41 +254,30 @@ struct LookSmem {
   u32 total[2];
 };
 
-// Rank a tile's valid records by bucket: zero the histogram, LDS atomics
-// (rank = place within the bucket), exclusive scan into lofs and total.
-// Every thread of the block calls it; it ends synchronised.
-template <int NT, int R>
-__device__ __forceinline__ void rank_tile(u32* hist, u32* lofs, u32* total, const u32 (&dig)[R], u32 valid,
-                                          u32 nbins, u32 (&rank)[R]) {
+// Exclusive scan of nbins bucket counts into lofs, their sum into *total;
+// called by the block's first wavefront (threads 0-63) only
+__device__ __forceinline__ void scan_bins(const u32* hist, u32* lofs, u32* total, u32 nbins) {
   const u32 t = threadIdx.x;
-  for (u32 b = t; b < nbins; b += NT) hist[b] = 0;
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < R; ++j) rank[j] = ((valid >> j) & 1u) ? atomicAdd(&hist[dig[j]], 1u) : 0u;
-  __syncthreads();
-  if (t < 64) {
-    const u32 per = (nbins + 63) / 64;
-    u32 sum = 0;
-    for (u32 k = 0; k < per; ++k) {
-      const u32 b = t * per + k;
-      if (b < nbins) sum += hist[b];
-    }
-    u32 incl = sum;
-    for (int o = 1; o < 64; o <<= 1) {
-      const u32 v = (u32)__shfl_up((int)incl, o, 64);
-      if ((int)t >= o) incl += v;
-    }
-    u32 run = incl - sum;
-    for (u32 k = 0; k < per; ++k) {
-      const u32 b = t * per + k;
-      if (b < nbins) {
-        lofs[b] = run;
-        run += hist[b];
-      }
-    }
-    if (t == 63) *total = incl;
+  const u32 per = (nbins + 63) / 64;
+  u32 sum = 0;
+  for (u32 k = 0; k < per; ++k) {
+    const u32 b = t * per + k;
+    if (b < nbins) sum += hist[b];
   }
-  __syncthreads();
+  u32 incl = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const u32 v = (u32)__shfl_up((int)incl, o, 64);
+    if ((int)t >= o) incl += v;
+  }
+  u32 run = incl - sum;
+  for (u32 k = 0; k < per; ++k) {
+    const u32 b = t * per + k;
+    if (b < nbins) {
+      lofs[b] = run;
+      run += hist[b];
+    }
+  }
+  if (t == 63) *total = incl;
 }
 
 // Each loaded record's cell of pass p (in slice [c0, c0 + len)) from LDS into
@@ -318,10 +307,10 @@ __device__ __forceinline__ u32 look_tile(const PartGeom& g, int p, const low_t* 
 
 // One block per fine bucket (= slice) of pass p < 2: each record's cell of
 // this pass from LDS into its partial xor, then on to the coarse bucket of
-// its next cell.  The tiles of the bucket go through a software pipeline, so
-// that no load or reservation round trip is waited for in full: while tile k
-// is staged and written out, tile k+1 is looked up, ranked and reserved and
-// tile k+2 is loaded.
+// its next cell.  The tiles of the bucket go through a software pipeline
+// with two barriers per tile, so that no load or reservation round trip is
+// waited for in full: while tile k is staged and written out, tile k+1 is
+// looked up, ranked, scanned and reserved and tile k+2 is loaded.
 template <int NB>
 __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g, PartSet in, PartSet out,
                                                                PartSide side) {
@@ -331,24 +320,30 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
   const u32 f = blockIdx.x;
   const u64 n = min((u64)in.fill[(u64)f * in.fill_stride], in.cap);
   if (n == 0) return;
+  const u32 t = threadIdx.x, nbins = g.nc, sub = f % out.subs;
+  // both count buffers start at zero (published by the slice's barrier)
+  for (u32 b = t; b < nbins; b += NT) s.hist[0][b] = s.hist[1][b] = 0;
   const u32 len = stage_slice<NT>(sl, g, p, f);
   const u64 c0 = (u64)f << g.slice_shift;
   const u64 off0 = (u64)f * in.cap;
-  const u32 t = threadIdx.x, nbins = g.nc, sub = f % out.subs;
   const u64 ntiles = (n + kTileRecs - 1) / kTileRecs;
   auto cnt_of = [&](u64 k) { return (u32)min((u64)kTileRecs, n - k * kTileRecs); };
   auto reserve = [&](u32 b, u32 c) {
     return c ? atomicAdd(out.fill + ((u64)b * out.subs + sub) * out.fill_stride, c) : 0u;
   };
-  // tile 0: loaded, looked up, ranked and reserved; tile 1 loading
+  // tile 0: loaded, looked up, ranked, scanned and reserved; tile 1 loading
   u64 hc[R];
   low_t ac[R];
   u32 dc[R], rc[R];
   load_tile<NT, R>(hc, in.h + off0, cnt_of(0));
   load_tile<NT, R>(ac, in.a ? in.a + off0 : nullptr, cnt_of(0));
   u32 vc = look_tile<NT, R>(g, p, sl, c0, len, cnt_of(0), hc, ac, dc, side);
-  rank_tile<NT, R>(s.hist[0], s.lofs[0], &s.total[0], dc, vc, nbins, rc);
+#pragma unroll
+  for (int j = 0; j < R; ++j) rc[j] = ((vc >> j) & 1u) ? atomicAdd(&s.hist[0][dc[j]], 1u) : 0u;
+  __syncthreads();
+  if (t < 64) scan_bins(s.hist[0], s.lofs[0], &s.total[0], nbins);
   u32 gb = t < nbins ? reserve(t, s.hist[0][t]) : 0u;
+  __syncthreads();
   u64 hn[R];
   low_t an[R];
   if (ntiles > 1) {
@@ -374,7 +369,11 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
         load_tile<NT, R>(an, in.a ? in.a + o : nullptr, cnt_of(k + 2));
       }
     }
-    // tile k: reservation and staging (published by the next barrier)
+    // tile k's counts were last read by the scan and the reservations of the
+    // previous iteration (before its last barrier): zeroed here for tile k+2
+    for (u32 b = t; b < nbins; b += NT) s.hist[cur][b] = 0;
+    // tile k: reservation and staging; tile k+1: ranks (its count buffer was
+    // zeroed in the previous iteration)
     if (t < nbins) s.gbase[t] = gb;
 #pragma unroll
     for (int j = 0; j < R; ++j)
@@ -384,10 +383,17 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
         s.a[pos] = ac[j];
         s.d[pos] = (u16)dc[j];
       }
-    if (more)
-      rank_tile<NT, R>(s.hist[nxt], s.lofs[nxt], &s.total[nxt], dx, vx, nbins, rx);
-    else
-      __syncthreads();
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) rx[j] = ((vx >> j) & 1u) ? atomicAdd(&s.hist[nxt][dx[j]], 1u) : 0u;
+    }
+    __syncthreads();
+    // tile k+1: offsets and reservation (the offsets are used after the
+    // iteration's last barrier, the reservation in the next iteration)
+    if (more) {
+      if (t < 64) scan_bins(s.hist[nxt], s.lofs[nxt], &s.total[nxt], nbins);
+      if (t < nbins) gb = reserve(t, s.hist[nxt][t]);
+    }
     // tile k: write-out in runs per bucket
     const u32 total = s.total[cur];
     for (u32 i = t; i < total; i += NT) {
@@ -401,9 +407,7 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
         side_push(side, s.h[i]);
       }
     }
-    // tile k+1: reservation, waited for only in the next iteration
-    if (more && t < nbins) gb = reserve(t, s.hist[nxt][t]);
-    __syncthreads();  // staging and gbase are reused by the next iteration
+    __syncthreads();  // staging, gbase and offsets are reused by the next iteration
     if (more) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
