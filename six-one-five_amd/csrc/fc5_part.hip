@@ -428,7 +428,7 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
 // block and flush instead of one per candidate (a single counter serialises
 // its atomics).
 constexpr u32 kCandBuf = 1024;
-__global__ __launch_bounds__(kLookThreads) void k_part_last(PartGeom g, PartSet in, PartSide side) {
+__global__ __launch_bounds__(kLookThreads, 8) void k_part_last(PartGeom g, PartSet in, PartSide side) {
   constexpr int NT = kLookThreads, R = 8;
   __shared__ __align__(16) low_t sl[kSliceCells];
   __shared__ u64 cand[kCandBuf];
