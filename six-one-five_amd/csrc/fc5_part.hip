@@ -426,7 +426,8 @@ __global__ __launch_bounds__(kLookThreads) void k_part_lookup(int p, PartGeom g,
 // candidates (2^-8 of the records) gather in an LDS buffer, and the block
 // reserves room in the side list once per flush: one global atomic per
 // block and flush instead of one per candidate (a single counter serialises
-// its atomics).
+// its atomics).  Held to 64 VGPRs (8 waves per SIMD), so that two blocks,
+// each with its 64 KiB slice and 8 KiB of candidates, share a CU.
 constexpr u32 kCandBuf = 1024;
 __global__ __launch_bounds__(kLookThreads, 8) void k_part_last(PartGeom g, PartSet in, PartSide side) {
   constexpr int NT = kLookThreads, R = 8;
